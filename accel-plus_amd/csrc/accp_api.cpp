@@ -1,0 +1,654 @@
+// accp_api.cpp -- the C ABI of include/accp.h: element-table compiler,
+// device memory, plain / batched / graph launches, beam statistics.
+//
+// Host code; built with hipcc (HIP runtime API) and -ffp-contract=off so the
+// per-element constants below are evaluated exactly as the reference
+// evaluates them at every pass.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/accp.h"
+#include "accp_adjoint.h"
+#include "accp_kernels.h"
+#include "accp_table.h"
+#include "accp_wake.h"
+
+using namespace accp;
+
+// ---- errors ---------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess) return fail(ACCP_E_HIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+extern "C" const char *accp_last_error(void) { return g_err.c_str(); }
+extern "C" int accp_abi_version(void) { return ACCP_ABI_VERSION; }
+
+extern "C" int accp_device_count(int *count) {
+  if (!count) return fail(ACCP_E_ARG, "count is null");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  *count = (e == hipSuccess) ? c : 0;
+  return ACCP_OK;
+}
+
+// ---- element-table compiler ---------------------------------------------------
+namespace {
+
+int highest_nonzero(const double *b, const double *a, int n) {
+  int hi = -1;
+  for (int i = 0; i < n; ++i)
+    if (b[i] != 0 || (a && a[i] != 0)) hi = i;
+  return hi;
+}
+
+struct Compiled {
+  std::vector<Hdr> hdr;
+  std::vector<double> par;
+  std::vector<int> wakes;  // element indices of LongWake records
+};
+
+int compile_lattice(const accp_elem *E, int n, Compiled &out) {
+  out.hdr.assign(n, Hdr{OP_NOP, 0, 0, 0});
+  out.par.clear();
+  out.wakes.clear();
+  for (int i = 0; i < n; ++i) {
+    const accp_elem &e = E[i];
+    Hdr h{OP_NOP, (int32_t)out.par.size(), 0, 0};
+    std::vector<double> &P = out.par;
+    switch (e.type) {
+      case ACCP_DRIFT:
+        h.op = OP_DRIFT;
+        P.push_back(e.len);
+        break;
+      case ACCP_QUAD:
+        if (e.k == 0) {
+          h.op = OP_QUAD_K0;
+          P.push_back(e.len);
+        } else {
+          h.op = OP_QUAD;
+          P.insert(P.end(), {e.len, e.k, std::fabs(e.k)});
+        }
+        break;
+      case ACCP_CORRECTOR:
+        h.op = OP_CORRECTOR;
+        P.insert(P.end(), {e.len, e.kickx, e.kicky});
+        break;
+      case ACCP_APERTURE:
+        h.op = OP_APERTURE;
+        P.insert(P.end(), {e.len, e.xlim[0], e.xlim[1], e.ylim[0], e.ylim[1]});
+        break;
+      case ACCP_MONITOR:
+        h.op = OP_NOP;
+        break;
+      case ACCP_BEND: {
+        // ABend: irho = theta/len (accel.h:170), Kx = angle/L in bend6
+        // (atpass.cpp:84), psi = irho*tan(e) in edge (atpass.cpp:192).
+        const double irho = e.theta / e.len;
+        h.op = OP_BEND;
+        P.insert(P.end(), {e.len, e.theta / e.len, e.k, irho, irho * std::tan(e.e1),
+                           irho * std::tan(e.e2)});
+        break;
+      }
+      case ACCP_BENDMPOLE: {
+        if (e.nslice < 1 || e.max_npole < 1 || e.max_npole > ACCP_MAX_MPOLE)
+          return fail(ACCP_E_LATTICE, "element %d: BendMPole needs nslice>=1, 1<=MaxNPole<=10", i);
+        const int mo = e.max_npole - 1;
+        // constants of ABendMPole::pass (accel.cpp:651-656) and
+        // bndthinkickrad (atpass.cpp:272-273, :319)
+        const double sl = e.len / e.nslice;
+        const double l1 = sl * DRIFT1, l2 = sl * DRIFT2, k1 = sl * KICK1, k2 = sl * KICK2;
+        const double irho = e.theta / e.len;
+        const double E0 = e.energy;
+        const double crad = CGAMMA * E0 * E0 * E0 / (TWOPI * 1e9);
+        const double uce0 = 0.665 * E0 * E0 / 1.0e6 / 2.99792458E8 * std::fabs(irho) * 1000;
+        const double cq = 55 * std::sqrt(3.0) / 72.0 * uce0;
+        const double hs1 = irho * l1, hs2 = irho * l2, half = 0.5 * sl, hsh = irho * half;
+        const double k1c = mo >= 1 ? e.polyB[1] : 0.0;
+        const double k2c = mo >= 2 ? e.polyB[2] : 0.0;
+        int hi = highest_nonzero(e.polyB, nullptr, mo + 1);
+        if (hi < 0) hi = 0;
+        const int plain = (k1c == 0 && k2c == 0) ? 1 : 0;
+        h.op = e.rad == ACCP_RAD_DAMP ? OP_BENDMP_RAD : e.rad == ACCP_RAD_QE ? OP_BENDMP_QE : OP_BENDMP;
+        h.a = e.nslice;
+        h.b = hi | (plain << 8);
+        double rec[BM_B];
+        rec[BM_PSI1] = irho * std::tan(e.e1);
+        rec[BM_PSI2] = irho * std::tan(e.e2);
+        rec[BM_IRHO] = irho;
+        rec[BM_L1] = l1; rec[BM_L2] = l2; rec[BM_K1] = k1; rec[BM_K2] = k2; rec[BM_SL] = sl;
+        rec[BM_CRAD] = crad; rec[BM_CQ] = cq;
+        rec[BM_HS1] = hs1; rec[BM_HQ1] = 1 / 4. * hs1 * l1;
+        rec[BM_HS2] = hs2; rec[BM_HQ2] = 1 / 4. * hs2 * l2;
+        rec[BM_HSH] = hsh; rec[BM_HQH] = 1 / 4. * hsh * half;
+        rec[BM_K1C] = k1c; rec[BM_K2C] = k2c;
+        P.insert(P.end(), rec, rec + BM_B);
+        for (int j = 0; j <= hi; ++j) P.push_back(e.polyB[j]);
+        break;
+      }
+      case ACCP_STRMPOLE: {
+        if (e.max_npole < 1 || e.max_npole > ACCP_MAX_MPOLE || (e.len != 0 && e.nslice < 1))
+          return fail(ACCP_E_LATTICE, "element %d: StrMPole needs 1<=MaxNPole<=10, nslice>=1", i);
+        const int hi = highest_nonzero(e.polyB, e.polyA, e.max_npole);
+        h.b = hi;
+        if (e.len == 0) {
+          h.op = OP_STRMP_THIN;  // one integrated kick, L = 1 (accel.cpp:824-827)
+        } else {
+          const double sl = e.len / e.nslice;  // accel.cpp:804-808
+          h.op = OP_STRMP;
+          h.a = e.nslice;
+          P.insert(P.end(), {sl * DRIFT1, sl * DRIFT2, sl * KICK1, sl * KICK2});
+        }
+        for (int j = 0; j <= hi; ++j) P.push_back(e.polyB[j]);
+        for (int j = 0; j <= hi; ++j) P.push_back(e.polyA[j]);
+        break;
+      }
+      case ACCP_CAVITY:
+        if (e.energy == 0) return fail(ACCP_E_LATTICE, "element %d: Cavity energy is zero", i);
+        h.op = OP_CAVITY;
+        h.a = e.on ? 1 : 0;
+        P.insert(P.end(), {e.len, e.volt / e.energy, TWOPI * e.freq / C0, e.phis});
+        break;
+      case ACCP_LONGWAKE:
+        if (e.nbins < 1 || e.nbins > WAKE_MAX_BINS || !(e.wake_zmax > e.wake_zmin) ||
+            e.wake_q <= 0.5 || e.energy == 0)
+          return fail(ACCP_E_LATTICE,
+                      "element %d: LongWake needs 1<=nbins<=%d, zmax>zmin, Q>1/2, energy>0", i,
+                      WAKE_MAX_BINS);
+        h.op = e.on ? OP_WAKE : OP_NOP;
+        if (e.on) out.wakes.push_back(i);
+        break;
+      default:
+        return fail(ACCP_E_LATTICE, "element %d: unsupported type code %d", i, e.type);
+    }
+    out.hdr[i] = h;
+  }
+  if (out.par.empty()) out.par.push_back(0.0);
+  return ACCP_OK;
+}
+
+size_t lds_bytes_for(size_t nelem, size_t npar) { return nelem * sizeof(Hdr) + npar * sizeof(double); }
+
+constexpr size_t LDS_LIMIT = 64 * 1024;
+
+// device buffer that only grows
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return ACCP_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(ACCP_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    cap = bytes;
+    return ACCP_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+}  // namespace
+
+// ---- lattice -------------------------------------------------------------------
+struct accp_lattice {
+  std::vector<accp_elem> elems;
+  Compiled tab;
+  DevBuf d_hdr, d_par, d_meta;  // meta: lat_hdr0, lat_par0, lat_npar of one lattice
+  DevBuf d_beam, d_snap, d_stat;
+  WakeState wake;
+};
+
+static int upload_lattice(accp_lattice *L) {
+  const size_t nh = L->tab.hdr.size(), np = L->tab.par.size();
+  int rc;
+  if ((rc = L->d_hdr.ensure(std::max<size_t>(nh, 1) * sizeof(Hdr)))) return rc;
+  if ((rc = L->d_par.ensure(np * sizeof(double)))) return rc;
+  if ((rc = L->d_meta.ensure(3 * sizeof(int32_t)))) return rc;
+  if (nh) HIPCHK(hipMemcpy(L->d_hdr.p, L->tab.hdr.data(), nh * sizeof(Hdr), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(L->d_par.p, L->tab.par.data(), np * sizeof(double), hipMemcpyHostToDevice));
+  int32_t meta[3] = {0, 0, (int32_t)np};
+  HIPCHK(hipMemcpy(L->d_meta.p, meta, sizeof meta, hipMemcpyHostToDevice));
+  return wake_prepare(L->wake, L->elems.data(), (int)L->elems.size(), L->tab.wakes);
+}
+
+extern "C" int accp_lattice_build(const accp_elem *elems, int n, accp_lattice **out) {
+  if (!elems || n <= 0 || !out) return fail(ACCP_E_ARG, "accp_lattice_build: bad arguments");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(ACCP_E_HIP, "no HIP device available");
+  accp_lattice *L = new accp_lattice;
+  L->elems.assign(elems, elems + n);
+  int rc = compile_lattice(elems, n, L->tab);
+  if (!rc) rc = upload_lattice(L);
+  if (rc) {
+    accp_lattice_free(L);
+    return rc;
+  }
+  *out = L;
+  return ACCP_OK;
+}
+
+extern "C" int accp_lattice_update(accp_lattice *L, const accp_elem *elems, int n) {
+  if (!L || !elems || n != (int)L->elems.size())
+    return fail(ACCP_E_ARG, "accp_lattice_update: lattice and element count must match");
+  for (int i = 0; i < n; ++i)
+    if (elems[i].type != L->elems[i].type)
+      return fail(ACCP_E_LATTICE, "accp_lattice_update: element %d changes type", i);
+  Compiled c;
+  int rc = compile_lattice(elems, n, c);
+  if (rc) return rc;
+  L->elems.assign(elems, elems + n);
+  L->tab = std::move(c);
+  return upload_lattice(L);
+}
+
+extern "C" int accp_lattice_nelem(const accp_lattice *L, int *n) {
+  if (!L || !n) return fail(ACCP_E_ARG, "null argument");
+  *n = (int)L->elems.size();
+  return ACCP_OK;
+}
+
+extern "C" int accp_lattice_free(accp_lattice *L) {
+  if (!L) return ACCP_OK;
+  L->d_hdr.release();
+  L->d_par.release();
+  L->d_meta.release();
+  L->d_beam.release();
+  L->d_snap.release();
+  L->d_stat.release();
+  wake_release(L->wake);
+  delete L;
+  return ACCP_OK;
+}
+
+// ---- launch shape ------------------------------------------------------------------
+static int resolve_tuning(const accp_tuning *t, size_t lds_need, int &block, int &ppl, bool &lds) {
+  block = (t && t->block) ? t->block : 64;
+  ppl = (t && t->ppl) ? t->ppl : 1;
+  const int st = t ? t->staging : ACCP_STAGE_AUTO;
+  if (block != 64 && block != 128 && block != 256) return fail(ACCP_E_ARG, "block must be 64, 128 or 256");
+  if (ppl != 1 && ppl != 2) return fail(ACCP_E_ARG, "ppl must be 1 or 2");
+  if (st == ACCP_STAGE_LDS && lds_need > LDS_LIMIT)
+    return fail(ACCP_E_ARG, "element table (%zu B) exceeds the LDS staging limit", lds_need);
+  lds = (st == ACCP_STAGE_LDS) || (st == ACCP_STAGE_AUTO && lds_need <= LDS_LIMIT);
+  return ACCP_OK;
+}
+
+// One plain pass of elements [st, ed) for nturn turns (no collective element
+// inside the range).
+static int launch_plain(accp_lattice *L, double *d_r, int64_t n, int nturn, int st, int ed,
+                        const accp_track_opts *o, double *snap, int nsnap, hipStream_t s) {
+  const size_t need = lds_bytes_for(L->tab.hdr.size(), L->tab.par.size());
+  int block, ppl;
+  bool lds;
+  int rc = resolve_tuning(o ? &o->tuning : nullptr, need, block, ppl, lds);
+  if (rc) return rc;
+  KernelCtx c{};
+  c.tiles = nullptr;
+  c.hdr = (const Hdr *)L->d_hdr.p;
+  c.par = (const double *)L->d_par.p;
+  c.lat_hdr0 = (const int32_t *)L->d_meta.p;
+  c.lat_par0 = (const int32_t *)L->d_meta.p + 1;
+  c.lat_npar = (const int32_t *)L->d_meta.p + 2;
+  c.nelem = (int32_t)L->tab.hdr.size();
+  c.nturn = nturn;
+  c.st = st;
+  c.ed = ed;
+  c.r = d_r;
+  c.n = n;
+  c.rng_seed = o ? o->rng_seed : 0;
+  c.turn0 = o ? o->turn0 : 0;
+  c.pid0 = o ? o->pid0 : 0;
+  c.snap = snap;
+  c.mon_interval = (snap && o) ? o->mon_interval : 1;
+  c.nsnap = nsnap;
+  const int64_t per = (int64_t)block * ppl;
+  const int64_t grid = (n + per - 1) / per;
+  if (grid > 0x7fffffff) return fail(ACCP_E_ARG, "too many particles for one launch");
+  HIPCHK(launch_track(c, (int)grid, block, ppl, lds, lds ? need : 0, s));
+  return ACCP_OK;
+}
+
+// Track, splitting every turn at active LongWake elements (collective: the
+// whole beam must reach the wake before its histogram is complete).
+static int track_device_impl(accp_lattice *L, double *d_r, int64_t n, int nturn, int st, int ed,
+                             const accp_track_opts *o, double *snap, int nsnap, hipStream_t s) {
+  std::vector<int> cuts;
+  for (int w : L->tab.wakes)
+    if (w >= st && w < ed) cuts.push_back(w);
+  if (cuts.empty()) return launch_plain(L, d_r, n, nturn, st, ed, o, snap, nsnap, s);
+  if (snap) return fail(ACCP_E_ARG, "monitor records with a wake element are not supported");
+  accp_track_opts oo{};
+  if (o) oo = *o;
+  for (int t = 0; t < nturn; ++t) {
+    oo.turn0 = (o ? o->turn0 : 0) + (uint64_t)t;
+    int a = st;
+    for (int w : cuts) {
+      if (w > a) {
+        int rc = launch_plain(L, d_r, n, 1, a, w, &oo, nullptr, 0, s);
+        if (rc) return rc;
+      }
+      int rc = wake_apply(L->wake, w, d_r, n, s);
+      if (rc) return fail(rc, "wake element %d: %s", w, wake_last_error());
+      a = w + 1;
+    }
+    if (ed > a) {
+      int rc = launch_plain(L, d_r, n, 1, a, ed, &oo, nullptr, 0, s);
+      if (rc) return rc;
+    }
+  }
+  return ACCP_OK;
+}
+
+static int check_range(accp_lattice *L, int n, int nturn, int st, int &ed) {
+  const int ne = (int)L->elems.size();
+  if (n < 0 || nturn < 0) return fail(ACCP_E_ARG, "negative particle or turn count");
+  if (ed < 0 || ed > ne) ed = ne;
+  if (st < 0 || st > ed) return fail(ACCP_E_ARG, "element range [%d,%d) invalid", st, ed);
+  return ACCP_OK;
+}
+
+extern "C" int accp_track_device(accp_lattice *L, double *d_r, int n, int nturn, int st, int ed,
+                                 const accp_track_opts *opts, void *stream) {
+  if (!L || (!d_r && n > 0)) return fail(ACCP_E_ARG, "accp_track_device: null argument");
+  int rc = check_range(L, n, nturn, st, ed);
+  if (rc) return rc;
+  if (opts && opts->mon_interval > 0)
+    return fail(ACCP_E_ARG, "monitor records need accp_track (host buffer)");
+  if (n == 0 || nturn == 0 || st == ed) return ACCP_OK;
+  return track_device_impl(L, d_r, n, nturn, st, ed, opts, nullptr, 0, (hipStream_t)stream);
+}
+
+extern "C" int accp_track(accp_lattice *L, double *r, int n, int nturn, int st, int ed,
+                          const accp_track_opts *opts) {
+  if (!L || (!r && n > 0)) return fail(ACCP_E_ARG, "accp_track: null argument");
+  int rc = check_range(L, n, nturn, st, ed);
+  if (rc) return rc;
+  if (n == 0 || nturn == 0) {
+    if (opts && opts->mon_count) *opts->mon_count = 0;
+    return ACCP_OK;
+  }
+  const size_t bytes = (size_t)n * 6 * sizeof(double);
+  if ((rc = L->d_beam.ensure(bytes))) return rc;
+  HIPCHK(hipMemcpy(L->d_beam.p, r, bytes, hipMemcpyHostToDevice));
+  int nsnap = 0;
+  double *snap = nullptr;
+  if (opts && opts->mon_interval > 0) {
+    if (!opts->mon_out || !opts->mon_count) return fail(ACCP_E_ARG, "monitor needs mon_out and mon_count");
+    nsnap = (nturn + opts->mon_interval - 1) / opts->mon_interval;
+    if (nsnap > opts->mon_max) nsnap = opts->mon_max;
+    if (nsnap > 0) {
+      if ((rc = L->d_snap.ensure((size_t)nsnap * bytes))) return rc;
+      snap = (double *)L->d_snap.p;
+    }
+  }
+  if (st < ed) {
+    rc = track_device_impl(L, (double *)L->d_beam.p, n, nturn, st, ed, opts, snap, nsnap, nullptr);
+    if (rc) return rc;
+  }
+  if (snap) {
+    if ((rc = L->d_stat.ensure((size_t)nsnap * ACCP_STAT_LEN * sizeof(double)))) return rc;
+    HIPCHK(launch_beam_stat(snap, n, nsnap, (double)opts->turn0, opts->mon_interval,
+                            (double *)L->d_stat.p, nullptr));
+    HIPCHK(hipMemcpy(opts->mon_out, L->d_stat.p, (size_t)nsnap * ACCP_STAT_LEN * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    *opts->mon_count = nsnap;
+  } else if (opts && opts->mon_count) {
+    *opts->mon_count = 0;
+  }
+  HIPCHK(hipMemcpy(r, L->d_beam.p, bytes, hipMemcpyDeviceToHost));
+  HIPCHK(hipDeviceSynchronize());
+  return ACCP_OK;
+}
+
+extern "C" int accp_beam_stat(const double *r, int n, double *stat) {
+  if (!r || n <= 0 || !stat) return fail(ACCP_E_ARG, "accp_beam_stat: bad arguments");
+  const size_t bytes = (size_t)n * 6 * sizeof(double);
+  DevBuf b, o;
+  int rc;
+  if ((rc = b.ensure(bytes)) || (rc = o.ensure(ACCP_STAT_LEN * sizeof(double)))) {
+    b.release();
+    o.release();
+    return rc;
+  }
+  hipError_t e = hipMemcpy(b.p, r, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_beam_stat((const double *)b.p, n, 1, 0.0, 1, (double *)o.p, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(stat, o.p, ACCP_STAT_LEN * sizeof(double), hipMemcpyDeviceToHost);
+  b.release();
+  o.release();
+  if (e != hipSuccess) return fail(ACCP_E_HIP, "accp_beam_stat: %s", hipGetErrorString(e));
+  return ACCP_OK;
+}
+
+// ---- batched multi-seed tracking -----------------------------------------------
+struct accp_batch {
+  int nseed = 0, nelem = 0;
+  size_t max_lds = 0;
+  std::vector<Hdr> hdr;
+  std::vector<double> par;
+  std::vector<int32_t> hdr0, par0, npar;
+  DevBuf d_hdr, d_par, d_meta, d_tiles, d_scratch;
+  std::vector<TileDesc> tiles;
+  std::vector<int64_t> tile_offsets;
+  int tile_block = 0, tile_ppl = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap_stream = nullptr;
+};
+
+extern "C" int accp_batch_create(accp_lattice *const *seeds, int nseed, accp_batch **out) {
+  if (!seeds || nseed <= 0 || !out) return fail(ACCP_E_ARG, "accp_batch_create: bad arguments");
+  *out = nullptr;
+  const int ne = (int)seeds[0]->elems.size();
+  for (int s = 0; s < nseed; ++s) {
+    if (!seeds[s]) return fail(ACCP_E_ARG, "seed %d is null", s);
+    if ((int)seeds[s]->elems.size() != ne) return fail(ACCP_E_LATTICE, "seed %d: element count differs", s);
+    if (!seeds[s]->tab.wakes.empty()) return fail(ACCP_E_LATTICE, "seed %d: wake elements are not batched", s);
+    for (int i = 0; i < ne; ++i)
+      if (seeds[s]->elems[i].type != seeds[0]->elems[i].type)
+        return fail(ACCP_E_LATTICE, "seed %d: element %d differs in type", s, i);
+  }
+  accp_batch *b = new accp_batch;
+  b->nseed = nseed;
+  b->nelem = ne;
+  for (int s = 0; s < nseed; ++s) {
+    const Compiled &c = seeds[s]->tab;
+    b->hdr0.push_back((int32_t)b->hdr.size());
+    b->par0.push_back((int32_t)b->par.size());
+    b->npar.push_back((int32_t)c.par.size());
+    b->hdr.insert(b->hdr.end(), c.hdr.begin(), c.hdr.end());
+    b->par.insert(b->par.end(), c.par.begin(), c.par.end());
+    b->max_lds = std::max(b->max_lds, lds_bytes_for(c.hdr.size(), c.par.size()));
+  }
+  std::vector<int32_t> meta;
+  meta.insert(meta.end(), b->hdr0.begin(), b->hdr0.end());
+  meta.insert(meta.end(), b->par0.begin(), b->par0.end());
+  meta.insert(meta.end(), b->npar.begin(), b->npar.end());
+  int rc = b->d_hdr.ensure(b->hdr.size() * sizeof(Hdr));
+  if (!rc) rc = b->d_par.ensure(b->par.size() * sizeof(double));
+  if (!rc) rc = b->d_meta.ensure(meta.size() * sizeof(int32_t));
+  if (rc) {
+    accp_batch_free(b);
+    return rc;
+  }
+  hipError_t e = hipMemcpy(b->d_hdr.p, b->hdr.data(), b->hdr.size() * sizeof(Hdr), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_par.p, b->par.data(), b->par.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_meta.p, meta.data(), meta.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    accp_batch_free(b);
+    return fail(ACCP_E_HIP, "batch upload: %s", hipGetErrorString(e));
+  }
+  *out = b;
+  return ACCP_OK;
+}
+
+// Tiles of block*ppl particles per seed; the hardware dispatcher hands them
+// to CUs as earlier workgroups retire, which balances uneven seeds.
+static int build_tiles(accp_batch *b, const int64_t *offsets, int block, int ppl) {
+  std::vector<int64_t> offs(offsets, offsets + b->nseed + 1);
+  if (b->tile_block == block && b->tile_ppl == ppl && offs == b->tile_offsets && !b->tiles.empty())
+    return ACCP_OK;
+  for (int s = 0; s < b->nseed; ++s)
+    if (offs[s + 1] < offs[s] || offs[s] < 0) return fail(ACCP_E_ARG, "offsets must be non-decreasing");
+  b->tiles.clear();
+  const int64_t per = (int64_t)block * ppl;
+  for (int s = 0; s < b->nseed; ++s)
+    for (int64_t a = offs[s]; a < offs[s + 1]; a += per)
+      b->tiles.push_back(TileDesc{s, 0, a, std::min(per, offs[s + 1] - a)});
+  if (b->tiles.empty()) return ACCP_OK;
+  int rc = b->d_tiles.ensure(b->tiles.size() * sizeof(TileDesc));
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(b->d_tiles.p, b->tiles.data(), b->tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice));
+  b->tile_offsets = offs;
+  b->tile_block = block;
+  b->tile_ppl = ppl;
+  return ACCP_OK;
+}
+
+static int batched_launch(accp_batch *b, double *d_r, const int64_t *offsets, int nturn,
+                          const accp_tuning *tuning, hipStream_t s) {
+  int block, ppl;
+  bool lds;
+  int rc = resolve_tuning(tuning, b->max_lds, block, ppl, lds);
+  if (rc) return rc;
+  if ((rc = build_tiles(b, offsets, block, ppl))) return rc;
+  if (b->tiles.empty() || nturn == 0) return ACCP_OK;
+  KernelCtx c{};
+  c.tiles = (const TileDesc *)b->d_tiles.p;
+  c.hdr = (const Hdr *)b->d_hdr.p;
+  c.par = (const double *)b->d_par.p;
+  c.lat_hdr0 = (const int32_t *)b->d_meta.p;
+  c.lat_par0 = c.lat_hdr0 + b->nseed;
+  c.lat_npar = c.lat_hdr0 + 2 * b->nseed;
+  c.nelem = b->nelem;
+  c.nturn = nturn;
+  c.st = 0;
+  c.ed = b->nelem;
+  c.r = d_r;
+  c.n = offsets[b->nseed];
+  c.mon_interval = 1;
+  HIPCHK(launch_track(c, (int)b->tiles.size(), block, ppl, lds, lds ? b->max_lds : 0, s));
+  return ACCP_OK;
+}
+
+extern "C" int accp_track_batched(accp_batch *b, double *d_r, const int64_t *offsets, int nturn,
+                                  const accp_tuning *tuning, void *stream) {
+  if (!b || !d_r || !offsets || nturn < 0) return fail(ACCP_E_ARG, "accp_track_batched: bad arguments");
+  return batched_launch(b, d_r, offsets, nturn, tuning, (hipStream_t)stream);
+}
+
+extern "C" int accp_batch_capture(accp_batch *b, double *d_r, const int64_t *offsets, int nturn,
+                                  const accp_tuning *tuning) {
+  if (!b || !d_r || !offsets || nturn < 0) return fail(ACCP_E_ARG, "accp_batch_capture: bad arguments");
+  int block, ppl;
+  bool lds;
+  int rc = resolve_tuning(tuning, b->max_lds, block, ppl, lds);
+  if (rc) return rc;
+  if ((rc = build_tiles(b, offsets, block, ppl))) return rc;  // uploads happen outside capture
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  if (!b->cap_stream) HIPCHK(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+  rc = batched_launch(b, d_r, offsets, nturn, tuning, b->cap_stream);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(ACCP_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  b->graph = g;
+  HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return ACCP_OK;
+}
+
+extern "C" int accp_batch_replay(accp_batch *b, void *stream) {
+  if (!b || !b->exec) return fail(ACCP_E_STATE, "accp_batch_replay: nothing captured");
+  HIPCHK(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return ACCP_OK;
+}
+
+extern "C" int accp_autotune(accp_batch *b, double *d_r, const int64_t *offsets, int nturn_probe,
+                             accp_tuning *best, double *best_ms) {
+  if (!b || !d_r || !offsets || !best || nturn_probe <= 0) return fail(ACCP_E_ARG, "accp_autotune: bad arguments");
+  const int64_t n = offsets[b->nseed];
+  const size_t bytes = (size_t)n * 6 * sizeof(double);
+  int rc = b->d_scratch.ensure(std::max<size_t>(bytes, 8));
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  double bestv = 1e300;
+  accp_tuning bt{64, 1, ACCP_STAGE_SCALAR, 0};
+  const int blocks[3] = {64, 128, 256}, ppls[2] = {1, 2};
+  const int stages[2] = {ACCP_STAGE_SCALAR, ACCP_STAGE_LDS};
+  for (int bl : blocks)
+    for (int pp : ppls)
+      for (int sg : stages) {
+        if (sg == ACCP_STAGE_LDS && b->max_lds > LDS_LIMIT) continue;
+        accp_tuning t{bl, pp, sg, 0};
+        HIPCHK(hipMemcpy(b->d_scratch.p, d_r, bytes, hipMemcpyDeviceToDevice));
+        // warm-up launch (tile upload, code object load), then the timed one
+        rc = batched_launch(b, (double *)b->d_scratch.p, offsets, 1, &t, nullptr);
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(b->d_scratch.p, d_r, bytes, hipMemcpyDeviceToDevice));
+        HIPCHK(hipEventRecord(e0, nullptr));
+        rc = batched_launch(b, (double *)b->d_scratch.p, offsets, nturn_probe, &t, nullptr);
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(e1, nullptr));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < bestv) {
+          bestv = ms;
+          bt = t;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *best = bt;
+  if (best_ms) *best_ms = bestv;
+  return ACCP_OK;
+}
+
+extern "C" int accp_batch_free(accp_batch *b) {
+  if (!b) return ACCP_OK;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  b->d_hdr.release();
+  b->d_par.release();
+  b->d_meta.release();
+  b->d_tiles.release();
+  b->d_scratch.release();
+  delete b;
+  return ACCP_OK;
+}

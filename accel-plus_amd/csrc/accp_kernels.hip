@@ -1,0 +1,404 @@
+// accp_kernels.hip -- gfx950 kernels of the tracking backend.
+//
+// k_track: persistent FP64 tracker.  One workgroup owns a tile of
+// particles of one lattice (a seed), each lane holds PPL particles whose 6D
+// state (plus 1/(1+delta)) stays in VGPRs for all turns; the element table is
+// read either through wave-uniform scalar loads or from a per-workgroup LDS
+// copy.  The same kernel serves the plain call (one seed, tiles implied by
+// blockIdx) and the batched call (a tile list over many seeds).
+//
+// Arithmetic is FP64 throughout; there is no dense contraction, so no MFMA.
+#include <hip/hip_runtime.h>
+
+#include "accp_kernels.h"
+#include "accp_physics.h"
+
+namespace accp {
+
+// ABendMPole::pass (accel.cpp:649-705): edge, nslice slices of the 4th-order
+// integrator (radiation off or classical, flag <= 1) or of the 2nd-order one
+// with quantum excitation (flag 2, accel.cpp:693-700), edge.
+template <int PPL, class PT>
+__device__ __forceinline__ void bendmp(Part (&p)[PPL], const Hdr h, const PT *P,
+                                       const KernelCtx &ctx, const uint64_t (&pid)[PPL],
+                                       uint64_t turn, int eidx) {
+  const int ns = h.a, mo = h.b & 0xff;
+  const bool plain = (h.b >> 8) & 1;
+  const double psi1 = P[BM_PSI1], psi2 = P[BM_PSI2], h0 = P[BM_IRHO];
+  const double k1c = P[BM_K1C], k2c = P[BM_K2C];
+  const PT *B = P + BM_B;
+  if (psi1 != 0) {
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) edge(p[k], psi1);
+  }
+  if (h.op == OP_BENDMP_QE) {
+    const double sl = P[BM_SL], crad = P[BM_CRAD], cq = P[BM_CQ];
+    const double half = 0.5 * sl, hs = P[BM_HSH], hq = P[BM_HQH];
+    for (int m = 0; m < ns; ++m) {
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        const double g = gauss_kick(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, (uint32_t)m);
+        bend_kick_rad(p[k], B, mo, sl, h0, k1c, k2c, crad, cq, 2, g);
+      }
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
+    }
+  } else {
+    const double l1 = P[BM_L1], l2 = P[BM_L2], k1 = P[BM_K1], k2 = P[BM_K2];
+    const double hs1 = P[BM_HS1], hq1 = P[BM_HQ1], hs2 = P[BM_HS2], hq2 = P[BM_HQ2];
+    const bool rad = h.op == OP_BENDMP_RAD;
+    const double crad = P[BM_CRAD];
+    for (int m = 0; m < ns; ++m) {
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) hxdrift(p[k], l1, h0, hs1, hq1);
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        if (rad) bend_kick_rad(p[k], B, mo, k1, h0, k1c, k2c, crad, 0.0, 1, 0.0);
+        else bend_kick(p[k], B, mo, plain, k1, h0, k1c, k2c);
+      }
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) hxdrift(p[k], l2, h0, hs2, hq2);
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        if (rad) bend_kick_rad(p[k], B, mo, k2, h0, k1c, k2c, crad, 0.0, 1, 0.0);
+        else bend_kick(p[k], B, mo, plain, k2, h0, k1c, k2c);
+      }
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) hxdrift(p[k], l2, h0, hs2, hq2);
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        if (rad) bend_kick_rad(p[k], B, mo, k1, h0, k1c, k2c, crad, 0.0, 1, 0.0);
+        else bend_kick(p[k], B, mo, plain, k1, h0, k1c, k2c);
+      }
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) hxdrift(p[k], l1, h0, hs1, hq1);
+    }
+  }
+  if (psi2 != 0) {
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) edge(p[k], psi2);
+  }
+}
+
+// ---- one element on PPL particles -------------------------------------------
+// Lost particles (x NaN) are computed on like the others and restored
+// afterwards, which keeps the PPL instruction streams interleaved; the
+// reference skips them (e.g. accel.cpp:166-167), with the same result.
+template <int PPL, class PT>
+__device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const PT *P,
+                                              const KernelCtx &ctx, const uint64_t (&pid)[PPL],
+                                              uint64_t turn, int eidx) {
+  Part keep[PPL];
+  bool live[PPL];
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) {
+    keep[k] = p[k];
+    live[k] = is_live(p[k]);
+  }
+  switch (h.op) {
+    case OP_DRIFT: {
+      const double L = P[0];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) drift_nl(p[k], L * p[k].ipd);
+      break;
+    }
+    case OP_QUAD: {
+      const double L = P[0], K = P[1], aK = P[2];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) quad(p[k], L, K, aK);
+      break;
+    }
+    case OP_QUAD_K0: {
+      const double L = P[0];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) drift_kzero(p[k], L);
+      break;
+    }
+    case OP_BEND: {
+      const double L = P[0], kx = P[1], grd = P[2], psi1 = P[4], psi2 = P[5];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        if (psi1 != 0) edge(p[k], psi1);
+        bend_linear(p[k], L, kx, grd);
+        if (psi2 != 0) edge(p[k], psi2);
+      }
+      break;
+    }
+    case OP_BENDMP:
+    case OP_BENDMP_RAD:
+    case OP_BENDMP_QE:
+      bendmp<PPL>(p, h, P, ctx, pid, turn, eidx);
+      break;
+    case OP_STRMP: {
+      const int ns = h.a, hi = h.b;
+      const double l1 = P[0], l2 = P[1], k1 = P[2], k2 = P[3];
+      const PT *B = P + 4;
+      const PT *A = B + (hi + 1);
+      for (int m = 0; m < ns; ++m) {
+        double n1[PPL], n2[PPL];
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+          n1[k] = l1 * p[k].ipd;
+          n2[k] = l2 * p[k].ipd;
+        }
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) drift_nl(p[k], n1[k]);
+        if (hi >= 0) {
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) mpole_kick(p[k], B, A, hi, k1);
+        }
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) drift_nl(p[k], n2[k]);
+        if (hi >= 0) {
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) mpole_kick(p[k], B, A, hi, k2);
+        }
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) drift_nl(p[k], n2[k]);
+        if (hi >= 0) {
+#pragma unroll
+          for (int k = 0; k < PPL; ++k) mpole_kick(p[k], B, A, hi, k1);
+        }
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) drift_nl(p[k], n1[k]);
+      }
+      break;
+    }
+    case OP_STRMP_THIN: {
+      const int hi = h.b;
+      if (hi >= 0) {
+        const PT *B = P;
+        const PT *A = B + (hi + 1);
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) mpole_kick(p[k], B, A, hi, 1.0);
+      }
+      break;
+    }
+    case OP_CAVITY: {
+      const double L = P[0], nv = P[1], w = P[2], phis = P[3];
+      const bool on = h.a != 0;
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) cavity(p[k], L, nv, w, phis, on);
+      break;
+    }
+    case OP_APERTURE: {
+      const double L = P[0], x0 = P[1], x1 = P[2], y0 = P[3], y1 = P[4];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) aperture(p[k], L, x0, x1, y0, y1);
+      break;
+    }
+    case OP_CORRECTOR: {
+      const double L = P[0], kx = P[1], ky = P[2];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) corrector(p[k], L, kx, ky);
+      break;
+    }
+    default:
+      break;
+  }
+#pragma unroll
+  for (int k = 0; k < PPL; ++k)
+    if (!live[k]) p[k] = keep[k];
+}
+
+// ---- the persistent tracker ---------------------------------------------------
+template <int BLOCK, int PPL, bool LDS>
+__global__ __launch_bounds__(BLOCK) void k_track(KernelCtx ctx) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+
+  // tile of this workgroup
+  int lat = 0;
+  int64_t start, count;
+  if (ctx.tiles) {
+    const TileDesc t = ctx.tiles[blockIdx.x];
+    lat = t.lat;
+    start = t.start;
+    count = t.count;
+  } else {
+    start = (int64_t)blockIdx.x * (BLOCK * PPL);
+    count = ctx.n - start;
+    if (count > BLOCK * PPL) count = BLOCK * PPL;
+  }
+  lat = __builtin_amdgcn_readfirstlane(lat);
+  const Hdr *__restrict__ ghdr = ctx.hdr + ctx.lat_hdr0[lat];
+  const double *__restrict__ gpar = ctx.par + ctx.lat_par0[lat];
+
+  const Hdr *hdr = ghdr;
+  const double *par = gpar;
+  if (LDS) {
+    Hdr *lh = reinterpret_cast<Hdr *>(lds_raw);
+    double *lp = reinterpret_cast<double *>(lds_raw + sizeof(Hdr) * ctx.nelem);
+    for (int i = threadIdx.x; i < ctx.nelem; i += BLOCK) lh[i] = ghdr[i];
+    const int np = ctx.lat_npar[lat];
+    for (int i = threadIdx.x; i < np; i += BLOCK) lp[i] = gpar[i];
+    __syncthreads();
+    hdr = lh;
+    par = lp;
+  }
+
+  Part p[PPL];
+  uint64_t pid[PPL];
+  bool inb[PPL];
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) {
+    const int64_t li = (int64_t)k * BLOCK + threadIdx.x;
+    inb[k] = li < count;
+    const int64_t gi = start + (inb[k] ? li : 0);
+    pid[k] = ctx.pid0 + (uint64_t)gi;
+    const double *src = ctx.r + 6 * gi;
+    p[k].x = src[0];
+    p[k].px = src[1];
+    p[k].y = src[2];
+    p[k].py = src[3];
+    p[k].z = src[4];
+    set_delta(p[k], src[5]);
+    if (!inb[k]) p[k].x = __builtin_nan("");  // padding lanes behave as lost
+  }
+
+  int t = 0;
+  for (; t < ctx.nturn; ++t) {
+    if (ctx.snap && t % ctx.mon_interval == 0) {
+      const int slot = t / ctx.mon_interval;
+#pragma unroll
+      for (int k = 0; k < PPL; ++k)
+        if (inb[k] && slot < ctx.nsnap) {
+          const int64_t gi = start + (int64_t)k * BLOCK + threadIdx.x;
+          double *dst = ctx.snap + ((int64_t)slot * ctx.n + gi) * 6;
+          dst[0] = p[k].x; dst[1] = p[k].px; dst[2] = p[k].y;
+          dst[3] = p[k].py; dst[4] = p[k].z; dst[5] = p[k].d;
+        }
+    }
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) any |= is_live(p[k]);
+    if (!__any(any)) break;  // every particle of this wave is lost: frozen from here on
+    const uint64_t turn = ctx.turn0 + (uint64_t)t;
+    for (int e = ctx.st; e < ctx.ed; ++e) {
+      Hdr h;
+      if (LDS) {
+        h.op = __builtin_amdgcn_readfirstlane(hdr[e].op);
+        h.off = __builtin_amdgcn_readfirstlane(hdr[e].off);
+        h.a = __builtin_amdgcn_readfirstlane(hdr[e].a);
+        h.b = __builtin_amdgcn_readfirstlane(hdr[e].b);
+      } else {
+        h = hdr[e];
+      }
+      apply_element<PPL>(p, h, par + h.off, ctx, pid, turn, e);
+    }
+  }
+  // monitor slots after an early exit hold the frozen state
+  if (ctx.snap) {
+    for (int tt = t; tt < ctx.nturn; ++tt) {
+      if (tt % ctx.mon_interval) continue;
+      const int slot = tt / ctx.mon_interval;
+#pragma unroll
+      for (int k = 0; k < PPL; ++k)
+        if (inb[k] && slot < ctx.nsnap) {
+          const int64_t gi = start + (int64_t)k * BLOCK + threadIdx.x;
+          double *dst = ctx.snap + ((int64_t)slot * ctx.n + gi) * 6;
+          dst[0] = p[k].x; dst[1] = p[k].px; dst[2] = p[k].y;
+          dst[3] = p[k].py; dst[4] = p[k].z; dst[5] = p[k].d;
+        }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) {
+    if (!inb[k]) continue;
+    const int64_t gi = start + (int64_t)k * BLOCK + threadIdx.x;
+    double *dst = ctx.r + 6 * gi;
+    dst[0] = p[k].x; dst[1] = p[k].px; dst[2] = p[k].y;
+    dst[3] = p[k].py; dst[4] = p[k].z; dst[5] = p[k].d;
+  }
+}
+
+// ---- calc_beam_stat on snapshots (utility.cpp:322-373) ----------------------
+// One workgroup per record; two passes (mean, then central moments) with a
+// fixed-order LDS tree reduction, so the result does not depend on timing.
+constexpr int STAT_BLOCK = 256;
+
+__device__ void block_sum(double (*buf)[STAT_BLOCK], int nv) {
+  for (int s = STAT_BLOCK / 2; s > 0; s >>= 1) {
+    __syncthreads();
+    if ((int)threadIdx.x < s)
+      for (int v = 0; v < nv; ++v) buf[v][threadIdx.x] += buf[v][threadIdx.x + s];
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(STAT_BLOCK) void k_beam_stat(const double *__restrict__ snap,
+                                                          int64_t n, double turn0, int interval,
+                                                          double *__restrict__ out) {
+  __shared__ double buf[13][STAT_BLOCK];
+  const double *r = snap + (int64_t)blockIdx.x * n * 6;
+  double s[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int64_t j = threadIdx.x; j < n; j += STAT_BLOCK) {
+    const double *q = r + 6 * j;
+    if (q[0] != q[0]) continue;
+    s[6] += 1.0;
+    for (int i = 0; i < 6; ++i) s[i] += q[i];
+  }
+  for (int v = 0; v < 7; ++v) buf[v][threadIdx.x] = s[v];
+  block_sum(buf, 7);
+  const double nl = buf[6][0];
+  double mean[6];
+  for (int i = 0; i < 6; ++i) mean[i] = buf[i][0] / nl;
+  __syncthreads();
+  const int pa[6] = {0, 2, 0, 1, 2, 3}, pb[6] = {1, 3, 5, 5, 4, 4};
+  double q2[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t j = threadIdx.x; j < n; j += STAT_BLOCK) {
+    const double *q = r + 6 * j;
+    if (q[0] != q[0]) continue;
+    double dv[6];
+    for (int i = 0; i < 6; ++i) {
+      dv[i] = q[i] - mean[i];
+      q2[i] += dv[i] * dv[i];
+    }
+    for (int i = 0; i < 6; ++i) q2[6 + i] += dv[pa[i]] * dv[pb[i]];
+  }
+  for (int v = 0; v < 12; ++v) buf[v][threadIdx.x] = q2[v];
+  block_sum(buf, 12);
+  if (threadIdx.x == 0) {
+    double *o = out + (int64_t)blockIdx.x * 20;
+    o[0] = turn0 + (double)blockIdx.x * interval;
+    o[1] = nl;
+    for (int i = 0; i < 6; ++i) {
+      o[2 + i] = mean[i];
+      o[8 + i] = sqrt(buf[i][0] / nl);
+      o[14 + i] = buf[6 + i][0] / nl;
+    }
+  }
+}
+
+// ---- launchers -------------------------------------------------------------------
+template <int BLOCK, int PPL, bool LDS>
+static hipError_t launch_one(const KernelCtx &ctx, int grid, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL((k_track<BLOCK, PPL, LDS>), dim3(grid), dim3(BLOCK), lds, s, ctx);
+  return hipGetLastError();
+}
+
+hipError_t launch_track(const KernelCtx &ctx, int grid, int block, int ppl, bool lds,
+                        size_t lds_bytes, hipStream_t s) {
+#define ACCP_CASE(B, P)                                                        \
+  if (block == B && ppl == P)                                                  \
+    return lds ? launch_one<B, P, true>(ctx, grid, lds_bytes, s)              \
+               : launch_one<B, P, false>(ctx, grid, 0, s);
+  ACCP_CASE(64, 1)
+  ACCP_CASE(128, 1)
+  ACCP_CASE(256, 1)
+  ACCP_CASE(64, 2)
+  ACCP_CASE(128, 2)
+  ACCP_CASE(256, 2)
+#undef ACCP_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_beam_stat(const double *snap, int64_t n, int nrec, double turn0, int interval,
+                            double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_beam_stat, dim3(nrec), dim3(STAT_BLOCK), 0, s, snap, n, turn0, interval,
+                     out);
+  return hipGetLastError();
+}
+
+}  // namespace accp

@@ -1,0 +1,317 @@
+// accp_physics.h -- element maps of the tracking path, one particle at a time.
+//
+// Each function restates one reference integrator primitive
+// (src/lib/accel/atpass.cpp) on a particle held in registers.  Differences
+// from the reference are confined to rounding: 1/(1+delta) is carried with
+// the particle and refreshed whenever delta changes (the reference divides
+// by (1+delta) in every drift), divisions by constants become products with
+// host-computed reciprocals, and hipcc may contract a*b+c into FMA.  The
+// per-coordinate tolerance of the parity tests (1e-12 relative after one
+// turn) covers these ulp-level effects.
+#pragma once
+#include <math.h>
+#include <stdint.h>
+
+#include "accp_table.h"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define ACCP_HD __host__ __device__ __forceinline__
+#else
+#define ACCP_HD inline
+#endif
+
+namespace accp {
+
+struct Part {
+  double x, px, y, py, z, d;
+  double ipd;  // 1/(1+d)
+};
+
+ACCP_HD void set_delta(Part &p, double d) {
+  p.d = d;
+  p.ipd = 1.0 / (1.0 + d);
+}
+
+ACCP_HD bool is_live(const Part &p) { return p.x == p.x; }
+
+// fastdrift with NormL = L/(1+delta) (atpass.cpp:416-428).
+ACCP_HD void drift_nl(Part &p, double nl) {
+  const double s = p.px * p.px + p.py * p.py;
+  p.x += nl * p.px;
+  p.y += nl * p.py;
+  p.z -= nl * s * (0.5 * p.ipd);
+}
+
+// drift() as quad6 uses it for K == 0 (atpass.cpp:6-13, :25-28), including
+// the reference's (px + y^2)^2 in the path-length term.
+ACCP_HD void drift_kzero(Part &p, double L) {
+  const double pn = p.ipd;
+  p.x += p.px * L;
+  p.y += p.py * L;
+  const double w = p.px + p.y * p.y;
+  p.z -= pn * pn * L * (w * w) * 0.5;
+}
+
+// quad6 (atpass.cpp:15-72) for K != 0; aK = |K|.
+ACCP_HD void quad(Part &p, double L, double K, double aK) {
+  const double pn = p.ipd;
+  const double x = p.x, xp = p.px * pn, y = p.y, yp = p.py * pn;
+  const double g = aK * pn;
+  const double t = sqrt(g);
+  const double it = 1.0 / t;
+  const double lt = L * t;
+  double sn, cs;
+  sincos(lt, &sn, &cs);
+  const double ch = cosh(lt), sh = sinh(lt);
+  double mhd, m12, m21, mvd, m34, m43;
+  if (K > 0) {
+    mhd = cs; m12 = sn * it; m21 = -m12 * g;
+    mvd = ch; m34 = sh * it; m43 = m34 * g;
+  } else {
+    mhd = ch; m12 = sh * it; m21 = m12 * g;
+    mvd = cs; m34 = sn * it; m43 = -m34 * g;
+  }
+  const double opd = 1.0 + p.d;
+  p.x = mhd * x + m12 * xp;
+  p.px = (m21 * x + mhd * xp) * opd;
+  p.y = mvd * y + m34 * yp;
+  p.py = (m43 * y + mvd * yp) * opd;
+  p.z -= g * (x * x * (L - mhd * m12) - y * y * (L - mvd * m34)) * 0.25;
+  p.z -= (xp * xp * (L + mhd * m12) + yp * yp * (L + mvd * m34)) * 0.25;
+  p.z -= (x * xp * m12 * m21 + y * yp * m34 * m43) * 0.5;
+}
+
+// Hard-edge fringe with psi = irho*tan(edge) precomputed (atpass.cpp:190-196).
+ACCP_HD void edge(Part &p, double psi) {
+  p.px += p.x * psi;
+  p.py -= p.y * psi;
+}
+
+// bend6 with ByError = 0 (atpass.cpp:76-187): linear sector bend with
+// gradient; kx = angle/L.
+ACCP_HD void bend_linear(Part &p, double L, double kx, double grd) {
+  const double pn = p.ipd;
+  const double g1 = (kx * kx + grd) * pn;
+  const double g2 = -grd * pn;
+  double mhd, m12, m21, mvd, m34, m43, s1 = 0, c1 = 1, sg1 = 0;
+  if (g1 == 0) {
+    mhd = 1; m12 = L; m21 = 0;
+  } else if (g1 > 0) {
+    sg1 = sqrt(g1);
+    sincos(L * sg1, &s1, &c1);
+    mhd = c1; m12 = s1 / sg1; m21 = -s1 * sg1;
+  } else {
+    sg1 = sqrt(-g1);
+    const double a1 = L * sg1;
+    s1 = sinh(a1); c1 = cosh(a1);
+    mhd = c1; m12 = s1 / sg1; m21 = s1 * sg1;
+  }
+  if (g2 == 0) {
+    mvd = 1; m34 = L; m43 = 0;
+  } else if (g2 > 0) {
+    const double sg2 = sqrt(g2);
+    double s2, c2;
+    sincos(L * sg2, &s2, &c2);
+    mvd = c2; m34 = s2 / sg2; m43 = -s2 * sg2;
+  } else {
+    const double sg2 = sqrt(-g2);
+    const double a2 = L * sg2;
+    const double s2 = sinh(a2);
+    mvd = cosh(a2); m34 = s2 / sg2; m43 = s2 * sg2;
+  }
+  const double x = p.x, xp = p.px * pn, y = p.y, yp = p.py * pn;
+  const double opd = 1.0 + p.d;
+  const double dp = p.d * pn;  // (delta*p_norm - ByError), ByError = 0
+  double nx = mhd * x + m12 * xp;
+  double npx = (m21 * x + mhd * xp) * opd;
+  if (g1 == 0) {
+    nx += dp * L * L * kx * 0.5;
+    npx += dp * L * kx * opd;
+  } else {
+    nx += dp * (1 - c1) * kx / g1;
+    npx += dp * s1 * kx / (sg1 * pn);
+  }
+  p.x = nx;
+  p.px = npx;
+  p.y = mvd * y + m34 * yp;
+  p.py = (m43 * y + mvd * yp) * opd;
+  double z = p.z - xp * xp * (L + mhd * m12) * 0.25;
+  if (g1 != 0) {
+    z -= (L - mhd * m12) * (x * x * g1 + dp * dp * kx * kx / g1 - 2 * x * kx * dp) * 0.25;
+    z -= m12 * m21 * (x * xp - xp * dp * kx / g1) * 0.5;
+    z -= kx * x * m12 + xp * (1 - mhd) * kx / g1 + dp * (L - m12) * kx * kx / g1;
+  }
+  z -= ((L - mvd * m34) * y * y * g2 + yp * yp * (L + mvd * m34)) * 0.25;
+  z -= m34 * m43 * y * yp * 0.5;
+  p.z = z;
+}
+
+// ATbendhxdrift6 (atpass.cpp:343-355); hsl4 = (1/4)*h*L*L precomputed,
+// hs = h*L.
+ACCP_HD void hxdrift(Part &p, double L, double h, double hs, double hsl4) {
+  const double i1 = p.ipd;
+  const double x = p.x, px = p.px, py = p.py;
+  const double a = 1.0 + h * x;
+  const double px2 = px * px, py2 = py * py;
+  const double s = px2 + py2;
+  p.x += a * px * i1 * L + hsl4 * (px2 - py2) * i1 * i1;
+  p.px -= hs * s * i1 * 0.5;
+  p.y += a * i1 * py * L * (1.0 + px * hs * 0.5);
+  p.z -= a * i1 * i1 * L * 0.5 * s;
+}
+
+// Complex Horner of sum_{n<=mo} B[n] (x+iy)^n (atpass.cpp:224-229).
+template <class PT>
+ACCP_HD void horner_b(const PT *B, int mo, double x, double y, double &re, double &im) {
+  double a = B[mo], b = 0.0;
+  for (int i = mo - 1; i >= 0; --i) {
+    const double t = a * x - b * y + B[i];
+    b = b * x + a * y;
+    a = t;
+  }
+  re = a;
+  im = b;
+}
+
+// bndthinkick (atpass.cpp:199-235); k1c = B[1], k2c = B[2] if mo >= 2.
+// `plain` = both curvature coefficients are zero (the terms vanish).
+template <class PT>
+ACCP_HD void bend_kick(Part &p, const PT *B, int mo, bool plain, double L, double h,
+                       double k1c, double k2c) {
+  const double x = p.x, y = p.y;
+  double re, im;
+  horner_b(B, mo, x, y, re, im);
+  if (plain) {
+    p.px -= L * (-h * p.d + re + h * (h * x));
+    if (mo > 0) p.py += L * im;
+  } else {
+    p.px -= L * (-h * p.d + re + h * (h * x + k1c * (x * x - 0.5 * y * y) +
+                                      k2c * (x * x * x - (4.0 / 3.0) * x * y * y)));
+    p.py += L * (im + h * (k1c * x * y + (4.0 / 3.0) * k2c * x * x * y +
+                           (h / 6.0 * k1c - k2c / 3.0) * y * y * y));
+  }
+  p.z -= L * h * x;
+}
+
+// bndthinkickrad (atpass.cpp:255-339).  crad = CGAMMA*E0^3/(2*pi*1e9);
+// cq = 55*sqrt(3)/72*uc/E0; flag 1 = classical loss, flag 2 = loss plus
+// gauss*sigma (quantum excitation).
+template <class PT>
+ACCP_HD void bend_kick_rad(Part &p, const PT *B, int mo, double L, double h, double k1c,
+                           double k2c, double crad, double cq, int flag, double gauss) {
+  const double pn = p.ipd;
+  const double x = p.x, xp = p.px * pn, y = p.y, yp = p.py * pn;
+  double re, im;
+  horner_b(B, mo, x, y, re, im);
+  const double hk = h * (k1c * h - k2c);
+  im += hk * y * y * y * (1.0 / 6.0);
+  re += -k1c * h * y * y * 0.5 + hk * x * y * y * 0.5;
+  const double a = 1.0 + x * h;
+  const double vn2 = 1.0 / (a * a + xp * xp + yp * yp);
+  const double by = re + h, bx = im;
+  const double t1 = by * a, t2 = bx * a, t3 = bx * yp - by * xp;
+  const double b2p = (t1 * t1 + t2 * t2 + t3 * t3) * vn2;
+  const double opd = 1.0 + p.d;
+  const double mean = crad * (opd * opd) * b2p * (a + (xp * xp + yp * yp) * 0.5) * L;
+  double dnew;
+  if (flag == 1) {
+    dnew = p.d - mean;
+  } else {
+    const double sig = sqrt(cq * fabs(mean));
+    dnew = p.d - (mean + gauss * sig);
+  }
+  set_delta(p, dnew);
+  const double opn = 1.0 + dnew;
+  p.px = xp * opn;
+  p.py = yp * opn;
+  p.px -= L * (-h * dnew + re + h * (h * x + k1c * (x * x - 0.5 * y * y) +
+                                     k2c * (x * x * x - (4.0 / 3.0) * x * y * y)));
+  p.py += L * (im + h * (k1c * x * y + (4.0 / 3.0) * k2c * x * x * y +
+                         (h / 6.0 * k1c - k2c / 3.0) * y * y * y));
+  p.z -= L * h * x;
+}
+
+// strthinkick (atpass.cpp:358-414), Horner started at the highest non-zero
+// order `hi` (leading zero coefficients contribute exact zeros).
+template <class PT>
+ACCP_HD void mpole_kick(Part &p, const PT *B, const PT *A, int hi, double L) {
+  const double x = p.x, y = p.y;
+  double re = B[hi], im = A[hi];
+  for (int i = hi - 1; i >= 0; --i) {
+    const double t = re * x - im * y + B[i];
+    im = im * x + re * y + A[i];
+    re = t;
+  }
+  p.px -= L * re;
+  p.py += L * im;
+}
+
+// ACavity::pass (accel.cpp:903-923); w_c0 = 2*pi*f/C0.
+ACCP_HD void cavity(Part &p, double L, double nv, double w_c0, double phis, bool on) {
+  if (L != 0) drift_nl(p, 0.5 * L * p.ipd);
+  if (on) set_delta(p, p.d + nv * sin(w_c0 * p.z + phis));
+  if (L != 0) drift_nl(p, 0.5 * L * p.ipd);
+}
+
+// AAperture::pass (accel.cpp:337-387): loss code into z, x = NaN.
+ACCP_HD bool ap_out(const Part &p, double x0, double x1, double y0, double y1) {
+  return p.x <= x0 || p.x >= x1 || p.y <= y0 || p.y >= y1;
+}
+ACCP_HD void ap_mark(Part &p, double x0, double x1, double y0, double y1) {
+  p.z = (double)((p.x <= x0) * 1 + (p.x >= x1) * 2 + (p.y <= y0) * 3 + (p.y >= y1) * 4);
+  p.x = __builtin_nan("");
+}
+ACCP_HD void aperture(Part &p, double L, double x0, double x1, double y0, double y1) {
+  if (ap_out(p, x0, x1, y0, y1)) {
+    ap_mark(p, x0, x1, y0, y1);
+    return;
+  }
+  if (L != 0) {
+    drift_nl(p, L * p.ipd);
+    if (ap_out(p, x0, x1, y0, y1)) ap_mark(p, x0, x1, y0, y1);
+  }
+}
+
+// ACorrector::pass (accel.cpp:270-292).
+ACCP_HD void corrector(Part &p, double L, double kx, double ky) {
+  if (L != 0) drift_nl(p, 0.5 * L * p.ipd);
+  p.px += kx * p.ipd;
+  p.py += ky * p.ipd;
+  if (L != 0) drift_nl(p, 0.5 * L * p.ipd);
+}
+
+// ---- counter-based normal variates (quantum excitation) -------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies with a
+// Weyl-sequence key schedule.
+ACCP_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int round = 0; round < 10; ++round) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[1] = (uint32_t)p1;
+    c[3] = (uint32_t)p0;
+    c[0] = n0;
+    c[2] = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// Standard normal for (seed, particle, turn, element, slice): two 53-bit
+// uniforms in (0,1) from the four Philox words, Box-Muller cosine branch.
+ACCP_HD double gauss_kick(uint64_t seed, uint64_t pid, uint64_t turn, uint32_t eidx,
+                          uint32_t slice) {
+  uint32_t c[4] = {(uint32_t)pid, (uint32_t)turn, eidx, slice};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const uint64_t k1 = ((uint64_t)(c[0] >> 5) << 26) | (uint64_t)(c[1] >> 6);
+  const uint64_t k2 = ((uint64_t)(c[2] >> 5) << 26) | (uint64_t)(c[3] >> 6);
+  const double u1 = ((double)k1 + 0.5) * (1.0 / 9007199254740992.0);
+  const double u2 = ((double)k2 + 0.5) * (1.0 / 9007199254740992.0);
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+}  // namespace accp

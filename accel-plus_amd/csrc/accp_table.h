@@ -1,0 +1,62 @@
+// accp_table.h -- the compiled element table shared by the host compiler and
+// the gfx950 kernels.
+//
+// A lattice is compiled into one flat table: a 16-byte header per element
+// {op, off, a, b} plus one packed double array of per-element constants.
+// Constants that the reference recomputes on every pass but that do not
+// depend on the particle (slice lengths, Forest-Ruth kick/drift lengths,
+// edge-focusing strength, radiation prefactors) are evaluated once on the
+// host, with the reference's operation order, so the kernels only do the
+// particle-dependent arithmetic.
+#pragma once
+#include <stdint.h>
+
+namespace accp {
+
+enum Op : int32_t {
+  OP_NOP = 0,     // Monitor, disabled wake
+  OP_DRIFT,       // [L]
+  OP_QUAD,        // [L, K, |K|]
+  OP_QUAD_K0,     // [L]              (quad6 with K == 0: the reference's drift())
+  OP_BEND,        // [L, angle/L, K, irho, psi1, psi2]
+  OP_BENDMP,      // BM_* constants then B0..Bmo             a=nslice b=mo | plain<<8
+  OP_BENDMP_RAD,  // same layout, classical radiation in the kicks (flag 1)
+  OP_BENDMP_QE,   // same layout, radiation + quantum excitation (flag 2, 2nd-order integrator)
+  OP_STRMP,       // [L1, L2, K1, K2, B0..Bhi, A0..Ahi]   a=nslice b=hi (-1: no field)
+  OP_STRMP_THIN,  // [B0..Bhi, A0..Ahi]                     b=hi
+  OP_CAVITY,      // [L, V/E, 2*pi*f, phis]                 a=on
+  OP_APERTURE,    // [L, xmin, xmax, ymin, ymax]
+  OP_CORRECTOR,   // [L, kickx, kicky]
+  OP_WAKE,        // collective: the host splits the turn here; no-op inside a pass
+  OP_COUNT
+};
+
+struct Hdr {
+  int32_t op;
+  int32_t off;  // first constant in the packed double array
+  int32_t a;
+  int32_t b;
+};
+
+// Offsets inside an OP_BENDMP* record.  HS* = irho*L and HQ* = (1/4)*HS*L
+// of the pseudo-drift lengths L1, L2 and SL/2; K1C/K2C are PolyB[1]/[2] as the
+// curvature terms of bndthinkick read them (zero beyond MaxNPole); CQ is
+// 55*sqrt(3)/72 * uc/E0 of the quantum-excitation sigma.
+enum {
+  BM_PSI1 = 0, BM_PSI2, BM_IRHO, BM_L1, BM_L2, BM_K1, BM_K2, BM_SL, BM_CRAD, BM_CQ,
+  BM_HS1, BM_HQ1, BM_HS2, BM_HQ2, BM_HSH, BM_HQH, BM_K1C, BM_K2C, BM_B
+};
+
+// Forest-Ruth / Yoshida 4th-order coefficients (src/lib/accel/accel.h:44-47,
+// same decimal literals).
+constexpr double DRIFT1 = 0.6756035959798286638;
+constexpr double DRIFT2 = -0.1756035959798286639;
+constexpr double KICK1 = 1.351207191959657328;
+constexpr double KICK2 = -1.702414383919314656;
+
+constexpr double PI = 3.1415926535897932384626;  // accel.h:23
+constexpr double TWOPI = 2 * PI;
+constexpr double C0 = 2.99792458e8;               // accel.h:29
+constexpr double CGAMMA = 8.846056192e-05;        // atpass.cpp:270
+
+}  // namespace accp

@@ -1,0 +1,225 @@
+"""ctypes binding of libaccp.so (include/accp.h).
+
+The library must exist (built by `make -C accel-plus_amd`); there is no
+fallback: a missing library or a failing call raises AccpError.
+"""
+import ctypes
+import os
+from ctypes import POINTER, byref, c_char_p, c_double, c_int, c_int32, c_int64, c_uint64, c_void_p
+
+import numpy as np
+
+from . import elements as el
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libaccp.so")
+
+STAGE_AUTO, STAGE_SCALAR, STAGE_LDS = 0, 1, 2
+STAT_LEN = 20
+
+
+class AccpError(RuntimeError):
+    pass
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("block", c_int32), ("ppl", c_int32), ("staging", c_int32), ("reserved", c_int32)]
+
+    def __repr__(self):
+        st = {0: "auto", 1: "scalar", 2: "lds"}.get(self.staging, self.staging)
+        return f"Tuning(block={self.block}, ppl={self.ppl}, staging={st})"
+
+
+class TrackOpts(ctypes.Structure):
+    _fields_ = [
+        ("rng_seed", c_uint64), ("turn0", c_uint64), ("pid0", c_uint64),
+        ("mon_interval", c_int32), ("mon_max", c_int32),
+        ("mon_out", POINTER(c_double)), ("mon_count", POINTER(c_int32)),
+        ("tuning", Tuning),
+    ]
+
+
+class Param(ctypes.Structure):
+    _fields_ = [("field", c_int32), ("index", c_int32), ("nelem", c_int32), ("elems", POINTER(c_int32))]
+
+
+# Every entry point include/accp.h declares.
+EXPORTS = (
+    "accp_last_error", "accp_abi_version", "accp_device_count",
+    "accp_lattice_build", "accp_lattice_update", "accp_lattice_nelem", "accp_lattice_free",
+    "accp_track", "accp_track_device", "accp_beam_stat",
+    "accp_batch_create", "accp_track_batched", "accp_batch_capture", "accp_batch_replay",
+    "accp_autotune", "accp_batch_free",
+    "accp_adjoint_create", "accp_adjoint_forward", "accp_adjoint_backward", "accp_adjoint_free",
+)
+
+DP = POINTER(c_double)
+
+
+def _dp(a):
+    return a.ctypes.data_as(DP)
+
+
+class Library:
+    def __init__(self, path=LIB_PATH):
+        if not os.path.exists(path):
+            raise AccpError(f"{path} is not built: run `make -C accel-plus_amd` (no CPU fallback exists)")
+        L = ctypes.CDLL(path)
+        self.path = path
+        E = POINTER(el.Elem)
+        VP = c_void_p
+        L.accp_last_error.restype = c_char_p
+        L.accp_abi_version.restype = c_int
+        L.accp_device_count.argtypes = [POINTER(c_int)]
+        L.accp_lattice_build.argtypes = [E, c_int, POINTER(VP)]
+        L.accp_lattice_update.argtypes = [VP, E, c_int]
+        L.accp_lattice_nelem.argtypes = [VP, POINTER(c_int)]
+        L.accp_lattice_free.argtypes = [VP]
+        L.accp_track.argtypes = [VP, DP, c_int, c_int, c_int, c_int, POINTER(TrackOpts)]
+        L.accp_track_device.argtypes = [VP, c_void_p, c_int, c_int, c_int, c_int, POINTER(TrackOpts), c_void_p]
+        L.accp_beam_stat.argtypes = [DP, c_int, DP]
+        L.accp_batch_create.argtypes = [POINTER(VP), c_int, POINTER(VP)]
+        L.accp_track_batched.argtypes = [VP, c_void_p, POINTER(c_int64), c_int, POINTER(Tuning), c_void_p]
+        L.accp_batch_capture.argtypes = [VP, c_void_p, POINTER(c_int64), c_int, POINTER(Tuning)]
+        L.accp_batch_replay.argtypes = [VP, c_void_p]
+        L.accp_autotune.argtypes = [VP, c_void_p, POINTER(c_int64), c_int, POINTER(Tuning), DP]
+        L.accp_batch_free.argtypes = [VP]
+        L.accp_adjoint_create.argtypes = [VP, c_int, POINTER(Param), POINTER(VP)]
+        L.accp_adjoint_forward.argtypes = [VP, DP, c_int, c_int, c_int, DP]
+        L.accp_adjoint_backward.argtypes = [VP, DP, DP, DP]
+        L.accp_adjoint_free.argtypes = [VP]
+        for nm in EXPORTS:
+            if nm not in ("accp_last_error", "accp_abi_version"):
+                getattr(L, nm).restype = c_int
+        self.lib = L
+
+    def check(self, rc, what=""):
+        if rc != 0:
+            raise AccpError(f"{what} failed ({rc}): {self.lib.accp_last_error().decode()}")
+
+    def device_count(self):
+        c = c_int(0)
+        self.check(self.lib.accp_device_count(byref(c)), "accp_device_count")
+        return c.value
+
+    def lattice(self, elems):
+        return Lattice(self, elems)
+
+    def beam_stat(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        out = np.zeros(STAT_LEN)
+        self.check(self.lib.accp_beam_stat(_dp(r), r.size // 6, _dp(out)), "accp_beam_stat")
+        return out
+
+
+class Lattice:
+    def __init__(self, lib, elems):
+        self.L = lib
+        self.arr = elems if isinstance(elems, ctypes.Array) else el.to_array(elems)
+        h = c_void_p()
+        lib.check(lib.lib.accp_lattice_build(self.arr, len(self.arr), byref(h)), "accp_lattice_build")
+        self.h = h
+
+    def update(self, elems):
+        self.arr = elems if isinstance(elems, ctypes.Array) else el.to_array(elems)
+        self.L.check(self.L.lib.accp_lattice_update(self.h, self.arr, len(self.arr)), "accp_lattice_update")
+
+    def free(self):
+        if self.h:
+            self.L.lib.accp_lattice_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def track(self, r, nturn=1, st=0, ed=-1, seed=0, turn0=0, pid0=0, tuning=None,
+              mon_interval=0, mon_max=0):
+        """ALine::pass semantics on a host AoS array; returns (r_out, records)."""
+        r = np.ascontiguousarray(r, dtype=np.float64).copy()
+        o = TrackOpts()
+        o.rng_seed, o.turn0, o.pid0 = seed, turn0, pid0
+        if tuning is not None:
+            o.tuning = tuning
+        recs = None
+        cnt = c_int32(0)
+        if mon_interval > 0:
+            mon_max = mon_max or (nturn + mon_interval - 1) // mon_interval
+            recs = np.zeros((mon_max, STAT_LEN))
+            o.mon_interval, o.mon_max = mon_interval, mon_max
+            o.mon_out = _dp(recs)
+            o.mon_count = ctypes.pointer(cnt)
+        self.L.check(self.L.lib.accp_track(self.h, _dp(r), r.size // 6, nturn, st, ed, byref(o)), "accp_track")
+        if recs is not None:
+            return r.reshape(-1, 6), recs[: cnt.value]
+        return r.reshape(-1, 6)
+
+    def track_device(self, d_ptr, n, nturn, st=0, ed=-1, seed=0, turn0=0, pid0=0, tuning=None, stream=None):
+        o = TrackOpts()
+        o.rng_seed, o.turn0, o.pid0 = seed, turn0, pid0
+        if tuning is not None:
+            o.tuning = tuning
+        self.L.check(self.L.lib.accp_track_device(self.h, c_void_p(d_ptr), n, nturn, st, ed, byref(o),
+                                                  c_void_p(stream) if stream else None),
+                     "accp_track_device")
+
+
+class Batch:
+    def __init__(self, lib, lattices):
+        self.L = lib
+        self.lats = lattices
+        hs = (c_void_p * len(lattices))(*[x.h.value for x in lattices])
+        h = c_void_p()
+        lib.check(lib.lib.accp_batch_create(hs, len(lattices), byref(h)), "accp_batch_create")
+        self.h = h
+
+    def track(self, d_ptr, offsets, nturn, tuning=None, stream=None):
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        self._offs = offs
+        t = byref(tuning) if tuning is not None else None
+        self.L.check(self.L.lib.accp_track_batched(self.h, c_void_p(d_ptr), offs.ctypes.data_as(POINTER(c_int64)),
+                                                   nturn, t, c_void_p(stream) if stream else None),
+                     "accp_track_batched")
+
+    def capture(self, d_ptr, offsets, nturn, tuning=None):
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        self._offs = offs
+        t = byref(tuning) if tuning is not None else None
+        self.L.check(self.L.lib.accp_batch_capture(self.h, c_void_p(d_ptr), offs.ctypes.data_as(POINTER(c_int64)),
+                                                   nturn, t), "accp_batch_capture")
+
+    def replay(self, stream=None):
+        self.L.check(self.L.lib.accp_batch_replay(self.h, c_void_p(stream) if stream else None), "accp_batch_replay")
+
+    def autotune(self, d_ptr, offsets, nturn_probe):
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        best = Tuning()
+        ms = c_double()
+        self.L.check(self.L.lib.accp_autotune(self.h, c_void_p(d_ptr), offs.ctypes.data_as(POINTER(c_int64)),
+                                              nturn_probe, byref(best), byref(ms)), "accp_autotune")
+        return best, ms.value
+
+    def free(self):
+        if self.h:
+            self.L.lib.accp_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+_LIB = None
+
+
+def load(path=None):
+    global _LIB
+    if path is not None:
+        return Library(path)
+    if _LIB is None:
+        _LIB = Library()
+    return _LIB

@@ -1,0 +1,178 @@
+"""Element records of the accp C ABI and the reference lattice-file grammar.
+
+`Elem` mirrors `accp_elem` in include/accp.h field for field.  The writer
+emits the Accel+ lattice-file format that AccelElement::deserialize and the
+per-type create() parsers read (src/lib/accel/accel.cpp:67-133, :147-157,
+:184-192, :236-244, :299-307, :394-407, :481-489, :560-573, :718-736,
+:860-869), so the compiled reference and this backend can load the same file.
+"""
+import ctypes
+from ctypes import c_char, c_double, c_int32
+
+import numpy as np
+
+MAX_MPOLE = 10
+
+DRIFT, QUAD, CORRECTOR, APERTURE, MONITOR, BEND, BENDMPOLE, STRMPOLE, CAVITY, LONGWAKE = range(1, 11)
+TYPE_NAMES = {
+    DRIFT: "Drift", QUAD: "Quad", CORRECTOR: "Corrector", APERTURE: "Aperture",
+    MONITOR: "Monitor", BEND: "Bend", BENDMPOLE: "BendMPole", STRMPOLE: "StrMPole",
+    CAVITY: "Cavity", LONGWAKE: "LongWake",
+}
+RAD_OFF, RAD_DAMP, RAD_QE = 0, 1, 2
+FIELD_K, FIELD_POLYB, FIELD_POLYA = 1, 2, 3
+DEFAULT_E0 = 3000.0  # MeV, accel.h:30
+C0 = 2.99792458e8  # accel.h:29
+
+
+class Elem(ctypes.Structure):
+    _fields_ = [
+        ("type", c_int32), ("nslice", c_int32), ("max_npole", c_int32),
+        ("rad", c_int32), ("on", c_int32), ("nbins", c_int32),
+        ("len", c_double), ("k", c_double), ("theta", c_double),
+        ("e1", c_double), ("e2", c_double), ("energy", c_double),
+        ("volt", c_double), ("freq", c_double), ("phis", c_double),
+        ("kickx", c_double), ("kicky", c_double),
+        ("xlim", c_double * 2), ("ylim", c_double * 2),
+        ("polyB", c_double * MAX_MPOLE), ("polyA", c_double * MAX_MPOLE),
+        ("wake_rs", c_double), ("wake_q", c_double), ("wake_fr", c_double),
+        ("wake_charge", c_double), ("wake_zmin", c_double), ("wake_zmax", c_double),
+        ("famname", c_char * 32),
+    ]
+
+    @property
+    def name(self):
+        return self.famname.decode()
+
+    def copy(self):
+        e = Elem()
+        ctypes.pointer(e)[0] = self
+        return e
+
+    def __repr__(self):
+        return f"Elem({TYPE_NAMES.get(self.type, self.type)} {self.name} L={self.len})"
+
+
+def _mk(t, name, L, **kw):
+    e = Elem()
+    e.type = t
+    e.len = float(L)
+    e.famname = name.encode()[:31]
+    for k, v in kw.items():
+        if k in ("polyB", "polyA"):
+            arr = getattr(e, k)
+            for i, x in enumerate(v):
+                arr[i] = float(x)
+        elif k in ("xlim", "ylim"):
+            arr = getattr(e, k)
+            arr[0], arr[1] = float(v[0]), float(v[1])
+        else:
+            setattr(e, k, v)
+    return e
+
+
+def drift(L, name="D"):
+    return _mk(DRIFT, name, L)
+
+
+def quad(L, K, name="Q"):
+    return _mk(QUAD, name, L, k=float(K))
+
+
+def corrector(L, kx, ky, name="COR"):
+    return _mk(CORRECTOR, name, L, kickx=float(kx), kicky=float(ky))
+
+
+def aperture(L, xlim, ylim, name="AP"):
+    return _mk(APERTURE, name, L, xlim=xlim, ylim=ylim)
+
+
+def monitor(name="MON"):
+    return _mk(MONITOR, name, 0.0)
+
+
+def bend(L, theta, e1=0.0, e2=0.0, K=0.0, name="B"):
+    return _mk(BEND, name, L, theta=float(theta), e1=float(e1), e2=float(e2), k=float(K))
+
+
+def bendmpole(L, theta, e1=0.0, e2=0.0, polyB=(0.0,), name="BM", nslice=10, energy=DEFAULT_E0, rad=0):
+    return _mk(BENDMPOLE, name, L, theta=float(theta), e1=float(e1), e2=float(e2),
+               polyB=polyB, max_npole=len(polyB), nslice=nslice, energy=float(energy), rad=rad)
+
+
+def strmpole(L, polyB, polyA=None, name="SM", nslice=10, max_npole=None):
+    n = max_npole if max_npole is not None else len(polyB)
+    polyA = polyA if polyA is not None else [0.0] * n
+    return _mk(STRMPOLE, name, L, polyB=polyB, polyA=polyA, max_npole=n, nslice=nslice)
+
+
+def cavity(L, volt, freq, phis=0.0, energy=DEFAULT_E0, on=1, name="RF"):
+    return _mk(CAVITY, name, L, volt=float(volt), freq=float(freq), phis=float(phis),
+               energy=float(energy), on=on)
+
+
+def longwake(rs, q, fr, charge, nbins, zmin, zmax, energy=DEFAULT_E0, on=1, name="WAKE"):
+    return _mk(LONGWAKE, name, 0.0, wake_rs=float(rs), wake_q=float(q), wake_fr=float(fr),
+               wake_charge=float(charge), nbins=int(nbins), wake_zmin=float(zmin),
+               wake_zmax=float(zmax), energy=float(energy), on=on)
+
+
+def to_array(elems):
+    arr = (Elem * len(elems))()
+    for i, e in enumerate(elems):
+        arr[i] = e
+    return arr
+
+
+def copy_list(elems):
+    return [e.copy() for e in elems]
+
+
+def _g(v):
+    return "%.17g" % v
+
+
+def write_lattice(elems, path):
+    """Write `elems` in the Accel+ lattice-file grammar (one record per
+    element, multipole coefficient lines after StrMPole/BendMPole headers,
+    terminated by RETURN).  LongWake has no reference record and is refused."""
+    lines = []
+    for e in elems:
+        n = e.name
+        t = e.type
+        if t == DRIFT:
+            lines.append(f"{n} : Drift {_g(e.len)}")
+        elif t == QUAD:
+            lines.append(f"{n} : Quad {_g(e.len)} {_g(e.k)}")
+        elif t == CORRECTOR:
+            lines.append(f"{n} : Corrector {_g(e.len)} {_g(e.kickx)} {_g(e.kicky)}")
+        elif t == APERTURE:
+            lines.append(f"{n} : Aperture {_g(e.len)} {_g(e.xlim[0])} {_g(e.xlim[1])} "
+                         f"{_g(e.ylim[0])} {_g(e.ylim[1])}")
+        elif t == MONITOR:
+            lines.append(f"{n} : Monitor 0 1 1")
+        elif t == BEND:
+            lines.append(f"{n} : Bend {_g(e.len)} {_g(e.theta)} {_g(e.e1)} {_g(e.e2)} {_g(e.k)}")
+        elif t == BENDMPOLE:
+            lines.append(f"{n} : BendMPole {_g(e.len)} {_g(e.theta)} {_g(e.e1)} {_g(e.e2)} {e.max_npole}")
+            lines.append("   " + " ".join(_g(e.polyB[i]) for i in range(e.max_npole)))
+        elif t == STRMPOLE:
+            lines.append(f"{n} : StrMPole {_g(e.len)} {e.max_npole}")
+            lines.append("   " + " ".join(_g(e.polyB[i]) for i in range(e.max_npole)))
+            lines.append("   " + " ".join(_g(e.polyA[i]) for i in range(e.max_npole)))
+        elif t == CAVITY:
+            lines.append(f"{n} : Cavity {_g(e.len)} {_g(e.volt)} {_g(e.freq)} {_g(e.phis)} {_g(e.energy)}")
+        else:
+            raise ValueError(f"element type {t} has no reference lattice record")
+    lines.append("RETURN")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def write_plain_beam(r, path):
+    """Particles as the `Distribution: Load f, format=plain` reader expects:
+    six numbers per line (process.cpp:1519-1545), full precision."""
+    r = np.asarray(r, dtype=np.float64).reshape(-1, 6)
+    with open(path, "w") as f:
+        for row in r:
+            f.write(" ".join(_g(v) for v in row) + "\n")

@@ -1,0 +1,214 @@
+/*
+ * accp.h -- C ABI of the MI355X (gfx950) tracking backend for Accel+.
+ *
+ * Plain C, no C++ types, no exceptions across the boundary.  Every entry
+ * point returns 0 on success and a negative ACCP_E* code on failure; the
+ * message of the last failure on the calling thread is accp_last_error().
+ *
+ * Particle layout is the reference's own: an AoS double[6*N] buffer of
+ * {x, px, y, py, z, delta} per particle, updated in place; a particle is lost
+ * when x is NaN (then z holds the aperture loss code), exactly as
+ * AccelElement::pass(double*, int) treats it (src/lib/accel/accel.h:70,
+ * e.g. src/lib/accel/accel.cpp:166-167, :350-354).
+ *
+ * Reference interfaces replaced (file:line, paths relative to the Accel+ tree):
+ *   accp_lattice_build / accp_lattice_update
+ *        <- ALine::deserialize + AccelElement::create (src/lib/accel/aline.cpp:202-219,
+ *           src/lib/accel/accel.cpp:67-133) and the flag/field setters
+ *           ALine::set_rad_onoff / set_cavity_onoff / set_wake_onoff / setfield
+ *           (src/lib/accel/aline.cpp:258-293, :429-463, :980-1031)
+ *   accp_track / accp_track_device
+ *        <- ALine::pass(double*, int, unsigned st, unsigned ed),
+ *           ALine::pass(double*, int), ALine::pass(double*, int, int nturn)
+ *           (src/lib/accel/accel.h:625-627, src/lib/accel/aline.cpp:313-363)
+ *   accp_beam_stat
+ *        <- calc_beam_stat (src/lib/accel/utility.cpp:322-373), as used by
+ *           AMonitor::pass (src/lib/accel/accel.cpp:437-475)
+ *   accp_batch_* / accp_track_batched
+ *        <- no reference counterpart (the reference tracks one lattice per
+ *           process); each seed behaves exactly as accp_track on that seed.
+ *   accp_adjoint_*
+ *        <- replaces the finite-difference responses of ALine::calctuneresp /
+ *           calcchromresp (src/lib/accel/aline.cpp:1187-1242, :1310-1355).
+ */
+#ifndef ACCP_H
+#define ACCP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ACCP_ABI_VERSION 1
+
+/* ---- error codes ------------------------------------------------------- */
+#define ACCP_OK 0
+#define ACCP_E_ARG -1     /* invalid argument (null pointer, bad size, range) */
+#define ACCP_E_HIP -2     /* HIP runtime failure (no device, launch, copy)     */
+#define ACCP_E_NOMEM -3   /* device or host allocation failed                   */
+#define ACCP_E_LATTICE -4 /* unsupported element type / inconsistent lattice   */
+#define ACCP_E_STATE -5   /* call out of order (e.g. backward before forward)  */
+
+/* ---- element type codes (one per reference element class on the path) --- */
+#define ACCP_DRIFT 1     /* ADrift      src/lib/accel/accel.cpp:158-177 */
+#define ACCP_QUAD 2      /* AQuad       src/lib/accel/accel.cpp:193-209 */
+#define ACCP_CORRECTOR 3 /* ACorrector  src/lib/accel/accel.cpp:270-292 */
+#define ACCP_APERTURE 4  /* AAperture   src/lib/accel/accel.cpp:337-387 */
+#define ACCP_MONITOR 5   /* AMonitor    src/lib/accel/accel.cpp:437-475 (no-op on coordinates) */
+#define ACCP_BEND 6      /* ABend       src/lib/accel/accel.cpp:523-550 */
+#define ACCP_BENDMPOLE 7 /* ABendMPole  src/lib/accel/accel.cpp:649-705 */
+#define ACCP_STRMPOLE 8  /* AStrMPole   src/lib/accel/accel.cpp:800-847 */
+#define ACCP_CAVITY 9    /* ACavity     src/lib/accel/accel.cpp:903-923 */
+#define ACCP_LONGWAKE 10 /* longitudinal resonator wake: new, no reference class */
+
+#define ACCP_MAX_MPOLE 10 /* MAX_MPOLE, src/lib/accel/accel.h:48 */
+#define ACCP_NAME_LEN 32
+
+/* Radiation flag values of ABendMPole::set_rad_onoff (src/lib/accel/accel.h:577-579). */
+#define ACCP_RAD_OFF 0
+#define ACCP_RAD_DAMP 1
+#define ACCP_RAD_QE 2
+
+/*
+ * One element, in the units of the reference lattice file (metres, radians,
+ * MV, Hz, MeV).  Fields that a type does not use are ignored.  A record is
+ * what AccelElement::create parses for that type plus the state the reference
+ * keeps outside the file (nslice, energy, radiation and on/off flags).
+ */
+typedef struct accp_elem {
+  int32_t type;      /* ACCP_* type code                                      */
+  int32_t nslice;    /* BendMPole/StrMPole slices (reference default 10)      */
+  int32_t max_npole; /* BendMPole/StrMPole MaxNPole (1..10)                   */
+  int32_t rad;       /* BendMPole radiation flag ACCP_RAD_*                   */
+  int32_t on;        /* Cavity flag_onoff, LongWake on/off                    */
+  int32_t nbins;     /* LongWake: number of longitudinal bins                 */
+  double len;        /* length [m]                                            */
+  double k;          /* Quad K [m^-2]; Bend gradient K                        */
+  double theta;      /* Bend/BendMPole bending angle [rad]                    */
+  double e1, e2;     /* Bend/BendMPole entrance/exit edge angles [rad]        */
+  double energy;     /* BendMPole/Cavity/LongWake beam energy [MeV]           */
+  double volt;       /* Cavity voltage [MV]                                   */
+  double freq;       /* Cavity frequency [Hz]                                 */
+  double phis;       /* Cavity synchronous phase [rad]                        */
+  double kickx, kicky;   /* Corrector kicks [rad]                             */
+  double xlim[2], ylim[2]; /* Aperture limits [m]                             */
+  double polyB[ACCP_MAX_MPOLE]; /* normal multipoles, B[n] in m^-(n+1)       */
+  double polyA[ACCP_MAX_MPOLE]; /* skew multipoles                           */
+  double wake_rs;    /* LongWake shunt impedance [Ohm]                        */
+  double wake_q;     /* LongWake quality factor                               */
+  double wake_fr;    /* LongWake resonant frequency [Hz]                      */
+  double wake_charge;/* LongWake bunch charge [C]                             */
+  double wake_zmin, wake_zmax; /* LongWake binning window in z [m]            */
+  char famname[ACCP_NAME_LEN]; /* family name (host-side lookups only)       */
+} accp_elem;
+
+typedef struct accp_lattice accp_lattice; /* device-resident element table   */
+typedef struct accp_batch accp_batch;     /* many seeds, one topology         */
+typedef struct accp_adjoint accp_adjoint; /* reverse-mode turn map context    */
+
+/* Element-table staging inside the tracking kernel. */
+#define ACCP_STAGE_AUTO 0   /* library default                                */
+#define ACCP_STAGE_SCALAR 1 /* wave-uniform scalar loads through the K$       */
+#define ACCP_STAGE_LDS 2    /* whole table copied to LDS per workgroup        */
+
+/* Launch shape of the tracking kernel; zero fields mean "library default". */
+typedef struct accp_tuning {
+  int32_t block;   /* threads per workgroup: 64, 128, 256                     */
+  int32_t ppl;     /* particles per lane: 1 or 2                              */
+  int32_t staging; /* ACCP_STAGE_*                                            */
+  int32_t reserved;
+} accp_tuning;
+
+/* Options of one tracking call. */
+typedef struct accp_track_opts {
+  uint64_t rng_seed;    /* key of the counter-based RNG (quantum excitation)  */
+  uint64_t turn0;       /* turn counter of the first tracked turn             */
+  uint64_t pid0;        /* particle id of r[0] (RNG stream = pid0 + i)        */
+  int32_t mon_interval; /* >0: record the beam every mon_interval turns at    */
+                        /* turn start (AMonitor at element 0), else 0         */
+  int32_t mon_max;      /* capacity of mon_out in records                     */
+  double *mon_out;      /* host: mon_max records of ACCP_STAT_LEN doubles     */
+  int32_t *mon_count;   /* host: number of records written                    */
+  accp_tuning tuning;   /* launch shape (zeros = default)                     */
+} accp_track_opts;
+
+/* One beam statistics record, the quantities calc_beam_stat returns:
+ * [0]=turn, [1]=nlive, [2..7]=mean of x..delta, [8..13]=rms of x..delta,
+ * [14..19]=<xx'>,<yy'>,<x delta>,<x' delta>,<y z>,<y' z> (central moments). */
+#define ACCP_STAT_LEN 20
+
+/* ---- library ----------------------------------------------------------- */
+const char *accp_last_error(void);
+int accp_abi_version(void);
+/* Number of HIP devices visible (0 on a machine without a GPU). */
+int accp_device_count(int *count);
+
+/* ---- lattice ----------------------------------------------------------- */
+int accp_lattice_build(const accp_elem *elems, int n, accp_lattice **out);
+/* Same topology, new parameters or flags (set_flags / modify_element). */
+int accp_lattice_update(accp_lattice *lat, const accp_elem *elems, int n);
+int accp_lattice_nelem(const accp_lattice *lat, int *n);
+int accp_lattice_free(accp_lattice *lat);
+
+/* ---- tracking ---------------------------------------------------------- */
+/* ALine::pass over elements [st, ed) repeated nturn times on a host AoS
+ * buffer; synchronous.  ed < 0 means "to the end of the line". */
+int accp_track(accp_lattice *lat, double *r, int n, int nturn, int st, int ed,
+               const accp_track_opts *opts);
+/* Same, on a device-resident AoS buffer, enqueued on `stream` (a
+ * hipStream_t, NULL = default stream).  Monitor records need the host
+ * variant. */
+int accp_track_device(accp_lattice *lat, double *d_r, int n, int nturn, int st,
+                      int ed, const accp_track_opts *opts, void *stream);
+
+/* calc_beam_stat on a host AoS buffer, computed on the device. */
+int accp_beam_stat(const double *r, int n, double *stat /* ACCP_STAT_LEN */);
+
+/* ---- batched multi-seed tracking ---------------------------------------- */
+/* All seeds must share one topology (types, slices, orders). */
+int accp_batch_create(accp_lattice *const *seeds, int nseed, accp_batch **out);
+/* Track all seeds in one launch.  d_r is one device AoS buffer; particles of
+ * seed s are [offsets[s], offsets[s+1]).  offsets is a host array of nseed+1. */
+int accp_track_batched(accp_batch *b, double *d_r, const int64_t *offsets,
+                       int nturn, const accp_tuning *tuning, void *stream);
+/* Capture accp_track_batched into a HIP graph (replay = one launch). */
+int accp_batch_capture(accp_batch *b, double *d_r, const int64_t *offsets,
+                       int nturn, const accp_tuning *tuning);
+int accp_batch_replay(accp_batch *b, void *stream);
+/* Time candidate launch shapes on a short run and return the fastest. */
+int accp_autotune(accp_batch *b, double *d_r, const int64_t *offsets,
+                  int nturn_probe, accp_tuning *best, double *best_ms);
+int accp_batch_free(accp_batch *b);
+
+/* ---- reverse-mode adjoint of the turn map -------------------------------- */
+#define ACCP_FIELD_K 1     /* Quad K                                          */
+#define ACCP_FIELD_POLYB 2 /* StrMPole PolyB[index]                           */
+#define ACCP_FIELD_POLYA 3 /* StrMPole PolyA[index]                           */
+
+/* One differentiable knob: a family of elements whose field moves together
+ * (a shift of the knob adds to every member, as calctuneresp does). */
+typedef struct accp_param {
+  int32_t field;        /* ACCP_FIELD_*                                       */
+  int32_t index;        /* multipole index for POLYB/POLYA                    */
+  int32_t nelem;        /* number of member elements                          */
+  const int32_t *elems; /* member element indices                             */
+} accp_param;
+
+int accp_adjoint_create(accp_lattice *lat, int nparam, const accp_param *params,
+                        accp_adjoint **out);
+/* Forward: track r0 (host AoS, n particles) for nturn turns, keeping a
+ * checkpoint every ckpt_every turns; r_out receives the final coordinates. */
+int accp_adjoint_forward(accp_adjoint *adj, const double *r0, int n, int nturn,
+                         int ckpt_every, double *r_out);
+/* Backward: given dL/dr_out (host AoS), return dL/dparam[nparam] and,
+ * if dl_dr0 is not NULL, dL/dr0 (host AoS). */
+int accp_adjoint_backward(accp_adjoint *adj, const double *dl_dr_out,
+                          double *dl_dparam, double *dl_dr0);
+int accp_adjoint_free(accp_adjoint *adj);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ACCP_H */

@@ -1,0 +1,106 @@
+/*
+ * accp_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement of the Accel+ tracking formulas, used by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker.
+ * Nothing in the product (the accp C ABI, its HIP kernels, its host driver)
+ * links, loads or calls this code.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the Accel+ tree).  Built with -O2 -ffp-contract=off and no -march, the
+ * same arithmetic the reference gets from g++ -O2 on x86-64 (no FMA).
+ */
+#ifndef ACCP_ORACLE_H
+#define ACCP_ORACLE_H
+
+#include <stdint.h>
+#include "../include/accp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- integrator primitives on one r[6] (src/lib/accel/atpass.cpp) ------ */
+void orc_drift_kzero(double *r, double L);                            /* :6-13    */
+void orc_quad6(double *r, double L, double K);                        /* :15-72   */
+void orc_bend6(double *r, double L, double angle, double grd, double by_err); /* :76-187 */
+void orc_edge(double *r, double inv_rho, double edge_angle);          /* :190-196 */
+void orc_bndthinkick(double *r, const double *B, double L, double irho, int max_order); /* :199-235 */
+double orc_b2perp(double bx, double by, double irho, double x, double xpr,
+                  double y, double ypr);                              /* :237-253 */
+void orc_bndthinkickrad(double *r, const double *B, double L, double irho,
+                        double E0, int max_order, int flag_rad, double gauss); /* :255-339 */
+void orc_hxdrift6(double *r, double L, double h);                     /* :343-355 */
+void orc_strthinkick(double *r, const double *A, const double *B, double L,
+                     int max_order);                                  /* :358-414 */
+void orc_fastdrift(double *r, double norm_l);                         /* :416-428 */
+
+/* ---- counter-based Gaussian stream (quantum excitation) ----------------- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* Standard normal for one radiation kick.  Counter = (particle id, turn,
+ * element index, slice), key = seed; Philox4x32-10 -> two 53-bit uniforms
+ * in (0,1) -> Box-Muller cosine branch. */
+double orc_gauss(uint64_t seed, uint64_t pid, uint64_t turn, uint32_t eidx,
+                 uint32_t slice);
+
+/* ---- one element over a beam (src/lib/accel/accel.cpp) ------------------ */
+typedef struct orc_ctx {
+  uint64_t seed;  /* RNG key                                  */
+  uint64_t turn;  /* absolute turn number                     */
+  uint64_t pid0;  /* particle id of r[0]                      */
+  int32_t eidx;   /* element index in the line                */
+} orc_ctx;
+
+void orc_elem_pass(const accp_elem *e, double *r, int n, const orc_ctx *ctx);
+
+/* ---- line (src/lib/accel/aline.cpp:313-363) ----------------------------- */
+/* Elements [st, ed) on all particles, nturn times; ed < 0 = whole line. */
+void orc_line_pass(const accp_elem *elems, int nelem, double *r, int n, int st,
+                   int ed, int nturn, uint64_t seed, uint64_t turn0, uint64_t pid0);
+/* As orc_line_pass, with OpenMP over contiguous particle blocks the way
+ * ALine::pass(r,N,st,ed) partitions them (aline.cpp:313-329).  Elements
+ * that need the whole beam (LongWake) are rejected (returns -1). */
+int orc_line_pass_omp(const accp_elem *elems, int nelem, double *r, int n,
+                      int nturn, int nthreads, uint64_t seed, uint64_t pid0);
+
+/* ---- optics by tracking (src/lib/accel/aline.cpp) ----------------------- */
+void orc_getm66(const accp_elem *elems, int nelem, int st, int ed, double dpp,
+                double delta, double m66[36]);                        /* :900-927  */
+void orc_calctune(const accp_elem *elems, int nelem, double dpp, double delta,
+                  double *tunex, double *tuney);                      /* :1032-1043 */
+void orc_calcchrom(const accp_elem *elems, int nelem, double dp, double delta,
+                   double *chromx, double *chromy);                   /* :1108-1126 */
+/* Knob kinds of calctuneresp / calcchromresp. */
+#define ORC_KNOB_TUNE 1  /* Quad K or StrMPole PolyB[1] (+1e-5)  */
+#define ORC_KNOB_CHROM 2 /* StrMPole PolyB[2] (+1e-5)            */
+void orc_calcresp(accp_elem *elems, int nelem, int kind, const int *fam1, int n1,
+                  const int *fam2, int n2, double resp[4]);           /* :1187-1242, :1310-1355 */
+/* One correction step; target in/out as in ALine::correcttune/correctchrom
+ * (:1243-1309, :1356-1418).  Modifies elems in place. */
+void orc_correct(accp_elem *elems, int nelem, int kind, const int *fam1, int n1,
+                 const int *fam2, int n2, double *tx, double *ty);
+
+/* ---- studies ------------------------------------------------------------ */
+/* ALine::setrfphase (aline.cpp:465-501): returns phis, writes it into every
+ * cavity; flags are restored the way the reference leaves them
+ * (radiation off, cavities off). */
+double orc_setrfphase(accp_elem *elems, int nelem);
+/* ALine::trackDA without the loss-turn printout (aline.cpp:607-715). */
+double orc_trackda(const accp_elem *elems, int nelem, double xmax, double xmin,
+                   double ymax, int nturn, int nray, int nstep, double *rx,
+                   double *ry, int negweight);
+/* calc_beam_stat (utility.cpp:322-373) into an ACCP_STAT_LEN record. */
+int orc_beam_stat(const double *r, int n, double turn, double *stat);
+
+/* ---- longitudinal resonator wake (new element; closed form) ------------- */
+/* W(tau) of a resonator: 2*alpha*Rs*exp(-alpha*tau)*(cos(wb*tau) -
+ * alpha/wb*sin(wb*tau)) for tau > 0 and alpha*Rs at tau = 0. */
+double orc_wake_w(double rs, double q, double fr, double tau);
+/* Brute-force O(N^2) energy kick of every live particle from all others
+ * (no binning): used to check the binned element. */
+void orc_wake_pairwise(const accp_elem *e, const double *r, int n, double *ddelta);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

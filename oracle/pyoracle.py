@@ -1,0 +1,309 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes views of the CPU checker.
+
+`Oracle` wraps liboracle.so (the plain-C restatement in accp_oracle.c);
+`Reference` wraps _ref/libaccel_ref.so (the reference's own sources plus
+ref_harness.cpp).  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module.
+"""
+import ctypes
+import os
+import tempfile
+from ctypes import POINTER, c_char_p, c_double, c_int, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DP = POINTER(c_double)
+IP = POINTER(c_int)
+
+
+def _dp(a):
+    return a.ctypes.data_as(DP)
+
+
+def _ip(a):
+    return a.ctypes.data_as(IP)
+
+
+class Oracle:
+    def __init__(self, path=None):
+        import pyaccp.elements as el
+
+        self.el = el
+        path = path or os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (make -C oracle)")
+        L = ctypes.CDLL(path)
+        E = POINTER(el.Elem)
+        L.orc_line_pass.argtypes = [E, c_int, DP, c_int, c_int, c_int, c_int, c_uint64, c_uint64, c_uint64]
+        L.orc_line_pass_omp.argtypes = [E, c_int, DP, c_int, c_int, c_int, c_uint64, c_uint64]
+        L.orc_line_pass_omp.restype = c_int
+        L.orc_elem_pass.argtypes = [E, DP, c_int, c_void_p]
+        L.orc_getm66.argtypes = [E, c_int, c_int, c_int, c_double, c_double, DP]
+        L.orc_calctune.argtypes = [E, c_int, c_double, c_double, DP, DP]
+        L.orc_calcchrom.argtypes = [E, c_int, c_double, c_double, DP, DP]
+        L.orc_calcresp.argtypes = [E, c_int, c_int, IP, c_int, IP, c_int, DP]
+        L.orc_correct.argtypes = [E, c_int, c_int, IP, c_int, IP, c_int, DP, DP]
+        L.orc_setrfphase.argtypes = [E, c_int]
+        L.orc_setrfphase.restype = c_double
+        L.orc_trackda.argtypes = [E, c_int, c_double, c_double, c_double, c_int, c_int, c_int, DP, DP, c_int]
+        L.orc_trackda.restype = c_double
+        L.orc_beam_stat.argtypes = [DP, c_int, c_double, DP]
+        L.orc_beam_stat.restype = c_int
+        L.orc_philox4x32_10.argtypes = [POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32)]
+        L.orc_gauss.argtypes = [c_uint64, c_uint64, c_uint64, c_uint32, c_uint32]
+        L.orc_gauss.restype = c_double
+        L.orc_wake_w.argtypes = [c_double, c_double, c_double, c_double]
+        L.orc_wake_w.restype = c_double
+        L.orc_wake_pairwise.argtypes = [E, DP, c_int, DP]
+        for nm in ("orc_quad6", "orc_drift_kzero", "orc_fastdrift"):
+            getattr(L, nm).argtypes = [DP, c_double] + ([c_double] if nm == "orc_quad6" else [])
+        self.lib = L
+
+    def _arr(self, elems):
+        return elems if isinstance(elems, ctypes.Array) else self.el.to_array(elems)
+
+    def track(self, elems, r, nturn=1, st=0, ed=-1, seed=0, turn0=0, pid0=0):
+        a = self._arr(elems)
+        r = np.ascontiguousarray(r, dtype=np.float64).copy()
+        n = r.size // 6
+        self.lib.orc_line_pass(a, len(a), _dp(r), n, st, ed, nturn, seed, turn0, pid0)
+        return r.reshape(-1, 6)
+
+    def track_omp(self, elems, r, nturn, nthreads, seed=0):
+        a = self._arr(elems)
+        r = np.ascontiguousarray(r, dtype=np.float64).copy()
+        rc = self.lib.orc_line_pass_omp(a, len(a), _dp(r), r.size // 6, nturn, nthreads, seed, 0)
+        if rc != 0:
+            raise ValueError("collective element in an OpenMP pass")
+        return r.reshape(-1, 6)
+
+    def getm66(self, elems, dpp=0.0, delta=1e-8, st=0, ed=-1):
+        a = self._arr(elems)
+        m = np.zeros(36)
+        self.lib.orc_getm66(a, len(a), st, ed, dpp, delta, _dp(m))
+        return m.reshape(6, 6)
+
+    def tune(self, elems, dpp=0.0, delta=1e-8):
+        a = self._arr(elems)
+        x, y = c_double(), c_double()
+        self.lib.orc_calctune(a, len(a), dpp, delta, ctypes.byref(x), ctypes.byref(y))
+        return x.value, y.value
+
+    def chrom(self, elems, dp=0.0, delta=1e-8):
+        a = self._arr(elems)
+        x, y = c_double(), c_double()
+        self.lib.orc_calcchrom(a, len(a), dp, delta, ctypes.byref(x), ctypes.byref(y))
+        return x.value, y.value
+
+    def resp(self, elems_arr, kind, fam1, fam2):
+        f1 = np.asarray(fam1, dtype=np.int32)
+        f2 = np.asarray(fam2, dtype=np.int32)
+        out = np.zeros(4)
+        self.lib.orc_calcresp(elems_arr, len(elems_arr), kind, _ip(f1), len(f1), _ip(f2), len(f2), _dp(out))
+        return out.reshape(2, 2)
+
+    def correct(self, elems_arr, kind, fam1, fam2, tx, ty):
+        f1 = np.asarray(fam1, dtype=np.int32)
+        f2 = np.asarray(fam2, dtype=np.int32)
+        x, y = c_double(tx), c_double(ty)
+        self.lib.orc_correct(elems_arr, len(elems_arr), kind, _ip(f1), len(f1), _ip(f2), len(f2),
+                             ctypes.byref(x), ctypes.byref(y))
+        return x.value, y.value
+
+    def setrfphase(self, elems_arr):
+        return self.lib.orc_setrfphase(elems_arr, len(elems_arr))
+
+    def trackda(self, elems, xmax, xmin, ymax, nturn, nray, nstep, negweight=False):
+        a = self._arr(elems)
+        rx = np.zeros(51)
+        ry = np.zeros(51)
+        s = self.lib.orc_trackda(a, len(a), xmax, xmin, ymax, nturn, nray, nstep, _dp(rx), _dp(ry), int(negweight))
+        nr = min(max(nray, 3), 51)
+        return s, rx[:nr], ry[:nr]
+
+    def beam_stat(self, r, turn=0.0):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        out = np.zeros(20)
+        self.lib.orc_beam_stat(_dp(r), r.size // 6, turn, _dp(out))
+        return out
+
+    def philox(self, ctr, key):
+        c = (c_uint32 * 4)(*ctr)
+        k = (c_uint32 * 2)(*key)
+        o = (c_uint32 * 4)()
+        self.lib.orc_philox4x32_10(c, k, o)
+        return list(o)
+
+    def gauss(self, seed, pid, turn, eidx, slc):
+        return self.lib.orc_gauss(seed, pid, turn, eidx, slc)
+
+    def wake_w(self, rs, q, fr, tau):
+        return self.lib.orc_wake_w(rs, q, fr, tau)
+
+    def wake_pairwise(self, e, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        out = np.zeros(r.size // 6)
+        self.lib.orc_wake_pairwise(ctypes.byref(e), _dp(r), r.size // 6, _dp(out))
+        return out
+
+
+class Reference:
+    """The compiled reference (oracle/_ref/libaccel_ref.so)."""
+
+    def __init__(self, path=None):
+        path = path or os.path.join(HERE, "_ref", "libaccel_ref.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (make -C oracle ref)")
+        L = ctypes.CDLL(path)
+        L.ref_load.argtypes = [c_char_p]
+        L.ref_load.restype = c_void_p
+        L.ref_free.argtypes = [c_void_p]
+        L.ref_nelem.argtypes = [c_void_p]
+        L.ref_set_rad.argtypes = [c_void_p, c_int]
+        L.ref_set_cavity.argtypes = [c_void_p, c_int]
+        L.ref_set_threads.argtypes = [c_int]
+        L.ref_pass.argtypes = [c_void_p, DP, c_int, c_int]
+        L.ref_pass_range.argtypes = [c_void_p, DP, c_int, c_int, c_int]
+        L.ref_elem_pass.argtypes = [c_void_p, c_int, DP, c_int]
+        L.ref_setrfphase.argtypes = [c_void_p]
+        L.ref_setrfphase.restype = c_double
+        L.ref_getm66.argtypes = [c_void_p, c_double, c_double, DP]
+        L.ref_calctune.argtypes = [c_void_p, DP, DP]
+        L.ref_calcchrom.argtypes = [c_void_p, DP, DP]
+        L.ref_correct.argtypes = [c_void_p, c_int, c_char_p, c_char_p, c_char_p, DP, DP]
+        L.ref_calcresp.argtypes = [c_void_p, c_int, c_char_p, c_char_p, c_char_p, DP]
+        L.ref_trackda.argtypes = [c_void_p, c_double, c_double, c_double, c_int, c_int, c_int, DP, DP, c_int]
+        L.ref_trackda.restype = c_double
+        L.ref_beam_stat.argtypes = [DP, c_int, DP, DP, DP]
+        L.ref_beam_stat.restype = c_int
+        L.ref_setfield.argtypes = [c_void_p, c_int, c_char_p, c_double]
+        L.ref_setfield_int.argtypes = [c_void_p, c_int, c_char_p, c_int]
+        L.ref_set_mpole.argtypes = [c_void_p, c_int, c_char_p, c_int, c_double]
+        L.ref_getfield.argtypes = [c_void_p, c_int, c_char_p]
+        L.ref_getfield.restype = c_double
+        L.ref_get_mpole.argtypes = [c_void_p, c_int, c_char_p, c_int]
+        L.ref_get_mpole.restype = c_double
+        self.lib = L
+
+    def load(self, elems):
+        """Write `elems` as a lattice file, load it, then apply the state the
+        file cannot carry (nslice, BendMPole energy, radiation, cavity flags)."""
+        import pyaccp.elements as el
+
+        fd, path = tempfile.mkstemp(suffix=".lat")
+        os.close(fd)
+        try:
+            el.write_lattice(elems, path)
+            h = self.lib.ref_load(path.encode())
+        finally:
+            os.unlink(path)
+        if not h:
+            raise RuntimeError("reference failed to load lattice")
+        line = RefLine(self.lib, h)
+        if self.lib.ref_nelem(h) != len(elems):
+            raise RuntimeError(f"reference parsed {self.lib.ref_nelem(h)} of {len(elems)} elements")
+        for i, e in enumerate(elems):
+            if e.type in (el.BENDMPOLE, el.STRMPOLE) and e.nslice != 10:
+                self.lib.ref_setfield_int(h, i, b"NSlice", e.nslice)
+            if e.type == el.BENDMPOLE and e.energy != el.DEFAULT_E0:
+                self.lib.ref_setfield(h, i, b"energy", e.energy)
+        rads = {e.rad for e in elems if e.type == el.BENDMPOLE}
+        if len(rads) > 1:
+            raise ValueError("reference sets one radiation flag for the whole line")
+        line.set_rad(rads.pop() if rads else 0)
+        cav = [e.on for e in elems if e.type == el.CAVITY]
+        if cav:
+            if len(set(cav)) > 1:
+                raise ValueError("reference sets one cavity flag for the whole line")
+            line.set_cavity(cav[0])
+        return line
+
+
+class RefLine:
+    def __init__(self, lib, h):
+        self.lib, self.h = lib, h
+
+    def __del__(self):
+        try:
+            self.lib.ref_free(self.h)
+        except Exception:
+            pass
+
+    def set_rad(self, flag):
+        self.lib.ref_set_rad(self.h, flag)
+
+    def set_cavity(self, on):
+        self.lib.ref_set_cavity(self.h, int(on))
+
+    def set_threads(self, n):
+        self.lib.ref_set_threads(n)
+
+    def track(self, r, nturn=1):
+        r = np.ascontiguousarray(r, dtype=np.float64).copy()
+        self.lib.ref_pass(self.h, _dp(r), r.size // 6, nturn)
+        return r.reshape(-1, 6)
+
+    def track_range(self, r, st, ed):
+        r = np.ascontiguousarray(r, dtype=np.float64).copy()
+        self.lib.ref_pass_range(self.h, _dp(r), r.size // 6, st, ed)
+        return r.reshape(-1, 6)
+
+    def elem_pass(self, idx, r):
+        r = np.ascontiguousarray(r, dtype=np.float64).copy()
+        self.lib.ref_elem_pass(self.h, idx, _dp(r), r.size // 6)
+        return r.reshape(-1, 6)
+
+    def setrfphase(self):
+        return self.lib.ref_setrfphase(self.h)
+
+    def getm66(self, dpp=0.0, delta=1e-8):
+        m = np.zeros(36)
+        self.lib.ref_getm66(self.h, dpp, delta, _dp(m))
+        return m.reshape(6, 6)
+
+    def tune(self):
+        x, y = c_double(), c_double()
+        self.lib.ref_calctune(self.h, ctypes.byref(x), ctypes.byref(y))
+        return x.value, y.value
+
+    def chrom(self):
+        x, y = c_double(), c_double()
+        self.lib.ref_calcchrom(self.h, ctypes.byref(x), ctypes.byref(y))
+        return x.value, y.value
+
+    def resp(self, kind, etype, fam1, fam2):
+        out = np.zeros(4)
+        self.lib.ref_calcresp(self.h, kind, etype.encode(), fam1.encode(), fam2.encode(), _dp(out))
+        return out.reshape(2, 2)
+
+    def correct(self, kind, etype, fam1, fam2, tx, ty):
+        x, y = c_double(tx), c_double(ty)
+        rc = self.lib.ref_correct(self.h, kind, etype.encode(), fam1.encode(), fam2.encode(),
+                                  ctypes.byref(x), ctypes.byref(y))
+        if rc:
+            raise RuntimeError("families not found")
+        return x.value, y.value
+
+    def trackda(self, xmax, xmin, ymax, nturn, nray, nstep, negweight=False):
+        rx = np.zeros(51)
+        ry = np.zeros(51)
+        s = self.lib.ref_trackda(self.h, xmax, xmin, ymax, nturn, nray, nstep, _dp(rx), _dp(ry), int(negweight))
+        nr = min(max(nray, 3), 51)
+        return s, rx[:nr], ry[:nr]
+
+    def getfield(self, idx, name):
+        return self.lib.ref_getfield(self.h, idx, name.encode())
+
+    def get_mpole(self, idx, name, index):
+        return self.lib.ref_get_mpole(self.h, idx, name.encode(), index)
+
+    def setfield(self, idx, name, v):
+        self.lib.ref_setfield(self.h, idx, name.encode(), v)
+
+
+def beam_stat_ref(ref, r):
+    r = np.ascontiguousarray(r, dtype=np.float64).copy()
+    ar, sg, sr = np.zeros(6), np.zeros(6), np.zeros(6)
+    nl = ref.lib.ref_beam_stat(_dp(r), r.size // 6, _dp(ar), _dp(sg), _dp(sr))
+    return nl, ar, sg, sr

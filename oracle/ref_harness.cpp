@@ -1,0 +1,151 @@
+// ref_harness.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// A small extern "C" shim over the compiled Accel+ reference library (its
+// own sources, unmodified, compiled by oracle/Makefile into
+// oracle/_ref/libaccel_ref.so).  Tests load it with ctypes to write golden
+// fixtures and to pin the oracle; bench.py times ref_pass as the CPU
+// baseline.  Nothing in the product links it.
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <string>
+
+#include <omp.h>
+
+#include "accel.h"
+#include "matvec.h"
+#include "utility.h"
+
+namespace {
+bool g_map_ready = false;
+void ensure_map() {
+  if (!g_map_ready) {
+    AccelElement::initializemap();
+    g_map_ready = true;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+// ALine::deserialize on a lattice file (aline.cpp:202-219).
+void *ref_load(const char *path) {
+  ensure_map();
+  std::ifstream fin(path);
+  if (!fin.good()) return nullptr;
+  ALine *line = ALine::deserialize(fin);
+  return line;
+}
+
+void ref_free(void *h) { delete static_cast<ALine *>(h); }
+
+int ref_nelem(void *h) { return (int)static_cast<ALine *>(h)->getnumber(); }
+
+void ref_set_rad(void *h, int flag) { static_cast<ALine *>(h)->set_rad_onoff((unsigned)flag); }
+
+void ref_set_cavity(void *h, int on) { static_cast<ALine *>(h)->set_cavity_onoff(on != 0); }
+
+void ref_set_threads(int n) { omp_set_num_threads(n < 1 ? 1 : n); }
+
+// ALine::pass(r, N, nturn) (aline.cpp:359-363).
+void ref_pass(void *h, double *r, int n, int nturn) { static_cast<ALine *>(h)->pass(r, n, nturn); }
+
+// ALine::pass(r, N, st, ed) (aline.cpp:313-329).
+void ref_pass_range(void *h, double *r, int n, int st, int ed) {
+  static_cast<ALine *>(h)->pass(r, n, (unsigned)st, (unsigned)ed);
+}
+
+// One element's pass on its own (AccelElement::pass, accel.h:70).
+void ref_elem_pass(void *h, int idx, double *r, int n) {
+  static_cast<ALine *>(h)->getelement((unsigned)idx)->pass(r, n);
+}
+
+double ref_setrfphase(void *h) { return static_cast<ALine *>(h)->setrfphase(); }
+
+void ref_getm66(void *h, double dpp, double delta, double *m66) {
+  Mat m(6, 6);
+  static_cast<ALine *>(h)->getm66(m, dpp, delta);
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 6; j++) m66[i * 6 + j] = m[i + 1][j + 1];
+}
+
+void ref_calctune(void *h, double *tx, double *ty) { static_cast<ALine *>(h)->calctune(*tx, *ty); }
+
+void ref_calcchrom(void *h, double *cx, double *cy) { static_cast<ALine *>(h)->calcchrom(*cx, *cy); }
+
+// kind 1: correcttune, kind 2: correctchrom; families looked up by type and
+// name the way proc_lattice does (process.cpp:624-639, :737-751).
+int ref_correct(void *h, int kind, const char *type, const char *fam1, const char *fam2,
+                double *tx, double *ty) {
+  ALine *line = static_cast<ALine *>(h);
+  unsigned n = line->getnumber();
+  unsigned *l1 = new unsigned[n], *l2 = new unsigned[n];
+  unsigned n1 = line->findelements(l1, std::string(type), std::string(fam1));
+  unsigned n2 = line->findelements(l2, std::string(type), std::string(fam2));
+  if (n1 < 1 || n2 < 1) {
+    delete[] l1;
+    delete[] l2;
+    return -1;
+  }
+  if (kind == 1) line->correcttune(*tx, *ty, n1, n2, l1, l2);
+  else line->correctchrom(*tx, *ty, n1, n2, l1, l2);
+  delete[] l1;
+  delete[] l2;
+  return 0;
+}
+
+// Tune / chromaticity response matrices (aline.cpp:1187-1242, :1310-1355).
+int ref_calcresp(void *h, int kind, const char *type, const char *fam1, const char *fam2,
+                 double *resp) {
+  ALine *line = static_cast<ALine *>(h);
+  unsigned n = line->getnumber();
+  unsigned *l1 = new unsigned[n], *l2 = new unsigned[n];
+  unsigned nf[2];
+  nf[0] = line->findelements(l1, std::string(type), std::string(fam1));
+  nf[1] = line->findelements(l2, std::string(type), std::string(fam2));
+  unsigned *lf[2] = {l1, l2};
+  double r[2][2];
+  if (kind == 1) line->calctuneresp(r, 2, nf, lf);
+  else line->calcchromresp(r, 2, nf, lf);
+  resp[0] = r[0][0]; resp[1] = r[0][1]; resp[2] = r[1][0]; resp[3] = r[1][1];
+  delete[] l1;
+  delete[] l2;
+  return 0;
+}
+
+double ref_trackda(void *h, double xmax, double xmin, double ymax, int nturn, int nray, int nstep,
+                   double *rx, double *ry, int negweight) {
+  return static_cast<ALine *>(h)->trackDA(xmax, xmin, ymax, nturn, (unsigned)nray, nstep, rx, ry,
+                                          negweight != 0);
+}
+
+int ref_beam_stat(double *r, int n, double *ar, double *sig, double *sigrr) {
+  return calc_beam_stat(r, n, ar, sig, sigrr);
+}
+
+// Element field setters (AccelElement::setfield and the indexed multipole
+// variants, accel.cpp:210-219, :773-785).
+void ref_setfield(void *h, int idx, const char *name, double v) {
+  static_cast<ALine *>(h)->getelement((unsigned)idx)->setfield(&v, std::string(name));
+}
+void ref_setfield_int(void *h, int idx, const char *name, int v) {
+  static_cast<ALine *>(h)->getelement((unsigned)idx)->setfield(&v, std::string(name));
+}
+void ref_set_mpole(void *h, int idx, const char *name, int index, double v) {
+  AccelElement *e = static_cast<ALine *>(h)->getelement((unsigned)idx);
+  if (e->gettype() == "StrMPole") static_cast<AStrMPole *>(e)->setfield(&v, std::string(name), index);
+  else if (e->gettype() == "BendMPole") static_cast<ABendMPole *>(e)->setfield(&v, std::string(name), index);
+}
+double ref_getfield(void *h, int idx, const char *name) {
+  double v = 0;
+  static_cast<ALine *>(h)->getelement((unsigned)idx)->getfield(&v, std::string(name));
+  return v;
+}
+double ref_get_mpole(void *h, int idx, const char *name, int index) {
+  double v = 0;
+  AccelElement *e = static_cast<ALine *>(h)->getelement((unsigned)idx);
+  if (e->gettype() == "StrMPole") static_cast<AStrMPole *>(e)->getfield(&v, std::string(name), index);
+  else if (e->gettype() == "BendMPole") static_cast<ABendMPole *>(e)->getfield(&v, std::string(name), index);
+  return v;
+}
+}

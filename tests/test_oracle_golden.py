@@ -1,0 +1,114 @@
+"""The oracle against the reference: bit-exact on every committed fixture
+(written by tests/golden/make_golden.py from the compiled reference), and
+against the live reference library where it is built."""
+import numpy as np
+import pytest
+
+import pyaccp.elements as el
+import pyaccp.lattices as lat
+from conftest import golden
+from helpers import same_bits
+
+make_golden = pytest.importorskip("golden.make_golden")
+
+
+def test_element_fixtures_bit_exact(oracle):
+    g = golden("elements.npz")
+    P = g["particles"]
+    for name, e in make_golden.element_cases():
+        out = oracle.track([e], P, 1)
+        assert same_bits(out, g[name]), name
+
+
+def test_ring_turns_bit_exact(oracle):
+    g = golden("ring_turns.npz")
+    ring = lat.fodo_ring()
+    assert same_bits(g["beam"], lat.gaussian_beam(1000, seed=0))
+    assert same_bits(oracle.track(ring, g["beam"], 1), g["turn1"])
+    assert same_bits(oracle.track(ring, g["beam"], 10), g["turn10"])
+
+
+def test_da_grid_bit_exact(oracle):
+    g = golden("da_grid.npz")
+    out = oracle.track(lat.da_ring(0), g["grid"], 256)
+    assert same_bits(out, g["turn256"])
+    lost = np.isnan(out[:, 0])
+    codes = out[lost, 4]
+    # aperture codes (accel.cpp:352), or NaN for orbits that overflowed
+    # before reaching the aperture
+    finite = codes[~np.isnan(codes)]
+    assert set(np.unique(finite)).issubset({1.0, 2.0, 3.0, 4.0, 5.0, 6.0, 7.0})
+
+
+def test_optics_bit_exact(oracle):
+    g = golden("optics.npz")
+    ring = lat.fodo_ring()
+    assert same_bits(oracle.getm66(ring, 0.0, 1e-8), g["m66"])
+    assert same_bits(oracle.getm66(ring, 1e-3, 1e-8), g["m66_dpp"])
+    assert same_bits(oracle.tune(ring), g["tune"])
+    assert same_bits(oracle.chrom(ring), g["chrom"])
+    arr = el.to_array(ring)
+    qf, qd = lat.family(ring, "QF"), lat.family(ring, "QD")
+    sf, sd = lat.family(ring, "SF"), lat.family(ring, "SD")
+    assert same_bits(oracle.resp(arr, 1, qf, qd), g["resp_tune"])
+    assert same_bits(oracle.resp(arr, 2, sf, sd), g["resp_chrom"])
+    ct = oracle.correct(arr, 1, qf, qd, 0.28, 0.24)
+    assert same_bits(ct, g["corrected_tune"])
+    assert same_bits([arr[qf[0]].k, arr[qd[0]].k], g["k_after"])
+    arr2 = el.to_array(lat.fodo_ring())
+    cc = oracle.correct(arr2, 2, sf, sd, 2.0, 1.0)
+    assert same_bits(cc, g["corrected_chrom"])
+    assert same_bits([arr2[sf[0]].polyB[2], arr2[sd[0]].polyB[2]], g["b2_after"])
+    da = lat.da_ring(0)
+    lat.set_cavity(da, 0)
+    assert same_bits(oracle.tune(da), g["tune_da"])
+
+
+def test_radiation_and_rf_phase_bit_exact(oracle):
+    g = golden("radiation.npz")
+    ring = lat.fodo_ring()
+    lat.set_rad(ring, 1)
+    assert same_bits(oracle.track(ring, g["beam"], 1), g["turn1"])
+    arr = el.to_array(lat.da_ring(0))
+    assert same_bits([oracle.setrfphase(arr)], g["phis"])
+    # the reference leaves radiation and cavities as setrfphase sets them
+    assert all(arr[i].rad == 0 for i in range(len(arr)) if arr[i].type == el.BENDMPOLE)
+
+
+def test_trackda_bit_exact(oracle):
+    g = golden("trackda.npz")
+    for k, neg in enumerate((False, True)):
+        arr = el.to_array(lat.da_ring(0))
+        oracle.setrfphase(arr)
+        for i in range(len(arr)):
+            if arr[i].type == el.BENDMPOLE:
+                arr[i].rad = 1
+            if arr[i].type == el.CAVITY:
+                arr[i].on = 1
+        area, rx, ry = oracle.trackda(arr, 0.02, 0.001, 0.01, 64, 11, 10, negweight=neg)
+        assert same_bits([area], g["area"][k:k + 1])
+        if not neg:
+            assert same_bits(rx, g["rx"]) and same_bits(ry, g["ry"])
+
+
+def test_beam_stat_bit_exact(oracle):
+    g = golden("beamstat.npz")
+    s = oracle.beam_stat(g["beam"])
+    assert s[1] == g["nlive"][0]
+    assert same_bits(s[2:8], g["mean"])
+    assert same_bits(s[8:14], g["sigma"])
+    assert same_bits(s[14:20], g["cross"])
+
+
+def test_oracle_matches_live_reference(oracle, reference):
+    """Fresh inputs the fixtures do not hold: the DA lattice of seed 3 with
+    strength errors, radiation on, 5 turns."""
+    da = lat.da_ring(3, strength_errors=True, extra_quads=True)
+    lat.set_rad(da, 1)
+    beam = lat.gaussian_beam(300, seed=7)
+    line = reference.load(da)
+    assert same_bits(oracle.track(da, beam, 5), line.track(beam, 5))
+    ring = lat.fodo_ring()
+    line = reference.load(ring)
+    # ALine::pass(r, N, st, ed): one pass over an element range
+    assert same_bits(oracle.track(ring, beam, 1, 17, 100), line.track_range(beam, 17, 100))
