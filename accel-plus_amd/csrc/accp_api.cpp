@@ -214,13 +214,159 @@ struct DevBuf {
 
 }  // namespace
 
+// ---- working set and the chunked run ---------------------------------------------
+namespace {
+
+// Device bookkeeping of one tracking call: the index list of particles still
+// alive (double-buffered), per-seed live counts, the tile list built from
+// them, and the seed offsets.
+struct WorkSet {
+  DevBuf idx[2], cnt[2], tiles, ntiles, offs;
+  std::vector<int64_t> h_offs;  // offsets last uploaded
+  void release() {
+    h_offs.clear();
+    for (int i = 0; i < 2; ++i) {
+      idx[i].release();
+      cnt[i].release();
+    }
+    tiles.release();
+    ntiles.release();
+    offs.release();
+  }
+};
+
+// Device-resident element tables of nseed lattices.
+struct TableRef {
+  const Hdr *hdr;
+  const double *par;
+  const int32_t *meta;  // lat_hdr0[nseed], lat_par0[nseed], lat_npar[nseed]
+  int nseed, nelem;
+  size_t lds_need;
+  bool has_qe;
+};
+
+struct RunSpec {
+  int nturn = 0, st = 0, ed = 0;
+  uint64_t rng_seed = 0, turn0 = 0, pid0 = 0;
+  int32_t *d_loss_turn = nullptr;
+  double *snap = nullptr;
+  int nsnap = 0, mon_interval = 1;
+};
+
+int resolve_tuning(const accp_tuning *t, size_t lds_need, int &block, int &ppl, bool &lds, int &chunk_max) {
+  block = (t && t->block) ? t->block : 64;
+  ppl = (t && t->ppl) ? t->ppl : 1;
+  const int st = t ? t->staging : ACCP_STAGE_AUTO;
+  chunk_max = (t && t->compact) ? t->compact : 256;
+  if (block != 64 && block != 128 && block != 256) return fail(ACCP_E_ARG, "block must be 64, 128 or 256");
+  if (ppl != 1 && ppl != 2) return fail(ACCP_E_ARG, "ppl must be 1 or 2");
+  if (st == ACCP_STAGE_LDS && lds_need > LDS_LIMIT)
+    return fail(ACCP_E_ARG, "element table (%zu B) exceeds the LDS staging limit", lds_need);
+  lds = (st == ACCP_STAGE_LDS) || (st == ACCP_STAGE_AUTO && lds_need <= LDS_LIMIT);
+  return ACCP_OK;
+}
+
+// Buffers for n particles over nseed seeds with host offsets; uploads the
+// offsets synchronously (so callers do it before any graph capture).
+int prepare_workset(WorkSet &ws, const int64_t *h_offs, int nseed, int per, int &max_tiles) {
+  const int64_t n = h_offs[nseed];
+  int64_t mt = 0;
+  for (int s = 0; s < nseed; ++s) {
+    const int64_t c = h_offs[s + 1] - h_offs[s];
+    if (c < 0 || h_offs[s] < 0) return fail(ACCP_E_ARG, "seed offsets must be non-decreasing from 0");
+    mt += (c + per - 1) / per;
+  }
+  if (n > 0x7fffffff || mt > 0x7fffffff) return fail(ACCP_E_ARG, "too many particles for one call");
+  int rc;
+  const size_t ni = (size_t)std::max<int64_t>(n, 1);
+  const void *old_offs = ws.offs.p;
+  if ((rc = ws.idx[0].ensure(ni * 4)) || (rc = ws.idx[1].ensure(ni * 4)) ||
+      (rc = ws.cnt[0].ensure(nseed * 4)) || (rc = ws.cnt[1].ensure(nseed * 4)) ||
+      (rc = ws.tiles.ensure(std::max<int64_t>(mt, 1) * sizeof(TileDesc))) || (rc = ws.ntiles.ensure(4)) ||
+      (rc = ws.offs.ensure((nseed + 1) * sizeof(int64_t))))
+    return rc;
+  if (ws.offs.p != old_offs) ws.h_offs.clear();
+  std::vector<int64_t> want(h_offs, h_offs + nseed + 1);
+  if (want != ws.h_offs) {
+    HIPCHK(hipMemcpy(ws.offs.p, h_offs, (nseed + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    ws.h_offs = want;
+  }
+  max_tiles = (int)mt;
+  return ACCP_OK;
+}
+
+// Enqueue a whole tracking run on stream s (no host synchronisation, no
+// memset: capturable).  Turns run in chunks of 1, 2, 4, ... up to
+// chunk_max; between chunks the lost particles leave the working set.
+int enqueue_run(const TableRef &T, WorkSet &ws, int max_tiles, double *d_r, int64_t n,
+                const RunSpec &R, const accp_tuning *tuning, hipStream_t s) {
+  int block, ppl, chunk_max;
+  bool lds;
+  int rc = resolve_tuning(tuning, T.lds_need, block, ppl, lds, chunk_max);
+  if (rc) return rc;
+  const int per = block * ppl;
+  const int64_t *d_offs = (const int64_t *)ws.offs.p;
+  int cur = 0;
+  HIPCHK(launch_chunk_init((int32_t *)ws.idx[0].p, n, R.d_loss_turn, R.nturn, (int32_t *)ws.cnt[0].p, d_offs,
+                           T.nseed, s));
+  HIPCHK(launch_make_tiles((const int32_t *)ws.cnt[0].p, d_offs, T.nseed, per, (TileDesc *)ws.tiles.p,
+                           (int32_t *)ws.ntiles.p, s));
+  KernelCtx c{};
+  c.tiles = (const TileDesc *)ws.tiles.p;
+  c.ntiles = (const int32_t *)ws.ntiles.p;
+  c.hdr = T.hdr;
+  c.par = T.par;
+  c.lat_hdr0 = T.meta;
+  c.lat_par0 = T.meta + T.nseed;
+  c.lat_npar = T.meta + 2 * T.nseed;
+  c.nelem = T.nelem;
+  c.st = R.st;
+  c.ed = R.ed;
+  c.r = d_r;
+  c.n = n;
+  c.rng_seed = R.rng_seed;
+  c.pid0 = R.pid0;
+  c.snap = R.snap;
+  c.nsnap = R.nsnap;
+  c.mon_interval = R.snap ? R.mon_interval : 1;
+  c.loss_turn = R.d_loss_turn;
+  const bool compact = chunk_max > 0 && !R.snap;
+  int t = 0, chunk = 1;
+  while (t < R.nturn) {
+    const int cn = compact ? std::min(chunk, R.nturn - t) : R.nturn;
+    c.idx = (const int32_t *)ws.idx[cur].p;
+    c.turn0 = R.turn0 + (uint64_t)t;
+    c.turn_rel0 = t;
+    c.nturn = cn;
+    HIPCHK(launch_track(c, max_tiles, block, ppl, lds, lds ? T.lds_need : 0, s));
+    t += cn;
+    if (t < R.nturn) {
+      HIPCHK(launch_compact((const TileDesc *)ws.tiles.p, (const int32_t *)ws.ntiles.p, max_tiles,
+                            (const int32_t *)ws.idx[cur].p, d_r, d_offs, (int32_t *)ws.cnt[1 - cur].p,
+                            (int32_t *)ws.idx[1 - cur].p, T.nseed, s));
+      HIPCHK(launch_make_tiles((const int32_t *)ws.cnt[1 - cur].p, d_offs, T.nseed, per,
+                               (TileDesc *)ws.tiles.p, (int32_t *)ws.ntiles.p, s));
+      cur = 1 - cur;
+      chunk = std::min(chunk * 2, chunk_max);
+    }
+  }
+  return ACCP_OK;
+}
+
+}  // namespace
+
 // ---- lattice -------------------------------------------------------------------
 struct accp_lattice {
   std::vector<accp_elem> elems;
   Compiled tab;
   DevBuf d_hdr, d_par, d_meta;  // meta: lat_hdr0, lat_par0, lat_npar of one lattice
-  DevBuf d_beam, d_snap, d_stat;
+  DevBuf d_beam, d_snap, d_stat, d_loss;
+  WorkSet ws;
   WakeState wake;
+  TableRef table() const {
+    return TableRef{(const Hdr *)d_hdr.p, (const double *)d_par.p, (const int32_t *)d_meta.p, 1,
+                    (int)tab.hdr.size(), lds_bytes_for(tab.hdr.size(), tab.par.size()), false};
+  }
 };
 
 static int upload_lattice(accp_lattice *L) {
@@ -282,85 +428,50 @@ extern "C" int accp_lattice_free(accp_lattice *L) {
   L->d_beam.release();
   L->d_snap.release();
   L->d_stat.release();
+  L->d_loss.release();
+  L->ws.release();
   wake_release(L->wake);
   delete L;
   return ACCP_OK;
 }
 
-// ---- launch shape ------------------------------------------------------------------
-static int resolve_tuning(const accp_tuning *t, size_t lds_need, int &block, int &ppl, bool &lds) {
-  block = (t && t->block) ? t->block : 64;
-  ppl = (t && t->ppl) ? t->ppl : 1;
-  const int st = t ? t->staging : ACCP_STAGE_AUTO;
-  if (block != 64 && block != 128 && block != 256) return fail(ACCP_E_ARG, "block must be 64, 128 or 256");
-  if (ppl != 1 && ppl != 2) return fail(ACCP_E_ARG, "ppl must be 1 or 2");
-  if (st == ACCP_STAGE_LDS && lds_need > LDS_LIMIT)
-    return fail(ACCP_E_ARG, "element table (%zu B) exceeds the LDS staging limit", lds_need);
-  lds = (st == ACCP_STAGE_LDS) || (st == ACCP_STAGE_AUTO && lds_need <= LDS_LIMIT);
-  return ACCP_OK;
-}
-
-// One plain pass of elements [st, ed) for nturn turns (no collective element
-// inside the range).
-static int launch_plain(accp_lattice *L, double *d_r, int64_t n, int nturn, int st, int ed,
-                        const accp_track_opts *o, double *snap, int nsnap, hipStream_t s) {
-  const size_t need = lds_bytes_for(L->tab.hdr.size(), L->tab.par.size());
-  int block, ppl;
+// ---- tracking -------------------------------------------------------------------
+// Run on one lattice; turns are split at active LongWake elements, which need
+// the whole beam (histogram over all particles) before anyone passes them.
+static int track_device_impl(accp_lattice *L, double *d_r, int64_t n, const RunSpec &R,
+                             const accp_tuning *tuning, hipStream_t s) {
+  const int64_t offs[2] = {0, n};
+  int block, ppl, chunk_max;
   bool lds;
-  int rc = resolve_tuning(o ? &o->tuning : nullptr, need, block, ppl, lds);
+  const TableRef T = L->table();
+  int rc = resolve_tuning(tuning, T.lds_need, block, ppl, lds, chunk_max);
   if (rc) return rc;
-  KernelCtx c{};
-  c.tiles = nullptr;
-  c.hdr = (const Hdr *)L->d_hdr.p;
-  c.par = (const double *)L->d_par.p;
-  c.lat_hdr0 = (const int32_t *)L->d_meta.p;
-  c.lat_par0 = (const int32_t *)L->d_meta.p + 1;
-  c.lat_npar = (const int32_t *)L->d_meta.p + 2;
-  c.nelem = (int32_t)L->tab.hdr.size();
-  c.nturn = nturn;
-  c.st = st;
-  c.ed = ed;
-  c.r = d_r;
-  c.n = n;
-  c.rng_seed = o ? o->rng_seed : 0;
-  c.turn0 = o ? o->turn0 : 0;
-  c.pid0 = o ? o->pid0 : 0;
-  c.snap = snap;
-  c.mon_interval = (snap && o) ? o->mon_interval : 1;
-  c.nsnap = nsnap;
-  const int64_t per = (int64_t)block * ppl;
-  const int64_t grid = (n + per - 1) / per;
-  if (grid > 0x7fffffff) return fail(ACCP_E_ARG, "too many particles for one launch");
-  HIPCHK(launch_track(c, (int)grid, block, ppl, lds, lds ? need : 0, s));
-  return ACCP_OK;
-}
-
-// Track, splitting every turn at active LongWake elements (collective: the
-// whole beam must reach the wake before its histogram is complete).
-static int track_device_impl(accp_lattice *L, double *d_r, int64_t n, int nturn, int st, int ed,
-                             const accp_track_opts *o, double *snap, int nsnap, hipStream_t s) {
+  int max_tiles = 0;
+  if ((rc = prepare_workset(L->ws, offs, 1, block * ppl, max_tiles))) return rc;
   std::vector<int> cuts;
   for (int w : L->tab.wakes)
-    if (w >= st && w < ed) cuts.push_back(w);
-  if (cuts.empty()) return launch_plain(L, d_r, n, nturn, st, ed, o, snap, nsnap, s);
-  if (snap) return fail(ACCP_E_ARG, "monitor records with a wake element are not supported");
-  accp_track_opts oo{};
-  if (o) oo = *o;
-  for (int t = 0; t < nturn; ++t) {
-    oo.turn0 = (o ? o->turn0 : 0) + (uint64_t)t;
-    int a = st;
-    for (int w : cuts) {
-      if (w > a) {
-        int rc = launch_plain(L, d_r, n, 1, a, w, &oo, nullptr, 0, s);
-        if (rc) return rc;
+    if (w >= R.st && w < R.ed) cuts.push_back(w);
+  if (cuts.empty()) return enqueue_run(T, L->ws, max_tiles, d_r, n, R, tuning, s);
+  if (R.snap) return fail(ACCP_E_ARG, "monitor records with a wake element are not supported");
+  accp_tuning one = tuning ? *tuning : accp_tuning{0, 0, 0, 0};
+  one.compact = -1;  // every segment sees the whole beam
+  for (int t = 0; t < R.nturn; ++t) {
+    RunSpec seg = R;
+    seg.nturn = 1;
+    seg.turn0 = R.turn0 + (uint64_t)t;
+    seg.d_loss_turn = nullptr;
+    int a = R.st;
+    for (size_t k = 0; k <= cuts.size(); ++k) {
+      const int b = k < cuts.size() ? cuts[k] : R.ed;
+      if (b > a) {
+        seg.st = a;
+        seg.ed = b;
+        if ((rc = enqueue_run(T, L->ws, max_tiles, d_r, n, seg, &one, s))) return rc;
       }
-      int rc = wake_apply(L->wake, w, d_r, n, s);
-      if (rc) return fail(rc, "wake element %d: %s", w, wake_last_error());
-      a = w + 1;
-    }
-    if (ed > a) {
-      int rc = launch_plain(L, d_r, n, 1, a, ed, &oo, nullptr, 0, s);
-      if (rc) return rc;
+      if (k < cuts.size()) {
+        if ((rc = wake_apply(L->wake, b, d_r, n, s))) return fail(rc, "wake element %d: %s", b, wake_last_error());
+        a = b + 1;
+      }
     }
   }
   return ACCP_OK;
@@ -374,6 +485,19 @@ static int check_range(accp_lattice *L, int n, int nturn, int st, int &ed) {
   return ACCP_OK;
 }
 
+static RunSpec spec_from(const accp_track_opts *o, int nturn, int st, int ed) {
+  RunSpec R;
+  R.nturn = nturn;
+  R.st = st;
+  R.ed = ed;
+  if (o) {
+    R.rng_seed = o->rng_seed;
+    R.turn0 = o->turn0;
+    R.pid0 = o->pid0;
+  }
+  return R;
+}
+
 extern "C" int accp_track_device(accp_lattice *L, double *d_r, int n, int nturn, int st, int ed,
                                  const accp_track_opts *opts, void *stream) {
   if (!L || (!d_r && n > 0)) return fail(ACCP_E_ARG, "accp_track_device: null argument");
@@ -382,7 +506,9 @@ extern "C" int accp_track_device(accp_lattice *L, double *d_r, int n, int nturn,
   if (opts && opts->mon_interval > 0)
     return fail(ACCP_E_ARG, "monitor records need accp_track (host buffer)");
   if (n == 0 || nturn == 0 || st == ed) return ACCP_OK;
-  return track_device_impl(L, d_r, n, nturn, st, ed, opts, nullptr, 0, (hipStream_t)stream);
+  RunSpec R = spec_from(opts, nturn, st, ed);
+  R.d_loss_turn = opts ? opts->loss_turn : nullptr;
+  return track_device_impl(L, d_r, n, R, opts ? &opts->tuning : nullptr, (hipStream_t)stream);
 }
 
 extern "C" int accp_track(accp_lattice *L, double *r, int n, int nturn, int st, int ed,
@@ -390,38 +516,44 @@ extern "C" int accp_track(accp_lattice *L, double *r, int n, int nturn, int st, 
   if (!L || (!r && n > 0)) return fail(ACCP_E_ARG, "accp_track: null argument");
   int rc = check_range(L, n, nturn, st, ed);
   if (rc) return rc;
-  if (n == 0 || nturn == 0) {
-    if (opts && opts->mon_count) *opts->mon_count = 0;
+  if (opts && opts->mon_count) *opts->mon_count = 0;
+  if (n == 0) return ACCP_OK;
+  if (nturn == 0 || st == ed) {
+    if (opts && opts->loss_turn)
+      for (int i = 0; i < n; ++i) opts->loss_turn[i] = (r[6 * (size_t)i] == r[6 * (size_t)i]) ? nturn : -1;
     return ACCP_OK;
   }
   const size_t bytes = (size_t)n * 6 * sizeof(double);
   if ((rc = L->d_beam.ensure(bytes))) return rc;
   HIPCHK(hipMemcpy(L->d_beam.p, r, bytes, hipMemcpyHostToDevice));
-  int nsnap = 0;
-  double *snap = nullptr;
+  RunSpec R = spec_from(opts, nturn, st, ed);
+  if (opts && opts->loss_turn) {
+    if ((rc = L->d_loss.ensure((size_t)n * 4))) return rc;
+    R.d_loss_turn = (int32_t *)L->d_loss.p;
+  }
   if (opts && opts->mon_interval > 0) {
     if (!opts->mon_out || !opts->mon_count) return fail(ACCP_E_ARG, "monitor needs mon_out and mon_count");
-    nsnap = (nturn + opts->mon_interval - 1) / opts->mon_interval;
+    int nsnap = (nturn + opts->mon_interval - 1) / opts->mon_interval;
     if (nsnap > opts->mon_max) nsnap = opts->mon_max;
     if (nsnap > 0) {
       if ((rc = L->d_snap.ensure((size_t)nsnap * bytes))) return rc;
-      snap = (double *)L->d_snap.p;
+      R.snap = (double *)L->d_snap.p;
+      R.nsnap = nsnap;
+      R.mon_interval = opts->mon_interval;
     }
   }
-  if (st < ed) {
-    rc = track_device_impl(L, (double *)L->d_beam.p, n, nturn, st, ed, opts, snap, nsnap, nullptr);
-    if (rc) return rc;
-  }
-  if (snap) {
-    if ((rc = L->d_stat.ensure((size_t)nsnap * ACCP_STAT_LEN * sizeof(double)))) return rc;
-    HIPCHK(launch_beam_stat(snap, n, nsnap, (double)opts->turn0, opts->mon_interval,
+  rc = track_device_impl(L, (double *)L->d_beam.p, n, R, opts ? &opts->tuning : nullptr, nullptr);
+  if (rc) return rc;
+  if (R.snap) {
+    if ((rc = L->d_stat.ensure((size_t)R.nsnap * ACCP_STAT_LEN * sizeof(double)))) return rc;
+    HIPCHK(launch_beam_stat(R.snap, n, R.nsnap, (double)opts->turn0, opts->mon_interval,
                             (double *)L->d_stat.p, nullptr));
-    HIPCHK(hipMemcpy(opts->mon_out, L->d_stat.p, (size_t)nsnap * ACCP_STAT_LEN * sizeof(double),
+    HIPCHK(hipMemcpy(opts->mon_out, L->d_stat.p, (size_t)R.nsnap * ACCP_STAT_LEN * sizeof(double),
                      hipMemcpyDeviceToHost));
-    *opts->mon_count = nsnap;
-  } else if (opts && opts->mon_count) {
-    *opts->mon_count = 0;
+    *opts->mon_count = R.nsnap;
   }
+  if (R.d_loss_turn)
+    HIPCHK(hipMemcpy(opts->loss_turn, R.d_loss_turn, (size_t)n * 4, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(r, L->d_beam.p, bytes, hipMemcpyDeviceToHost));
   HIPCHK(hipDeviceSynchronize());
   return ACCP_OK;
@@ -450,24 +582,24 @@ extern "C" int accp_beam_stat(const double *r, int n, double *stat) {
 struct accp_batch {
   int nseed = 0, nelem = 0;
   size_t max_lds = 0;
-  std::vector<Hdr> hdr;
-  std::vector<double> par;
-  std::vector<int32_t> hdr0, par0, npar;
-  DevBuf d_hdr, d_par, d_meta, d_tiles, d_scratch;
-  std::vector<TileDesc> tiles;
-  std::vector<int64_t> tile_offsets;
-  int tile_block = 0, tile_ppl = 0;
+  DevBuf d_hdr, d_par, d_meta, d_scratch;
+  WorkSet ws;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t cap_stream = nullptr;
+  TableRef table() const {
+    return TableRef{(const Hdr *)d_hdr.p, (const double *)d_par.p, (const int32_t *)d_meta.p, nseed, nelem,
+                    max_lds, false};
+  }
 };
 
 extern "C" int accp_batch_create(accp_lattice *const *seeds, int nseed, accp_batch **out) {
   if (!seeds || nseed <= 0 || !out) return fail(ACCP_E_ARG, "accp_batch_create: bad arguments");
   *out = nullptr;
+  for (int s = 0; s < nseed; ++s)
+    if (!seeds[s]) return fail(ACCP_E_ARG, "seed %d is null", s);
   const int ne = (int)seeds[0]->elems.size();
   for (int s = 0; s < nseed; ++s) {
-    if (!seeds[s]) return fail(ACCP_E_ARG, "seed %d is null", s);
     if ((int)seeds[s]->elems.size() != ne) return fail(ACCP_E_LATTICE, "seed %d: element count differs", s);
     if (!seeds[s]->tab.wakes.empty()) return fail(ACCP_E_LATTICE, "seed %d: wake elements are not batched", s);
     for (int i = 0; i < ne; ++i)
@@ -477,28 +609,30 @@ extern "C" int accp_batch_create(accp_lattice *const *seeds, int nseed, accp_bat
   accp_batch *b = new accp_batch;
   b->nseed = nseed;
   b->nelem = ne;
+  std::vector<Hdr> hdr;
+  std::vector<double> par;
+  std::vector<int32_t> h0, p0, np;
   for (int s = 0; s < nseed; ++s) {
     const Compiled &c = seeds[s]->tab;
-    b->hdr0.push_back((int32_t)b->hdr.size());
-    b->par0.push_back((int32_t)b->par.size());
-    b->npar.push_back((int32_t)c.par.size());
-    b->hdr.insert(b->hdr.end(), c.hdr.begin(), c.hdr.end());
-    b->par.insert(b->par.end(), c.par.begin(), c.par.end());
+    h0.push_back((int32_t)hdr.size());
+    p0.push_back((int32_t)par.size());
+    np.push_back((int32_t)c.par.size());
+    hdr.insert(hdr.end(), c.hdr.begin(), c.hdr.end());
+    par.insert(par.end(), c.par.begin(), c.par.end());
     b->max_lds = std::max(b->max_lds, lds_bytes_for(c.hdr.size(), c.par.size()));
   }
-  std::vector<int32_t> meta;
-  meta.insert(meta.end(), b->hdr0.begin(), b->hdr0.end());
-  meta.insert(meta.end(), b->par0.begin(), b->par0.end());
-  meta.insert(meta.end(), b->npar.begin(), b->npar.end());
-  int rc = b->d_hdr.ensure(b->hdr.size() * sizeof(Hdr));
-  if (!rc) rc = b->d_par.ensure(b->par.size() * sizeof(double));
+  std::vector<int32_t> meta(h0);
+  meta.insert(meta.end(), p0.begin(), p0.end());
+  meta.insert(meta.end(), np.begin(), np.end());
+  int rc = b->d_hdr.ensure(hdr.size() * sizeof(Hdr));
+  if (!rc) rc = b->d_par.ensure(par.size() * sizeof(double));
   if (!rc) rc = b->d_meta.ensure(meta.size() * sizeof(int32_t));
   if (rc) {
     accp_batch_free(b);
     return rc;
   }
-  hipError_t e = hipMemcpy(b->d_hdr.p, b->hdr.data(), b->hdr.size() * sizeof(Hdr), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(b->d_par.p, b->par.data(), b->par.size() * sizeof(double), hipMemcpyHostToDevice);
+  hipError_t e = hipMemcpy(b->d_hdr.p, hdr.data(), hdr.size() * sizeof(Hdr), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_par.p, par.data(), par.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(b->d_meta.p, meta.data(), meta.size() * sizeof(int32_t), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     accp_batch_free(b);
@@ -508,76 +642,50 @@ extern "C" int accp_batch_create(accp_lattice *const *seeds, int nseed, accp_bat
   return ACCP_OK;
 }
 
-// Tiles of block*ppl particles per seed; the hardware dispatcher hands them
-// to CUs as earlier workgroups retire, which balances uneven seeds.
-static int build_tiles(accp_batch *b, const int64_t *offsets, int block, int ppl) {
-  std::vector<int64_t> offs(offsets, offsets + b->nseed + 1);
-  if (b->tile_block == block && b->tile_ppl == ppl && offs == b->tile_offsets && !b->tiles.empty())
-    return ACCP_OK;
-  for (int s = 0; s < b->nseed; ++s)
-    if (offs[s + 1] < offs[s] || offs[s] < 0) return fail(ACCP_E_ARG, "offsets must be non-decreasing");
-  b->tiles.clear();
-  const int64_t per = (int64_t)block * ppl;
-  for (int s = 0; s < b->nseed; ++s)
-    for (int64_t a = offs[s]; a < offs[s + 1]; a += per)
-      b->tiles.push_back(TileDesc{s, 0, a, std::min(per, offs[s + 1] - a)});
-  if (b->tiles.empty()) return ACCP_OK;
-  int rc = b->d_tiles.ensure(b->tiles.size() * sizeof(TileDesc));
-  if (rc) return rc;
-  HIPCHK(hipMemcpy(b->d_tiles.p, b->tiles.data(), b->tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice));
-  b->tile_offsets = offs;
-  b->tile_block = block;
-  b->tile_ppl = ppl;
-  return ACCP_OK;
-}
-
-static int batched_launch(accp_batch *b, double *d_r, const int64_t *offsets, int nturn,
-                          const accp_tuning *tuning, hipStream_t s) {
-  int block, ppl;
+static int batched_run(accp_batch *b, double *d_r, const int64_t *offsets, int nturn,
+                       const accp_tuning *tuning, int32_t *d_loss_turn, hipStream_t s, bool prepared,
+                       int *max_tiles_io) {
+  int block, ppl, chunk_max;
   bool lds;
-  int rc = resolve_tuning(tuning, b->max_lds, block, ppl, lds);
+  const TableRef T = b->table();
+  int rc = resolve_tuning(tuning, T.lds_need, block, ppl, lds, chunk_max);
   if (rc) return rc;
-  if ((rc = build_tiles(b, offsets, block, ppl))) return rc;
-  if (b->tiles.empty() || nturn == 0) return ACCP_OK;
-  KernelCtx c{};
-  c.tiles = (const TileDesc *)b->d_tiles.p;
-  c.hdr = (const Hdr *)b->d_hdr.p;
-  c.par = (const double *)b->d_par.p;
-  c.lat_hdr0 = (const int32_t *)b->d_meta.p;
-  c.lat_par0 = c.lat_hdr0 + b->nseed;
-  c.lat_npar = c.lat_hdr0 + 2 * b->nseed;
-  c.nelem = b->nelem;
-  c.nturn = nturn;
-  c.st = 0;
-  c.ed = b->nelem;
-  c.r = d_r;
-  c.n = offsets[b->nseed];
-  c.mon_interval = 1;
-  HIPCHK(launch_track(c, (int)b->tiles.size(), block, ppl, lds, lds ? b->max_lds : 0, s));
-  return ACCP_OK;
+  int max_tiles = max_tiles_io ? *max_tiles_io : 0;
+  if (!prepared && (rc = prepare_workset(b->ws, offsets, b->nseed, block * ppl, max_tiles))) return rc;
+  if (max_tiles_io) *max_tiles_io = max_tiles;
+  const int64_t n = offsets[b->nseed];
+  if (n == 0 || nturn == 0) return ACCP_OK;
+  RunSpec R;
+  R.nturn = nturn;
+  R.st = 0;
+  R.ed = b->nelem;
+  R.d_loss_turn = d_loss_turn;
+  return enqueue_run(T, b->ws, max_tiles, d_r, n, R, tuning, s);
 }
 
 extern "C" int accp_track_batched(accp_batch *b, double *d_r, const int64_t *offsets, int nturn,
-                                  const accp_tuning *tuning, void *stream) {
+                                  const accp_tuning *tuning, int32_t *d_loss_turn, void *stream) {
   if (!b || !d_r || !offsets || nturn < 0) return fail(ACCP_E_ARG, "accp_track_batched: bad arguments");
-  return batched_launch(b, d_r, offsets, nturn, tuning, (hipStream_t)stream);
+  return batched_run(b, d_r, offsets, nturn, tuning, d_loss_turn, (hipStream_t)stream, false, nullptr);
 }
 
 extern "C" int accp_batch_capture(accp_batch *b, double *d_r, const int64_t *offsets, int nturn,
-                                  const accp_tuning *tuning) {
+                                  const accp_tuning *tuning, int32_t *d_loss_turn) {
   if (!b || !d_r || !offsets || nturn < 0) return fail(ACCP_E_ARG, "accp_batch_capture: bad arguments");
-  int block, ppl;
+  int block, ppl, chunk_max;
   bool lds;
-  int rc = resolve_tuning(tuning, b->max_lds, block, ppl, lds);
+  int rc = resolve_tuning(tuning, b->max_lds, block, ppl, lds, chunk_max);
   if (rc) return rc;
-  if ((rc = build_tiles(b, offsets, block, ppl))) return rc;  // uploads happen outside capture
+  int max_tiles = 0;
+  // offsets and buffers go to the device before the capture starts
+  if ((rc = prepare_workset(b->ws, offsets, b->nseed, block * ppl, max_tiles))) return rc;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
   if (!b->cap_stream) HIPCHK(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
   HIPCHK(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
-  rc = batched_launch(b, d_r, offsets, nturn, tuning, b->cap_stream);
+  rc = batched_run(b, d_r, offsets, nturn, tuning, d_loss_turn, b->cap_stream, true, &max_tiles);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
   if (rc) {
@@ -615,14 +723,16 @@ extern "C" int accp_autotune(accp_batch *b, double *d_r, const int64_t *offsets,
       for (int sg : stages) {
         if (sg == ACCP_STAGE_LDS && b->max_lds > LDS_LIMIT) continue;
         accp_tuning t{bl, pp, sg, 0};
+        // untimed launch (code object load, buffers), then the timed probe
         HIPCHK(hipMemcpy(b->d_scratch.p, d_r, bytes, hipMemcpyDeviceToDevice));
-        // warm-up launch (tile upload, code object load), then the timed one
-        rc = batched_launch(b, (double *)b->d_scratch.p, offsets, 1, &t, nullptr);
-        if (rc) return rc;
+        if ((rc = batched_run(b, (double *)b->d_scratch.p, offsets, 1, &t, nullptr, nullptr, false, nullptr)))
+          return rc;
         HIPCHK(hipMemcpy(b->d_scratch.p, d_r, bytes, hipMemcpyDeviceToDevice));
+        int mt = 0;
+        if ((rc = prepare_workset(b->ws, offsets, b->nseed, bl * pp, mt))) return rc;
         HIPCHK(hipEventRecord(e0, nullptr));
-        rc = batched_launch(b, (double *)b->d_scratch.p, offsets, nturn_probe, &t, nullptr);
-        if (rc) return rc;
+        if ((rc = batched_run(b, (double *)b->d_scratch.p, offsets, nturn_probe, &t, nullptr, nullptr, true, &mt)))
+          return rc;
         HIPCHK(hipEventRecord(e1, nullptr));
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0;
@@ -647,8 +757,8 @@ extern "C" int accp_batch_free(accp_batch *b) {
   b->d_hdr.release();
   b->d_par.release();
   b->d_meta.release();
-  b->d_tiles.release();
   b->d_scratch.release();
+  b->ws.release();
   delete b;
   return ACCP_OK;
 }

@@ -8,17 +8,19 @@
 
 namespace accp {
 
-// One workgroup's share of a batched launch: particles [start, start+count)
-// of the concatenated beam, tracked through lattice `lat`.
+// One workgroup's share of a launch: positions [start, start+count) of the
+// working-set index list (particles idx[start..]), tracked through lattice
+// `lat`.
 struct TileDesc {
   int32_t lat;
-  int32_t pad;
+  int32_t count;
   int64_t start;
-  int64_t count;
 };
 
 struct KernelCtx {
-  const TileDesc *tiles;     // null: one lattice, tile = blockIdx.x * BLOCK*PPL
+  const TileDesc *tiles;     // tile list (device)
+  const int32_t *ntiles;     // number of valid tiles (device); later blocks exit
+  const int32_t *idx;        // working set: particle index of each position
   const Hdr *hdr;            // headers of all lattices
   const double *par;         // constants of all lattices
   const int32_t *lat_hdr0;   // first header of lattice s
@@ -31,7 +33,19 @@ struct KernelCtx {
   uint64_t rng_seed, turn0, pid0;
   double *snap;              // monitor snapshots (plain calls only) or null
   int32_t mon_interval, nsnap;
+  int32_t *loss_turn;        // per particle: turn lost in (relative to the call) or null
+  int32_t turn_rel0;         // turn of this launch relative to the call start
 };
+
+// Chunked driver helpers (device-resident bookkeeping, no host round trips,
+// so a whole run can be captured in a HIP graph).
+hipError_t launch_chunk_init(int32_t *idx, int64_t n, int32_t *loss_turn, int32_t nturn,
+                             int32_t *cnt, const int64_t *offs, int nseed, hipStream_t s);
+hipError_t launch_make_tiles(const int32_t *cnt, const int64_t *offs, int nseed, int per,
+                             TileDesc *tiles, int32_t *ntiles, hipStream_t s);
+hipError_t launch_compact(const TileDesc *tiles, const int32_t *ntiles, int max_tiles,
+                          const int32_t *idx_in, const double *r, const int64_t *offs,
+                          int32_t *cnt_out, int32_t *idx_out, int nseed, hipStream_t s);
 
 hipError_t launch_track(const KernelCtx &ctx, int grid, int block, int ppl, bool lds,
                         size_t lds_bytes, hipStream_t s);

@@ -10,6 +10,8 @@
 // Arithmetic is FP64 throughout; there is no dense contraction, so no MFMA.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "accp_kernels.h"
 #include "accp_physics.h"
 
@@ -209,19 +211,11 @@ __global__ __launch_bounds__(BLOCK) void k_track(KernelCtx ctx) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
 
   // tile of this workgroup
-  int lat = 0;
-  int64_t start, count;
-  if (ctx.tiles) {
-    const TileDesc t = ctx.tiles[blockIdx.x];
-    lat = t.lat;
-    start = t.start;
-    count = t.count;
-  } else {
-    start = (int64_t)blockIdx.x * (BLOCK * PPL);
-    count = ctx.n - start;
-    if (count > BLOCK * PPL) count = BLOCK * PPL;
-  }
-  lat = __builtin_amdgcn_readfirstlane(lat);
+  if ((int)blockIdx.x >= *ctx.ntiles) return;
+  const TileDesc tile = ctx.tiles[blockIdx.x];
+  const int lat = __builtin_amdgcn_readfirstlane(tile.lat);
+  const int64_t start = tile.start;
+  const int count = tile.count;
   const Hdr *__restrict__ ghdr = ctx.hdr + ctx.lat_hdr0[lat];
   const double *__restrict__ gpar = ctx.par + ctx.lat_par0[lat];
 
@@ -240,20 +234,27 @@ __global__ __launch_bounds__(BLOCK) void k_track(KernelCtx ctx) {
 
   Part p[PPL];
   uint64_t pid[PPL];
+  int64_t gidx[PPL];
   bool inb[PPL];
+  int lost_at[PPL];
+  bool record[PPL];
 #pragma unroll
   for (int k = 0; k < PPL; ++k) {
-    const int64_t li = (int64_t)k * BLOCK + threadIdx.x;
+    const int li = k * BLOCK + (int)threadIdx.x;
     inb[k] = li < count;
-    const int64_t gi = start + (inb[k] ? li : 0);
-    pid[k] = ctx.pid0 + (uint64_t)gi;
-    const double *src = ctx.r + 6 * gi;
+    gidx[k] = ctx.idx[start + (inb[k] ? li : 0)];
+    pid[k] = ctx.pid0 + (uint64_t)gidx[k];
+    const double *src = ctx.r + 6 * gidx[k];
     p[k].x = src[0];
     p[k].px = src[1];
     p[k].y = src[2];
     p[k].py = src[3];
     p[k].z = src[4];
     set_delta(p[k], src[5]);
+    // a particle already lost when this launch starts was recorded earlier,
+    // unless this is the first launch of the call (lost on input: -1)
+    record[k] = inb[k] && ctx.loss_turn && (is_live(p[k]) || ctx.turn_rel0 == 0);
+    lost_at[k] = 0x7fffffff;
     if (!inb[k]) p[k].x = __builtin_nan("");  // padding lanes behave as lost
   }
 
@@ -264,15 +265,18 @@ __global__ __launch_bounds__(BLOCK) void k_track(KernelCtx ctx) {
 #pragma unroll
       for (int k = 0; k < PPL; ++k)
         if (inb[k] && slot < ctx.nsnap) {
-          const int64_t gi = start + (int64_t)k * BLOCK + threadIdx.x;
-          double *dst = ctx.snap + ((int64_t)slot * ctx.n + gi) * 6;
+          double *dst = ctx.snap + ((int64_t)slot * ctx.n + gidx[k]) * 6;
           dst[0] = p[k].x; dst[1] = p[k].px; dst[2] = p[k].y;
           dst[3] = p[k].py; dst[4] = p[k].z; dst[5] = p[k].d;
         }
     }
     bool any = false;
 #pragma unroll
-    for (int k = 0; k < PPL; ++k) any |= is_live(p[k]);
+    for (int k = 0; k < PPL; ++k) {
+      const bool lv = is_live(p[k]);
+      any |= lv;
+      if (!lv && lost_at[k] == 0x7fffffff) lost_at[k] = ctx.turn_rel0 + t - 1;
+    }
     if (!__any(any)) break;  // every particle of this wave is lost: frozen from here on
     const uint64_t turn = ctx.turn0 + (uint64_t)t;
     for (int e = ctx.st; e < ctx.ed; ++e) {
@@ -296,8 +300,7 @@ __global__ __launch_bounds__(BLOCK) void k_track(KernelCtx ctx) {
 #pragma unroll
       for (int k = 0; k < PPL; ++k)
         if (inb[k] && slot < ctx.nsnap) {
-          const int64_t gi = start + (int64_t)k * BLOCK + threadIdx.x;
-          double *dst = ctx.snap + ((int64_t)slot * ctx.n + gi) * 6;
+          double *dst = ctx.snap + ((int64_t)slot * ctx.n + gidx[k]) * 6;
           dst[0] = p[k].x; dst[1] = p[k].px; dst[2] = p[k].y;
           dst[3] = p[k].py; dst[4] = p[k].z; dst[5] = p[k].d;
         }
@@ -306,10 +309,14 @@ __global__ __launch_bounds__(BLOCK) void k_track(KernelCtx ctx) {
 #pragma unroll
   for (int k = 0; k < PPL; ++k) {
     if (!inb[k]) continue;
-    const int64_t gi = start + (int64_t)k * BLOCK + threadIdx.x;
-    double *dst = ctx.r + 6 * gi;
+    double *dst = ctx.r + 6 * gidx[k];
     dst[0] = p[k].x; dst[1] = p[k].px; dst[2] = p[k].y;
     dst[3] = p[k].py; dst[4] = p[k].z; dst[5] = p[k].d;
+    if (record[k]) {
+      int lt = lost_at[k];
+      if (lt == 0x7fffffff && !is_live(p[k])) lt = ctx.turn_rel0 + t - 1;  // lost in the last turn run
+      if (lt != 0x7fffffff) ctx.loss_turn[gidx[k]] = lt;
+    }
   }
 }
 
@@ -371,6 +378,80 @@ __global__ __launch_bounds__(STAT_BLOCK) void k_beam_stat(const double *__restri
   }
 }
 
+// ---- working-set bookkeeping ---------------------------------------------------
+__global__ __launch_bounds__(256) void k_chunk_init(int32_t *idx, int64_t n, int32_t *loss_turn,
+                                                    int32_t nturn, int32_t *cnt,
+                                                    const int64_t *offs, int nseed) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    idx[i] = (int32_t)i;
+    if (loss_turn) loss_turn[i] = nturn;
+  }
+  if (blockIdx.x == 0)
+    for (int s = threadIdx.x; s < nseed; s += 256) cnt[s] = (int32_t)(offs[s + 1] - offs[s]);
+}
+
+// Tile list from per-seed working-set sizes: seed s owns positions
+// [offs[s], offs[s] + cnt[s]) of the index list.
+__global__ __launch_bounds__(256) void k_make_tiles(const int32_t *cnt, const int64_t *offs,
+                                                    int nseed, int per, TileDesc *tiles,
+                                                    int32_t *ntiles) {
+  __shared__ int32_t base[257];
+  int32_t total = 0;
+  for (int s0 = 0; s0 < nseed; s0 += 256) {
+    const int s = s0 + threadIdx.x;
+    const int32_t nt = s < nseed ? (cnt[s] + per - 1) / per : 0;
+    // block-wide exclusive scan (256 entries, fixed order)
+    base[threadIdx.x] = nt;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+      const int32_t v = threadIdx.x >= (unsigned)d ? base[threadIdx.x - d] : 0;
+      __syncthreads();
+      base[threadIdx.x] += v;
+      __syncthreads();
+    }
+    const int32_t first = total + base[threadIdx.x] - nt;
+    if (s < nseed)
+      for (int32_t k = 0; k < nt; ++k) {
+        const int32_t left = cnt[s] - k * per;
+        tiles[first + k] = TileDesc{s, left < per ? left : per, offs[s] + (int64_t)k * per};
+      }
+    total += base[255];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *ntiles = total;
+}
+
+__global__ __launch_bounds__(256) void k_zero_i32(int32_t *p, int n) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = 0;
+}
+
+// Keep the live particles of each tile: one atomic per wave on the seed's
+// counter, lanes placed by ballot prefix.  Which slot a particle lands in
+// depends on timing, its trajectory does not.
+__global__ __launch_bounds__(256) void k_compact(const TileDesc *tiles, const int32_t *ntiles,
+                                                 const int32_t *idx_in, const double *r,
+                                                 const int64_t *offs, int32_t *cnt_out,
+                                                 int32_t *idx_out) {
+  if ((int)blockIdx.x >= *ntiles) return;
+  const TileDesc t = tiles[blockIdx.x];
+  const int lane = __lane_id();
+  for (int j0 = 0; j0 < t.count; j0 += 256) {
+    const int j = j0 + threadIdx.x;
+    int32_t i = -1;
+    bool live = false;
+    if (j < t.count) {
+      i = idx_in[t.start + j];
+      const double x = r[6 * (int64_t)i];
+      live = x == x;
+    }
+    const unsigned long long m = __ballot(live);
+    int base = 0;
+    if (lane == 0 && m) base = atomicAdd(&cnt_out[t.lat], __popcll(m));
+    base = __shfl(base, 0);
+    if (live) idx_out[offs[t.lat] + base + __popcll(m & ((1ull << lane) - 1ull))] = i;
+  }
+}
+
 // ---- launchers -------------------------------------------------------------------
 template <int BLOCK, int PPL, bool LDS>
 static hipError_t launch_one(const KernelCtx &ctx, int grid, size_t lds, hipStream_t s) {
@@ -392,6 +473,29 @@ hipError_t launch_track(const KernelCtx &ctx, int grid, int block, int ppl, bool
   ACCP_CASE(256, 2)
 #undef ACCP_CASE
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_chunk_init(int32_t *idx, int64_t n, int32_t *loss_turn, int32_t nturn,
+                             int32_t *cnt, const int64_t *offs, int nseed, hipStream_t s) {
+  const int grid = (int)std::min<int64_t>((n + 255) / 256 + 1, 4096);
+  hipLaunchKernelGGL(k_chunk_init, dim3(grid), dim3(256), 0, s, idx, n, loss_turn, nturn, cnt, offs, nseed);
+  return hipGetLastError();
+}
+
+hipError_t launch_make_tiles(const int32_t *cnt, const int64_t *offs, int nseed, int per,
+                             TileDesc *tiles, int32_t *ntiles, hipStream_t s) {
+  hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(256), 0, s, cnt, offs, nseed, per, tiles, ntiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_compact(const TileDesc *tiles, const int32_t *ntiles, int max_tiles,
+                          const int32_t *idx_in, const double *r, const int64_t *offs,
+                          int32_t *cnt_out, int32_t *idx_out, int nseed, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_i32, dim3((nseed + 255) / 256), dim3(256), 0, s, cnt_out, nseed);
+  if (max_tiles > 0)
+    hipLaunchKernelGGL(k_compact, dim3(max_tiles), dim3(256), 0, s, tiles, ntiles, idx_in, r, offs,
+                       cnt_out, idx_out);
+  return hipGetLastError();
 }
 
 hipError_t launch_beam_stat(const double *snap, int64_t n, int nrec, double turn0, int interval,

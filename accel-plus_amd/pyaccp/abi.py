@@ -23,11 +23,16 @@ class AccpError(RuntimeError):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("block", c_int32), ("ppl", c_int32), ("staging", c_int32), ("reserved", c_int32)]
+    _fields_ = [("block", c_int32), ("ppl", c_int32), ("staging", c_int32), ("compact", c_int32)]
 
     def __repr__(self):
         st = {0: "auto", 1: "scalar", 2: "lds"}.get(self.staging, self.staging)
-        return f"Tuning(block={self.block}, ppl={self.ppl}, staging={st})"
+        return f"Tuning(block={self.block}, ppl={self.ppl}, staging={st}, compact={self.compact})"
+
+    def as_dict(self):
+        return {"block": self.block, "ppl": self.ppl,
+                "staging": {0: "auto", 1: "scalar", 2: "lds"}.get(self.staging, self.staging),
+                "compact": self.compact}
 
 
 class TrackOpts(ctypes.Structure):
@@ -35,6 +40,7 @@ class TrackOpts(ctypes.Structure):
         ("rng_seed", c_uint64), ("turn0", c_uint64), ("pid0", c_uint64),
         ("mon_interval", c_int32), ("mon_max", c_int32),
         ("mon_out", POINTER(c_double)), ("mon_count", POINTER(c_int32)),
+        ("loss_turn", POINTER(c_int32)),
         ("tuning", Tuning),
     ]
 
@@ -79,8 +85,8 @@ class Library:
         L.accp_track_device.argtypes = [VP, c_void_p, c_int, c_int, c_int, c_int, POINTER(TrackOpts), c_void_p]
         L.accp_beam_stat.argtypes = [DP, c_int, DP]
         L.accp_batch_create.argtypes = [POINTER(VP), c_int, POINTER(VP)]
-        L.accp_track_batched.argtypes = [VP, c_void_p, POINTER(c_int64), c_int, POINTER(Tuning), c_void_p]
-        L.accp_batch_capture.argtypes = [VP, c_void_p, POINTER(c_int64), c_int, POINTER(Tuning)]
+        L.accp_track_batched.argtypes = [VP, c_void_p, POINTER(c_int64), c_int, POINTER(Tuning), c_void_p, c_void_p]
+        L.accp_batch_capture.argtypes = [VP, c_void_p, POINTER(c_int64), c_int, POINTER(Tuning), c_void_p]
         L.accp_batch_replay.argtypes = [VP, c_void_p]
         L.accp_autotune.argtypes = [VP, c_void_p, POINTER(c_int64), c_int, POINTER(Tuning), DP]
         L.accp_batch_free.argtypes = [VP]
@@ -136,13 +142,19 @@ class Lattice:
             pass
 
     def track(self, r, nturn=1, st=0, ed=-1, seed=0, turn0=0, pid0=0, tuning=None,
-              mon_interval=0, mon_max=0):
-        """ALine::pass semantics on a host AoS array; returns (r_out, records)."""
+              mon_interval=0, mon_max=0, loss_turn=False):
+        """ALine::pass semantics on a host AoS array.  Returns r_out, plus the
+        monitor records if mon_interval > 0, plus the per-particle loss turns
+        if loss_turn."""
         r = np.ascontiguousarray(r, dtype=np.float64).copy()
         o = TrackOpts()
         o.rng_seed, o.turn0, o.pid0 = seed, turn0, pid0
         if tuning is not None:
             o.tuning = tuning
+        lt = None
+        if loss_turn:
+            lt = np.zeros(r.size // 6, dtype=np.int32)
+            o.loss_turn = lt.ctypes.data_as(POINTER(c_int32))
         recs = None
         cnt = c_int32(0)
         if mon_interval > 0:
@@ -152,13 +164,19 @@ class Lattice:
             o.mon_out = _dp(recs)
             o.mon_count = ctypes.pointer(cnt)
         self.L.check(self.L.lib.accp_track(self.h, _dp(r), r.size // 6, nturn, st, ed, byref(o)), "accp_track")
+        out = (r.reshape(-1, 6),)
         if recs is not None:
-            return r.reshape(-1, 6), recs[: cnt.value]
-        return r.reshape(-1, 6)
+            out += (recs[: cnt.value],)
+        if lt is not None:
+            out += (lt,)
+        return out if len(out) > 1 else out[0]
 
-    def track_device(self, d_ptr, n, nturn, st=0, ed=-1, seed=0, turn0=0, pid0=0, tuning=None, stream=None):
+    def track_device(self, d_ptr, n, nturn, st=0, ed=-1, seed=0, turn0=0, pid0=0, tuning=None, stream=None,
+                     d_loss_turn=None):
         o = TrackOpts()
         o.rng_seed, o.turn0, o.pid0 = seed, turn0, pid0
+        if d_loss_turn:
+            o.loss_turn = ctypes.cast(c_void_p(d_loss_turn), POINTER(c_int32))
         if tuning is not None:
             o.tuning = tuning
         self.L.check(self.L.lib.accp_track_device(self.h, c_void_p(d_ptr), n, nturn, st, ed, byref(o),
@@ -175,20 +193,22 @@ class Batch:
         lib.check(lib.lib.accp_batch_create(hs, len(lattices), byref(h)), "accp_batch_create")
         self.h = h
 
-    def track(self, d_ptr, offsets, nturn, tuning=None, stream=None):
+    def track(self, d_ptr, offsets, nturn, tuning=None, stream=None, d_loss_turn=None):
         offs = np.ascontiguousarray(offsets, dtype=np.int64)
         self._offs = offs
         t = byref(tuning) if tuning is not None else None
         self.L.check(self.L.lib.accp_track_batched(self.h, c_void_p(d_ptr), offs.ctypes.data_as(POINTER(c_int64)),
-                                                   nturn, t, c_void_p(stream) if stream else None),
+                                                   nturn, t, c_void_p(d_loss_turn) if d_loss_turn else None,
+                                                   c_void_p(stream) if stream else None),
                      "accp_track_batched")
 
-    def capture(self, d_ptr, offsets, nturn, tuning=None):
+    def capture(self, d_ptr, offsets, nturn, tuning=None, d_loss_turn=None):
         offs = np.ascontiguousarray(offsets, dtype=np.int64)
         self._offs = offs
         t = byref(tuning) if tuning is not None else None
         self.L.check(self.L.lib.accp_batch_capture(self.h, c_void_p(d_ptr), offs.ctypes.data_as(POINTER(c_int64)),
-                                                   nturn, t), "accp_batch_capture")
+                                                   nturn, t, c_void_p(d_loss_turn) if d_loss_turn else None),
+                     "accp_batch_capture")
 
     def replay(self, stream=None):
         self.L.check(self.L.lib.accp_batch_replay(self.h, c_void_p(stream) if stream else None), "accp_batch_replay")

@@ -113,12 +113,15 @@ typedef struct accp_adjoint accp_adjoint; /* reverse-mode turn map context    */
 #define ACCP_STAGE_SCALAR 1 /* wave-uniform scalar loads through the K$       */
 #define ACCP_STAGE_LDS 2    /* whole table copied to LDS per workgroup        */
 
-/* Launch shape of the tracking kernel; zero fields mean "library default". */
+/* Launch shape of the tracking kernel; zero fields mean "library default".
+ * Lost particles are dropped from the working set between chunks of turns
+ * (chunks of 1, 2, 4, ... turns up to `compact` turns; 0 = default 256,
+ * negative = never compact). */
 typedef struct accp_tuning {
   int32_t block;   /* threads per workgroup: 64, 128, 256                     */
   int32_t ppl;     /* particles per lane: 1 or 2                              */
   int32_t staging; /* ACCP_STAGE_*                                            */
-  int32_t reserved;
+  int32_t compact; /* longest chunk of turns between compactions              */
 } accp_tuning;
 
 /* Options of one tracking call. */
@@ -131,6 +134,10 @@ typedef struct accp_track_opts {
   int32_t mon_max;      /* capacity of mon_out in records                     */
   double *mon_out;      /* host: mon_max records of ACCP_STAT_LEN doubles     */
   int32_t *mon_count;   /* host: number of records written                    */
+  int32_t *loss_turn;   /* optional, n entries (host for accp_track, device   */
+                        /* for accp_track_device): turn of this call in which */
+                        /* the particle was lost, -1 if lost on input, nturn  */
+                        /* if it survived                                     */
   accp_tuning tuning;   /* launch shape (zeros = default)                     */
 } accp_track_opts;
 
@@ -169,13 +176,16 @@ int accp_beam_stat(const double *r, int n, double *stat /* ACCP_STAT_LEN */);
 /* ---- batched multi-seed tracking ---------------------------------------- */
 /* All seeds must share one topology (types, slices, orders). */
 int accp_batch_create(accp_lattice *const *seeds, int nseed, accp_batch **out);
-/* Track all seeds in one launch.  d_r is one device AoS buffer; particles of
- * seed s are [offsets[s], offsets[s+1]).  offsets is a host array of nseed+1. */
+/* Track all seeds in one launch sequence.  d_r is one device AoS buffer;
+ * particles of seed s are [offsets[s], offsets[s+1]) (offsets: host array of
+ * nseed+1).  d_loss_turn: optional device int32[n], as in accp_track_opts. */
 int accp_track_batched(accp_batch *b, double *d_r, const int64_t *offsets,
-                       int nturn, const accp_tuning *tuning, void *stream);
+                       int nturn, const accp_tuning *tuning,
+                       int32_t *d_loss_turn, void *stream);
 /* Capture accp_track_batched into a HIP graph (replay = one launch). */
 int accp_batch_capture(accp_batch *b, double *d_r, const int64_t *offsets,
-                       int nturn, const accp_tuning *tuning);
+                       int nturn, const accp_tuning *tuning,
+                       int32_t *d_loss_turn);
 int accp_batch_replay(accp_batch *b, void *stream);
 /* Time candidate launch shapes on a short run and return the fastest. */
 int accp_autotune(accp_batch *b, double *d_r, const int64_t *offsets,

@@ -191,3 +191,28 @@ def test_autotune_picks_a_valid_shape(gpu_lib):
     best, ms = B.autotune(d.data_ptr(), offs, 2)
     assert best.block in (64, 128, 256) and best.ppl in (1, 2) and best.staging in (1, 2) and ms > 0
     assert np.array_equal(d.cpu().numpy(), beam)  # probes ran on a scratch copy
+
+
+def test_compaction_and_loss_turns(gpu_lib, oracle):
+    """Dropping lost particles between chunks of turns changes nothing in the
+    coordinates; the recorded loss turn is the turn (0-based) in which the
+    oracle first shows the particle lost (-1: lost on input)."""
+    import pyaccp.abi as abi
+
+    da = lat.da_ring(0)
+    grid = lat.da_grid(21, 11)
+    grid[5, 0] = np.nan
+    L = gpu_lib.lattice(da)
+    a, lt = L.track(grid, 64, loss_turn=True)
+    b = L.track(grid, 64, tuning=abi.Tuning(64, 1, 0, -1))
+    c = L.track(grid, 64, tuning=abi.Tuning(128, 2, 2, 4))
+    assert np.array_equal(np.nan_to_num(a, nan=9.0), np.nan_to_num(b, nan=9.0))
+    assert np.array_equal(np.isnan(a), np.isnan(c))
+    want = np.full(len(grid), 64, dtype=np.int32)
+    want[5] = -1
+    r = grid.copy()
+    for t in range(64):
+        r = oracle.track(da, r, 1)
+        newly = np.isnan(r[:, 0]) & (want == 64)
+        want[newly] = t
+    assert np.array_equal(lt, want)
