@@ -102,18 +102,17 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
     import pyaccp
     import pyaccp.abi as abi
     import pyaccp.flops as flops
     import pyaccp.lattices as lat
+    from pyaccp.hipmem import DeviceArray, Event, Stream, synchronize
 
     lib = pyaccp.load()
     if lib.device_count() < 1:
         raise SystemExit("no HIP device")
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream(dev)
+    stream = Stream()
 
     log("building workload")
     lats, beam, offs = lat.batched_workload(args.nseed)
@@ -121,40 +120,37 @@ def main():
     nturn = args.nturn
     glats = [lib.lattice(x) for x in lats]
     B = abi.Batch(lib, glats)
-    d_pristine = torch.tensor(beam, dtype=torch.float64, device=dev)
-    d_r = d_pristine.clone()
-    d_lt = torch.zeros(n, dtype=torch.int32, device=dev)
+    d_pristine = DeviceArray(beam)
+    d_r = DeviceArray(beam)
+    d_lt = DeviceArray(np.zeros(n, dtype=np.int32))
 
     log("autotuning")
-    tuning, probe_ms = B.autotune(d_r.data_ptr(), offs, 8)
+    tuning, probe_ms = B.autotune(d_r.ptr, offs, 8)
     log(f"autotune: {tuning} ({probe_ms:.2f} ms for 8 turns)")
 
-    B.capture(d_r.data_ptr(), offs, nturn, tuning=tuning, d_loss_turn=d_lt.data_ptr())
-
-    def step():
-        d_r.copy_(d_pristine)
-        B.replay(stream.cuda_stream)
+    B.capture(d_r.ptr, offs, nturn, tuning=tuning, d_loss_turn=d_lt.ptr)
 
     for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        d_r.copy_from(d_pristine, stream.handle)
+        B.replay(stream.handle)
+    synchronize()
+    ev0 = [Event() for _ in range(args.steps)]
+    ev1 = [Event() for _ in range(args.steps)]
     log(f"timing {args.steps} steps")
-    torch.cuda.synchronize()
+    synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        d_r.copy_(d_pristine)
-        ev0[k].record(stream)
-        B.replay(stream.cuda_stream)
-        ev1[k].record(stream)
-    torch.cuda.synchronize()
+        d_r.copy_from(d_pristine, stream.handle)
+        ev0[k].record(stream.handle)
+        B.replay(stream.handle)
+        ev1[k].record(stream.handle)
+    synchronize()
     wall = time.perf_counter() - t0
-    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev0, ev1)]))
+    kernel_ms = float(np.mean([a.elapsed_ms(b) for a, b in zip(ev0, ev1)]))
     ms_per_step = wall / args.steps * 1e3
 
-    out = d_r.cpu().numpy()
-    lt = d_lt.cpu().numpy()
+    out = d_r.numpy()
+    lt = d_lt.numpy()
     alive_turns = np.where(lt >= nturn, nturn, np.maximum(lt + 1, 0)).astype(np.int64)
     survived = ~np.isnan(out[:, 0])
     launched_pt = n * nturn
@@ -208,15 +204,15 @@ def main():
 
     if not args.no_plain:
         log("plain per-seed GPU path")
-        d_p = d_pristine.clone()
-        torch.cuda.synchronize()
+        d_p = DeviceArray(beam)
+        synchronize()
         t0 = time.perf_counter()
         for s in range(len(lats)):
             cnt = int(offs[s + 1] - offs[s])
-            glats[s].track_device(d_p.data_ptr() + int(offs[s]) * 48, cnt, nturn, stream=stream.cuda_stream)
-        torch.cuda.synchronize()
+            glats[s].track_device(d_p.ptr + int(offs[s]) * 48, cnt, nturn, stream=stream.handle)
+        synchronize()
         plain_s = time.perf_counter() - t0
-        pout = d_p.cpu().numpy()
+        pout = d_p.numpy()
         same = bool(np.array_equal(np.isnan(pout[:, 0]), np.isnan(out[:, 0])))
         result["plain_per_seed"] = {
             "value": launched_pt / plain_s,
