@@ -94,6 +94,7 @@ def main():
     ap.add_argument("--nturn", type=int, default=1024)
     ap.add_argument("--no-plain", action="store_true", help="skip the per-seed plain comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--tune", default="", help="block,ppl,staging(scalar|lds) instead of autotuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this backend has no "
@@ -124,9 +125,13 @@ def main():
     d_r = DeviceArray(beam)
     d_lt = DeviceArray(np.zeros(n, dtype=np.int32))
 
-    log("autotuning")
-    tuning, probe_ms = B.autotune(d_r.ptr, offs, 8)
-    log(f"autotune: {tuning} ({probe_ms:.2f} ms for 8 turns)")
+    if args.tune:
+        bl, pp, sg = args.tune.split(",")
+        tuning = abi.Tuning(int(bl), int(pp), {"scalar": 1, "lds": 2}[sg], 0)
+    else:
+        log("autotuning")
+        tuning, probe_ms = B.autotune(d_r.ptr, offs, 8)
+        log(f"autotune: {tuning} ({probe_ms:.2f} ms for 8 turns)")
 
     B.capture(d_r.ptr, offs, nturn, tuning=tuning, d_loss_turn=d_lt.ptr)
 
