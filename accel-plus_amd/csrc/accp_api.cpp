@@ -62,11 +62,46 @@ int highest_nonzero(const double *b, const double *a, int n) {
   return hi;
 }
 
+// 1/j! rounded to double (long double product, one final rounding).
+double inv_factorial(int j) {
+  long double f = 1.0L;
+  for (int i = 2; i <= j; ++i) f *= (long double)i;
+  return (double)(1.0L / f);
+}
+
+// Smallest supported term count NT with umax^NT/(2NT)! < 1e-18 (the
+// truncation of the cos/cosh/sin/sinh power series), 0 if none.
+int quad_series_terms(double umax) {
+  for (int nt = 4; nt <= 16; nt += 2) {
+    double t = 1.0;
+    for (int j = 1; j <= nt; ++j) t *= umax / ((2.0 * j - 1) * (2.0 * j));
+    if (t < 1e-18) return nt;
+  }
+  return 0;
+}
+
 struct Compiled {
   std::vector<Hdr> hdr;
   std::vector<double> par;
   std::vector<int> wakes;  // element indices of LongWake records
+  bool full = false;       // needs the kernel variant with every element map
 };
+
+// Elements the lean kernel variant does not carry.
+bool needs_full(const Hdr &h) {
+  switch (h.op) {
+    case OP_BEND:
+    case OP_BENDMP_RAD:
+    case OP_BENDMP_QE:
+      return true;
+    case OP_QUAD:
+      return h.a == 0;
+    case OP_BENDMP:
+      return (h.b & 0xff) > 3 || (!((h.b >> 8) & 1) && (h.b & 0xff) == 0);
+    default:
+      return false;
+  }
+}
 
 int compile_lattice(const accp_elem *E, int n, Compiled &out) {
   out.hdr.assign(n, Hdr{OP_NOP, 0, 0, 0});
@@ -86,8 +121,16 @@ int compile_lattice(const accp_elem *E, int n, Compiled &out) {
           h.op = OP_QUAD_K0;
           P.push_back(e.len);
         } else {
+          // series length for u = L^2|K|/(1+delta) up to delta = -1/2
+          // (0: outside the series range, trigonometric path)
           h.op = OP_QUAD;
-          P.insert(P.end(), {e.len, e.k, std::fabs(e.k)});
+          h.a = quad_series_terms(2.0 * e.len * e.len * std::fabs(e.k));
+          P.insert(P.end(), {e.len, e.k, std::fabs(e.k), e.len * e.len});
+          const int M = h.a / 2;
+          for (int blk = 0; blk < 4; ++blk) {  // 1/(4m)!, 1/(4m+2)!, 1/(4m+1)!, 1/(4m+3)!
+            const int shift = blk == 0 ? 0 : blk == 1 ? 2 : blk == 2 ? 1 : 3;
+            for (int m = 0; m < M; ++m) P.push_back(inv_factorial(4 * m + shift));
+          }
         }
         break;
       case ACCP_CORRECTOR:
@@ -182,6 +225,7 @@ int compile_lattice(const accp_elem *E, int n, Compiled &out) {
         return fail(ACCP_E_LATTICE, "element %d: unsupported type code %d", i, e.type);
     }
     out.hdr[i] = h;
+    out.full = out.full || needs_full(h);
   }
   if (out.par.empty()) out.par.push_back(0.0);
   return ACCP_OK;
@@ -242,7 +286,7 @@ struct TableRef {
   const int32_t *meta;  // lat_hdr0[nseed], lat_par0[nseed], lat_npar[nseed]
   int nseed, nelem;
   size_t lds_need;
-  bool has_qe;
+  bool full;  // kernel variant with every element map
 };
 
 struct RunSpec {
@@ -338,7 +382,7 @@ int enqueue_run(const TableRef &T, WorkSet &ws, int max_tiles, double *d_r, int6
     c.turn0 = R.turn0 + (uint64_t)t;
     c.turn_rel0 = t;
     c.nturn = cn;
-    HIPCHK(launch_track(c, max_tiles, block, ppl, lds, lds ? T.lds_need : 0, s));
+    HIPCHK(launch_track(c, max_tiles, block, ppl, lds, lds ? T.lds_need : 0, T.full, s));
     t += cn;
     if (t < R.nturn) {
       HIPCHK(launch_compact((const TileDesc *)ws.tiles.p, (const int32_t *)ws.ntiles.p, max_tiles,
@@ -365,7 +409,7 @@ struct accp_lattice {
   WakeState wake;
   TableRef table() const {
     return TableRef{(const Hdr *)d_hdr.p, (const double *)d_par.p, (const int32_t *)d_meta.p, 1,
-                    (int)tab.hdr.size(), lds_bytes_for(tab.hdr.size(), tab.par.size()), false};
+                    (int)tab.hdr.size(), lds_bytes_for(tab.hdr.size(), tab.par.size()), tab.full};
   }
 };
 
@@ -582,6 +626,7 @@ extern "C" int accp_beam_stat(const double *r, int n, double *stat) {
 struct accp_batch {
   int nseed = 0, nelem = 0;
   size_t max_lds = 0;
+  bool full = false;
   DevBuf d_hdr, d_par, d_meta, d_scratch;
   WorkSet ws;
   hipGraph_t graph = nullptr;
@@ -589,7 +634,7 @@ struct accp_batch {
   hipStream_t cap_stream = nullptr;
   TableRef table() const {
     return TableRef{(const Hdr *)d_hdr.p, (const double *)d_par.p, (const int32_t *)d_meta.p, nseed, nelem,
-                    max_lds, false};
+                    max_lds, full};
   }
 };
 
@@ -620,6 +665,7 @@ extern "C" int accp_batch_create(accp_lattice *const *seeds, int nseed, accp_bat
     hdr.insert(hdr.end(), c.hdr.begin(), c.hdr.end());
     par.insert(par.end(), c.par.begin(), c.par.end());
     b->max_lds = std::max(b->max_lds, lds_bytes_for(c.hdr.size(), c.par.size()));
+    b->full = b->full || c.full;
   }
   std::vector<int32_t> meta(h0);
   meta.insert(meta.end(), p0.begin(), p0.end());

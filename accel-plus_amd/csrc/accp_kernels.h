@@ -47,8 +47,10 @@ hipError_t launch_compact(const TileDesc *tiles, const int32_t *ntiles, int max_
                           const int32_t *idx_in, const double *r, const int64_t *offs,
                           int32_t *cnt_out, int32_t *idx_out, int nseed, hipStream_t s);
 
+// full: use the kernel variant with every element map (radiation, linear
+// Bend, trigonometric quads, high-order BendMPole); otherwise the lean one.
 hipError_t launch_track(const KernelCtx &ctx, int grid, int block, int ppl, bool lds,
-                        size_t lds_bytes, hipStream_t s);
+                        size_t lds_bytes, bool full, hipStream_t s);
 hipError_t launch_beam_stat(const double *snap, int64_t n, int nrec, double turn0, int interval,
                             double *out, hipStream_t s);
 

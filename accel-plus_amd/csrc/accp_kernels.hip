@@ -84,11 +84,171 @@ __device__ __forceinline__ void bendmp(Part (&p)[PPL], const Hdr h, const PT *P,
   }
 }
 
+// ABendMPole::pass without radiation: delta is fixed inside the element, so
+// the pseudo-drift and kick constants that depend on it are formed once;
+// PolyB[0..MO] live in registers.
+template <int PPL, int MO, bool PLAIN, class PT>
+__device__ __forceinline__ void bendmp_fixed(Part (&p)[PPL], const Hdr h, const PT *P) {
+  const int ns = h.a;
+  const double psi1 = P[BM_PSI1], psi2 = P[BM_PSI2], h0 = P[BM_IRHO];
+  const double l1 = P[BM_L1], l2 = P[BM_L2], k1 = P[BM_K1], k2 = P[BM_K2];
+  const double hs1 = P[BM_HS1], hq1 = P[BM_HQ1], hs2 = P[BM_HS2], hq2 = P[BM_HQ2];
+  const double k1c = P[BM_K1C], k2c = P[BM_K2C];
+  double b[MO + 1];
+#pragma unroll
+  for (int i = 0; i <= MO; ++i) b[i] = P[BM_B + i];
+  const double c4a = hs1 * 0.5, c4b = hs2 * 0.5;
+  const double lh1 = k1 * h0, lh2 = k2 * h0;
+  HxC ca[PPL], cb[PPL];
+  double lhd1[PPL], lhd2[PPL];
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) {
+    if (psi1 != 0) edge(p[k], psi1);
+    ca[k] = hx_consts(p[k].ipd, l1, hs1, hq1);
+    cb[k] = hx_consts(p[k].ipd, l2, hs2, hq2);
+    lhd1[k] = lh1 * p[k].d;
+    lhd2[k] = lh2 * p[k].d;
+  }
+  for (int m = 0; m < ns; ++m) {
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) hxdrift_c(p[k], ca[k], h0, c4a);
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) bend_kick_c<MO, PLAIN>(p[k], b, k1, h0, lhd1[k], lh1, k1c, k2c);
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) hxdrift_c(p[k], cb[k], h0, c4b);
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) bend_kick_c<MO, PLAIN>(p[k], b, k2, h0, lhd2[k], lh2, k1c, k2c);
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) hxdrift_c(p[k], cb[k], h0, c4b);
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) bend_kick_c<MO, PLAIN>(p[k], b, k1, h0, lhd1[k], lh1, k1c, k2c);
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) hxdrift_c(p[k], ca[k], h0, c4a);
+  }
+  if (psi2 != 0) {
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) edge(p[k], psi2);
+  }
+}
+
+// AStrMPole::pass with PolyB/PolyA[0..HI] in registers; thick (len > 0,
+// nslice Forest-Ruth slices, accel.cpp:829-843) or thin (one integrated kick,
+// accel.cpp:824-827).
+template <int PPL, int HI, bool THICK, class PT>
+__device__ __forceinline__ void strmp_fixed(Part (&p)[PPL], const Hdr h, const PT *P) {
+  const PT *C = THICK ? P + 4 : P;
+  double b[HI + 1], a[HI + 1];
+#pragma unroll
+  for (int i = 0; i <= HI; ++i) {
+    b[i] = C[i];
+    a[i] = C[HI + 1 + i];
+  }
+  if (!THICK) {
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+      double re, im;
+      horner_reg<HI>(b, a, p[k].x, p[k].y, re, im);
+      p[k].px -= re;
+      p[k].py += im;
+    }
+    return;
+  }
+  const int ns = h.a;
+  const double l1 = P[0], l2 = P[1], k1 = P[2], k2 = P[3];
+  double n1[PPL], n2[PPL], z1[PPL], z2[PPL];
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) {
+    n1[k] = l1 * p[k].ipd;
+    n2[k] = l2 * p[k].ipd;
+    z1[k] = n1[k] * (0.5 * p[k].ipd);
+    z2[k] = n2[k] * (0.5 * p[k].ipd);
+  }
+  for (int m = 0; m < ns; ++m) {
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+      double re, im;
+      drift_c(p[k], n1[k], z1[k]);
+      horner_reg<HI>(b, a, p[k].x, p[k].y, re, im);
+      p[k].px -= k1 * re;
+      p[k].py += k1 * im;
+      drift_c(p[k], n2[k], z2[k]);
+      horner_reg<HI>(b, a, p[k].x, p[k].y, re, im);
+      p[k].px -= k2 * re;
+      p[k].py += k2 * im;
+      drift_c(p[k], n2[k], z2[k]);
+      horner_reg<HI>(b, a, p[k].x, p[k].y, re, im);
+      p[k].px -= k1 * re;
+      p[k].py += k1 * im;
+      drift_c(p[k], n1[k], z1[k]);
+    }
+  }
+}
+
+// StrMPole with no field: the slices reduce to drifts of total length len.
+template <int PPL, class PT>
+__device__ __forceinline__ void strmp_empty(Part (&p)[PPL], const Hdr h, const PT *P) {
+  const int ns = h.a;
+  const double l1 = P[0], l2 = P[1];
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) {
+    const double n1 = l1 * p[k].ipd, n2 = l2 * p[k].ipd;
+    const double z1 = n1 * (0.5 * p[k].ipd), z2 = n2 * (0.5 * p[k].ipd);
+    for (int m = 0; m < ns; ++m) {
+      drift_c(p[k], n1, z1);
+      drift_c(p[k], n2, z2);
+      drift_c(p[k], n2, z2);
+      drift_c(p[k], n1, z1);
+    }
+  }
+}
+
+template <int PPL, bool THICK, class PT>
+__device__ __forceinline__ void strmp_dispatch(Part (&p)[PPL], const Hdr h, const PT *P) {
+  switch (h.b) {
+    case -1:
+      if (THICK) strmp_empty<PPL>(p, h, P);
+      break;
+    case 0: strmp_fixed<PPL, 0, THICK>(p, h, P); break;
+    case 1: strmp_fixed<PPL, 1, THICK>(p, h, P); break;
+    case 2: strmp_fixed<PPL, 2, THICK>(p, h, P); break;
+    case 3: strmp_fixed<PPL, 3, THICK>(p, h, P); break;
+    case 4: strmp_fixed<PPL, 4, THICK>(p, h, P); break;
+    case 5: strmp_fixed<PPL, 5, THICK>(p, h, P); break;
+    case 6: strmp_fixed<PPL, 6, THICK>(p, h, P); break;
+    case 7: strmp_fixed<PPL, 7, THICK>(p, h, P); break;
+    case 8: strmp_fixed<PPL, 8, THICK>(p, h, P); break;
+    default: strmp_fixed<PPL, 9, THICK>(p, h, P); break;
+  }
+}
+
+template <int PPL, bool FULL, class PT>
+__device__ __forceinline__ void quad_dispatch(Part (&p)[PPL], const Hdr h, const PT *P) {
+  const double L = P[0], K = P[1], aK = P[2], L2 = P[3];
+  const bool foc = K > 0;
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) {
+    switch (h.a) {
+      case 4: quad_series<4>(p[k], P + 4, L, L2, foc, aK); break;
+      case 6: quad_series<6>(p[k], P + 4, L, L2, foc, aK); break;
+      case 8: quad_series<8>(p[k], P + 4, L, L2, foc, aK); break;
+      case 10: quad_series<10>(p[k], P + 4, L, L2, foc, aK); break;
+      case 12: quad_series<12>(p[k], P + 4, L, L2, foc, aK); break;
+      case 14: quad_series<14>(p[k], P + 4, L, L2, foc, aK); break;
+      case 16: quad_series<16>(p[k], P + 4, L, L2, foc, aK); break;
+      default:
+        // outside the series range: only the full kernel carries the
+        // trigonometric map (the host never sends such a quad to the lean one)
+        if (FULL) quad(p[k], L, K, aK);
+        break;
+    }
+  }
+}
+
 // ---- one element on PPL particles -------------------------------------------
 // Lost particles (x NaN) are computed on like the others and restored
 // afterwards, which keeps the PPL instruction streams interleaved; the
 // reference skips them (e.g. accel.cpp:166-167), with the same result.
-template <int PPL, class PT>
+template <int PPL, bool FULL, class PT>
 __device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const PT *P,
                                               const KernelCtx &ctx, const uint64_t (&pid)[PPL],
                                               uint64_t turn, int eidx) {
@@ -106,12 +266,9 @@ __device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const
       for (int k = 0; k < PPL; ++k) drift_nl(p[k], L * p[k].ipd);
       break;
     }
-    case OP_QUAD: {
-      const double L = P[0], K = P[1], aK = P[2];
-#pragma unroll
-      for (int k = 0; k < PPL; ++k) quad(p[k], L, K, aK);
+    case OP_QUAD:
+      quad_dispatch<PPL, FULL>(p, h, P);
       break;
-    }
     case OP_QUAD_K0: {
       const double L = P[0];
 #pragma unroll
@@ -119,6 +276,7 @@ __device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const
       break;
     }
     case OP_BEND: {
+      if (!FULL) break;
       const double L = P[0], kx = P[1], grd = P[2], psi1 = P[4], psi2 = P[5];
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
@@ -128,56 +286,41 @@ __device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const
       }
       break;
     }
-    case OP_BENDMP:
+    case OP_BENDMP: {
+      const int mo = h.b & 0xff;
+      const bool plain = (h.b >> 8) & 1;
+      if (plain) {
+        switch (mo) {
+          case 0: bendmp_fixed<PPL, 0, true>(p, h, P); break;
+          case 1: bendmp_fixed<PPL, 1, true>(p, h, P); break;
+          case 2: bendmp_fixed<PPL, 2, true>(p, h, P); break;
+          case 3: bendmp_fixed<PPL, 3, true>(p, h, P); break;
+          default:
+            if (FULL) bendmp<PPL>(p, h, P, ctx, pid, turn, eidx);
+            break;
+        }
+      } else {
+        switch (mo) {
+          case 1: bendmp_fixed<PPL, 1, false>(p, h, P); break;
+          case 2: bendmp_fixed<PPL, 2, false>(p, h, P); break;
+          case 3: bendmp_fixed<PPL, 3, false>(p, h, P); break;
+          default:
+            if (FULL) bendmp<PPL>(p, h, P, ctx, pid, turn, eidx);
+            break;
+        }
+      }
+      break;
+    }
     case OP_BENDMP_RAD:
     case OP_BENDMP_QE:
-      bendmp<PPL>(p, h, P, ctx, pid, turn, eidx);
+      if (FULL) bendmp<PPL>(p, h, P, ctx, pid, turn, eidx);
       break;
-    case OP_STRMP: {
-      const int ns = h.a, hi = h.b;
-      const double l1 = P[0], l2 = P[1], k1 = P[2], k2 = P[3];
-      const PT *B = P + 4;
-      const PT *A = B + (hi + 1);
-      for (int m = 0; m < ns; ++m) {
-        double n1[PPL], n2[PPL];
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) {
-          n1[k] = l1 * p[k].ipd;
-          n2[k] = l2 * p[k].ipd;
-        }
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) drift_nl(p[k], n1[k]);
-        if (hi >= 0) {
-#pragma unroll
-          for (int k = 0; k < PPL; ++k) mpole_kick(p[k], B, A, hi, k1);
-        }
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) drift_nl(p[k], n2[k]);
-        if (hi >= 0) {
-#pragma unroll
-          for (int k = 0; k < PPL; ++k) mpole_kick(p[k], B, A, hi, k2);
-        }
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) drift_nl(p[k], n2[k]);
-        if (hi >= 0) {
-#pragma unroll
-          for (int k = 0; k < PPL; ++k) mpole_kick(p[k], B, A, hi, k1);
-        }
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) drift_nl(p[k], n1[k]);
-      }
+    case OP_STRMP:
+      strmp_dispatch<PPL, true>(p, h, P);
       break;
-    }
-    case OP_STRMP_THIN: {
-      const int hi = h.b;
-      if (hi >= 0) {
-        const PT *B = P;
-        const PT *A = B + (hi + 1);
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) mpole_kick(p[k], B, A, hi, 1.0);
-      }
+    case OP_STRMP_THIN:
+      strmp_dispatch<PPL, false>(p, h, P);
       break;
-    }
     case OP_CAVITY: {
       const double L = P[0], nv = P[1], w = P[2], phis = P[3];
       const bool on = h.a != 0;
@@ -205,8 +348,26 @@ __device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const
     if (!live[k]) p[k] = keep[k];
 }
 
+template <bool LDS>
+__device__ __forceinline__ Hdr load_hdr(const Hdr *hdr, int e) {
+  Hdr h;
+  if (LDS) {
+    const Hdr v = hdr[e];
+    h.op = __builtin_amdgcn_readfirstlane(v.op);
+    h.off = __builtin_amdgcn_readfirstlane(v.off);
+    h.a = __builtin_amdgcn_readfirstlane(v.a);
+    h.b = __builtin_amdgcn_readfirstlane(v.b);
+  } else {
+    h = hdr[e];
+  }
+  return h;
+}
+
 // ---- the persistent tracker ---------------------------------------------------
-template <int BLOCK, int PPL, bool LDS>
+// FULL = false is the lean variant for lattices whose quads are all in the
+// series range and that have no radiation, no linear Bend and no BendMPole
+// above octupole order; the host picks the variant per lattice.
+template <int BLOCK, int PPL, bool LDS, bool FULL>
 __global__ __launch_bounds__(BLOCK) void k_track(KernelCtx ctx) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
 
@@ -279,17 +440,12 @@ __global__ __launch_bounds__(BLOCK) void k_track(KernelCtx ctx) {
     }
     if (!__any(any)) break;  // every particle of this wave is lost: frozen from here on
     const uint64_t turn = ctx.turn0 + (uint64_t)t;
+    // header of the next element is fetched while the current one runs
+    Hdr h = load_hdr<LDS>(hdr, ctx.st);
     for (int e = ctx.st; e < ctx.ed; ++e) {
-      Hdr h;
-      if (LDS) {
-        h.op = __builtin_amdgcn_readfirstlane(hdr[e].op);
-        h.off = __builtin_amdgcn_readfirstlane(hdr[e].off);
-        h.a = __builtin_amdgcn_readfirstlane(hdr[e].a);
-        h.b = __builtin_amdgcn_readfirstlane(hdr[e].b);
-      } else {
-        h = hdr[e];
-      }
-      apply_element<PPL>(p, h, par + h.off, ctx, pid, turn, e);
+      const Hdr hn = load_hdr<LDS>(hdr, e + 1 < ctx.ed ? e + 1 : e);
+      apply_element<PPL, FULL>(p, h, par + h.off, ctx, pid, turn, e);
+      h = hn;
     }
   }
   // monitor slots after an early exit hold the frozen state
@@ -454,17 +610,20 @@ __global__ __launch_bounds__(256) void k_compact(const TileDesc *tiles, const in
 
 // ---- launchers -------------------------------------------------------------------
 template <int BLOCK, int PPL, bool LDS>
-static hipError_t launch_one(const KernelCtx &ctx, int grid, size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL((k_track<BLOCK, PPL, LDS>), dim3(grid), dim3(BLOCK), lds, s, ctx);
+static hipError_t launch_one(const KernelCtx &ctx, int grid, size_t lds, bool full, hipStream_t s) {
+  if (full)
+    hipLaunchKernelGGL((k_track<BLOCK, PPL, LDS, true>), dim3(grid), dim3(BLOCK), lds, s, ctx);
+  else
+    hipLaunchKernelGGL((k_track<BLOCK, PPL, LDS, false>), dim3(grid), dim3(BLOCK), lds, s, ctx);
   return hipGetLastError();
 }
 
 hipError_t launch_track(const KernelCtx &ctx, int grid, int block, int ppl, bool lds,
-                        size_t lds_bytes, hipStream_t s) {
+                        size_t lds_bytes, bool full, hipStream_t s) {
 #define ACCP_CASE(B, P)                                                        \
   if (block == B && ppl == P)                                                  \
-    return lds ? launch_one<B, P, true>(ctx, grid, lds_bytes, s)              \
-               : launch_one<B, P, false>(ctx, grid, 0, s);
+    return lds ? launch_one<B, P, true>(ctx, grid, lds_bytes, full, s)        \
+               : launch_one<B, P, false>(ctx, grid, 0, full, s);
   ACCP_CASE(64, 1)
   ACCP_CASE(128, 1)
   ACCP_CASE(256, 1)

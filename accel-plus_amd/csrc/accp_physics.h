@@ -281,6 +281,135 @@ ACCP_HD void corrector(Part &p, double L, double kx, double ky) {
   if (L != 0) drift_nl(p, 0.5 * L * p.ipd);
 }
 
+// ---- fast paths: constants that depend only on delta hoisted per element ----
+// Inside an element without radiation delta does not change, so 1/(1+delta)
+// and everything built from it is computed once per particle per element
+// instead of once per sub-step.
+
+// fastdrift with z coefficient zc = NormL/(2(1+delta)).
+ACCP_HD void drift_c(Part &p, double nl, double zc) {
+  const double s = p.px * p.px + p.py * p.py;
+  p.x += nl * p.px;
+  p.y += nl * p.py;
+  p.z -= zc * s;
+}
+
+// Per-particle constants of ATbendhxdrift6 for one step length L:
+// c1 = L/(1+d), c2 = (h L^2/4)/(1+d)^2, c3 = (h L/2)/(1+d), c5 = L/(2(1+d)^2);
+// c4 = h L/2 is particle independent.
+struct HxC {
+  double c1, c2, c3, c5;
+};
+ACCP_HD HxC hx_consts(double ipd, double L, double hs, double hsl4) {
+  return HxC{ipd * L, hsl4 * ipd * ipd, hs * 0.5 * ipd, ipd * ipd * L * 0.5};
+}
+ACCP_HD void hxdrift_c(Part &p, const HxC &c, double h, double c4) {
+  const double x = p.x, px = p.px, py = p.py;
+  const double a = 1.0 + h * x;
+  const double px2 = px * px, py2 = py * py;
+  const double s = px2 + py2;
+  p.x += a * px * c.c1 + c.c2 * (px2 - py2);
+  p.px -= c.c3 * s;
+  p.y += a * py * c.c1 * (1.0 + px * c4);
+  p.z -= a * c.c5 * s;
+}
+
+// Complex Horner of order HI over coefficients held in registers.
+template <int HI>
+ACCP_HD void horner_reg(const double (&b)[HI + 1], const double (&a)[HI + 1], double x, double y,
+                        double &re, double &im) {
+  re = b[HI];
+  im = a[HI];
+#pragma unroll
+  for (int i = HI - 1; i >= 0; --i) {
+    const double t = re * x - im * y + b[i];
+    im = im * x + re * y + a[i];
+    re = t;
+  }
+}
+template <int HI>
+ACCP_HD void horner_b_reg(const double (&b)[HI + 1], double x, double y, double &re, double &im) {
+  re = b[HI];
+  im = 0.0;
+#pragma unroll
+  for (int i = HI - 1; i >= 0; --i) {
+    const double t = re * x - im * y + b[i];
+    im = im * x + re * y;
+    re = t;
+  }
+}
+
+// bndthinkick with delta fixed: lhd = L*h*delta precomputed per element.
+template <int MO, bool PLAIN>
+ACCP_HD void bend_kick_c(Part &p, const double (&b)[MO + 1], double L, double h, double lhd,
+                         double lh, double k1c, double k2c) {
+  const double x = p.x, y = p.y;
+  double re, im;
+  horner_b_reg<MO>(b, x, y, re, im);
+  if (PLAIN) {
+    p.px -= L * (re + h * (h * x)) - lhd;
+    if (MO > 0) p.py += L * im;
+  } else {
+    p.px -= L * (re + h * (h * x + k1c * (x * x - 0.5 * y * y) +
+                           k2c * (x * x * x - (4.0 / 3.0) * x * y * y))) - lhd;
+    p.py += L * (im + h * (k1c * x * y + (4.0 / 3.0) * k2c * x * x * y +
+                           (h / 6.0 * k1c - k2c / 3.0) * y * y * y));
+  }
+  p.z -= lh * x;
+}
+
+// quad6 for K != 0 from power series in u = L^2 |K|/(1+delta):
+//   cos(sqrt u) = E(-u), cosh(sqrt u) = E(u),       E(v) = sum v^k/(2k)!
+//   sin(sqrt u)/sqrt u = O(-u), sinh(sqrt u)/sqrt u = O(u),  O(v) = sum v^k/(2k+1)!
+// split into even/odd k so both signs come from two polynomials in u^2:
+// no sqrt, division or transcendental.  NT terms are exact to double
+// rounding for u <= umax chosen by the host.
+// C holds the series coefficients in four blocks of M = NT/2 values:
+// 1/(4m)!, 1/(4m+2)!, 1/(4m+1)!, 1/(4m+3)! for m = 0..M-1 (host-computed, so
+// no constant lives in registers across the kernel).
+template <int NT, class PT>
+ACCP_HD void cs_series(const PT *C, double u, double &c, double &ch, double &s, double &sh) {
+  constexpr int M = NT / 2;
+  const double w = u * u;
+  double ee = C[M - 1], eo = C[2 * M - 1], oe = C[3 * M - 1], oo = C[4 * M - 1];
+#pragma unroll
+  for (int m = M - 2; m >= 0; --m) {
+    ee = ee * w + C[m];
+    eo = eo * w + C[M + m];
+    oe = oe * w + C[2 * M + m];
+    oo = oo * w + C[3 * M + m];
+  }
+  c = ee - u * eo;
+  ch = ee + u * eo;
+  s = oe - u * oo;
+  sh = oe + u * oo;
+}
+
+template <int NT, class PT>
+ACCP_HD void quad_series(Part &p, const PT *C, double L, double L2, bool focusing, double aK) {
+  const double pn = p.ipd;
+  const double x = p.x, xp = p.px * pn, y = p.y, yp = p.py * pn;
+  const double g = aK * pn;
+  double c, ch, s, sh;
+  cs_series<NT>(C, L2 * g, c, ch, s, sh);
+  double mhd, m12, m21, mvd, m34, m43;
+  if (focusing) {
+    mhd = c; m12 = L * s; m21 = -m12 * g;
+    mvd = ch; m34 = L * sh; m43 = m34 * g;
+  } else {
+    mhd = ch; m12 = L * sh; m21 = m12 * g;
+    mvd = c; m34 = L * s; m43 = -m34 * g;
+  }
+  const double opd = 1.0 + p.d;
+  p.x = mhd * x + m12 * xp;
+  p.px = (m21 * x + mhd * xp) * opd;
+  p.y = mvd * y + m34 * yp;
+  p.py = (m43 * y + mvd * yp) * opd;
+  p.z -= g * (x * x * (L - mhd * m12) - y * y * (L - mvd * m34)) * 0.25;
+  p.z -= (xp * xp * (L + mhd * m12) + yp * yp * (L + mvd * m34)) * 0.25;
+  p.z -= (x * xp * m12 * m21 + y * yp * m34 * m43) * 0.5;
+}
+
 // ---- counter-based normal variates (quantum excitation) -------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
 // easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies with a
