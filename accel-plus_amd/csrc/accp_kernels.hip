@@ -245,20 +245,10 @@ __device__ __forceinline__ void quad_dispatch(Part (&p)[PPL], const Hdr h, const
 }
 
 // ---- one element on PPL particles -------------------------------------------
-// Lost particles (x NaN) are computed on like the others and restored
-// afterwards, which keeps the PPL instruction streams interleaved; the
-// reference skips them (e.g. accel.cpp:166-167), with the same result.
 template <int PPL, bool FULL, class PT>
-__device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const PT *P,
-                                              const KernelCtx &ctx, const uint64_t (&pid)[PPL],
-                                              uint64_t turn, int eidx) {
-  Part keep[PPL];
-  bool live[PPL];
-#pragma unroll
-  for (int k = 0; k < PPL; ++k) {
-    keep[k] = p[k];
-    live[k] = is_live(p[k]);
-  }
+__device__ __forceinline__ void element_body(Part (&p)[PPL], const Hdr h, const PT *P,
+                                             const KernelCtx &ctx, const uint64_t (&pid)[PPL],
+                                             uint64_t turn, int eidx) {
   switch (h.op) {
     case OP_DRIFT: {
       const double L = P[0];
@@ -343,6 +333,28 @@ __device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const
     default:
       break;
   }
+}
+
+// Lost particles (x NaN) stay frozen, as the reference skips them
+// (e.g. accel.cpp:166-167).  One particle per lane: the lane is masked off
+// (exec mask, scalar cost only).  Several per lane: all are computed, so the
+// PPL instruction streams stay interleaved, and the lost ones are restored.
+template <int PPL, bool FULL, class PT>
+__device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const PT *P,
+                                              const KernelCtx &ctx, const uint64_t (&pid)[PPL],
+                                              uint64_t turn, int eidx) {
+  if (PPL == 1) {
+    if (is_live(p[0])) element_body<PPL, FULL>(p, h, P, ctx, pid, turn, eidx);
+    return;
+  }
+  Part keep[PPL];
+  bool live[PPL];
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) {
+    keep[k] = p[k];
+    live[k] = is_live(p[k]);
+  }
+  element_body<PPL, FULL>(p, h, P, ctx, pid, turn, eidx);
 #pragma unroll
   for (int k = 0; k < PPL; ++k)
     if (!live[k]) p[k] = keep[k];
