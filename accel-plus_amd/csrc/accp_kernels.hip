@@ -336,17 +336,14 @@ __device__ __forceinline__ void element_body(Part (&p)[PPL], const Hdr h, const 
 }
 
 // Lost particles (x NaN) stay frozen, as the reference skips them
-// (e.g. accel.cpp:166-167).  One particle per lane: the lane is masked off
-// (exec mask, scalar cost only).  Several per lane: all are computed, so the
-// PPL instruction streams stay interleaved, and the lost ones are restored.
+// (e.g. accel.cpp:166-167): every lane computes (the PPL instruction streams
+// stay interleaved, no divergent branch per element) and lost particles get
+// their state back afterwards.  (An exec-masked branch per element measured
+// slower on the latency-bound tail of a dynamic-aperture run.)
 template <int PPL, bool FULL, class PT>
 __device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const PT *P,
                                               const KernelCtx &ctx, const uint64_t (&pid)[PPL],
                                               uint64_t turn, int eidx) {
-  if (PPL == 1) {
-    if (is_live(p[0])) element_body<PPL, FULL>(p, h, P, ctx, pid, turn, eidx);
-    return;
-  }
   Part keep[PPL];
   bool live[PPL];
 #pragma unroll

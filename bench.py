@@ -130,8 +130,9 @@ def main():
         tuning = abi.Tuning(int(bl), int(pp), {"scalar": 1, "lds": 2}[sg], 0)
     else:
         log("autotuning")
-        tuning, probe_ms = B.autotune(d_r.ptr, offs, 8)
-        log(f"autotune: {tuning} ({probe_ms:.2f} ms for 8 turns)")
+        # the probe spans the dense first turns and the sparse tail
+        tuning, probe_ms = B.autotune(d_r.ptr, offs, 64)
+        log(f"autotune: {tuning} ({probe_ms:.2f} ms for 64 turns)")
 
     B.capture(d_r.ptr, offs, nturn, tuning=tuning, d_loss_turn=d_lt.ptr)
 
