@@ -1,0 +1,207 @@
+// accp_adjoint.hip -- reverse sweep of the turn map on gfx950.
+//
+// One particle per lane walks the turns backwards.  Inside a turn it walks
+// the elements backwards: each element's input state is rebuilt from its
+// output with the closed-form inverse map (accp_adjoint.h) while the adjoint
+// dL/d(state) and the knob gradients are carried back through the element's
+// vector-Jacobian product.  At every turn boundary the state is reset to the
+// exact turn-start state recomputed from the checkpoints, so inversion
+// round-off never accumulates beyond one turn.
+#include <hip/hip_runtime.h>
+
+#include "accp_adjoint.h"
+#include "accp_adjoint_kernels.h"
+
+namespace accp {
+
+template <int MO, class PT>
+__device__ __forceinline__ void bendmp_rev(Part &p, Adj &l, const Hdr h, const PT *P) {
+  const int ns = h.a;
+  const double psi1 = P[BM_PSI1], psi2 = P[BM_PSI2], h0 = P[BM_IRHO];
+  const double l1 = P[BM_L1], l2 = P[BM_L2], k1 = P[BM_K1], k2 = P[BM_K2];
+  const double hs1 = P[BM_HS1], hq1 = P[BM_HQ1], hs2 = P[BM_HS2], hq2 = P[BM_HQ2];
+  double b[MO + 1];
+#pragma unroll
+  for (int i = 0; i <= MO; ++i) b[i] = P[BM_B + i];
+  const double c4a = hs1 * 0.5, c4b = hs2 * 0.5, lh1 = k1 * h0, lh2 = k2 * h0;
+  if (psi2 != 0) edge_rev(p, l, psi2);
+  const HxC ca = hx_consts(p.ipd, l1, hs1, hq1), cb = hx_consts(p.ipd, l2, hs2, hq2);
+  const double lhd1 = lh1 * p.d, lhd2 = lh2 * p.d;
+  for (int m = 0; m < ns; ++m) {
+    hxdrift_c_rev(p, l, ca, h0, c4a);
+    bend_kick_rev<MO>(p, l, b, k1, h0, lhd1, lh1);
+    hxdrift_c_rev(p, l, cb, h0, c4b);
+    bend_kick_rev<MO>(p, l, b, k2, h0, lhd2, lh2);
+    hxdrift_c_rev(p, l, cb, h0, c4b);
+    bend_kick_rev<MO>(p, l, b, k1, h0, lhd1, lh1);
+    hxdrift_c_rev(p, l, ca, h0, c4a);
+  }
+  if (psi1 != 0) edge_rev(p, l, psi1);
+}
+
+template <int HI, bool THICK, class PT>
+__device__ __forceinline__ void strmp_rev(Part &p, Adj &l, const Hdr h, const PT *P, const Knob &kn,
+                                          double *grad) {
+  const PT *C = THICK ? P + 4 : P;
+  double b[HI + 1], a[HI + 1];
+#pragma unroll
+  for (int i = 0; i <= HI; ++i) {
+    b[i] = C[i];
+    a[i] = C[HI + 1 + i];
+  }
+  if (!THICK) {
+    mpole_kick_rev<HI>(p, l, b, a, 1.0, kn, grad);
+    return;
+  }
+  const int ns = h.a;
+  const double l1 = P[0], l2 = P[1], k1 = P[2], k2 = P[3];
+  const double n1 = l1 * p.ipd, n2 = l2 * p.ipd;
+  const double z1 = n1 * (0.5 * p.ipd), z2 = n2 * (0.5 * p.ipd);
+  for (int m = 0; m < ns; ++m) {
+    drift_c_rev(p, l, n1, z1);
+    mpole_kick_rev<HI>(p, l, b, a, k1, kn, grad);
+    drift_c_rev(p, l, n2, z2);
+    mpole_kick_rev<HI>(p, l, b, a, k2, kn, grad);
+    drift_c_rev(p, l, n2, z2);
+    mpole_kick_rev<HI>(p, l, b, a, k1, kn, grad);
+    drift_c_rev(p, l, n1, z1);
+  }
+}
+
+// StrMPole with no field: drifts, plus the knob derivative if a knob sits on it.
+template <class PT>
+__device__ __forceinline__ void strmp_empty_rev(Part &p, Adj &l, const Hdr h, const PT *P, bool thick,
+                                                const Knob &kn, double *grad) {
+  if (!thick) {
+    mpole_knob_only(p, l, 1.0, kn, grad);
+    return;
+  }
+  const int ns = h.a;
+  const double l1 = P[0], l2 = P[1], k1 = P[2], k2 = P[3];
+  const double n1 = l1 * p.ipd, n2 = l2 * p.ipd;
+  const double z1 = n1 * (0.5 * p.ipd), z2 = n2 * (0.5 * p.ipd);
+  for (int m = 0; m < ns; ++m) {
+    drift_c_rev(p, l, n1, z1);
+    mpole_knob_only(p, l, k1, kn, grad);
+    drift_c_rev(p, l, n2, z2);
+    mpole_knob_only(p, l, k2, kn, grad);
+    drift_c_rev(p, l, n2, z2);
+    mpole_knob_only(p, l, k1, kn, grad);
+    drift_c_rev(p, l, n1, z1);
+  }
+}
+
+template <bool THICK, class PT>
+__device__ __forceinline__ void strmp_rev_dispatch(Part &p, Adj &l, const Hdr h, const PT *P,
+                                                   const Knob &kn, double *grad) {
+  switch (h.b) {
+    case -1: strmp_empty_rev(p, l, h, P, THICK, kn, grad); break;
+    case 0: strmp_rev<0, THICK>(p, l, h, P, kn, grad); break;
+    case 1: strmp_rev<1, THICK>(p, l, h, P, kn, grad); break;
+    case 2: strmp_rev<2, THICK>(p, l, h, P, kn, grad); break;
+    case 3: strmp_rev<3, THICK>(p, l, h, P, kn, grad); break;
+    case 4: strmp_rev<4, THICK>(p, l, h, P, kn, grad); break;
+    case 5: strmp_rev<5, THICK>(p, l, h, P, kn, grad); break;
+    case 6: strmp_rev<6, THICK>(p, l, h, P, kn, grad); break;
+    case 7: strmp_rev<7, THICK>(p, l, h, P, kn, grad); break;
+    case 8: strmp_rev<8, THICK>(p, l, h, P, kn, grad); break;
+    default: strmp_rev<9, THICK>(p, l, h, P, kn, grad); break;
+  }
+}
+
+template <class PT>
+__device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const PT *P, const Knob &kn,
+                                            double *grad) {
+  switch (h.op) {
+    case OP_DRIFT:
+      drift_nl_rev(p, l, P[0]);
+      break;
+    case OP_QUAD: {
+      const double L = P[0], K = P[1], aK = P[2], L2 = P[3];
+      const bool foc = K > 0;
+      const double sK = foc ? 1.0 : -1.0;
+      switch (h.a) {
+        case 4: quad_series_rev<4>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
+        case 6: quad_series_rev<6>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
+        case 8: quad_series_rev<8>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
+        case 10: quad_series_rev<10>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
+        case 12: quad_series_rev<12>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
+        case 14: quad_series_rev<14>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
+        default: quad_series_rev<16>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
+      }
+      break;
+    }
+    case OP_BENDMP:
+      switch (h.b & 0xff) {
+        case 0: bendmp_rev<0>(p, l, h, P); break;
+        case 1: bendmp_rev<1>(p, l, h, P); break;
+        case 2: bendmp_rev<2>(p, l, h, P); break;
+        default: bendmp_rev<3>(p, l, h, P); break;
+      }
+      break;
+    case OP_STRMP:
+      strmp_rev_dispatch<true>(p, l, h, P, kn, grad);
+      break;
+    case OP_STRMP_THIN:
+      strmp_rev_dispatch<false>(p, l, h, P, kn, grad);
+      break;
+    case OP_CAVITY:
+      cavity_rev(p, l, P[0], P[1], P[2], P[3], h.a != 0);
+      break;
+    case OP_APERTURE:
+      // a particle that reaches the end alive passed every aperture: the
+      // element acts on it as its drift (or the identity for len = 0)
+      if (P[0] != 0) drift_nl_rev(p, l, P[0]);
+      break;
+    case OP_CORRECTOR:
+      corrector_rev(p, l, P[0], P[1], P[2]);
+      break;
+    default:
+      break;
+  }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_adj_reverse(AdjCtx c) {
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= c.n) return;
+  double *pe = c.r_end + 6 * i;
+  double *pl = c.lam + 6 * i;
+  Part p;
+  p.x = pe[0]; p.px = pe[1]; p.y = pe[2]; p.py = pe[3]; p.z = pe[4];
+  set_delta(p, pe[5]);
+  Adj l{pl[0], pl[1], pl[2], pl[3], pl[4], pl[5]};
+  double grad[ADJ_MAX_PARAM];
+#pragma unroll
+  for (int k = 0; k < ADJ_MAX_PARAM; ++k) grad[k] = 0.0;
+  if (p.x != p.x) {
+    // lost particles are left out of the gradient
+    for (int k = 0; k < 6; ++k) pl[k] = 0.0;
+    return;
+  }
+  for (int t = c.t_hi - 1; t >= c.t_lo; --t) {
+    for (int e = c.ed - 1; e >= c.st; --e) {
+      const Hdr h = c.hdr[e];
+      const Knob kn{c.knob_slot[e], c.knob_kind[e] & 0xff, c.knob_kind[e] >> 8};
+      element_rev(p, l, h, c.par + h.off, kn, grad);
+    }
+    // exact state at the start of turn t
+    const int seg = t / c.ckpt_every, j = t % c.ckpt_every;
+    const double *s = c.states + ((int64_t)j * c.group_n + (int64_t)(seg - c.seg0) * c.n + i) * 6;
+    p.x = s[0]; p.px = s[1]; p.y = s[2]; p.py = s[3]; p.z = s[4];
+    set_delta(p, s[5]);
+  }
+  pe[0] = p.x; pe[1] = p.px; pe[2] = p.y; pe[3] = p.py; pe[4] = p.z; pe[5] = p.d;
+  pl[0] = l.x; pl[1] = l.px; pl[2] = l.y; pl[3] = l.py; pl[4] = l.z; pl[5] = l.d;
+  double *g = c.grad + i * c.nparam;
+  for (int k = 0; k < c.nparam; ++k) g[k] += grad[k];
+}
+
+hipError_t launch_adj_reverse(const AdjCtx &c, hipStream_t s) {
+  constexpr int B = 64;
+  const int64_t grid = (c.n + B - 1) / B;
+  hipLaunchKernelGGL(k_adj_reverse<B>, dim3((unsigned)grid), dim3(B), 0, s, c);
+  return hipGetLastError();
+}
+
+}  // namespace accp

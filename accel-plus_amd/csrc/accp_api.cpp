@@ -15,7 +15,7 @@
 #include <vector>
 
 #include "../../include/accp.h"
-#include "accp_adjoint.h"
+#include "accp_adjoint_kernels.h"
 #include "accp_kernels.h"
 #include "accp_table.h"
 #include "accp_wake.h"
@@ -67,6 +67,18 @@ double inv_factorial(int j) {
   long double f = 1.0L;
   for (int i = 2; i <= j; ++i) f *= (long double)i;
   return (double)(1.0L / f);
+}
+
+// Series coefficients of a quad record: 1/(4m)!, 1/(4m+2)!, 1/(4m+1)!,
+// 1/(4m+3)! (cos/cosh, sin/sinh), then (2m+1)/(4m+3)!, (2m+2)/(4m+5)! (the
+// derivative series the adjoint uses), m = 0..NT/2-1.
+void append_quad_series(std::vector<double> &P, int nt) {
+  const int M = nt / 2;
+  const int shift[4] = {0, 2, 1, 3};
+  for (int blk = 0; blk < 4; ++blk)
+    for (int m = 0; m < M; ++m) P.push_back(inv_factorial(4 * m + shift[blk]));
+  for (int m = 0; m < M; ++m) P.push_back((2.0 * m + 1) * inv_factorial(4 * m + 3));
+  for (int m = 0; m < M; ++m) P.push_back((2.0 * m + 2) * inv_factorial(4 * m + 5));
 }
 
 // Smallest supported term count NT with umax^NT/(2NT)! < 1e-18 (the
@@ -126,11 +138,7 @@ int compile_lattice(const accp_elem *E, int n, Compiled &out) {
           h.op = OP_QUAD;
           h.a = quad_series_terms(2.0 * e.len * e.len * std::fabs(e.k));
           P.insert(P.end(), {e.len, e.k, std::fabs(e.k), e.len * e.len});
-          const int M = h.a / 2;
-          for (int blk = 0; blk < 4; ++blk) {  // 1/(4m)!, 1/(4m+2)!, 1/(4m+1)!, 1/(4m+3)!
-            const int shift = blk == 0 ? 0 : blk == 1 ? 2 : blk == 2 ? 1 : 3;
-            for (int m = 0; m < M; ++m) P.push_back(inv_factorial(4 * m + shift));
-          }
+          append_quad_series(P, h.a);
         }
         break;
       case ACCP_CORRECTOR:
@@ -806,5 +814,195 @@ extern "C" int accp_batch_free(accp_batch *b) {
   b->d_scratch.release();
   b->ws.release();
   delete b;
+  return ACCP_OK;
+}
+
+// ---- reverse-mode adjoint of the turn map ------------------------------------------
+struct accp_adjoint {
+  accp_lattice *L = nullptr;
+  int nparam = 0;
+  std::vector<int32_t> knob_slot, knob_kind;
+  DevBuf d_knob_slot, d_knob_kind;
+  int n = 0, nturn = 0, ckpt = 1, nseg = 0;
+  bool forward_done = false;
+  DevBuf d_r, d_ckpt, d_rec, d_states, d_end, d_lam, d_grad;
+};
+
+// Element maps the reverse sweep carries (the others make create fail).
+static bool adjoint_supports(const Hdr &h) {
+  switch (h.op) {
+    case OP_NOP: case OP_DRIFT: case OP_STRMP: case OP_STRMP_THIN:
+    case OP_CAVITY: case OP_APERTURE: case OP_CORRECTOR:
+      return true;
+    case OP_QUAD:
+      return h.a > 0;
+    case OP_BENDMP:
+      return ((h.b >> 8) & 1) && (h.b & 0xff) <= 3;
+    default:
+      return false;
+  }
+}
+
+extern "C" int accp_adjoint_create(accp_lattice *L, int nparam, const accp_param *params,
+                                   accp_adjoint **out) {
+  if (!L || !out || nparam < 0 || (nparam > 0 && !params))
+    return fail(ACCP_E_ARG, "accp_adjoint_create: bad arguments");
+  *out = nullptr;
+  if (nparam > ADJ_MAX_PARAM) return fail(ACCP_E_ARG, "at most %d knobs", ADJ_MAX_PARAM);
+  const int ne = (int)L->elems.size();
+  for (int e = 0; e < ne; ++e)
+    if (!adjoint_supports(L->tab.hdr[e]))
+      return fail(ACCP_E_LATTICE,
+                  "element %d (type %d): no reverse map (radiation, wake, linear Bend, K=0 or "
+                  "out-of-range Quad, combined-function or high-order BendMPole)",
+                  e, L->elems[e].type);
+  std::vector<int32_t> slot(ne, -1), kind(ne, 0);
+  for (int k = 0; k < nparam; ++k) {
+    const accp_param &pr = params[k];
+    if (pr.nelem < 0 || (pr.nelem > 0 && !pr.elems)) return fail(ACCP_E_ARG, "knob %d: bad member list", k);
+    for (int m = 0; m < pr.nelem; ++m) {
+      const int e = pr.elems[m];
+      if (e < 0 || e >= ne) return fail(ACCP_E_ARG, "knob %d: element %d out of range", k, e);
+      if (slot[e] >= 0) return fail(ACCP_E_ARG, "element %d belongs to two knobs", e);
+      const int t = L->elems[e].type;
+      if (pr.field == ACCP_FIELD_K) {
+        if (t != ACCP_QUAD) return fail(ACCP_E_ARG, "knob %d: K needs Quad elements (element %d)", k, e);
+        kind[e] = 1;
+      } else if (pr.field == ACCP_FIELD_POLYB || pr.field == ACCP_FIELD_POLYA) {
+        if (t != ACCP_STRMPOLE) return fail(ACCP_E_ARG, "knob %d: PolyB/PolyA need StrMPole (element %d)", k, e);
+        if (pr.index < 0 || pr.index >= ACCP_MAX_MPOLE) return fail(ACCP_E_ARG, "knob %d: multipole index", k);
+        kind[e] = (pr.field == ACCP_FIELD_POLYB ? 2 : 3) | (pr.index << 8);
+      } else {
+        return fail(ACCP_E_ARG, "knob %d: unknown field %d", k, pr.field);
+      }
+      slot[e] = k;
+    }
+  }
+  accp_adjoint *A = new accp_adjoint;
+  A->L = L;
+  A->nparam = nparam;
+  A->knob_slot = slot;
+  A->knob_kind = kind;
+  int rc = A->d_knob_slot.ensure(ne * 4);
+  if (!rc) rc = A->d_knob_kind.ensure(ne * 4);
+  if (!rc && (hipMemcpy(A->d_knob_slot.p, slot.data(), ne * 4, hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(A->d_knob_kind.p, kind.data(), ne * 4, hipMemcpyHostToDevice) != hipSuccess))
+    rc = fail(ACCP_E_HIP, "knob table upload failed");
+  if (rc) {
+    accp_adjoint_free(A);
+    return rc;
+  }
+  *out = A;
+  return ACCP_OK;
+}
+
+extern "C" int accp_adjoint_forward(accp_adjoint *A, const double *r0, int n, int nturn, int ckpt_every,
+                                    double *r_out) {
+  if (!A || !r0 || n <= 0 || nturn <= 0 || ckpt_every <= 0)
+    return fail(ACCP_E_ARG, "accp_adjoint_forward: bad arguments");
+  accp_lattice *L = A->L;
+  const size_t bytes = (size_t)n * 6 * sizeof(double);
+  A->forward_done = false;
+  A->n = n;
+  A->nturn = nturn;
+  A->ckpt = ckpt_every;
+  A->nseg = (nturn + ckpt_every - 1) / ckpt_every;
+  int rc;
+  if ((rc = A->d_r.ensure(bytes)) || (rc = A->d_ckpt.ensure(bytes * A->nseg))) return rc;
+  HIPCHK(hipMemcpy(A->d_r.p, r0, bytes, hipMemcpyHostToDevice));
+  RunSpec R;
+  R.nturn = nturn;
+  R.st = 0;
+  R.ed = (int)L->elems.size();
+  R.snap = (double *)A->d_ckpt.p;  // state at turns 0, C, 2C, ...
+  R.nsnap = A->nseg;
+  R.mon_interval = ckpt_every;
+  if ((rc = track_device_impl(L, (double *)A->d_r.p, n, R, nullptr, nullptr))) return rc;
+  if (r_out) HIPCHK(hipMemcpy(r_out, A->d_r.p, bytes, hipMemcpyDeviceToHost));
+  HIPCHK(hipDeviceSynchronize());
+  A->forward_done = true;
+  return ACCP_OK;
+}
+
+extern "C" int accp_adjoint_backward(accp_adjoint *A, const double *dl_dr_out, double *dl_dparam,
+                                     double *dl_dr0) {
+  if (!A || !dl_dr_out) return fail(ACCP_E_ARG, "accp_adjoint_backward: bad arguments");
+  if (!A->forward_done) return fail(ACCP_E_STATE, "accp_adjoint_backward before accp_adjoint_forward");
+  accp_lattice *L = A->L;
+  const int n = A->n, C = A->ckpt, nseg = A->nseg, np = A->nparam;
+  const size_t bytes = (size_t)n * 6 * sizeof(double);
+  // segments recomputed together: all of them unless that exceeds 2 GiB
+  const size_t per_seg = (size_t)C * bytes;
+  const int G = (int)std::max<size_t>(1, std::min<size_t>(nseg, ((size_t)2 << 30) / per_seg));
+  int rc;
+  if ((rc = A->d_end.ensure(bytes)) || (rc = A->d_lam.ensure(bytes)) ||
+      (rc = A->d_grad.ensure(std::max<size_t>((size_t)n * np, 1) * sizeof(double))) ||
+      (rc = A->d_rec.ensure(bytes * G)) || (rc = A->d_states.ensure(per_seg * G)))
+    return rc;
+  HIPCHK(hipMemcpy(A->d_lam.p, dl_dr_out, bytes, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(A->d_end.p, A->d_r.p, bytes, hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemset(A->d_grad.p, 0, std::max<size_t>((size_t)n * np, 1) * sizeof(double)));
+  const int ne = (int)L->elems.size();
+  for (int k1 = nseg; k1 > 0;) {
+    const int k0 = std::max(0, k1 - G), g = k1 - k0;
+    // recompute every turn-start state of segments [k0, k1) in one launch:
+    // segment checkpoints side by side form one beam of g*n particles
+    HIPCHK(hipMemcpy(A->d_rec.p, (const char *)A->d_ckpt.p + (size_t)k0 * bytes, (size_t)g * bytes,
+                     hipMemcpyDeviceToDevice));
+    RunSpec R;
+    R.nturn = C;
+    R.st = 0;
+    R.ed = ne;
+    R.snap = (double *)A->d_states.p;
+    R.nsnap = C;
+    R.mon_interval = 1;
+    if ((rc = track_device_impl(L, (double *)A->d_rec.p, (int64_t)g * n, R, nullptr, nullptr))) return rc;
+    AdjCtx c{};
+    c.hdr = (const Hdr *)L->d_hdr.p;
+    c.par = (const double *)L->d_par.p;
+    c.knob_slot = (const int32_t *)A->d_knob_slot.p;
+    c.knob_kind = (const int32_t *)A->d_knob_kind.p;
+    c.st = 0;
+    c.ed = ne;
+    c.t_lo = k0 * C;
+    c.t_hi = std::min(k1 * C, A->nturn);
+    c.ckpt_every = C;
+    c.seg0 = k0;
+    c.n = n;
+    c.group_n = (int64_t)g * n;
+    c.states = (const double *)A->d_states.p;
+    c.r_end = (double *)A->d_end.p;
+    c.lam = (double *)A->d_lam.p;
+    c.grad = (double *)A->d_grad.p;
+    c.nparam = np;
+    HIPCHK(launch_adj_reverse(c, nullptr));
+    k1 = k0;
+  }
+  HIPCHK(hipDeviceSynchronize());
+  if (dl_dr0) HIPCHK(hipMemcpy(dl_dr0, A->d_lam.p, bytes, hipMemcpyDeviceToHost));
+  if (dl_dparam && np > 0) {
+    std::vector<double> g((size_t)n * np);
+    HIPCHK(hipMemcpy(g.data(), A->d_grad.p, g.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < np; ++k) {
+      double s = 0;
+      for (int i = 0; i < n; ++i) s += g[(size_t)i * np + k];  // fixed order
+      dl_dparam[k] = s;
+    }
+  }
+  return ACCP_OK;
+}
+
+extern "C" int accp_adjoint_free(accp_adjoint *A) {
+  if (!A) return ACCP_OK;
+  A->d_knob_slot.release();
+  A->d_knob_kind.release();
+  A->d_r.release();
+  A->d_ckpt.release();
+  A->d_rec.release();
+  A->d_states.release();
+  A->d_end.release();
+  A->d_lam.release();
+  A->d_grad.release();
+  delete A;
   return ACCP_OK;
 }

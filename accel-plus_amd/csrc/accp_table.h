@@ -16,7 +16,7 @@ namespace accp {
 enum Op : int32_t {
   OP_NOP = 0,     // Monitor, disabled wake
   OP_DRIFT,       // [L]
-  OP_QUAD,        // [L, K, |K|, L^2, 2*NT series coefs]   a=NT series terms (0: trig)
+  OP_QUAD,        // [L, K, |K|, L^2, 3*NT series coefs]   a=NT series terms (0: trig)
   OP_QUAD_K0,     // [L]              (quad6 with K == 0: the reference's drift())
   OP_BEND,        // [L, angle/L, K, irho, psi1, psi2]
   OP_BENDMP,      // BM_* constants then B0..Bmo             a=nslice b=mo | plain<<8

@@ -243,3 +243,47 @@ def load(path=None):
     if _LIB is None:
         _LIB = Library()
     return _LIB
+
+
+class Adjoint:
+    """accp_adjoint_* : gradient of a loss on the final coordinates with
+    respect to family knobs.  params: list of (field, index, element list),
+    field one of elements.FIELD_K / FIELD_POLYB / FIELD_POLYA."""
+
+    def __init__(self, lib, lattice, params):
+        self.L = lib
+        self.lattice = lattice
+        self._members = [np.ascontiguousarray(m, dtype=np.int32) for _, _, m in params]
+        arr = (Param * max(len(params), 1))()
+        for k, (field, index, _) in enumerate(params):
+            arr[k] = Param(field, index, len(self._members[k]),
+                           self._members[k].ctypes.data_as(POINTER(c_int32)))
+        h = c_void_p()
+        lib.check(lib.lib.accp_adjoint_create(lattice.h, len(params), arr, byref(h)), "accp_adjoint_create")
+        self.h = h
+        self.nparam = len(params)
+
+    def forward(self, r0, nturn, ckpt_every=16):
+        r0 = np.ascontiguousarray(r0, dtype=np.float64)
+        out = np.empty_like(r0)
+        self.L.check(self.L.lib.accp_adjoint_forward(self.h, _dp(r0), r0.size // 6, nturn, ckpt_every, _dp(out)),
+                     "accp_adjoint_forward")
+        return out.reshape(-1, 6)
+
+    def backward(self, dl_dr_out):
+        g = np.ascontiguousarray(dl_dr_out, dtype=np.float64)
+        grad = np.zeros(max(self.nparam, 1))
+        dr0 = np.empty_like(g)
+        self.L.check(self.L.lib.accp_adjoint_backward(self.h, _dp(g), _dp(grad), _dp(dr0)), "accp_adjoint_backward")
+        return grad[: self.nparam], dr0.reshape(-1, 6)
+
+    def free(self):
+        if self.h:
+            self.L.lib.accp_adjoint_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

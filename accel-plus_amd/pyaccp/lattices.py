@@ -163,3 +163,30 @@ def batched_workload(nseed=64):
         beams.append(g)
         offs.append(offs[-1] + len(g))
     return lats, np.ascontiguousarray(np.concatenate(beams)), np.asarray(offs, dtype=np.int64)
+
+
+def adjoint_knobs(elems):
+    """The 8 family knobs of the adjoint workload: QF, QD K; SF, SD
+    PolyB[2]; QM1..QM4 K.  Returns [(name, field, index, members)]."""
+    out = []
+    for name, field, idx in (("QF", el.FIELD_K, 0), ("QD", el.FIELD_K, 0), ("SF", el.FIELD_POLYB, 2),
+                             ("SD", el.FIELD_POLYB, 2), ("QM1", el.FIELD_K, 0), ("QM2", el.FIELD_K, 0),
+                             ("QM3", el.FIELD_K, 0), ("QM4", el.FIELD_K, 0)):
+        t = el.QUAD if field == el.FIELD_K else el.STRMPOLE
+        m = family(elems, name, t)
+        if m:
+            out.append((name, field, idx, m))
+    return out
+
+
+def shift_knob(elems, field, index, members, delta):
+    """Copy of elems with the knob moved by delta on every member."""
+    out = [e.copy() for e in elems]
+    for i in members:
+        if field == el.FIELD_K:
+            out[i].k += delta
+        elif field == el.FIELD_POLYB:
+            out[i].polyB[index] += delta
+        else:
+            out[i].polyA[index] += delta
+    return out

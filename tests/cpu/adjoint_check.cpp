@@ -1,0 +1,167 @@
+// CPU check of the reverse-mode primitives of the tracking kernels
+// (accel-plus_amd/csrc/accp_adjoint.h, compiled here for the host): for each
+// element map, (1) the inverse recovers the input state, (2) the
+// vector-Jacobian product equals central finite differences of the forward
+// map, including the knob derivatives.  Prints one line per check and exits
+// non-zero on the first failure.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+#include <random>
+#include <vector>
+
+#include "accp_adjoint.h"
+
+using namespace accp;
+
+static std::mt19937_64 rng(12345);
+static double U(double a, double b) { return std::uniform_real_distribution<double>(a, b)(rng); }
+
+static double inv_fact(int j) {
+  long double f = 1.0L;
+  for (int i = 2; i <= j; ++i) f *= i;
+  return (double)(1.0L / f);
+}
+static std::vector<double> quad_coefs(int nt) {
+  std::vector<double> P;
+  const int M = nt / 2, sh[4] = {0, 2, 1, 3};
+  for (int b = 0; b < 4; ++b)
+    for (int m = 0; m < M; ++m) P.push_back(inv_fact(4 * m + sh[b]));
+  for (int m = 0; m < M; ++m) P.push_back((2.0 * m + 1) * inv_fact(4 * m + 3));
+  for (int m = 0; m < M; ++m) P.push_back((2.0 * m + 2) * inv_fact(4 * m + 5));
+  return P;
+}
+
+static Part mk(const double v[6]) {
+  Part p{v[0], v[1], v[2], v[3], v[4], 0, 0};
+  set_delta(p, v[5]);
+  return p;
+}
+static void unpack(const Part &p, double v[6]) {
+  v[0] = p.x; v[1] = p.px; v[2] = p.y; v[3] = p.py; v[4] = p.z; v[5] = p.d;
+}
+
+static int failures = 0;
+
+// fwd(state, knob) -> state; rev(state_out, adj, grad) -> state_in, adj_in
+// knob_rel: relative finite-difference step on the knob (maps linear in
+// their knob take a large one, which is exact for them)
+static void check(const char *name, std::function<void(Part &, double)> fwd,
+                  std::function<void(Part &, Adj &, double *)> rev, double knob, bool has_knob,
+                  const double in[6], double knob_rel = 1e-7, double tol = 2e-7) {
+  // forward and inverse round trip
+  Part p = mk(in);
+  fwd(p, knob);
+  double out[6];
+  unpack(p, out);
+  const double lw[6] = {U(-1, 1), U(-1, 1), U(-1, 1), U(-1, 1), U(-1, 1), U(-1, 1)};
+  Adj l{lw[0], lw[1], lw[2], lw[3], lw[4], lw[5]};
+  double grad[4] = {0, 0, 0, 0};
+  rev(p, l, grad);
+  double back[6];
+  unpack(p, back);
+  double rt = 0;
+  for (int i = 0; i < 6; ++i) rt = std::fmax(rt, std::fabs(back[i] - in[i]) / (std::fabs(in[i]) + 1e-6));
+  // VJP against central differences
+  const double lin[6] = {l.x, l.px, l.y, l.py, l.z, l.d};
+  double worst = 0;
+  for (int i = 0; i <= 6; ++i) {
+    if (i == 6 && !has_knob) break;
+    const double hstep = i < 6 ? 1e-7 * (std::fabs(in[i]) + 1e-4) : knob_rel * (std::fabs(knob) + 1e-3);
+    double vp[6], vm[6];
+    for (int k = 0; k < 6; ++k) vp[k] = vm[k] = in[k];
+    double kp = knob, km = knob;
+    if (i < 6) { vp[i] += hstep; vm[i] -= hstep; } else { kp += hstep; km -= hstep; }
+    Part a = mk(vp), b = mk(vm);
+    fwd(a, kp);
+    fwd(b, km);
+    double oa[6], ob[6];
+    unpack(a, oa);
+    unpack(b, ob);
+    double fd = 0;
+    for (int k = 0; k < 6; ++k) fd += lw[k] * (oa[k] - ob[k]) / (2 * hstep);
+    const double ad = i < 6 ? lin[i] : grad[0];
+    const double err = std::fabs(fd - ad) / (std::fabs(fd) + std::fabs(ad) + 1e-9);
+    worst = std::fmax(worst, err);
+  }
+  const bool ok = rt < 1e-11 && worst < tol;
+  std::printf("%-28s roundtrip %.2e  vjp-vs-fd %.2e  %s\n", name, rt, worst, ok ? "ok" : "FAIL");
+  if (!ok) ++failures;
+}
+
+int main() {
+  for (int trial = 0; trial < 3; ++trial) {
+    const double d = trial == 0 ? -0.03 : trial == 1 ? 0.0 : 0.027;
+    const double s0[6] = {U(-3e-3, 3e-3), U(-3e-4, 3e-4), U(-2e-3, 2e-3), U(-2e-4, 2e-4), U(-1e-3, 1e-3), d};
+
+    check("drift", [](Part &p, double) { drift_nl(p, 0.7 * p.ipd); },
+          [](Part &p, Adj &l, double *) { drift_nl_rev(p, l, 0.7); }, 0, false, s0);
+
+    check("edge", [](Part &p, double) { edge(p, 0.013); }, [](Part &p, Adj &l, double *) { edge_rev(p, l, 0.013); },
+          0, false, s0);
+
+    {  // pseudo-drift of a bend
+      const double h = 0.157 / 1.5, L = 0.15 * DRIFT1, hs = h * L, hq = 1 / 4. * hs * L;
+      check("hxdrift", [=](Part &p, double) { hxdrift_c(p, hx_consts(p.ipd, L, hs, hq), h, hs * 0.5); },
+            [=](Part &p, Adj &l, double *) { hxdrift_c_rev(p, l, hx_consts(p.ipd, L, hs, hq), h, hs * 0.5); }, 0,
+            false, s0);
+    }
+    {  // bend kicks, dipole and with normal multipoles
+      const double h = 0.157 / 1.5, L = 0.15 * KICK1, lh = L * h;
+      const double b0[1] = {0.0};
+      check("bend kick MO=0", [=](Part &p, double) { bend_kick_c<0, true>(p, b0, L, h, lh * p.d, lh, 0, 0); },
+            [=](Part &p, Adj &l, double *) { bend_kick_rev<0>(p, l, b0, L, h, lh * p.d, lh); }, 0, false, s0);
+      const double b3[4] = {1e-3, 0.0, 5.0, 20.0};
+      check("bend kick MO=3", [=](Part &p, double) { bend_kick_c<3, true>(p, b3, L, h, lh * p.d, lh, 0, 0); },
+            [=](Part &p, Adj &l, double *) { bend_kick_rev<3>(p, l, b3, L, h, lh * p.d, lh); }, 0, false, s0);
+    }
+    {  // multipole kicks with knobs on PolyB[2], PolyB[1], PolyA[3]
+      const int kinds[3][2] = {{2, 2}, {2, 1}, {3, 3}};
+      for (auto &kk : kinds) {
+        const int kind = kk[0], j = kk[1];
+        char nm[64];
+        std::snprintf(nm, sizeof nm, "mpole kick %s[%d]", kind == 2 ? "B" : "A", j);
+        check(nm,
+              [=](Part &p, double kv) {
+                double b[4] = {1e-4, 0.5, -8.0, 30.0}, a[4] = {2e-4, 0.1, 3.0, -10.0};
+                (kind == 2 ? b : a)[j] = kv;
+                double re, im;
+                horner_reg<3>(b, a, p.x, p.y, re, im);
+                p.px -= 0.03 * re;
+                p.py += 0.03 * im;
+              },
+              [=](Part &p, Adj &l, double *g) {
+                double b[4] = {1e-4, 0.5, -8.0, 30.0}, a[4] = {2e-4, 0.1, 3.0, -10.0};
+                (kind == 2 ? b : a)[j] = 7.5;
+                mpole_kick_rev<3>(p, l, b, a, 0.03, Knob{0, kind, j}, g);
+              },
+              7.5, true, s0, 0.1);
+      }
+    }
+    {  // series quads, focusing and defocusing, K knob
+      const std::vector<double> C = quad_coefs(8);
+      for (double K : {1.2, -1.1}) {
+        const double L = 0.3;
+        check(K > 0 ? "quad series K>0" : "quad series K<0",
+              [=](Part &p, double kv) { quad_series<8>(p, C.data(), L, L * L, kv > 0, std::fabs(kv)); },
+              [=](Part &p, Adj &l, double *g) {
+                quad_series_rev<8>(p, l, C.data(), L, L * L, K > 0, std::fabs(K), K > 0 ? 1.0 : -1.0,
+                                   Knob{0, 1, 0}, g);
+              },
+              K, true, s0);
+      }
+    }
+    {
+      const double w = TWOPI * 562e6 / C0;
+      check("cavity thin", [=](Part &p, double) { cavity(p, 0.0, 5e-4, w, 0.1, true); },
+            [=](Part &p, Adj &l, double *) { cavity_rev(p, l, 0.0, 5e-4, w, 0.1, true); }, 0, false, s0);
+      check("cavity thick", [=](Part &p, double) { cavity(p, 0.4, 5e-4, w, 0.1, true); },
+            [=](Part &p, Adj &l, double *) { cavity_rev(p, l, 0.4, 5e-4, w, 0.1, true); }, 0, false, s0);
+      check("corrector", [=](Part &p, double) { corrector(p, 0.2, 1e-4, -2e-4); },
+            [=](Part &p, Adj &l, double *) { corrector_rev(p, l, 0.2, 1e-4, -2e-4); }, 0, false, s0);
+    }
+  }
+  std::printf("%s\n", failures ? "FAILED" : "all adjoint primitives ok");
+  return failures ? 1 : 0;
+}
