@@ -85,6 +85,80 @@ def cpu_baseline(lats, beam, offs, nturn, budget_s=20.0):
     }
 
 
+def adjoint_leg(lib, stream):
+    """Adjoint gradient workload: loss = sum of squared final coordinates of
+    4096 particles after 512 turns of the DA lattice with 20 extra quads
+    (radiation off, cavity on); gradient w.r.t. 8 family knobs, checkpoints
+    every 16 turns; checked against central differences (h = 1e-7 relative)
+    computed with the same GPU forward map."""
+    import numpy as np
+
+    import pyaccp.abi as abi
+    import pyaccp.elements as el
+    import pyaccp.lattices as lat
+    from pyaccp.hipmem import DeviceArray, Event, synchronize
+
+    ring = lat.da_ring(0, extra_quads=True)
+    lat.set_rad(ring, 0)
+    lat.set_cavity(ring, 1)
+    knobs = lat.adjoint_knobs(ring)
+    beam = lat.gaussian_beam(4096, seed=1)
+    nturn = 512
+    L = lib.lattice(ring)
+    d0 = DeviceArray(beam)
+    d = DeviceArray(beam)
+    e0, e1 = Event(), Event()
+
+    def forward_ms(lattice):
+        d.copy_from(d0, stream.handle)
+        e0.record(stream.handle)
+        lattice.track_device(d.ptr, len(beam), nturn, stream=stream.handle)
+        e1.record(stream.handle)
+        return e0.elapsed_ms(e1)
+
+    forward_ms(L)
+    fwd = min(forward_ms(L) for _ in range(3))
+    A = abi.Adjoint(lib, L, [(f, i, m) for _, f, i, m in knobs])
+    times = []
+    for _ in range(3):
+        synchronize()
+        t0 = time.perf_counter()
+        out = A.forward(beam, nturn, ckpt_every=16)
+        grad, _ = A.backward(2.0 * out)
+        synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    adj = min(times)
+    synchronize()
+    t0 = time.perf_counter()
+    fd = []
+    for name, f, i, m in knobs:
+        base = ring[m[0]].k if f == el.FIELD_K else ring[m[0]].polyB[i]
+        h = 1e-7 * max(abs(base), 1e-2)
+        vals = []
+        for sgn in (1, -1):
+            Lh = lib.lattice(lat.shift_knob(ring, f, i, m, sgn * h))
+            d.copy_from(d0, stream.handle)
+            Lh.track_device(d.ptr, len(beam), nturn, stream=stream.handle)
+            r = d.numpy()
+            vals.append(float(np.sum(r * r)))
+        fd.append((vals[0] - vals[1]) / (2 * h))
+    synchronize()
+    fd_ms = (time.perf_counter() - t0) * 1e3
+    rel = [abs(g - f) / abs(f) for g, f in zip(grad, fd)]
+    return {
+        "workload": f"4096 particles x {nturn} turns, {len(ring)} elements, {len(knobs)} knobs "
+                    "(QF,QD K; SF,SD PolyB[2]; QM1-4 K), checkpoints every 16 turns",
+        "forward_ms": round(fwd, 3),
+        "adjoint_ms": round(adj, 3),
+        "adjoint_over_forward": round(adj / fwd, 3),
+        "fd_central_16_forwards_ms": round(fd_ms, 3),
+        "fd_over_adjoint": round(fd_ms / adj, 3),
+        "max_rel_diff_vs_fd": max(rel),
+        "knobs": [k[0] for k in knobs],
+        "gradient": [float(g) for g in grad],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +169,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the per-seed plain comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--tune", default="", help="block,ppl,staging(scalar|lds) instead of autotuning")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this backend has no "
@@ -234,6 +309,10 @@ def main():
         "achieved_tflops": round(sat_fl / (sat_best * 1e-3) / 1e12, 3),
         "frac_of_fp64_peak": round(sat_fl / (sat_best * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
     }
+
+    if not args.no_adjoint:
+        log("adjoint leg")
+        result["adjoint"] = adjoint_leg(lib, stream)
 
     if not args.no_plain:
         log("plain per-seed GPU path")
