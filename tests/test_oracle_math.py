@@ -1,0 +1,110 @@
+"""Pins of the oracle beyond the reference's fixtures: closed forms,
+invariants and published known-answer vectors that a plausible mistake in
+the restatement (a dropped term, a sign, a transposed operand) would break.
+CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+import pyaccp.elements as el
+import pyaccp.lattices as lat
+
+
+def test_philox_known_answers(oracle):
+    # Random123 known-answer vectors for philox4x32-10 (Salmon et al., SC'11)
+    assert oracle.philox([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert oracle.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert oracle.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == \
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_gauss_stream_moments(oracle):
+    g = np.array([oracle.gauss(5, pid, 3, 7, 1) for pid in range(40000)])
+    assert abs(g.mean()) < 0.02 and abs(g.std() - 1) < 0.02
+    assert abs(np.mean(g ** 4) - 3) < 0.1
+    assert oracle.gauss(5, 1, 3, 7, 1) != oracle.gauss(5, 1, 3, 7, 2)
+    assert oracle.gauss(5, 1, 3, 7, 1) == oracle.gauss(5, 1, 3, 7, 1)
+
+
+def test_one_turn_matrix_is_symplectic(oracle):
+    J = np.zeros((6, 6))
+    for i in range(3):
+        J[2 * i, 2 * i + 1] = 1
+        J[2 * i + 1, 2 * i] = -1
+    for ring in (lat.fodo_ring(), lat.da_ring(0, extra_quads=True)):
+        lat.set_cavity(ring, 0)
+        m = oracle.getm66(ring)
+        assert np.abs(m.T @ J @ m - J).max() < 1e-6
+        assert abs(np.linalg.det(m) - 1) < 1e-6
+
+
+def test_strmpole_quad_converges_to_quad6(oracle):
+    """A StrMPole with PolyB[1] = K only is the 4th-order integration of the
+    linear quadrupole: at delta = 0 its transverse map approaches quad6's
+    as nslice^-4."""
+    K, L = 1.2, 0.3
+    P = np.array([[1e-3, 2e-4, -5e-4, 1e-4, 0, 0]])
+    ref = oracle.track([el.quad(L, K)], P, 1)
+    errs = []
+    for ns in (2, 4, 8):
+        out = oracle.track([el.strmpole(L, [0.0, K], nslice=ns)], P, 1)
+        errs.append(np.abs(out[0, :4] - ref[0, :4]).max())
+    assert errs[1] < errs[0] / 12 and errs[2] < errs[1] / 12
+
+
+def test_drift_and_cavity_closed_forms(oracle):
+    r = np.array([[1e-3, 2e-3, -1e-3, 5e-4, 1e-4, 0.01]])
+    out = oracle.track([el.drift(2.0)], r, 1)[0]
+    nl = 2.0 / 1.01
+    assert out[0] == pytest.approx(1e-3 + nl * 2e-3, rel=1e-15)
+    assert out[4] == pytest.approx(1e-4 - nl * (4e-6 + 2.5e-7) / (2 * 1.01), rel=1e-12)
+    cav = el.cavity(0.0, 1.5, 500e6, 0.3, 3000.0)
+    out = oracle.track([cav], r, 1)[0]
+    assert out[5] == pytest.approx(0.01 + 1.5 / 3000 * math.sin(2 * math.pi * 500e6 * 1e-4 / 2.99792458e8 + 0.3),
+                                   rel=1e-14)
+
+
+def test_radiation_loss_per_turn_matches_u0(oracle):
+    """On axis, the classical loss per turn through the bends is
+    U0 = Cgamma E^4 / rho (Cgamma = 8.846e-5 m/GeV^3) to the accuracy of
+    the integrator."""
+    ring = lat.fodo_ring()
+    lat.set_rad(ring, 1)
+    out = oracle.track(ring, np.zeros((1, 6)), 1)[0]
+    rho = 1.5 / (2 * math.pi / 40)
+    u0 = 8.846056192e-05 * 3.0 ** 4 / rho  # GeV
+    assert -out[5] * 3.0 == pytest.approx(u0, rel=2e-3)
+
+
+def test_wake_binned_vs_pairwise(oracle):
+    beam = lat.gaussian_beam(2000, seed=9)
+    e = el.longwake(5e3, 1.0, 10e9, 2e-9, 256, -0.015, 0.015, energy=3000.0)
+    binned = oracle.track([e], beam, 1)[:, 5] - beam[:, 5]
+    exact = oracle.wake_pairwise(e, beam)
+    scale = np.abs(exact).max()
+    assert scale > 0 and np.abs(binned - exact).max() < 0.05 * scale
+
+
+def test_resonator_closed_form(oracle):
+    rs, q, fr = 5e3, 1.0, 10e9
+    wr = 2 * math.pi * fr
+    alpha = wr / (2 * q)
+    assert oracle.wake_w(rs, q, fr, 0.0) == pytest.approx(alpha * rs, rel=1e-15)
+    assert oracle.wake_w(rs, q, fr, 1e-18) == pytest.approx(2 * alpha * rs, rel=1e-6)  # W(0+) = 2 W(0)
+    # W'' + 2 alpha W' + wr^2 W = 0 for tau > 0
+    t, h = 3.7e-11, 1e-15
+    w = [oracle.wake_w(rs, q, fr, t + k * h) for k in (-1, 0, 1)]
+    wp = (w[2] - w[0]) / (2 * h)
+    wpp = (w[2] - 2 * w[1] + w[0]) / h ** 2
+    assert abs(wpp + 2 * alpha * wp + wr * wr * w[1]) < 1e-4 * wr * wr * abs(w[1])
+
+
+def test_beam_stat_closed_form(oracle):
+    r = lat.gaussian_beam(500, seed=2)
+    r[7, 0] = np.nan
+    s = oracle.beam_stat(r)
+    live = r[~np.isnan(r[:, 0])]
+    assert s[1] == 499
+    np.testing.assert_allclose(s[2:8], live.mean(0), rtol=1e-12, atol=1e-22)
+    np.testing.assert_allclose(s[8:14], live.std(0), rtol=1e-12)
