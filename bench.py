@@ -123,7 +123,10 @@ def adjoint_leg(lib, stream):
         synchronize()
         t0 = time.perf_counter()
         out = A.forward(beam, nturn, ckpt_every=16)
-        grad, _ = A.backward(2.0 * out)
+        # loss = sum of squares over the particles that survive (lost ones
+        # carry no gradient)
+        alive = ~np.isnan(out[:, 0])
+        grad, _ = A.backward(np.where(alive[:, None], 2.0 * out, 0.0))
         synchronize()
         times.append((time.perf_counter() - t0) * 1e3)
     adj = min(times)
@@ -139,7 +142,7 @@ def adjoint_leg(lib, stream):
                 Lh = lib.lattice(lat.shift_knob(ring, f, i, m, sgn * h))
                 d.copy_from(d0, stream.handle)
                 Lh.track_device(d.ptr, len(beam), nturn, stream=stream.handle)
-                r = d.numpy()
+                r = d.numpy()[alive]
                 vals.append(float(np.sum(r * r)))
             out.append((vals[0] - vals[1]) / (2 * h))
         return out
@@ -165,6 +168,7 @@ def adjoint_leg(lib, stream):
         "max_rel_diff_vs_fd_h1e-7": max(rel),
         "rel_diff_vs_fd_h1e-7": [float(x) for x in rel],
         "max_rel_diff_vs_fd_h1e-5rel": max(rel_bal),
+        "survivors": int(alive.sum()),
         "knobs": [k[0] for k in knobs],
         "gradient": [float(g) for g in grad],
     }

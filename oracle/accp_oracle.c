@@ -473,7 +473,7 @@ void orc_line_pass(const accp_elem *elems, int nelem, double *r, int n, int st, 
 /* aline.cpp:313-329: per turn, one parallel region; thread t owns particles
  * [t*N/T, (t+1)*N/T) and pushes them through every element. */
 int orc_line_pass_omp(const accp_elem *elems, int nelem, double *r, int n, int nturn,
-                      int nthreads, uint64_t seed, uint64_t pid0) {
+                      int nthreads, uint64_t seed, uint64_t turn0, uint64_t pid0) {
   for (int k = 0; k < nelem; k++)
     if (elems[k].type == ACCP_LONGWAKE && elems[k].on) return -1;
 #ifdef _OPENMP
@@ -492,7 +492,7 @@ int orc_line_pass_omp(const accp_elem *elems, int nelem, double *r, int n, int n
 #endif
       int s = (int)((long long)tid * n / nth);
       int e = (int)((long long)(tid + 1) * n / nth);
-      orc_ctx ctx = {seed, (uint64_t)t, pid0 + (uint64_t)s, 0};
+      orc_ctx ctx = {seed, turn0 + (uint64_t)t, pid0 + (uint64_t)s, 0};
       for (int k = 0; k < nelem; k++) {
         ctx.eidx = k;
         orc_elem_pass(&elems[k], r + 6 * (size_t)s, e - s, &ctx);

@@ -61,7 +61,7 @@ void orc_line_pass(const accp_elem *elems, int nelem, double *r, int n, int st,
  * ALine::pass(r,N,st,ed) partitions them (aline.cpp:313-329).  Elements
  * that need the whole beam (LongWake) are rejected (returns -1). */
 int orc_line_pass_omp(const accp_elem *elems, int nelem, double *r, int n,
-                      int nturn, int nthreads, uint64_t seed, uint64_t pid0);
+                      int nturn, int nthreads, uint64_t seed, uint64_t turn0, uint64_t pid0);
 
 /* ---- optics by tracking (src/lib/accel/aline.cpp) ----------------------- */
 void orc_getm66(const accp_elem *elems, int nelem, int st, int ed, double dpp,

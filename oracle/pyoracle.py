@@ -36,7 +36,7 @@ class Oracle:
         L = ctypes.CDLL(path)
         E = POINTER(el.Elem)
         L.orc_line_pass.argtypes = [E, c_int, DP, c_int, c_int, c_int, c_int, c_uint64, c_uint64, c_uint64]
-        L.orc_line_pass_omp.argtypes = [E, c_int, DP, c_int, c_int, c_int, c_uint64, c_uint64]
+        L.orc_line_pass_omp.argtypes = [E, c_int, DP, c_int, c_int, c_int, c_uint64, c_uint64, c_uint64]
         L.orc_line_pass_omp.restype = c_int
         L.orc_elem_pass.argtypes = [E, DP, c_int, c_void_p]
         L.orc_getm66.argtypes = [E, c_int, c_int, c_int, c_double, c_double, DP]
@@ -70,10 +70,10 @@ class Oracle:
         self.lib.orc_line_pass(a, len(a), _dp(r), n, st, ed, nturn, seed, turn0, pid0)
         return r.reshape(-1, 6)
 
-    def track_omp(self, elems, r, nturn, nthreads, seed=0):
+    def track_omp(self, elems, r, nturn, nthreads, seed=0, turn0=0, pid0=0):
         a = self._arr(elems)
         r = np.ascontiguousarray(r, dtype=np.float64).copy()
-        rc = self.lib.orc_line_pass_omp(a, len(a), _dp(r), r.size // 6, nturn, nthreads, seed, 0)
+        rc = self.lib.orc_line_pass_omp(a, len(a), _dp(r), r.size // 6, nturn, nthreads, seed, turn0, pid0)
         if rc != 0:
             raise ValueError("collective element in an OpenMP pass")
         return r.reshape(-1, 6)
