@@ -161,8 +161,81 @@ __device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const 
   }
 }
 
-template <int BLOCK>
+template <bool LDS>
+__device__ __forceinline__ void load_rev(const Hdr *hdr, const int2 *knob, int e, Hdr &h, Knob &kn) {
+  Hdr v = hdr[e];
+  int2 k = knob[e];
+  if (LDS) {
+    v.op = __builtin_amdgcn_readfirstlane(v.op);
+    v.off = __builtin_amdgcn_readfirstlane(v.off);
+    v.a = __builtin_amdgcn_readfirstlane(v.a);
+    v.b = __builtin_amdgcn_readfirstlane(v.b);
+    k.x = __builtin_amdgcn_readfirstlane(k.x);
+    k.y = __builtin_amdgcn_readfirstlane(k.y);
+  }
+  h = v;
+  kn = Knob{k.x, k.y & 0xff, k.y >> 8};
+}
+
+// Element table pointers of a reverse kernel.  With LDS the table (headers,
+// knob slot/kind pairs, parameters) is copied into LDS once per workgroup.
+struct RevTable {
+  const Hdr *hdr;
+  const int2 *knob;
+  const double *par;
+};
+
+template <int BLOCK, bool LDS>
+__device__ __forceinline__ RevTable stage_table(const AdjCtx &c, unsigned char *lds_raw) {
+  RevTable T{c.hdr, reinterpret_cast<const int2 *>(c.knob), c.par};
+  if (LDS) {
+    Hdr *lh = reinterpret_cast<Hdr *>(lds_raw);
+    int2 *lk = reinterpret_cast<int2 *>(lds_raw + sizeof(Hdr) * c.ed);
+    double *lp = reinterpret_cast<double *>(lds_raw + (sizeof(Hdr) + sizeof(int2)) * c.ed);
+    for (int e = threadIdx.x; e < c.ed; e += BLOCK) {
+      lh[e] = T.hdr[e];
+      lk[e] = T.knob[e];
+    }
+    for (int k = threadIdx.x; k < c.npar; k += BLOCK) lp[k] = T.par[k];
+    __syncthreads();
+    T = RevTable{lh, lk, lp};
+  }
+  return T;
+}
+
+// Walks turns t_hi-1 .. t_lo of one particle backwards.  `q` indexes the
+// particle inside the recomputed group of segments; at the start of every
+// turn the state is reset to the exact recomputed one.  The next element's
+// header and knob are fetched while the current element runs.
+template <bool LDS>
+__device__ __forceinline__ void reverse_turns(const AdjCtx &c, const RevTable &T, int64_t q, int t_lo,
+                                              int t_hi, Part &p, Adj &l, double *grad) {
+  for (int t = t_hi - 1; t >= t_lo; --t) {
+    // exact state at the start of turn t, loaded early: nothing below
+    // depends on it until the turn is done
+    const int j = t % c.ckpt_every;
+    const double *s = c.states + ((int64_t)j * c.group_n + q) * 6;
+    const double s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3], s4 = s[4], s5 = s[5];
+    Hdr h;
+    Knob kn;
+    load_rev<LDS>(T.hdr, T.knob, c.ed - 1, h, kn);
+    for (int e = c.ed - 1; e >= c.st; --e) {
+      Hdr hn;
+      Knob kn_next;
+      load_rev<LDS>(T.hdr, T.knob, e > c.st ? e - 1 : e, hn, kn_next);
+      element_rev(p, l, h, T.par + h.off, kn, grad);
+      h = hn;
+      kn = kn_next;
+    }
+    p.x = s0; p.px = s1; p.y = s2; p.py = s3; p.z = s4;
+    set_delta(p, s5);
+  }
+}
+
+template <int BLOCK, bool LDS>
 __global__ __launch_bounds__(BLOCK) void k_adj_reverse(AdjCtx c) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const RevTable T = stage_table<BLOCK, LDS>(c, lds_raw);
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i >= c.n) return;
   double *pe = c.r_end + 6 * i;
@@ -179,17 +252,12 @@ __global__ __launch_bounds__(BLOCK) void k_adj_reverse(AdjCtx c) {
     for (int k = 0; k < 6; ++k) pl[k] = 0.0;
     return;
   }
-  for (int t = c.t_hi - 1; t >= c.t_lo; --t) {
-    for (int e = c.ed - 1; e >= c.st; --e) {
-      const Hdr h = c.hdr[e];
-      const Knob kn{c.knob_slot[e], c.knob_kind[e] & 0xff, c.knob_kind[e] >> 8};
-      element_rev(p, l, h, c.par + h.off, kn, grad);
-    }
-    // exact state at the start of turn t
-    const int seg = t / c.ckpt_every, j = t % c.ckpt_every;
-    const double *s = c.states + ((int64_t)j * c.group_n + (int64_t)(seg - c.seg0) * c.n + i) * 6;
-    p.x = s[0]; p.px = s[1]; p.y = s[2]; p.py = s[3]; p.z = s[4];
-    set_delta(p, s[5]);
+  // turns t_hi-1 .. t_lo, segment by segment
+  for (int t1 = c.t_hi; t1 > c.t_lo;) {
+    const int seg = (t1 - 1) / c.ckpt_every;
+    const int t0 = seg * c.ckpt_every > c.t_lo ? seg * c.ckpt_every : c.t_lo;
+    reverse_turns<LDS>(c, T, (int64_t)(seg - c.seg0) * c.n + i, t0, t1, p, l, grad);
+    t1 = t0;
   }
   pe[0] = p.x; pe[1] = p.px; pe[2] = p.y; pe[3] = p.py; pe[4] = p.z; pe[5] = p.d;
   pl[0] = l.x; pl[1] = l.px; pl[2] = l.y; pl[3] = l.py; pl[4] = l.z; pl[5] = l.d;
@@ -197,10 +265,87 @@ __global__ __launch_bounds__(BLOCK) void k_adj_reverse(AdjCtx c) {
   for (int k = 0; k < c.nparam; ++k) g[k] += grad[k];
 }
 
+template <int BLOCK, bool LDS>
+__global__ __launch_bounds__(BLOCK) void k_adj_segjac(AdjCtx c) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const RevTable T = stage_table<BLOCK, LDS>(c, lds_raw);
+  const int64_t gid = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (gid >= c.group_n * 6) return;
+  const int col = (int)(gid % 6);
+  const int64_t q = gid / 6;  // segment-in-group * n + particle
+  const int seg = c.seg0 + (int)(q / c.n);
+  const double *pe = c.r_end + 6 * q;
+  Part p;
+  p.x = pe[0]; p.px = pe[1]; p.y = pe[2]; p.py = pe[3]; p.z = pe[4];
+  set_delta(p, pe[5]);
+  // a particle lost by the end of its segment is lost at the end of the run
+  // and is never read back
+  if (p.x != p.x) return;
+  Adj l{col == 0 ? 1.0 : 0.0, col == 1 ? 1.0 : 0.0, col == 2 ? 1.0 : 0.0,
+        col == 3 ? 1.0 : 0.0, col == 4 ? 1.0 : 0.0, col == 5 ? 1.0 : 0.0};
+  double grad[ADJ_MAX_PARAM];
+#pragma unroll
+  for (int k = 0; k < ADJ_MAX_PARAM; ++k) grad[k] = 0.0;
+  const int t0 = seg * c.ckpt_every;
+  const int t1 = t0 + c.ckpt_every < c.nturn ? t0 + c.ckpt_every : c.nturn;
+  reverse_turns<LDS>(c, T, q, t0, t1, p, l, grad);
+  double *o = c.jac + (q * 6 + col) * (6 + c.nparam);
+  o[0] = l.x; o[1] = l.px; o[2] = l.y; o[3] = l.py; o[4] = l.z; o[5] = l.d;
+  for (int k = 0; k < c.nparam; ++k) o[6 + k] = grad[k];
+}
+
+__global__ __launch_bounds__(64) void k_adj_combine(AdjCtx c) {
+  const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (i >= c.n) return;
+  double *pl = c.lam + 6 * i;
+  if (c.r_final[6 * i] != c.r_final[6 * i]) {
+    for (int k = 0; k < 6; ++k) pl[k] = 0.0;
+    return;
+  }
+  const int w = 6 + c.nparam;
+  double lam[6] = {pl[0], pl[1], pl[2], pl[3], pl[4], pl[5]};
+  double *g = c.grad + i * c.nparam;
+  for (int s = c.nseg_g - 1; s >= 0; --s) {
+    const double *J = c.jac + ((int64_t)s * c.n + i) * 6 * w;
+    for (int k = 0; k < c.nparam; ++k) {
+      double acc = g[k];
+      for (int col = 0; col < 6; ++col) acc += lam[col] * J[col * w + 6 + k];
+      g[k] = acc;
+    }
+    double nl[6];
+    for (int r = 0; r < 6; ++r) {
+      double acc = 0.0;
+      for (int col = 0; col < 6; ++col) acc += lam[col] * J[col * w + r];
+      nl[r] = acc;
+    }
+    for (int r = 0; r < 6; ++r) lam[r] = nl[r];
+  }
+  for (int k = 0; k < 6; ++k) pl[k] = lam[k];
+}
+
 hipError_t launch_adj_reverse(const AdjCtx &c, hipStream_t s) {
   constexpr int B = 64;
   const int64_t grid = (c.n + B - 1) / B;
-  hipLaunchKernelGGL(k_adj_reverse<B>, dim3((unsigned)grid), dim3(B), 0, s, c);
+  if (c.lds_bytes > 0)
+    hipLaunchKernelGGL((k_adj_reverse<B, true>), dim3((unsigned)grid), dim3(B), c.lds_bytes, s, c);
+  else
+    hipLaunchKernelGGL((k_adj_reverse<B, false>), dim3((unsigned)grid), dim3(B), 0, s, c);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_segjac(const AdjCtx &c, hipStream_t s) {
+  constexpr int B = 64;
+  const int64_t grid = (c.group_n * 6 + B - 1) / B;
+  if (c.lds_bytes > 0)
+    hipLaunchKernelGGL((k_adj_segjac<B, true>), dim3((unsigned)grid), dim3(B), c.lds_bytes, s, c);
+  else
+    hipLaunchKernelGGL((k_adj_segjac<B, false>), dim3((unsigned)grid), dim3(B), 0, s, c);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_combine(const AdjCtx &c, hipStream_t s) {
+  const int64_t grid = (c.n + 63) / 64;
+  hipLaunchKernelGGL(k_adj_combine, dim3((unsigned)grid), dim3(64), 0, s, c);
   return hipGetLastError();
 }
 

@@ -12,8 +12,7 @@ constexpr int ADJ_MAX_PARAM = 16;
 struct AdjCtx {
   const Hdr *hdr;
   const double *par;
-  const int32_t *knob_slot;  // per element: gradient slot or -1
-  const int32_t *knob_kind;  // per element: kind | (multipole index << 8)
+  const int32_t *knob;       // per element: {gradient slot or -1, kind | (multipole index << 8)}
   int32_t st, ed;            // element range of one turn
   int32_t t_lo, t_hi;        // turns of this launch, walked from t_hi-1 down to t_lo
   int32_t ckpt_every;        // turns per checkpoint segment
@@ -21,12 +20,35 @@ struct AdjCtx {
   int64_t n;                 // particles
   int64_t group_n;           // particles x segments held in `states`
   const double *states;      // [turn in segment][segment][particle][6] turn-start states
-  double *r_end;             // in: state at the end of turn t_hi-1; out: at the start of t_lo
+  double *r_end;             // sequential: in state at the end of turn t_hi-1, out at the start of
+                             // t_lo; segment Jacobians: end state of each segment of the group
   double *lam;               // in/out: dL/d(state), AoS
   double *grad;              // per particle accumulated knob gradient [n][nparam]
   int32_t nparam;
+  int32_t npar;              // doubles in `par`
+  int32_t lds_bytes;         // > 0: stage headers, knobs and parameters in LDS
+  // segment-Jacobian mode
+  int32_t nturn;             // turns of the whole forward run
+  int32_t nseg_g;            // segments held in `states` (group_n = nseg_g * n)
+  double *jac;               // [segment][particle][column][6 + nparam]
+  const double *r_final;     // state after the last turn (lost particles are NaN)
 };
 
+// LDS bytes the reverse sweep needs to stage a table of nelem headers and
+// npar parameters (headers, then knob slot/kind pairs, then parameters).
+inline size_t adj_lds_bytes(size_t nelem, size_t npar) {
+  return nelem * (sizeof(Hdr) + 2 * sizeof(int32_t)) + npar * sizeof(double);
+}
+
+// Sequential sweep: one lane per particle walks turns t_hi-1 .. t_lo.
 hipError_t launch_adj_reverse(const AdjCtx &c, hipStream_t s);
+// Segment Jacobians: one lane per (segment, particle, column c) walks its
+// segment back from the unit adjoint e_c, giving column c of the segment's
+// transposed Jacobian and of its knob derivatives; r_end holds each
+// segment's end state [segment][particle][6].
+hipError_t launch_adj_segjac(const AdjCtx &c, hipStream_t s);
+// Chains the segment Jacobians of the group from the last segment to the
+// first: lam <- J^T lam, grad += G^T lam, one lane per particle.
+hipError_t launch_adj_combine(const AdjCtx &c, hipStream_t s);
 
 }  // namespace accp

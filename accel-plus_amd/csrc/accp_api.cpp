@@ -822,10 +822,12 @@ struct accp_adjoint {
   accp_lattice *L = nullptr;
   int nparam = 0;
   std::vector<int32_t> knob_slot, knob_kind;
-  DevBuf d_knob_slot, d_knob_kind;
+  DevBuf d_knob;  // per element {slot, kind}
   int n = 0, nturn = 0, ckpt = 1, nseg = 0;
+  int reverse_mode = ACCP_REVERSE_AUTO;
+  int64_t resident_lanes = 0;  // lanes the reverse kernels keep in flight
   bool forward_done = false;
-  DevBuf d_r, d_ckpt, d_rec, d_states, d_end, d_lam, d_grad;
+  DevBuf d_r, d_ckpt, d_rec, d_states, d_end, d_lam, d_grad, d_jac;
 };
 
 // Element maps the reverse sweep carries (the others make create fail).
@@ -883,10 +885,13 @@ extern "C" int accp_adjoint_create(accp_lattice *L, int nparam, const accp_param
   A->nparam = nparam;
   A->knob_slot = slot;
   A->knob_kind = kind;
-  int rc = A->d_knob_slot.ensure(ne * 4);
-  if (!rc) rc = A->d_knob_kind.ensure(ne * 4);
-  if (!rc && (hipMemcpy(A->d_knob_slot.p, slot.data(), ne * 4, hipMemcpyHostToDevice) != hipSuccess ||
-              hipMemcpy(A->d_knob_kind.p, kind.data(), ne * 4, hipMemcpyHostToDevice) != hipSuccess))
+  std::vector<int32_t> knob(2 * (size_t)std::max(ne, 1));
+  for (int e = 0; e < ne; ++e) {
+    knob[2 * e] = slot[e];
+    knob[2 * e + 1] = kind[e];
+  }
+  int rc = A->d_knob.ensure(knob.size() * 4);
+  if (!rc && hipMemcpy(A->d_knob.p, knob.data(), knob.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
     rc = fail(ACCP_E_HIP, "knob table upload failed");
   if (rc) {
     accp_adjoint_free(A);
@@ -924,6 +929,23 @@ extern "C" int accp_adjoint_forward(accp_adjoint *A, const double *r0, int n, in
   return ACCP_OK;
 }
 
+// The reverse kernels hold about 230 VGPRs, so two 64-lane waves fit per
+// SIMD: CUs x 4 SIMDs x 2 waves x 64 lanes are resident at once.
+static int64_t reverse_resident_lanes() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  return (int64_t)cus * 4 * 2 * 64;
+}
+
+extern "C" int accp_adjoint_set_reverse(accp_adjoint *A, int mode) {
+  if (!A || mode < ACCP_REVERSE_AUTO || mode > ACCP_REVERSE_SEGMENT_JACOBIAN)
+    return fail(ACCP_E_ARG, "accp_adjoint_set_reverse: bad arguments");
+  A->reverse_mode = mode;
+  return ACCP_OK;
+}
+
 extern "C" int accp_adjoint_backward(accp_adjoint *A, const double *dl_dr_out, double *dl_dparam,
                                      double *dl_dr0) {
   if (!A || !dl_dr_out) return fail(ACCP_E_ARG, "accp_adjoint_backward: bad arguments");
@@ -931,13 +953,28 @@ extern "C" int accp_adjoint_backward(accp_adjoint *A, const double *dl_dr_out, d
   accp_lattice *L = A->L;
   const int n = A->n, C = A->ckpt, nseg = A->nseg, np = A->nparam;
   const size_t bytes = (size_t)n * 6 * sizeof(double);
-  // segments recomputed together: all of them unless that exceeds 2 GiB
-  const size_t per_seg = (size_t)C * bytes;
-  const int G = (int)std::max<size_t>(1, std::min<size_t>(nseg, ((size_t)2 << 30) / per_seg));
+  // Sequential sweep: n lanes, each walking every turn.  Segment Jacobians:
+  // 6 n lanes per segment, all segments of a group at once, six times the
+  // arithmetic.  The second wins when the first leaves the GPU mostly idle.
+  if (!A->resident_lanes) A->resident_lanes = reverse_resident_lanes();
+  const double P = (double)A->resident_lanes;
+  const size_t jac_seg = (size_t)n * 6 * (6 + np) * sizeof(double);
+  const size_t budget = (size_t)4 << 30;  // states + segment Jacobians of one group
+  bool segjac = A->reverse_mode == ACCP_REVERSE_SEGMENT_JACOBIAN;
+  if (A->reverse_mode == ACCP_REVERSE_AUTO) {
+    const size_t per = (size_t)C * bytes + bytes + jac_seg;
+    const int g = (int)std::max<size_t>(1, std::min<size_t>(nseg, budget / per));
+    const double t_seq = 1.0 / std::min<double>(n, P);
+    const double t_jac = 6.0 / std::min<double>(6.0 * n * g, P);
+    segjac = nseg > 1 && t_jac < t_seq;
+  }
+  const size_t per_seg = (size_t)C * bytes + bytes + (segjac ? jac_seg : 0);
+  const int G = (int)std::max<size_t>(1, std::min<size_t>(nseg, budget / per_seg));
   int rc;
   if ((rc = A->d_end.ensure(bytes)) || (rc = A->d_lam.ensure(bytes)) ||
       (rc = A->d_grad.ensure(std::max<size_t>((size_t)n * np, 1) * sizeof(double))) ||
-      (rc = A->d_rec.ensure(bytes * G)) || (rc = A->d_states.ensure(per_seg * G)))
+      (rc = A->d_rec.ensure(bytes * G)) || (rc = A->d_states.ensure((size_t)C * bytes * G)) ||
+      (segjac && (rc = A->d_jac.ensure(jac_seg * G))))
     return rc;
   HIPCHK(hipMemcpy(A->d_lam.p, dl_dr_out, bytes, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(A->d_end.p, A->d_r.p, bytes, hipMemcpyDeviceToDevice));
@@ -960,8 +997,7 @@ extern "C" int accp_adjoint_backward(accp_adjoint *A, const double *dl_dr_out, d
     AdjCtx c{};
     c.hdr = (const Hdr *)L->d_hdr.p;
     c.par = (const double *)L->d_par.p;
-    c.knob_slot = (const int32_t *)A->d_knob_slot.p;
-    c.knob_kind = (const int32_t *)A->d_knob_kind.p;
+    c.knob = (const int32_t *)A->d_knob.p;
     c.st = 0;
     c.ed = ne;
     c.t_lo = k0 * C;
@@ -971,11 +1007,33 @@ extern "C" int accp_adjoint_backward(accp_adjoint *A, const double *dl_dr_out, d
     c.n = n;
     c.group_n = (int64_t)g * n;
     c.states = (const double *)A->d_states.p;
-    c.r_end = (double *)A->d_end.p;
     c.lam = (double *)A->d_lam.p;
     c.grad = (double *)A->d_grad.p;
     c.nparam = np;
-    HIPCHK(launch_adj_reverse(c, nullptr));
+    c.npar = (int32_t)L->tab.par.size();
+    const size_t lds = adj_lds_bytes(ne, L->tab.par.size());
+    c.lds_bytes = lds <= LDS_LIMIT ? (int32_t)lds : 0;
+    c.nturn = A->nturn;
+    c.nseg_g = g;
+    c.r_final = (const double *)A->d_r.p;
+    if (segjac) {
+      // end state of each segment, exactly: the next checkpoint, or the
+      // final state for the last segment
+      const int inner = std::min(k1, nseg - 1) - k0;
+      if (inner > 0)
+        HIPCHK(hipMemcpy(A->d_rec.p, (const char *)A->d_ckpt.p + (size_t)(k0 + 1) * bytes,
+                         (size_t)inner * bytes, hipMemcpyDeviceToDevice));
+      if (k1 == nseg)
+        HIPCHK(hipMemcpy((char *)A->d_rec.p + (size_t)(g - 1) * bytes, A->d_r.p, bytes,
+                         hipMemcpyDeviceToDevice));
+      c.r_end = (double *)A->d_rec.p;
+      c.jac = (double *)A->d_jac.p;
+      HIPCHK(launch_adj_segjac(c, nullptr));
+      HIPCHK(launch_adj_combine(c, nullptr));
+    } else {
+      c.r_end = (double *)A->d_end.p;
+      HIPCHK(launch_adj_reverse(c, nullptr));
+    }
     k1 = k0;
   }
   HIPCHK(hipDeviceSynchronize());
@@ -994,8 +1052,7 @@ extern "C" int accp_adjoint_backward(accp_adjoint *A, const double *dl_dr_out, d
 
 extern "C" int accp_adjoint_free(accp_adjoint *A) {
   if (!A) return ACCP_OK;
-  A->d_knob_slot.release();
-  A->d_knob_kind.release();
+  A->d_knob.release();
   A->d_r.release();
   A->d_ckpt.release();
   A->d_rec.release();
@@ -1003,6 +1060,7 @@ extern "C" int accp_adjoint_free(accp_adjoint *A) {
   A->d_end.release();
   A->d_lam.release();
   A->d_grad.release();
+  A->d_jac.release();
   delete A;
   return ACCP_OK;
 }

@@ -56,7 +56,8 @@ EXPORTS = (
     "accp_track", "accp_track_device", "accp_beam_stat",
     "accp_batch_create", "accp_track_batched", "accp_batch_capture", "accp_batch_replay",
     "accp_autotune", "accp_batch_free",
-    "accp_adjoint_create", "accp_adjoint_forward", "accp_adjoint_backward", "accp_adjoint_free",
+    "accp_adjoint_create", "accp_adjoint_forward", "accp_adjoint_backward", "accp_adjoint_set_reverse",
+    "accp_adjoint_free",
 )
 
 DP = POINTER(c_double)
@@ -262,6 +263,12 @@ class Adjoint:
         lib.check(lib.lib.accp_adjoint_create(lattice.h, len(params), arr, byref(h)), "accp_adjoint_create")
         self.h = h
         self.nparam = len(params)
+
+    REVERSE_AUTO, REVERSE_SEQUENTIAL, REVERSE_SEGMENT_JACOBIAN = 0, 1, 2
+
+    def set_reverse(self, mode):
+        """accp_adjoint_set_reverse: AUTO, SEQUENTIAL or SEGMENT_JACOBIAN."""
+        self.L.check(self.L.lib.accp_adjoint_set_reverse(self.h, mode), "accp_adjoint_set_reverse")
 
     def forward(self, r0, nturn, ckpt_every=16):
         r0 = np.ascontiguousarray(r0, dtype=np.float64)

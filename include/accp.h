@@ -216,6 +216,17 @@ int accp_adjoint_forward(accp_adjoint *adj, const double *r0, int n, int nturn,
  * if dl_dr0 is not NULL, dL/dr0 (host AoS). */
 int accp_adjoint_backward(accp_adjoint *adj, const double *dl_dr_out,
                           double *dl_dparam, double *dl_dr0);
+/* How the backward pass walks the turns.  SEQUENTIAL: one GPU lane per
+ * particle carries dL/d(state) back through every turn.  SEGMENT_JACOBIAN:
+ * the transposed Jacobian of every checkpoint segment (and its knob
+ * derivatives) is built by 6 lanes per particle and segment, all segments
+ * at once, then chained: six times the arithmetic on many more lanes, for
+ * beams too small to fill the GPU.  AUTO (default) picks by lane count.
+ * Both give the same gradient up to round-off. */
+#define ACCP_REVERSE_AUTO 0
+#define ACCP_REVERSE_SEQUENTIAL 1
+#define ACCP_REVERSE_SEGMENT_JACOBIAN 2
+int accp_adjoint_set_reverse(accp_adjoint *adj, int mode);
 int accp_adjoint_free(accp_adjoint *adj);
 
 #ifdef __cplusplus
