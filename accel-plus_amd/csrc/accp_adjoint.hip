@@ -333,13 +333,19 @@ hipError_t launch_adj_reverse(const AdjCtx &c, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Many lanes here: 256-lane workgroups let four waves share one LDS copy of
+// the table (with a 64-lane workgroup a table of tens of KB would cap a CU
+// at two or three waves, below what the registers allow).
 hipError_t launch_adj_segjac(const AdjCtx &c, hipStream_t s) {
-  constexpr int B = 64;
-  const int64_t grid = (c.group_n * 6 + B - 1) / B;
-  if (c.lds_bytes > 0)
+  if (c.lds_bytes > 0) {
+    constexpr int B = 256;
+    const int64_t grid = (c.group_n * 6 + B - 1) / B;
     hipLaunchKernelGGL((k_adj_segjac<B, true>), dim3((unsigned)grid), dim3(B), c.lds_bytes, s, c);
-  else
+  } else {
+    constexpr int B = 64;
+    const int64_t grid = (c.group_n * 6 + B - 1) / B;
     hipLaunchKernelGGL((k_adj_segjac<B, false>), dim3((unsigned)grid), dim3(B), 0, s, c);
+  }
   return hipGetLastError();
 }
 

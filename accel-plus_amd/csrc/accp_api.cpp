@@ -306,15 +306,18 @@ struct RunSpec {
 };
 
 int resolve_tuning(const accp_tuning *t, size_t lds_need, int &block, int &ppl, bool &lds, int &chunk_max) {
-  block = (t && t->block) ? t->block : 64;
   ppl = (t && t->ppl) ? t->ppl : 1;
   const int st = t ? t->staging : ACCP_STAGE_AUTO;
   chunk_max = (t && t->compact) ? t->compact : 256;
-  if (block != 64 && block != 128 && block != 256) return fail(ACCP_E_ARG, "block must be 64, 128 or 256");
-  if (ppl != 1 && ppl != 2) return fail(ACCP_E_ARG, "ppl must be 1 or 2");
   if (st == ACCP_STAGE_LDS && lds_need > LDS_LIMIT)
     return fail(ACCP_E_ARG, "element table (%zu B) exceeds the LDS staging limit", lds_need);
   lds = (st == ACCP_STAGE_LDS) || (st == ACCP_STAGE_AUTO && lds_need <= LDS_LIMIT);
+  // default shape: with the table in LDS, 256-lane workgroups share one copy
+  // among four waves (a 64-lane workgroup holding tens of KB of LDS would
+  // cap a CU below the waves its registers allow)
+  block = (t && t->block) ? t->block : (lds ? 256 : 64);
+  if (block != 64 && block != 128 && block != 256) return fail(ACCP_E_ARG, "block must be 64, 128 or 256");
+  if (ppl != 1 && ppl != 2) return fail(ACCP_E_ARG, "ppl must be 1 or 2");
   return ACCP_OK;
 }
 

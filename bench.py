@@ -85,6 +85,41 @@ def cpu_baseline(lats, beam, offs, nturn, budget_s=20.0):
     }
 
 
+def saturated_leg(lib, glat, elems, tuning, stream):
+    """The same kernel at full occupancy: a Gaussian core beam that survives
+    (2^18 particles on the seed-0 lattice, 16 turns)."""
+    import numpy as np
+
+    import pyaccp.flops as flops
+    import pyaccp.lattices as lat
+    from pyaccp.hipmem import DeviceArray, Event
+
+    log("saturated leg")
+    nsat, tsat = 1 << 18, 16
+    sat_beam = lat.gaussian_beam(nsat, seed=11)
+    d_sat0 = DeviceArray(sat_beam)
+    d_sat = DeviceArray(sat_beam)
+    e0, e1 = Event(), Event()
+    sat_ms = []
+    for rep in range(3):
+        d_sat.copy_from(d_sat0, stream.handle)
+        e0.record(stream.handle)
+        glat.track_device(d_sat.ptr, nsat, tsat, tuning=tuning, stream=stream.handle)
+        e1.record(stream.handle)
+        sat_ms.append(e0.elapsed_ms(e1))
+    sat_out = d_sat.numpy()
+    sat_alive = int((~np.isnan(sat_out[:, 0])).sum())
+    sat_fl = nsat * tsat * flops.flops_per_turn(elems, "useful")
+    sat_best = min(sat_ms[1:])
+    return {
+        "workload": f"{nsat} Gaussian core particles, seed-0 DA lattice, {tsat} turns ({sat_alive} survive)",
+        "ms": round(sat_best, 3),
+        "particle_turns_per_s": nsat * tsat / (sat_best * 1e-3),
+        "achieved_tflops": round(sat_fl / (sat_best * 1e-3) / 1e12, 3),
+        "frac_of_fp64_peak": round(sat_fl / (sat_best * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+    }
+
+
 def adjoint_leg(lib, stream):
     """Adjoint gradient workload: loss = sum of squared final coordinates of
     4096 particles after 512 turns of the DA lattice with 20 extra quads
@@ -196,7 +231,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--tune", default="", help="block,ppl,staging(scalar|lds) instead of autotuning")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
+    ap.add_argument("--legs", default="saturated,adjoint,plain,cpu",
+                    help="comma list of the legs run after the batched one "
+                         "(saturated, adjoint, plain, cpu); 'none' for the batched leg alone")
     args = ap.parse_args()
+    legs = set() if args.legs == "none" else set(args.legs.split(","))
+    unknown = legs - {"saturated", "adjoint", "plain", "cpu"}
+    if unknown:
+        ap.error(f"unknown legs {sorted(unknown)}")
+    for flag, leg in ((args.no_plain, "plain"), (args.no_cpu, "cpu"), (args.no_adjoint, "adjoint")):
+        if flag:
+            legs.discard(leg)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this backend has no "
               f"multi-GPU path: seeds and particles are independent and would shard with zero exchange.",
@@ -309,38 +354,13 @@ def main():
         },
     }
 
-    # the same kernel at full occupancy: a Gaussian core beam that survives
-    # (2^18 particles on the seed-0 lattice, 16 turns)
-    log("saturated leg")
-    nsat, tsat = 1 << 18, 16
-    sat_beam = lat.gaussian_beam(nsat, seed=11)
-    d_sat0 = DeviceArray(sat_beam)
-    d_sat = DeviceArray(sat_beam)
-    e0, e1 = Event(), Event()
-    sat_ms = []
-    for rep in range(3):
-        d_sat.copy_from(d_sat0, stream.handle)
-        e0.record(stream.handle)
-        glats[0].track_device(d_sat.ptr, nsat, tsat, tuning=tuning, stream=stream.handle)
-        e1.record(stream.handle)
-        sat_ms.append(e0.elapsed_ms(e1))
-    sat_out = d_sat.numpy()
-    sat_alive = int((~np.isnan(sat_out[:, 0])).sum())
-    sat_fl = nsat * tsat * flops.flops_per_turn(lats[0], "useful")
-    sat_best = min(sat_ms[1:])
-    result["saturated"] = {
-        "workload": f"{nsat} Gaussian core particles, seed-0 DA lattice, {tsat} turns ({sat_alive} survive)",
-        "ms": round(sat_best, 3),
-        "particle_turns_per_s": nsat * tsat / (sat_best * 1e-3),
-        "achieved_tflops": round(sat_fl / (sat_best * 1e-3) / 1e12, 3),
-        "frac_of_fp64_peak": round(sat_fl / (sat_best * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
-    }
-
-    if not args.no_adjoint:
+    if "saturated" in legs:
+        result["saturated"] = saturated_leg(lib, glats[0], lats[0], tuning, stream)
+    if "adjoint" in legs:
         log("adjoint leg")
         result["adjoint"] = adjoint_leg(lib, stream)
 
-    if not args.no_plain:
+    if "plain" in legs:
         log("plain per-seed GPU path")
         d_p = DeviceArray(beam)
         synchronize()
@@ -359,7 +379,7 @@ def main():
             "identical_survival_maps": same,
         }
 
-    if not args.no_cpu:
+    if "cpu" in legs:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(lats, beam, offs, nturn)
     print(json.dumps(result), flush=True)
