@@ -29,6 +29,23 @@ for p in (os.path.join(ROOT, "accel-plus_amd"), os.path.join(ROOT, "oracle")):
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 
+# HBM bytes per k_track launch of the default batched workload (64 seeds,
+# 1024 turns, 12 launches per step): rocprofv3 FETCH_SIZE and WRITE_SIZE
+# passes, scaled by what each counter reads for a plain device copy of known
+# size (1.9955 and 1.0); profiles/k_track_batched_da_64seeds_1024turns.md.
+TRAFFIC_BYTES_PER_LAUNCH = 7.12e6
+
+
+def chunk_launches(nturn, chunk_max=256):
+    """Tracking launches of one call: chunks of 1, 2, 4, ... turns up to
+    chunk_max, compaction in between."""
+    k, c, t = 0, 1, 0
+    while t < nturn:
+        t += min(c, nturn - t)
+        k += 1
+        c = min(2 * c, chunk_max)
+    return k
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
@@ -314,7 +331,11 @@ def main():
         at = int(alive_turns[offs[s]:offs[s + 1]].sum())
         fl_useful += at * flops.flops_per_turn(lats[s], "useful")
         fl_ref += at * flops.flops_per_turn(lats[s], "reference")
-    achieved = fl_useful / (kernel_ms * 1e-3) / 1e12
+    launches = chunk_launches(nturn)
+    # per launch: algorithmic flops / average launch time = the same ratio
+    # over the whole step (the bookkeeping kernels are < 0.1% of it)
+    achieved = (fl_useful / launches) / (kernel_ms / launches * 1e-3) / 1e12
+    default_workload = args.nseed == 64 and nturn == 1024
 
     result = {
         "metric": "particle-turns/s FP64 (batched vs plain GPU)",
@@ -347,7 +368,11 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES_PER_LAUNCH if default_workload else None,
+            "traffic_unit": "HBM bytes per k_track launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "launches_per_step": launches,
+            "ms_per_launch": round(kernel_ms / launches, 3),
+            "flops_per_launch_useful": fl_useful / launches,
             "flops_per_step_useful": fl_useful,
             "flops_per_step_reference_algorithm": fl_ref,
             "kernel_ms": round(kernel_ms, 3),
