@@ -16,6 +16,7 @@
 
 #include "../../include/accp.h"
 #include "accp_adjoint_kernels.h"
+#include "accp_error.h"
 #include "accp_kernels.h"
 #include "accp_table.h"
 #include "accp_wake.h"
@@ -40,6 +41,12 @@ static int fail(int code, const char *fmt, ...) {
     hipError_t e_ = (x);                                                         \
     if (e_ != hipSuccess) return fail(ACCP_E_HIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
   } while (0)
+
+// Error hook for the other host translation units of the library.
+int accp::set_error(int code, const char *msg) {
+  g_err = msg;
+  return code;
+}
 
 extern "C" const char *accp_last_error(void) { return g_err.c_str(); }
 extern "C" int accp_abi_version(void) { return ACCP_ABI_VERSION; }
@@ -467,6 +474,13 @@ extern "C" int accp_lattice_update(accp_lattice *L, const accp_elem *elems, int 
   L->elems.assign(elems, elems + n);
   L->tab = std::move(c);
   return upload_lattice(L);
+}
+
+extern "C" int accp_lattice_get(const accp_lattice *L, accp_elem *out, int n) {
+  if (!L || !out || n != (int)L->elems.size())
+    return fail(ACCP_E_ARG, "accp_lattice_get: lattice and element count must match");
+  std::copy(L->elems.begin(), L->elems.end(), out);
+  return ACCP_OK;
 }
 
 extern "C" int accp_lattice_nelem(const accp_lattice *L, int *n) {

@@ -1,0 +1,896 @@
+// accp_run.cpp -- the Accel+ command-file driver on the GPU backend.
+//
+//   accp-run [--precision N] [--parse-only] command_file
+//
+// Reads the reference's command files (src/process.cpp) and runs their
+// Lattice, Optics, Track and DAMA blocks through libaccp: lattice files in
+// the reference grammar (src/lib/accel/accel.cpp:67-133 and each create()),
+// the same keys, defaults and printed blocks ($$$...), tracking, optics and
+// DA on the GPU.  Numbers print with 6 significant digits like the
+// reference's cout unless --precision asks for more.  --parse-only reads the
+// command file and the lattice and prints the element count without
+// touching the GPU.
+//
+// Commands handled (the path's callers in the reference):
+//   Lattice: file_name=F[, type=ERING]; set_flags: radiation=, cavity=, wake=;
+//            modify_element: type=, famname=, index=, field=, value=|delta=;
+//            correct_tune / correct_chrom: type=, famname=(F1 F2) [a, b]
+//            [REPEAT] [METHOD=FD|ADJOINT]; set_max_num_threads=N (accepted)
+//   Optics:  print_tune / print_chrom [dpp=, delta=]; print_matrix: type=RING
+//   Track:   Distribution: Zero NPar= | Load f[, format=plain|accel-export] |
+//            Coordinate r=(...;...) | Gaussian NPar= sigx= ... | Linear X=a:d:b
+//            [shift=(...)] [print]; Monitor: interval=, fraction=,
+//            printflag=Coordinate|Stat1|Stat2; Do_Track: NTurn=,
+//            radiation=, RFCavity=, PrintFinal=No|Yes|Full [, seed=]
+//   DAMA:    TRACK_DA: xmin=, xmax=, ymax=, nray=, nstep=, nturn=
+//            [, print=loss_turn] [, weightminusside=true]
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/accp.h"
+
+namespace {
+
+using std::string;
+
+int g_prec = 6;
+
+// ---- text helpers (process.cpp:109-160) ----------------------------------------
+string trim(const string &s) {
+  size_t a = s.find_first_not_of(" \t\r\n\f\v");
+  if (a == string::npos) return "";
+  size_t b = s.find_last_not_of(" \t\r\n\f\v");
+  return s.substr(a, b - a + 1);
+}
+string upper(string s) {
+  for (auto &c : s) c = (char)std::toupper((unsigned char)c);
+  return s;
+}
+bool ieq(const string &a, const string &b) { return upper(trim(a)) == upper(trim(b)); }
+
+void strip_comment(string &s) {
+  size_t p = s.find_first_of("#%");
+  if (p != string::npos) s.erase(p);
+}
+
+// One logical line: comments removed, '\' continues a line.
+bool get_logical_line(std::istream &in, string &line) {
+  if (!std::getline(in, line)) return false;
+  strip_comment(line);
+  line = trim(line);
+  while (!line.empty() && line.back() == '\\') {
+    line.back() = ' ';
+    string next;
+    if (!std::getline(in, next)) break;
+    strip_comment(next);
+    line += trim(next);
+  }
+  return true;
+}
+
+// The next line whose upper-cased text starts with one of keys; -1 at
+// STOP/RETURN or end of file (process.cpp:136-160).
+int look_for(std::istream &in, string &line, const std::vector<string> &keys) {
+  string raw;
+  while (get_logical_line(in, raw)) {
+    line = trim(raw);
+    const string up = upper(line);
+    if (up == "STOP" || up == "RETURN") return -1;
+    for (size_t i = 0; i < keys.size(); ++i)
+      if (up.compare(0, keys[i].size(), keys[i]) == 0) return (int)i;
+  }
+  line.clear();
+  return -1;
+}
+
+// "key=value" pairs after the first '=' or ':', commas as blanks.
+std::vector<std::pair<string, string>> key_values(string s) {
+  size_t p = s.find_first_of("=:");
+  s = p == string::npos ? "" : s.substr(p + 1);
+  std::replace(s.begin(), s.end(), ',', ' ');
+  std::vector<std::pair<string, string>> kv;
+  s = trim(s);
+  while (!s.empty()) {
+    p = s.find('=');
+    if (p == string::npos) {
+      kv.emplace_back(upper(trim(s)), "");
+      break;
+    }
+    string key = upper(trim(s.substr(0, p)));
+    s = trim(s.substr(p + 1));
+    size_t q = s.find_first_of(" \t");
+    kv.emplace_back(key, s.substr(0, q));
+    s = q == string::npos ? "" : trim(s.substr(q + 1));
+  }
+  return kv;
+}
+
+string fmt(double v) {
+  std::ostringstream o;
+  o << std::setprecision(g_prec) << v;
+  return o.str();
+}
+string fmt_sci(double v) {
+  std::ostringstream o;
+  o << std::scientific << std::setprecision(g_prec) << v;
+  return o.str();
+}
+
+void check(int rc, const char *what) {
+  if (rc != ACCP_OK) throw std::runtime_error(string(what) + ": " + accp_last_error());
+}
+
+// ---- lattice file (accel.cpp:67-133 and the create() of each type) ------------
+accp_elem new_elem(int type, const string &fam) {
+  accp_elem e;
+  std::memset(&e, 0, sizeof e);
+  e.type = type;
+  e.nslice = 10;      // accel.h:195
+  e.energy = 3000.0;  // BendMPole default energy [MeV]
+  e.on = 1;
+  std::strncpy(e.famname, fam.c_str(), ACCP_NAME_LEN - 1);
+  return e;
+}
+
+std::vector<accp_elem> read_lattice(const string &path) {
+  std::ifstream in(path);
+  if (!in) throw std::runtime_error("cannot open lattice file " + path);
+  std::vector<accp_elem> v;
+  string fam;
+  while (in >> fam) {
+    if (fam == "RETURN") break;
+    if (fam[0] == '!') {  // comment line
+      string skip;
+      std::getline(in, skip);
+      continue;
+    }
+    if (fam.back() == ':') fam.pop_back();
+    else {
+      string skip;
+      if (!std::getline(in, skip, ':')) break;
+    }
+    string type, rest;
+    if (!(in >> type)) break;
+    std::getline(in, rest);
+    std::istringstream ps(rest);
+    accp_elem e;
+    if (type == "Drift") {
+      e = new_elem(ACCP_DRIFT, fam);
+      ps >> e.len;
+    } else if (type == "Quad") {
+      e = new_elem(ACCP_QUAD, fam);
+      ps >> e.len >> e.k;
+    } else if (type == "Corrector") {
+      e = new_elem(ACCP_CORRECTOR, fam);
+      ps >> e.len >> e.kickx >> e.kicky;
+    } else if (type == "Aperture") {
+      e = new_elem(ACCP_APERTURE, fam);
+      ps >> e.len >> e.xlim[0] >> e.xlim[1] >> e.ylim[0] >> e.ylim[1];
+    } else if (type == "Monitor") {
+      e = new_elem(ACCP_MONITOR, fam);
+      ps >> e.len;
+    } else if (type == "Bend") {
+      e = new_elem(ACCP_BEND, fam);
+      ps >> e.len >> e.theta >> e.e1 >> e.e2 >> e.k;
+    } else if (type == "BendMPole") {
+      e = new_elem(ACCP_BENDMPOLE, fam);
+      ps >> e.len >> e.theta >> e.e1 >> e.e2 >> e.max_npole;
+      if (e.max_npole < 1 || e.max_npole > ACCP_MAX_MPOLE)
+        throw std::runtime_error("BendMPole " + fam + ": MaxNPole out of range");
+      for (int i = 0; i < e.max_npole; ++i) in >> e.polyB[i];
+      std::getline(in, rest);
+    } else if (type == "StrMPole") {
+      e = new_elem(ACCP_STRMPOLE, fam);
+      ps >> e.len >> e.max_npole;
+      if (e.max_npole < 1 || e.max_npole > ACCP_MAX_MPOLE)
+        throw std::runtime_error("StrMPole " + fam + ": MaxNPole out of range");
+      for (int i = 0; i < e.max_npole; ++i) in >> e.polyB[i];
+      std::getline(in, rest);
+      for (int i = 0; i < e.max_npole; ++i) in >> e.polyA[i];
+      std::getline(in, rest);
+    } else if (type == "Cavity") {
+      e = new_elem(ACCP_CAVITY, fam);
+      ps >> e.len >> e.volt >> e.freq >> e.phis >> e.energy;
+    } else if (type == "LongWake") {  // new element: L Rs Q fr charge zmin zmax nbins E
+      e = new_elem(ACCP_LONGWAKE, fam);
+      ps >> e.len >> e.wake_rs >> e.wake_q >> e.wake_fr >> e.wake_charge >> e.wake_zmin >> e.wake_zmax >>
+          e.nbins >> e.energy;
+    } else {
+      throw std::runtime_error("element " + fam + ": type " + type + " is not carried by this backend");
+    }
+    v.push_back(e);
+  }
+  if (v.empty()) throw std::runtime_error("no element read from " + path);
+  return v;
+}
+
+// ---- the line: host records plus the device table ------------------------------
+struct Line {
+  std::vector<accp_elem> e;
+  accp_lattice *lat = nullptr;
+  bool dirty = true;
+  bool parse_only = false;
+
+  ~Line() { accp_lattice_free(lat); }
+  accp_lattice *device() {
+    if (parse_only) throw std::runtime_error("--parse-only: no GPU work");
+    if (!lat) check(accp_lattice_build(e.data(), (int)e.size(), &lat), "accp_lattice_build");
+    else if (dirty) check(accp_lattice_update(lat, e.data(), (int)e.size()), "accp_lattice_update");
+    dirty = false;
+    return lat;
+  }
+  void pull() {  // after a library call changed the lattice
+    check(accp_lattice_get(lat, e.data(), (int)e.size()), "accp_lattice_get");
+    dirty = false;
+  }
+  static const char *type_name(int t) {
+    switch (t) {
+      case ACCP_DRIFT: return "Drift";
+      case ACCP_QUAD: return "Quad";
+      case ACCP_CORRECTOR: return "Corrector";
+      case ACCP_APERTURE: return "Aperture";
+      case ACCP_MONITOR: return "Monitor";
+      case ACCP_BEND: return "Bend";
+      case ACCP_BENDMPOLE: return "BendMPole";
+      case ACCP_STRMPOLE: return "StrMPole";
+      case ACCP_CAVITY: return "Cavity";
+      case ACCP_LONGWAKE: return "LongWake";
+    }
+    return "?";
+  }
+  // ALine::findelements (aline.cpp:520-555); index = 1 for the first match,
+  // 0 for all of them.
+  std::vector<int32_t> find(const string &type, const string &fam, unsigned index = 0) const {
+    std::vector<int32_t> out;
+    unsigned cnt = 0;
+    for (size_t i = 0; i < e.size(); ++i)
+      if (type == type_name(e[i].type) && (fam.empty() || fam == e[i].famname)) {
+        ++cnt;
+        if (index == 0) out.push_back((int32_t)i);
+        else if (index == cnt) return {(int32_t)i};
+      }
+    return out;
+  }
+  // ALine::set_rad_onoff / set_cavity_onoff / set_wake_onoff (aline.cpp:258-293, :429-463)
+  int set_rad(int flag) {
+    int n = 0;
+    for (auto &x : e)
+      if (x.type == ACCP_BENDMPOLE) x.rad = flag, ++n;
+    dirty = true;
+    return n;
+  }
+  int set_on(int type, int on) {
+    int n = 0;
+    for (auto &x : e)
+      if (x.type == type) x.on = on, ++n;
+    dirty = true;
+    return n;
+  }
+  double length() const {
+    double s = 0;
+    for (const auto &x : e) s += x.len;
+    return s;
+  }
+};
+
+// AccelElement::setfield/getfield names (accel.cpp) for modify_element.
+double *field_ptr(accp_elem &x, const string &f, unsigned index) {
+  if (f == "Length") return &x.len;
+  if (f == "K" && (x.type == ACCP_QUAD || x.type == ACCP_BEND)) return &x.k;
+  if (f == "BendingAngle") return &x.theta;
+  if (f == "EntranceAngle") return &x.e1;
+  if (f == "ExitAngle") return &x.e2;
+  if (f == "energy" || f == "Energy") return &x.energy;
+  if (f == "Voltage") return &x.volt;
+  if (f == "Frequency") return &x.freq;
+  if (f == "Phis") return &x.phis;
+  if (f == "KickX") return &x.kickx;
+  if (f == "KickY") return &x.kicky;
+  if ((f == "PolynomB" || f == "PolynomA") && index < ACCP_MAX_MPOLE &&
+      (x.type == ACCP_BENDMPOLE || x.type == ACCP_STRMPOLE)) {
+    if (f == "PolynomA" && x.type == ACCP_BENDMPOLE) return nullptr;  // accel.cpp:630-634
+    return f == "PolynomB" ? &x.polyB[index] : &x.polyA[index];
+  }
+  return nullptr;
+}
+
+// ---- Lattice block (process.cpp:173-988) ------------------------------------------
+void cmd_file_name(Line &L, const string &line) {
+  string s = line;
+  std::replace(s.begin(), s.end(), ',', ' ');
+  size_t p = s.find_first_of("=:");
+  s = trim(s.substr(p + 1));
+  size_t q = s.find_first_of(" \t");
+  const string file = s.substr(0, q);
+  L.e = read_lattice(file);
+  if (L.lat) {
+    accp_lattice_free(L.lat);
+    L.lat = nullptr;
+  }
+  L.dirty = true;
+  const string rest = q == string::npos ? "" : s.substr(q + 1);
+  for (const auto &kv : key_values("x=" + rest))
+    if (kv.first == "TYPE" && ieq(kv.second, "ERING") && !L.parse_only) {
+      double phis = 0;
+      check(accp_setrfphase(L.device(), &phis), "accp_setrfphase");
+      L.pull();
+      std::cout << "RF cavities phase set to: phi_s=" << fmt(phis) << " rad" << std::endl;
+      L.set_rad(ACCP_RAD_OFF);
+      std::cout << "radiation off" << std::endl;
+    }
+  const int ncav = L.set_on(ACCP_CAVITY, 0);
+  if (ncav > 0) std::cout << ncav << " cavity(ies) turned off" << std::endl;
+  const int nwake = L.set_on(ACCP_LONGWAKE, 0);
+  if (nwake > 0) std::cout << nwake << " wake element(s) turned off" << std::endl;
+}
+
+void cmd_set_flags(Line &L, const string &line) {
+  for (const auto &kv : key_values(line)) {
+    const string &v = kv.second;
+    if (kv.first == "RADIATION") {
+      if (ieq(v, "off")) L.set_rad(ACCP_RAD_OFF), std::cout << "radiation turned off" << std::endl;
+      else if (ieq(v, "rad") || ieq(v, "on"))
+        L.set_rad(ACCP_RAD_DAMP), std::cout << "radiation turned on without quantum excitation" << std::endl;
+      else if (ieq(v, "radQE"))
+        L.set_rad(ACCP_RAD_QE), std::cout << "radiation turned on with quantum excitation" << std::endl;
+    } else if (kv.first == "CAVITY" || kv.first == "WAKE") {
+      const int type = kv.first == "CAVITY" ? ACCP_CAVITY : ACCP_LONGWAKE;
+      const char *what = kv.first == "CAVITY" ? " cavity(ies) turned " : " wake element(s) turned ";
+      if (ieq(v, "off") || ieq(v, "on")) {
+        const int on = ieq(v, "on");
+        const int n = L.set_on(type, on);
+        if (n > 0) std::cout << n << what << (on ? "on" : "off") << std::endl;
+      }
+    }
+  }
+}
+
+void cmd_modify_element(Line &L, const string &line) {
+  string type, fam, field;
+  unsigned index = 0;
+  double value = 0;
+  int mode = 0;  // 1 value, 2 delta
+  for (const auto &kv : key_values(line)) {
+    if (kv.first == "TYPE") type = kv.second;
+    else if (kv.first == "FAMNAME") fam = kv.second;
+    else if (kv.first == "INDEX") index = (unsigned)std::stoul(kv.second);
+    else if (kv.first == "FIELD") field = kv.second;
+    else if (kv.first == "VALUE") value = std::stod(kv.second), mode = 1;
+    else if (kv.first == "DELTA") value = std::stod(kv.second), mode = 2;
+  }
+  // index selects the index-th family member (0: all) and, for PolynomB/A,
+  // is also the multipole index, as the reference reads it (process.cpp:839-870)
+  const std::vector<int32_t> list = L.find(type, fam, index);
+  if (list.empty() || !mode) {
+    if (list.empty()) std::cout << "No element is found: " << type << "\t" << fam << "\t" << index << std::endl;
+    if (!mode) std::cout << "No value is given" << std::endl;
+    return;
+  }
+  for (int32_t i : list) {
+    accp_elem &x = L.e[i];
+    if (field == "NSlice") {
+      x.nslice = mode == 2 ? x.nslice + (int)value : (int)value;
+    } else {
+      double *f = field_ptr(x, field, index);
+      if (!f) throw std::runtime_error("modify_element: field " + field + " not settable on " + type);
+      *f = mode == 2 ? *f + value : value;
+    }
+  }
+  L.dirty = true;
+}
+
+void cmd_correct(Line &L, const string &line, int quantity) {
+  const bool tune = quantity == ACCP_TUNE;
+  std::cout << (tune ? "$$$Correct Tunes***********************" : "$$$Correct chromaticities***********************")
+            << std::endl;
+  string s = line.substr(line.find_first_of("=:") + 1);
+  std::replace(s.begin(), s.end(), ',', ' ');
+  const string up = upper(s);
+  string type;
+  {
+    size_t p = up.find("TYPE");
+    size_t q = s.find('=', p);
+    std::istringstream(s.substr(q + 1)) >> type;
+  }
+  std::cout << type << std::endl;
+  string fam1, fam2;
+  {
+    size_t a = s.find('('), b = s.find(')');
+    if (a == string::npos || b == string::npos) throw std::runtime_error("correct: famname=(F1 F2) expected");
+    std::istringstream(s.substr(a + 1, b - a - 1)) >> fam1 >> fam2;
+  }
+  std::cout << fam1 << " " << fam2 << std::endl;
+  double target[2] = {0, 0};
+  {
+    size_t a = s.find('['), b = s.find(']');
+    if (a == string::npos || b == string::npos) throw std::runtime_error("correct: [target1 target2] expected");
+    std::istringstream(s.substr(a + 1, b - a - 1)) >> target[0] >> target[1];
+  }
+  const int method = up.find("ADJOINT") != string::npos ? ACCP_RESP_ADJOINT : ACCP_RESP_FD;
+  std::cout << (tune ? "Target tunes: " : "Target chromaticities: ") << fmt(target[0]) << "\t" << fmt(target[1])
+            << "\t" << std::endl;
+  L.set_rad(ACCP_RAD_OFF);
+  if (!tune) L.set_on(ACCP_CAVITY, 0);
+  double v[2];
+  if (tune) check(accp_calctune(L.device(), 0.0, ACCP_DELTA_DIFF, v), "accp_calctune");
+  else check(accp_calcchrom(L.device(), 0.0, ACCP_DELTA_DIFF, v), "accp_calcchrom");
+  std::cout << (tune ? "Initial tunes: " : "Initial chromaticities: ") << fmt(v[0]) << "\t" << fmt(v[1]) << "\t"
+            << std::endl;
+  const std::vector<int32_t> f1 = L.find(type, fam1), f2 = L.find(type, fam2);
+  if (f1.empty() || f2.empty()) {
+    std::cout << "error: no elements found " << f1.size() << "\t" << f2.size() << std::endl;
+    return;
+  }
+  std::cout << f1.size() << "\t" << f2.size() << std::endl;
+  const int steps = up.find("REPEAT") != string::npos ? 2 : 1;
+  for (int k = 0; k < steps; ++k) {
+    double t[2] = {target[0], target[1]};
+    check(accp_correct(L.device(), quantity, method, f1.data(), (int)f1.size(), f2.data(), (int)f2.size(), t),
+          "accp_correct");
+    L.pull();
+    std::cout << (tune ? "corrected tunes: " : "corrected chromaticities: ") << fmt(t[0]) << "\t" << fmt(t[1])
+              << "\t" << std::endl;
+  }
+  std::cout << "*********************************" << std::endl;
+}
+
+void proc_lattice(std::istream &in, Line &L) {
+  const std::vector<string> keys = {"FILE_NAME",      "CORRECT_TUNE", "CORRECT_CHROM", "SET_MAX_NUM_THREADS",
+                                    "MODIFY_ELEMENT", "SET_FLAGS",    "RESET_FIRST",   "PRINT_ELEM_VALUE",
+                                    "PRINT_LAT_SUMMARY", "EXPORT_ELEMENT", "APPLY_SETPT", "END"};
+  string line;
+  for (;;) {
+    const int k = look_for(in, line, keys);
+    if (k < 0 || keys[k] == "END") return;
+    if (k != 0 && L.e.empty()) throw std::runtime_error("Lattice file name needs to be given first");
+    if (k >= 6) {
+      std::cerr << "accp-run: " << keys[k] << " is not carried by this backend; line ignored" << std::endl;
+      continue;
+    }
+    if (L.parse_only && (k == 1 || k == 2)) continue;
+    switch (k) {
+      case 0: cmd_file_name(L, line); break;
+      case 1: cmd_correct(L, line, ACCP_TUNE); break;
+      case 2: cmd_correct(L, line, ACCP_CHROM); break;
+      case 3: {
+        int n = 1;
+        std::istringstream(line.substr(line.find_first_of("=:") + 1)) >> n;
+        std::cout << "Maximum thread number set to " << std::max(n, 1) << std::endl;  // host threads unused
+        break;
+      }
+      case 4: cmd_modify_element(L, line); break;
+      case 5: cmd_set_flags(L, line); break;
+    }
+  }
+}
+
+// ---- Optics block (process.cpp:1107-1395): tunes, chromaticities, one-turn matrix
+void proc_optics(std::istream &in, Line &L) {
+  const std::vector<string> keys = {"PRINT_MATRIX", "PRINT_TUNE", "PRINT_CHROM", "END"};
+  string line;
+  for (;;) {
+    const int k = look_for(in, line, keys);
+    if (k < 0 || keys[k] == "END") return;
+    double dpp = 0, delta = ACCP_DELTA_DIFF;
+    for (const auto &kv : key_values(line)) {
+      if (kv.first == "DPP") dpp = std::stod(kv.second);
+      else if (kv.first == "DELTA") delta = std::stod(kv.second);
+    }
+    if (L.parse_only) continue;
+    L.set_rad(ACCP_RAD_OFF);
+    if (k == 0) {
+      double m[36];
+      check(accp_getm66(L.device(), 0, -1, dpp, delta, m), "accp_getm66");
+      std::cout << "$$$OPTICS-MATRIX*******************ONE-TURN" << std::endl;
+      std::cout << "type=RING at " << Line::type_name(L.e[0].type) << ", famname=" << L.e[0].famname
+                << ", ring index=0, spos=" << fmt(L.length()) << std::endl;
+      for (int i = 0; i < 6; ++i) {
+        for (int j = 0; j < 6; ++j) std::cout << fmt(m[i * 6 + j]) << (j < 5 ? "\t" : "");
+        std::cout << std::endl;
+      }
+      std::cout << "****************************" << std::endl;
+    } else if (k == 1) {
+      double t[2];
+      check(accp_calctune(L.device(), dpp, delta, t), "accp_calctune");
+      std::cout << "Tunes: [" << fmt(t[0]) << "\t" << fmt(t[1]) << "], dpp=" << fmt(dpp) << std::endl;
+    } else {
+      double c[2];
+      check(accp_calcchrom(L.device(), dpp, delta, c), "accp_calcchrom");
+      std::cout << "Chromaticity: [" << fmt(c[0]) << "\t" << fmt(c[1]) << "], dpp=" << fmt(dpp) << std::endl;
+    }
+  }
+}
+
+// ---- Track block (process.cpp:1428-2220) -------------------------------------------
+double find_num(const string &sel, const string &key, bool &found) {
+  size_t p = sel.find(key);
+  found = p != string::npos;
+  if (!found) return 0;
+  string s = sel.substr(p + key.size());
+  s.erase(0, s.find('=') + 1);
+  double v = 0;
+  std::istringstream(s) >> v;
+  return v;
+}
+
+// Box-Muller normals from a counter-based stream (Philox4x32-10 is the
+// tracker's; here a splitmix64 counter suffices for initial beams).
+struct Normals {
+  uint64_t key, ctr = 0;
+  explicit Normals(uint64_t seed) : key(seed) {}
+  double uniform() {
+    uint64_t z = key + 0x9E3779B97F4A7C15ull * ++ctr;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  }
+  double next() {
+    const double u1 = uniform(), u2 = uniform();
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * 3.141592653589793 * u2);
+  }
+};
+
+std::vector<double> gen_distri(const string &line, uint64_t seed) {
+  std::cout << line << std::endl;
+  string sel = trim(line.substr(line.find_first_of("=:") + 1));
+  size_t pos = sel.find_first_of(" \t");
+  if (pos == string::npos && !ieq(sel, "ZERO")) return std::vector<double>(6, 0.0);  // one particle at zero
+  const string option = sel.substr(0, pos);
+  std::vector<double> r;
+  if (ieq(option, "ZERO")) {
+    int np = 0;
+    std::istringstream(sel.substr(sel.find('=') + 1)) >> np;
+    r.assign(6 * (size_t)std::max(np, 0), 0.0);
+  } else if (ieq(option, "LOAD")) {
+    sel.erase(0, pos + 1);
+    size_t q = sel.find_first_of(", \t\n");
+    const string file = trim(sel.substr(0, q));
+    std::cout << file << std::endl;
+    string format = "PLAIN";
+    string rest = q == string::npos ? "" : sel.substr(q + 1);
+    size_t e = rest.find('=');
+    if (e != string::npos && ieq(rest.substr(0, e), "FORMAT")) format = upper(trim(rest.substr(e + 1)));
+    std::ifstream fin(file);
+    if (!fin) throw std::runtime_error("cannot open distribution file " + file);
+    string sl;
+    if (format == "ACCEL-EXPORT")
+      while (get_logical_line(fin, sl))
+        if (sl.compare(0, 3, "$$$") == 0 && sl.find("Full export") != string::npos) {
+          get_logical_line(fin, sl);
+          break;
+        }
+    while (get_logical_line(fin, sl)) {
+      if (sl.compare(0, 5, "*****") == 0) break;
+      if (sl.size() <= 11) continue;
+      std::istringstream ls(sl);
+      double v[6];
+      if (format == "ACCEL-EXPORT") {
+        int turn;
+        ls >> turn;
+      }
+      for (double &x : v) {
+        string tok;
+        ls >> tok;
+        x = ieq(tok, "nan") || ieq(tok, "-nan") ? std::nan("") : std::stod(tok);
+      }
+      r.insert(r.end(), v, v + 6);
+    }
+  } else if (ieq(option, "COORDINATE")) {
+    size_t a = sel.find('('), b = sel.find(')');
+    string s = sel.substr(a + 1, b - a - 1);
+    std::replace(s.begin(), s.end(), ',', ' ');
+    std::istringstream ps(s);
+    string part;
+    while (std::getline(ps, part, ';')) {
+      std::istringstream qs(part);
+      double v[6] = {0, 0, 0, 0, 0, 0};
+      for (double &x : v) qs >> x;
+      r.insert(r.end(), v, v + 6);
+    }
+  } else if (ieq(option, "GAUSSIAN")) {
+    sel.erase(0, pos + 1);
+    bool f;
+    int np = (int)find_num(sel, "NPar", f);
+    if (!f) np = 1;
+    // sigx/sigxp/... as the reference reads them (process.cpp:1634-1750);
+    // emit_x/betx forms need twiss functions this backend does not carry
+    if (sel.find("emit_x") != string::npos || sel.find("emit_y") != string::npos)
+      throw std::runtime_error("Gaussian: emit_x/emit_y need calctwiss; give sigx/sigxp/sigy/sigyp");
+    const double sg[6] = {find_num(sel, "sigx", f),  find_num(sel, "sigxp", f), find_num(sel, "sigy", f),
+                          find_num(sel, "sigyp", f), find_num(sel, "sigz", f),  find_num(sel, "sigdpp", f)};
+    Normals N(seed);
+    r.resize(6 * (size_t)std::max(np, 0));
+    for (int i = 0; i < np; ++i)
+      for (int j = 0; j < 6; ++j) r[6 * i + j] = sg[j] * N.next();
+  } else if (ieq(option, "LINEAR")) {
+    sel = trim(sel.substr(pos + 1));
+    size_t e = sel.find('=');
+    const string coord = trim(sel.substr(0, e));
+    string s = trim(sel.substr(e + 1));
+    std::replace(s.begin(), s.end(), ':', ' ');
+    double st = 0, step = 0, ed = 0;
+    std::istringstream(s) >> st >> step >> ed;
+    if (step == 0) {
+      std::cout << "interval cannot be zero" << std::endl;
+      return r;
+    }
+    int np = 0;
+    for (double v = st; (v <= ed && step > 0) || (v >= ed && step < 0); v += step) ++np;
+    const char *names[6] = {"X", "XP", "Y", "YP", "Z", "DELTA"};
+    int j = 0;
+    for (int c = 0; c < 6; ++c)
+      if (ieq(coord, names[c])) j = c;
+    r.assign(6 * (size_t)np, 0.0);
+    for (int i = 0; i < np; ++i) r[6 * i + j] = st + step * i;
+  }
+  return r;
+}
+
+struct Monitor {
+  int interval = 1;
+  double fraction = 1.0;
+  int flag = 0;  // 1 Coordinate, 2 Stat1, 3 Stat2
+};
+
+void print_stat(const double *s, bool cross) {
+  std::cout << (long long)s[0] << "\t" << (long long)s[1];
+  for (int i = 0; i < 6; ++i) std::cout << "\t" << fmt_sci(s[2 + i]);
+  for (int i = 0; i < 6; ++i) std::cout << "\t" << fmt_sci(s[8 + i]);
+  if (cross)
+    for (int i = 0; i < 6; ++i) std::cout << "\t" << fmt_sci(s[14 + i]);
+  std::cout << std::endl;
+}
+
+void print_coords(const std::vector<double> &r, long long turn, double fraction) {
+  const int np = (int)(r.size() / 6);
+  int iskip = (int)(1.0 / fraction);
+  iskip = std::max(1, std::min(iskip, np));
+  for (int i = 0; i < np; i += iskip) {
+    std::cout << turn;
+    for (int j = 0; j < 6; ++j) std::cout << "\t" << fmt(r[6 * i + j]);
+    std::cout << std::endl;
+  }
+}
+
+void do_track(Line &L, std::vector<double> &r, const Monitor &mon, const string &line, uint64_t &seed) {
+  string str_rad, str_rf, printfinal = "NO";
+  int nturn = 0;
+  for (const auto &kv : key_values(line)) {
+    if (kv.first == "RADIATION") str_rad = kv.second;
+    else if (kv.first == "RFCAVITY") str_rf = kv.second;
+    else if (kv.first == "NTURN") nturn = std::stoi(kv.second);
+    else if (kv.first == "PRINTFINAL") printfinal = kv.second;
+    else if (kv.first == "SEED") seed = std::stoull(kv.second);
+  }
+  if (ieq(str_rad, "OFF")) L.set_rad(ACCP_RAD_OFF);
+  else if (ieq(str_rad, "RAD")) L.set_rad(ACCP_RAD_DAMP);
+  else if (ieq(str_rad, "RADQE")) L.set_rad(ACCP_RAD_QE);
+  else if (!str_rad.empty() && !ieq(str_rad, "NO"))
+    throw std::runtime_error("radiation=" + str_rad + " (lumped radiation) is not carried by this backend");
+  if (ieq(str_rf, "OFF")) L.set_on(ACCP_CAVITY, 0);
+  else if (ieq(str_rf, "ON")) L.set_on(ACCP_CAVITY, 1);
+  const int np = (int)(r.size() / 6);
+  if (np <= 0 || nturn <= 0) return;
+  const char *head[4] = {"", "$$$Track-Data: Coordinate**************************",
+                         "$$$Track-Data: Stat1******************************",
+                         "$$$Track-Data: Stat2******************************"};
+  if (mon.flag) {
+    std::cout << head[mon.flag] << std::endl;
+    std::cout << "#" << str_rad << " " << str_rf << " " << nturn << std::endl;
+    if (mon.flag == 1) {
+      int iskip = std::max(1, std::min((int)(1.0 / mon.fraction), np));
+      std::cout << "Np=" << np << ", Nprint=" << np / iskip << std::endl;
+    } else {
+      std::cout << "Np=" << np << std::endl;
+    }
+  }
+  accp_lattice *lat = L.device();
+  accp_track_opts o{};
+  o.rng_seed = seed;
+  if (mon.flag == 2 || mon.flag == 3) {
+    // statistics at the start of every interval-th turn, on the device
+    const int nrec = (nturn + mon.interval - 1) / mon.interval;
+    std::vector<double> rec((size_t)nrec * ACCP_STAT_LEN);
+    int32_t cnt = 0;
+    o.mon_interval = mon.interval;
+    o.mon_max = nrec;
+    o.mon_out = rec.data();
+    o.mon_count = &cnt;
+    check(accp_track(lat, r.data(), np, nturn, 0, -1, &o), "accp_track");
+    if (np >= 2)
+      for (int k = 0; k < cnt; ++k) print_stat(&rec[(size_t)k * ACCP_STAT_LEN], mon.flag == 3);
+  } else if (mon.flag == 1) {
+    // coordinates at the start of every interval-th turn: track in pieces
+    for (int t = 0; t < nturn; t += mon.interval) {
+      print_coords(r, t, mon.fraction);
+      o.turn0 = (uint64_t)t;
+      check(accp_track(lat, r.data(), np, std::min(mon.interval, nturn - t), 0, -1, &o), "accp_track");
+    }
+  } else {
+    check(accp_track(lat, r.data(), np, nturn, 0, -1, &o), "accp_track");
+  }
+  if (ieq(printfinal, "Full")) {
+    std::cout << "*************************************" << std::endl;
+    std::cout << "$$$Track-Data: Full export ************************" << std::endl;
+    std::cout << "Np=" << np << ", Nprint=" << np << std::endl;
+    for (int i = 0; i < np; ++i) {
+      std::cout << nturn << "\t";
+      for (int j = 0; j < 6; ++j) std::cout << fmt(r[6 * i + j]) << "\t";
+      std::cout << std::endl;
+    }
+  } else if (ieq(printfinal, "Yes") || ieq(printfinal, "same")) {
+    if (mon.flag == 1) print_coords(r, nturn, mon.fraction);
+    else if ((mon.flag == 2 || mon.flag == 3) && np >= 2) {
+      double s[ACCP_STAT_LEN];
+      check(accp_beam_stat(r.data(), np, s), "accp_beam_stat");
+      s[0] = nturn;
+      print_stat(s, mon.flag == 3);
+    }
+  }
+  std::cout << "*************************************" << std::endl;
+}
+
+void proc_track(std::istream &in, Line &L) {
+  const std::vector<string> keys = {"DISTRIBUTION", "MONITOR", "DO_TRACK", "END"};
+  std::vector<double> r;
+  Monitor mon;
+  uint64_t seed = 0;
+  string line;
+  for (;;) {
+    const int k = look_for(in, line, keys);
+    if (k < 0 || keys[k] == "END") return;
+    if (k == 0) {
+      r = gen_distri(line, seed);
+      const string up = upper(line);
+      size_t p = up.find("SHIFT");
+      if (p != string::npos && !r.empty()) {
+        size_t a = up.find('(', p), b = up.find(')', a);
+        string s = line.substr(a + 1, b - a - 1);
+        std::replace(s.begin(), s.end(), ',', ' ');
+        double d[6] = {0, 0, 0, 0, 0, 0};
+        std::istringstream qs(s);
+        for (double &x : d) qs >> x;
+        for (size_t i = 0; i < r.size(); ++i) r[i] += d[i % 6];
+      }
+      if (up.find("PRINT") != string::npos) {
+        std::cout << "$$$Init-Distribution*************" << std::endl;
+        for (size_t i = 0; i < r.size(); i += 6) {
+          for (int j = 0; j < 6; ++j) std::cout << fmt(r[i + j]) << " ";
+          std::cout << std::endl;
+        }
+        std::cout << "***********************" << std::endl;
+      }
+    } else if (k == 1) {
+      mon = Monitor();
+      mon.flag = 1;
+      for (const auto &kv : key_values(line)) {
+        if (kv.first == "INTERVAL") mon.interval = std::max(1, std::stoi(kv.second));
+        else if (kv.first == "FRACTION") mon.fraction = std::min(1.0, std::max(0.01, std::stod(kv.second)));
+        else if (kv.first == "PRINTFLAG")
+          mon.flag = ieq(kv.second, "STAT1") ? 2 : ieq(kv.second, "STAT2") ? 3 : 1;
+      }
+    } else if (k == 2) {
+      if (L.parse_only) continue;
+      do_track(L, r, mon, line, seed);
+    }
+  }
+}
+
+// ---- DAMA block (process.cpp:2393-2500) --------------------------------------------
+void proc_dama(std::istream &in, Line &L) {
+  const std::vector<string> keys = {"TRACK_DA", "END"};
+  string line;
+  for (;;) {
+    const int k = look_for(in, line, keys);
+    if (k < 0 || keys[k] == "END") return;
+    int nray = 0, nstep = 0, nturn = 0;
+    double xmin = 0, xmax = 0, ymax = 0;
+    bool loss = false, neg = false;
+    for (const auto &kv : key_values(line)) {
+      if (kv.first == "XMIN") xmin = std::stod(kv.second);
+      else if (kv.first == "XMAX") xmax = std::stod(kv.second);
+      else if (kv.first == "YMAX") ymax = std::stod(kv.second);
+      else if (kv.first == "NRAY") nray = std::stoi(kv.second);
+      else if (kv.first == "NSTEP") nstep = std::stoi(kv.second);
+      else if (kv.first == "NTURN") nturn = std::stoi(kv.second);
+      else if (kv.first == "PRINT") loss = ieq(kv.second, "loss_turn");
+      else if (kv.first == "WEIGHTMINUSSIDE") neg = ieq(kv.second, "true");
+    }
+    if (nray < 2 || nray > 51) std::cout << "Error: number of rays: " << nray << std::endl;
+    std::cout << "#" << fmt(xmin) << " " << fmt(xmax) << " " << fmt(ymax) << " " << nray << " " << nstep << " "
+              << nturn << " " << neg << std::endl;
+    if (L.parse_only) continue;
+    L.set_rad(ACCP_RAD_DAMP);
+    L.set_on(ACCP_CAVITY, 1);
+    const int nr = std::min(std::max(nray, 3), 51), ns = std::max(nstep, 2);
+    double rx[51], ry[51], area = 0;
+    std::vector<int32_t> lt((size_t)nr * ns);
+    check(accp_trackda(L.device(), xmax, xmin, ymax, nturn, nray, nstep, neg, rx, ry, &area, lt.data()),
+          "accp_trackda");
+    if (loss) {
+      // the reference's per-turn scan (aline.cpp:636-662) records the turn
+      // after which a point is first seen lost, 0 for survivors; a point
+      // lost in turn 0 is first recorded at turn 1
+      std::cout << "$$$DA-Loss " << std::endl;
+      const double dx = xmax - xmin;
+      for (int i = 0; i < nr; ++i)
+        for (int j = 0; j < ns; ++j) {
+          const double rad = xmin + dx * j / (ns - 1);
+          const double xl = rad * std::cos(i * 3.1415926535897932384626 / (nr - 1));
+          const double yl = rad * std::sin(i * 3.1415926535897932384626 / (nr - 1)) * ymax / xmax;
+          const int t = lt[(size_t)i * ns + j];
+          const int rep = t >= nturn ? 0 : (t == 0 ? (nturn > 1 ? 1 : 0) : t);
+          std::cout << fmt(xl) << "\t" << fmt(yl) << "\t" << rep << std::endl;
+        }
+    }
+    std::cout << "$$$DA-profile " << nturn << " turns, " << (neg ? "3/4 DA area=" : "DA area=") << fmt(area * 1.0e6)
+              << " mm^2" << std::endl;
+    for (int i = 0; i < nr; ++i) std::cout << fmt(rx[i]) << " \t" << fmt(ry[i]) << " \t" << std::endl;
+    std::cout << "******************************" << std::endl;
+  }
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  string cmdfile;
+  bool parse_only = false;
+  for (int i = 1; i < argc; ++i) {
+    const string a = argv[i];
+    if (a == "--precision" && i + 1 < argc) g_prec = std::max(1, std::min(17, std::atoi(argv[++i])));
+    else if (a == "--parse-only") parse_only = true;
+    else cmdfile = a;
+  }
+  if (cmdfile.empty()) {
+    std::cerr << "usage: accp-run [--precision N] [--parse-only] command_file" << std::endl;
+    return 2;
+  }
+  std::ifstream in(cmdfile);
+  if (!in) {
+    std::cerr << "cannot open " << cmdfile << std::endl;
+    return 2;
+  }
+  const std::vector<string> blocks = {"LATTICE", "OPTICS", "SURVEY", "DAMA", "TRACK"};
+  Line L;
+  L.parse_only = parse_only;
+  try {
+    string line;
+    int k = look_for(in, line, blocks);
+    if (k != 0) {
+      std::cout << (k > 0 ? "The first block must be LATTICE." : "No LATTICE block is found.") << std::endl;
+      return 1;
+    }
+    proc_lattice(in, L);
+    std::cout << "Total elements: " << L.e.size() << std::endl;
+    while ((k = look_for(in, line, blocks)) >= 0) {
+      std::cout << blocks[k] << std::endl;
+      switch (k) {
+        case 0: proc_lattice(in, L); break;
+        case 1: proc_optics(in, L); break;
+        case 3: proc_dama(in, L); break;
+        case 4: proc_track(in, L); break;
+        default: {  // Survey: nothing on the tracking path
+          std::vector<string> end = {"END"};
+          look_for(in, line, end);
+        }
+      }
+    }
+  } catch (const std::exception &ex) {
+    std::cout.flush();
+    std::cerr << "accp-run: " << ex.what() << std::endl;
+    return 1;
+  }
+  return 0;
+}

@@ -58,6 +58,8 @@ EXPORTS = (
     "accp_autotune", "accp_batch_free",
     "accp_adjoint_create", "accp_adjoint_forward", "accp_adjoint_backward", "accp_adjoint_set_reverse",
     "accp_adjoint_free",
+    "accp_lattice_get", "accp_getm66", "accp_calctune", "accp_calcchrom", "accp_calcresp", "accp_correct",
+    "accp_setrfphase", "accp_trackda",
 )
 
 DP = POINTER(c_double)
@@ -141,6 +143,73 @@ class Lattice:
             self.free()
         except Exception:
             pass
+
+    # ---- ALine's lattice services (accp_getm66 ... accp_trackda) ----------
+    TUNE, CHROM = 1, 2
+    RESP_FD, RESP_ADJOINT = 0, 1
+
+    def get(self):
+        """The lattice's element records (accp_lattice_get)."""
+        out = (el.Elem * len(self.arr))()
+        self.L.check(self.L.lib.accp_lattice_get(self.h, out, len(out)), "accp_lattice_get")
+        return out
+
+    def getm66(self, st=0, ed=-1, dp=0.0, delta=1e-8):
+        m = np.zeros(36)
+        self.L.check(self.L.lib.accp_getm66(self.h, st, ed, c_double(dp), c_double(delta), _dp(m)), "accp_getm66")
+        return m.reshape(6, 6)
+
+    def calctune(self, dp=0.0, delta=1e-8):
+        t = np.zeros(2)
+        self.L.check(self.L.lib.accp_calctune(self.h, c_double(dp), c_double(delta), _dp(t)), "accp_calctune")
+        return t
+
+    def calcchrom(self, dp=0.0, delta=1e-8):
+        c = np.zeros(2)
+        self.L.check(self.L.lib.accp_calcchrom(self.h, c_double(dp), c_double(delta), _dp(c)), "accp_calcchrom")
+        return c
+
+    @staticmethod
+    def _fam(f):
+        a = np.ascontiguousarray(f, dtype=np.int32)
+        return a, a.ctypes.data_as(POINTER(c_int32)), len(a)
+
+    def calcresp(self, quantity, fam1, fam2, method=0):
+        """resp[k, p] = d(tune or chromaticity of plane p)/d(knob k)."""
+        a1, p1, n1 = self._fam(fam1)
+        a2, p2, n2 = self._fam(fam2)
+        r = np.zeros(4)
+        self.L.check(self.L.lib.accp_calcresp(self.h, quantity, method, p1, n1, p2, n2, _dp(r)), "accp_calcresp")
+        return r.reshape(2, 2)
+
+    def correct(self, quantity, fam1, fam2, target, method=0):
+        """One correction step; returns the tunes (chromaticities) reached."""
+        a1, p1, n1 = self._fam(fam1)
+        a2, p2, n2 = self._fam(fam2)
+        t = np.array(target, dtype=np.float64)
+        self.L.check(self.L.lib.accp_correct(self.h, quantity, method, p1, n1, p2, n2, _dp(t)), "accp_correct")
+        self.arr = self.get()
+        return t
+
+    def setrfphase(self):
+        ph = c_double(0.0)
+        self.L.check(self.L.lib.accp_setrfphase(self.h, byref(ph)), "accp_setrfphase")
+        self.arr = self.get()
+        return ph.value
+
+    def trackda(self, xmax, xmin, ymax, nturn, nray, nstep, negweight=False, loss_turn=False):
+        """Returns (area, rx[nray], ry[nray]) and, if loss_turn, the loss turn
+        of every grid point."""
+        rx, ry = np.zeros(51), np.zeros(51)
+        area = c_double(0.0)
+        nr = min(max(nray, 3), 51)
+        lt = np.zeros(nr * max(nstep, 2), dtype=np.int32) if loss_turn else None
+        self.L.check(self.L.lib.accp_trackda(
+            self.h, c_double(xmax), c_double(xmin), c_double(ymax), nturn, nray, nstep, int(negweight),
+            _dp(rx), _dp(ry), byref(area), lt.ctypes.data_as(POINTER(c_int32)) if lt is not None else None),
+            "accp_trackda")
+        out = (area.value, rx[:nr], ry[:nr])
+        return out + (lt,) if loss_turn else out
 
     def track(self, r, nturn=1, st=0, ed=-1, seed=0, turn0=0, pid0=0, tuning=None,
               mon_interval=0, mon_max=0, loss_turn=False):

@@ -229,6 +229,65 @@ int accp_adjoint_backward(accp_adjoint *adj, const double *dl_dr_out,
 int accp_adjoint_set_reverse(accp_adjoint *adj, int mode);
 int accp_adjoint_free(accp_adjoint *adj);
 
+/* ---- ALine's lattice services on the GPU tracker -------------------------- */
+/* These mirror the ALine methods the command-file driver calls
+ * (src/lib/accel/accel.h:627-642), with the same arguments and arithmetic;
+ * every ray or particle is tracked by accp_track.  They change the lattice
+ * the way the reference changes its ALine (family strengths, cavity phase,
+ * flags); accp_lattice_get returns the records afterwards. */
+
+/* Copy of the lattice's element records (n = accp_lattice_nelem). */
+int accp_lattice_get(const accp_lattice *lat, accp_elem *out, int n);
+
+#define ACCP_DELTA_DIFF 1.0e-8 /* DELTA_DIFF_TRACK, src/lib/accel/accel.h:586 */
+
+/* ALine::getm66(m66, st, ed, dp, delta) (src/lib/accel/aline.cpp:891-927):
+ * transfer matrix of elements [st, ed) (ed < 0: to the end) from 12 rays at
+ * +-delta around (0,0,0,0,0,dp); m66 row-major, m66[6*i+j] = dr_i/dr0_j. */
+int accp_getm66(accp_lattice *lat, int st, int ed, double dp, double delta,
+                double m66[36]);
+/* ALine::calctune (src/lib/accel/aline.cpp:1032-1043): acos(trace/2)/2pi
+ * per plane of getm66. */
+int accp_calctune(accp_lattice *lat, double dp, double delta, double tune[2]);
+/* ALine::calcchrom (src/lib/accel/aline.cpp:1108-1126): tune difference
+ * between dp-1e-5 and dp+1e-5 over 2e-5. */
+int accp_calcchrom(accp_lattice *lat, double dp, double delta, double chrom[2]);
+
+#define ACCP_TUNE 1  /* knobs: Quad K, or StrMPole PolyB[1]                  */
+#define ACCP_CHROM 2 /* knobs: StrMPole PolyB[2]                             */
+#define ACCP_RESP_FD 0      /* the reference's +1e-5 forward difference       */
+#define ACCP_RESP_ADJOINT 1 /* reverse-mode adjoint through the getm66 rays   */
+
+/* ALine::calctuneresp / calcchromresp (src/lib/accel/aline.cpp:1187-1242,
+ * :1310-1355): resp[2*k + p] = d(tune, or chromaticity, of plane p) / d(knob
+ * k) for the two families fam1 and fam2 (element indices; a knob shift moves
+ * every member).  ACCP_RESP_ADJOINT differentiates the same getm66-based
+ * tune exactly instead of by a forward difference; its lattice must be one
+ * accp_adjoint_create accepts. */
+int accp_calcresp(accp_lattice *lat, int quantity, int method, const int32_t *fam1,
+                  int n1, const int32_t *fam2, int n2, double resp[4]);
+/* ALine::correcttune / correctchrom (src/lib/accel/aline.cpp:1243-1309,
+ * :1356-1418): one 2x2 Newton step of the two families toward target[2],
+ * applied to the lattice; target[] then receives the tunes (chromaticities)
+ * reached, as the reference returns them. */
+int accp_correct(accp_lattice *lat, int quantity, int method, const int32_t *fam1,
+                 int n1, const int32_t *fam2, int n2, double target[2]);
+/* ALine::setrfphase() (src/lib/accel/aline.cpp:465-501): one turn of a
+ * particle at the origin with classical radiation and the cavities off gives
+ * the energy loss U0/E; every cavity's phase becomes asin(U0/V_total).  Like
+ * the reference it leaves the cavities off and radiation off.  *phis = 0 and
+ * nothing changes when the lattice has no cavity. */
+int accp_setrfphase(accp_lattice *lat, double *phis);
+/* ALine::trackDA (src/lib/accel/aline.cpp:607-715): nray (3..51) polar rays
+ * of nstep points from xmin to xmax (y scaled by ymax/xmax) at z = 1e-5,
+ * tracked nturn turns together; rx, ry (51 entries) receive each ray's first
+ * lost point (or its last point), *area the enclosed area (positive-x side
+ * at half weight if negweight).  loss_turn (optional, nray*nstep) receives
+ * the turn in which each point was lost, nturn if it survived. */
+int accp_trackda(accp_lattice *lat, double xmax, double xmin, double ymax, int nturn,
+                 int nray, int nstep, int negweight, double *rx, double *ry,
+                 double *area, int32_t *loss_turn);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,128 @@
+"""The command-file driver (accp-run) on the GPU.
+
+Command files in the reference's grammar (src/process.cpp) run through the
+driver, which prints the reference's blocks; numbers are read back at 17
+digits and compared with the oracle on the same lattice and beam, and with
+the compiled reference's fixtures where those exist."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyaccp.elements as el
+import pyaccp.lattices as lat
+from conftest import ROOT, golden
+from helpers import assert_coords_close
+
+pytestmark = pytest.mark.gpu
+
+RUN = os.path.join(ROOT, "accel-plus_amd", "accp-run")
+
+
+def run(tmp_path, text):
+    (tmp_path / "cmd.txt").write_text(text)
+    print("accp-run", tmp_path / "cmd.txt", flush=True)
+    p = subprocess.run([RUN, "--precision", "17", "cmd.txt"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr
+    return p.stdout.splitlines()
+
+
+def after(lines, head, n):
+    i = next(k for k, s in enumerate(lines) if s.startswith(head))
+    return lines[i + 1:i + 1 + n]
+
+
+def numbers(s):
+    """The numeric tokens of one printed line."""
+    out = []
+    for t in s.replace("[", " ").replace("]", " ").replace(",", " ").split():
+        try:
+            out.append(float(t))
+        except ValueError:
+            pass
+    return out
+
+
+def test_ering_tune_and_tracking(tmp_path, oracle):
+    ring = lat.da_ring(0)
+    el.write_lattice(ring, str(tmp_path / "da.lat"))
+    beam = lat.gaussian_beam(200, seed=4)
+    el.write_plain_beam(beam, str(tmp_path / "beam.txt"))
+    out = run(tmp_path, """Lattice
+  file_name = da.lat, type=ERING
+End
+Optics
+  print_tune
+End
+Track
+  Distribution: Load beam.txt, format=plain
+  Monitor: interval=5, printflag=Stat1
+  Do_Track: NTurn=20, radiation=off, RFCavity=on, PrintFinal=Full
+End
+STOP
+""")
+    arr = el.to_array(ring)
+    phis = oracle.setrfphase(arr)
+    got = float(next(s for s in out if s.startswith("RF cavities phase")).split("=")[1].split()[0])
+    assert abs(got - phis) <= 1e-12 * abs(phis)
+    tune = numbers(next(s for s in out if s.startswith("Tunes:")))[:2]
+    assert np.allclose(tune, oracle.tune(arr), rtol=0, atol=1e-12)
+    # tracking with the cavity switched on by Do_Track
+    for i in range(len(arr)):
+        if arr[i].type == el.CAVITY:
+            arr[i].on = 1
+    stats = [numbers(s) for s in after(out, "$$$Track-Data: Stat1", 6)[2:6]]
+    r = beam.copy()
+    for k, rec in enumerate(stats):
+        ref = oracle.beam_stat(r)
+        assert rec[0] == 5 * k and rec[1] == ref[1]
+        assert np.allclose(rec[2:14], ref[2:14], rtol=1e-10, atol=1e-18), (k, rec, ref)
+        r = oracle.track(arr, r, 5)
+    final = np.array([numbers(s)[1:7] for s in after(out, "$$$Track-Data: Full export", 201)[1:]])
+    assert final.shape == (200, 6)
+    assert_coords_close(final, r, 1e-10, what="20 turns through the driver")
+
+
+def test_track_da_with_loss_turns(tmp_path):
+    """TRACK_DA sets radiation (classical) and the cavity on, like the
+    reference's DAMA block; with ERING phasing this is the trackda fixture's
+    setup."""
+    g = golden("trackda.npz")
+    el.write_lattice(lat.da_ring(0), str(tmp_path / "da.lat"))
+    out = run(tmp_path, """Lattice
+  file_name = da.lat, type=ERING
+End
+DAMA
+  TRACK_DA: xmin=0.001, xmax=0.02, ymax=0.01, nray=11, nstep=10, nturn=64, print=loss_turn
+End
+""")
+    loss = np.array([numbers(s) for s in after(out, "$$$DA-Loss", 110)])
+    assert loss.shape == (110, 3) and np.all((loss[:, 2] >= 0) & (loss[:, 2] < 64))
+    prof = np.array([numbers(s) for s in after(out, "$$$DA-profile", 11)])
+    step = (0.02 - 0.001) / 9
+    moved = np.abs(np.hypot(prof[:, 0], prof[:, 1] * 2) - np.hypot(g["rx"], g["ry"] * 2))
+    assert np.all(moved <= step * (1 + 1e-9)), moved
+    area = numbers(next(s for s in out if s.startswith("$$$DA-profile")).split("=")[1])[0]
+    if not moved.any():
+        assert abs(area - g["area"][0] * 1e6) <= 1e-9 * abs(g["area"][0] * 1e6)
+
+
+def test_correct_tune_and_chrom(tmp_path):
+    g = golden("optics.npz")
+    el.write_lattice(lat.fodo_ring(), str(tmp_path / "fodo.lat"))
+    out = run(tmp_path, """Lattice
+  file_name = fodo.lat
+  correct_tune: type=Quad, famname=(QF QD) [0.28, 0.24]
+  correct_chrom: type=StrMPole, famname=(SF SD) [2.0, 1.0] REPEAT method=adjoint
+  correct_tune: type=Quad, famname=(QF QD) [0.27, 0.25] REPEAT method=adjoint
+End
+""")
+    tunes = [numbers(s) for s in out if s.startswith("corrected tunes:")]
+    chroms = [numbers(s) for s in out if s.startswith("corrected chromaticities:")]
+    # the first step is the reference's own (finite-difference responses)
+    assert np.allclose(tunes[0], g["corrected_tune"], rtol=0, atol=1e-9)
+    # adjoint responses: two Newton steps reach the targets
+    assert np.allclose(chroms[-1], [2.0, 1.0], rtol=0, atol=1e-5)
+    assert np.allclose(tunes[-1], [0.27, 0.25], rtol=0, atol=1e-8)
