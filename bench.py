@@ -137,6 +137,54 @@ def saturated_leg(lib, glat, elems, tuning, stream):
     }
 
 
+def collective_leg(lib, stream, nturn=200):
+    """Collective tracking (radiation with quantum excitation, cavity at the
+    synchronous phase from the GPU's setrfphase, longitudinal resonator wake
+    of 5 kOhm, Q = 1, 10 GHz, 2 nC in 256 bins) of 1e5 Gaussian particles on
+    the seed-0 DA lattice; every turn splits at the wake for the histogram,
+    convolution and kick."""
+    import numpy as np
+
+    import pyaccp.elements as el
+    import pyaccp.flops as flops
+    import pyaccp.lattices as lat
+    from pyaccp.hipmem import DeviceArray, Event
+
+    log("collective leg")
+    ring = lat.da_ring(0) + [el.longwake(5e3, 1.0, 10e9, 2e-9, 256, -0.02, 0.02, energy=3000.0, on=1)]
+    L = lib.lattice(ring)
+    phis = L.setrfphase()
+    arr = L.get()
+    for i in range(len(arr)):
+        if arr[i].type == el.BENDMPOLE:
+            arr[i].rad = el.RAD_QE
+        if arr[i].type in (el.CAVITY, el.LONGWAKE):
+            arr[i].on = 1
+    L.update(arr)
+    n = 100000
+    beam = lat.gaussian_beam(n, seed=13)
+    d0 = DeviceArray(beam)
+    d = DeviceArray(beam)
+    e0, e1 = Event(), Event()
+    L.track_device(d.ptr, n, 10, seed=3, stream=stream.handle)  # warmup
+    d.copy_from(d0, stream.handle)
+    e0.record(stream.handle)
+    L.track_device(d.ptr, n, nturn, seed=3, stream=stream.handle)
+    e1.record(stream.handle)
+    ms = e0.elapsed_ms(e1)
+    out = d.numpy()
+    alive = int((~np.isnan(out[:, 0])).sum())
+    fl = n * nturn * flops.flops_per_turn([arr[i] for i in range(len(arr))], "useful")
+    return {
+        "workload": f"{n} particles, RADQE + cavity (phi_s = {phis:.6f} from setrfphase) + LongWake "
+                    f"(256 bins), {len(ring)} elements, {nturn} turns ({alive} survive)",
+        "ms": round(ms, 3),
+        "ms_per_turn": round(ms / nturn, 4),
+        "particle_turns_per_s": n * nturn / (ms * 1e-3),
+        "achieved_tflops": round(fl / (ms * 1e-3) / 1e12, 3),
+    }
+
+
 def adjoint_leg(lib, stream):
     """Adjoint gradient workload: loss = sum of squared final coordinates of
     4096 particles after 512 turns of the DA lattice with 20 extra quads
@@ -248,12 +296,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--tune", default="", help="block,ppl,staging(scalar|lds) instead of autotuning")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
-    ap.add_argument("--legs", default="saturated,adjoint,plain,cpu",
+    ap.add_argument("--legs", default="saturated,collective,adjoint,plain,cpu",
                     help="comma list of the legs run after the batched one "
-                         "(saturated, adjoint, plain, cpu); 'none' for the batched leg alone")
+                         "(saturated, collective, adjoint, plain, cpu); 'none' for the batched leg alone")
     args = ap.parse_args()
     legs = set() if args.legs == "none" else set(args.legs.split(","))
-    unknown = legs - {"saturated", "adjoint", "plain", "cpu"}
+    unknown = legs - {"saturated", "collective", "adjoint", "plain", "cpu"}
     if unknown:
         ap.error(f"unknown legs {sorted(unknown)}")
     for flag, leg in ((args.no_plain, "plain"), (args.no_cpu, "cpu"), (args.no_adjoint, "adjoint")):
@@ -381,6 +429,8 @@ def main():
 
     if "saturated" in legs:
         result["saturated"] = saturated_leg(lib, glats[0], lats[0], tuning, stream)
+    if "collective" in legs:
+        result["collective"] = collective_leg(lib, stream)
     if "adjoint" in legs:
         log("adjoint leg")
         result["adjoint"] = adjoint_leg(lib, stream)

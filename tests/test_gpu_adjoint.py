@@ -134,3 +134,29 @@ def test_reverse_modes_agree(gpu_lib):
     assert np.all(np.abs(d1 - d2) <= 1e-11 * scale), np.abs(d1 - d2).max(axis=0) / scale
     with pytest.raises(Exception):
         A.set_reverse(7)
+
+
+def test_adjoint_jacobian_matches_reference_getm66(gpu_lib):
+    """The reference's own numerical derivatives: getm66 differences 12 rays
+    at +-1e-8 (aline.cpp:900-926; fixture optics.npz from the compiled
+    reference).  Six backward passes with dL/dr_out = e_i on one particle
+    give row i of the exact one-turn Jacobian; the central difference
+    differs from it by O(delta^2) truncation and 1e-16/1e-8 round-off, so
+    1e-7 of the largest entry, at delta = 0 and on the dp = 1e-3 orbit."""
+    from conftest import golden
+    g = golden("optics.npz")
+    ring = lat.fodo_ring()
+    L = gpu_lib.lattice(ring)
+    A = abi.Adjoint(gpu_lib, L, [(el.FIELD_K, 0, lat.family(ring, "QF"))])
+    for dp, ref in ((0.0, g["m66"]), (1e-3, g["m66_dpp"])):
+        r0 = np.zeros((1, 6))
+        r0[0, 5] = dp
+        A.forward(r0, 1, ckpt_every=1)
+        J = np.zeros((6, 6))
+        for i in range(6):
+            lam = np.zeros((1, 6))
+            lam[0, i] = 1.0
+            J[i] = A.backward(lam)[1][0]
+        err = np.abs(J - ref).max() / np.abs(ref).max()
+        print(f"dp={dp}: adjoint Jacobian vs reference getm66 {err:.2e}")
+        assert err <= 1e-7
