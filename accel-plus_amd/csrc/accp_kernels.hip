@@ -560,13 +560,19 @@ __global__ __launch_bounds__(256) void k_chunk_init(int32_t *idx, int64_t n, int
 __global__ __launch_bounds__(256) void k_make_tiles(const int32_t *cnt, const int64_t *offs,
                                                     int nseed, int per, TileDesc *tiles,
                                                     int32_t *ntiles) {
-  __shared__ int32_t base[257];
+  // Seeds in chunks of 256: an inclusive scan of their tile counts, then
+  // every thread writes tiles of the chunk (tile t belongs to the first seed
+  // whose inclusive count exceeds t).  Seed-major, tile order fixed.
+  __shared__ int32_t base[256];
+  __shared__ int32_t cnts[256];
+  __shared__ int64_t offs_s[256];
   int32_t total = 0;
   for (int s0 = 0; s0 < nseed; s0 += 256) {
     const int s = s0 + threadIdx.x;
-    const int32_t nt = s < nseed ? (cnt[s] + per - 1) / per : 0;
-    // block-wide exclusive scan (256 entries, fixed order)
-    base[threadIdx.x] = nt;
+    const int32_t c = s < nseed ? cnt[s] : 0;
+    cnts[threadIdx.x] = c;
+    offs_s[threadIdx.x] = s < nseed ? offs[s] : 0;
+    base[threadIdx.x] = (c + per - 1) / per;
     __syncthreads();
     for (int d = 1; d < 256; d <<= 1) {
       const int32_t v = threadIdx.x >= (unsigned)d ? base[threadIdx.x - d] : 0;
@@ -574,13 +580,19 @@ __global__ __launch_bounds__(256) void k_make_tiles(const int32_t *cnt, const in
       base[threadIdx.x] += v;
       __syncthreads();
     }
-    const int32_t first = total + base[threadIdx.x] - nt;
-    if (s < nseed)
-      for (int32_t k = 0; k < nt; ++k) {
-        const int32_t left = cnt[s] - k * per;
-        tiles[first + k] = TileDesc{s, left < per ? left : per, offs[s] + (int64_t)k * per};
+    const int32_t chunk = base[255];
+    for (int32_t t = threadIdx.x; t < chunk; t += 256) {
+      int lo = 0, hi = 255;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (base[mid] > t) hi = mid;
+        else lo = mid + 1;
       }
-    total += base[255];
+      const int32_t k = t - (lo ? base[lo - 1] : 0);
+      const int32_t left = cnts[lo] - k * per;
+      tiles[total + t] = TileDesc{s0 + lo, left < per ? left : per, offs_s[lo] + (int64_t)k * per};
+    }
+    total += chunk;
     __syncthreads();
   }
   if (threadIdx.x == 0) *ntiles = total;

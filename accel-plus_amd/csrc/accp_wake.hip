@@ -53,12 +53,22 @@ __global__ __launch_bounds__(WB) void k_wake_hist(const double *__restrict__ r, 
     if (h[i]) atomicAdd(&cnt[i], h[i]);
 }
 
+// V_k = qm * sum_{m >= k} cnt_m W((m - k) dz / c), summed in increasing m
+// like the oracle; counts and the wake table are staged in LDS first.
 __global__ __launch_bounds__(WB) void k_wake_conv(const uint32_t *__restrict__ cnt,
                                                   const double *__restrict__ wt, int nb, double qm,
                                                   double *__restrict__ v) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double *lw = reinterpret_cast<double *>(lds_raw);
+  uint32_t *lc = reinterpret_cast<uint32_t *>(lw + nb);
+  for (int i = threadIdx.x; i < nb; i += WB) {
+    lw[i] = wt[i];
+    lc[i] = cnt[i];
+  }
+  __syncthreads();
   for (int k = blockIdx.x * WB + threadIdx.x; k < nb; k += gridDim.x * WB) {
     double s = 0;
-    for (int m = k; m < nb; ++m) s += (double)cnt[m] * wt[m - k];
+    for (int m = k; m < nb; ++m) s += (double)lc[m] * lw[m - k];
     v[k] = qm * s;
   }
 }
@@ -120,7 +130,8 @@ int wake_apply(WakeState &w, int eidx, double *d_r, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_wake_zero, dim3((nb + WB - 1) / WB), dim3(WB), 0, s, we->d_cnt, nb);
   hipLaunchKernelGGL(k_wake_hist, dim3(grid), dim3(WB), sizeof(uint32_t) * nb, s, d_r, n, we->zmin,
                      we->dz, nb, we->d_cnt);
-  hipLaunchKernelGGL(k_wake_conv, dim3((nb + WB - 1) / WB), dim3(WB), 0, s, we->d_cnt, we->d_wt, nb,
+  hipLaunchKernelGGL(k_wake_conv, dim3((nb + WB - 1) / WB), dim3(WB), (sizeof(double) + sizeof(uint32_t)) * nb, s, we->d_cnt,
+                     we->d_wt, nb,
                      we->qm_charge / (double)n, we->d_v);
   hipLaunchKernelGGL(k_wake_kick, dim3(grid), dim3(WB), 0, s, d_r, n, we->zmin, we->dz, nb, we->d_v,
                      we->e0ev);
