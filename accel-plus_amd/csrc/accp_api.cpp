@@ -669,12 +669,12 @@ extern "C" int accp_batch_create(accp_lattice *const *seeds, int nseed, accp_bat
   for (int s = 0; s < nseed; ++s)
     if (!seeds[s]) return fail(ACCP_E_ARG, "seed %d is null", s);
   const int ne = (int)seeds[0]->elems.size();
+  // every workgroup reads its own seed's table, so the seeds need only the
+  // same element count (error seeds of one lattice, or one ring started at
+  // different elements)
   for (int s = 0; s < nseed; ++s) {
     if ((int)seeds[s]->elems.size() != ne) return fail(ACCP_E_LATTICE, "seed %d: element count differs", s);
     if (!seeds[s]->tab.wakes.empty()) return fail(ACCP_E_LATTICE, "seed %d: wake elements are not batched", s);
-    for (int i = 0; i < ne; ++i)
-      if (seeds[s]->elems[i].type != seeds[0]->elems[i].type)
-        return fail(ACCP_E_LATTICE, "seed %d: element %d differs in type", s, i);
   }
   accp_batch *b = new accp_batch;
   b->nseed = nseed;

@@ -8,10 +8,13 @@
 // keeps the reference's arithmetic (operation order, constants) so that its
 // results follow the reference's up to the tracking tolerance; built with
 // -ffp-contract=off like the rest of the host code.
+#include <hip/hip_runtime.h>
+
 #include <cmath>
 #include <cstdio>
 #include <exception>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/accp.h"
@@ -181,6 +184,68 @@ int resp_adjoint(accp_lattice *L, const Records &R, int quantity, const int32_t 
   return ACCP_OK;
 }
 
+// Solve A x = b (n <= 6) by Gaussian elimination with partial pivoting.
+void solve_small(int n, double *A, double *b) {
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(A[r * n + c]) > std::fabs(A[piv * n + c])) piv = r;
+    if (piv != c) {
+      for (int k = 0; k < n; ++k) std::swap(A[c * n + k], A[piv * n + k]);
+      std::swap(b[c], b[piv]);
+    }
+    for (int r = c + 1; r < n; ++r) {
+      const double f = A[r * n + c] / A[c * n + c];
+      for (int k = c; k < n; ++k) A[r * n + k] -= f * A[c * n + k];
+      b[r] -= f * b[c];
+    }
+  }
+  for (int r = n - 1; r >= 0; --r) {
+    double sum = b[r];
+    for (int k = r + 1; k < n; ++k) sum -= A[r * n + k] * b[k];
+    b[r] = sum / A[r * n + r];
+  }
+}
+
+// ALine::set_rad_onoff + set_cavity_onoff on the host records.
+int set_flags(accp_lattice *L, Records &R, int rad, int cav) {
+  for (auto &e : R.e) {
+    if (e.type == ACCP_BENDMPOLE) e.rad = rad;
+    if (e.type == ACCP_CAVITY) e.on = cav;
+  }
+  return R.store(L);
+}
+
+// Closed orbit rc = (I - M)^-1 r1 of the leading n x n block, r1 the
+// one-turn image of (0,0,0,0,0,dp).
+int closed_orbit(accp_lattice *L, int n, double dp, double *rc) {
+  double m[36], r[6] = {0, 0, 0, 0, 0, dp}, A[36];
+  int rc_ = accp_getm66(L, 0, -1, dp, ACCP_DELTA_DIFF, m);
+  accp_track_opts o{};
+  if (!rc_) rc_ = accp_track(L, r, 1, 1, 0, -1, &o);
+  if (rc_) return rc_;
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) A[i * n + j] = (i == j ? 1.0 : 0.0) - m[i * 6 + j];
+    rc[i] = r[i];
+  }
+  solve_small(n, A, rc);
+  return ACCP_OK;
+}
+
+// Least-squares slope of y on x, centred on the mean of x.
+double lsq_slope(int n, const double *x, const double *y) {
+  double sx = 0;
+  for (int i = 0; i < n; ++i) sx += x[i];
+  const double mx = sx / n;
+  double stt = 0, sty = 0;
+  for (int i = 0; i < n; ++i) {
+    const double t = x[i] - mx;
+    stt += t * t;
+    sty += t * y[i];
+  }
+  return sty / stt;
+}
+
 }  // namespace
 
 extern "C" int accp_getm66(accp_lattice *L, int st, int ed, double dp, double delta, double m66[36]) {
@@ -335,5 +400,163 @@ extern "C" int accp_trackda(accp_lattice *L, double xmax, double xmin, double ym
     else s += .5 * rd1 * rd2 * std::sin(dphi);
   }
   *area = s;
+  return ACCP_OK;
+}
+
+extern "C" int accp_getco4(accp_lattice *L, double dp, double rc[4]) {
+  if (!L || !rc) return fail(ACCP_E_ARG, "accp_getco4: null argument");
+  Records R;
+  int rc_ = R.load(L);
+  if (!rc_) rc_ = set_flags(L, R, ACCP_RAD_OFF, 0);  // aline.cpp:841-842
+  if (rc_) return rc_;
+  return closed_orbit(L, 4, dp, rc);
+}
+
+extern "C" int accp_getco6(accp_lattice *L, double rc[6]) {
+  if (!L || !rc) return fail(ACCP_E_ARG, "accp_getco6: null argument");
+  Records R;
+  int rc_ = R.load(L);
+  if (!rc_) rc_ = set_flags(L, R, ACCP_RAD_DAMP, 1);  // aline.cpp:874-875
+  if (rc_) return rc_;
+  return closed_orbit(L, 6, 0.0, rc);
+}
+
+extern "C" int accp_calctwiss(accp_lattice *L, const int32_t *idx, int n, double dp, double delta,
+                              double *tw) {
+  if (!L || !tw || n < 0 || (n > 0 && !idx)) return fail(ACCP_E_ARG, "accp_calctwiss: bad arguments");
+  Records R;
+  int rc = R.load(L);
+  if (rc) return rc;
+  const int ne = (int)R.e.size();
+  for (int i = 0; i < n; ++i)
+    if (idx[i] < 0 || idx[i] > ne || (i && idx[i] < idx[i - 1]))
+      return fail(ACCP_E_ARG, "accp_calctwiss: indices must be increasing and within the line");
+  // aline.cpp:1097-1106: periodic functions from the one-turn matrix
+  double m[36];
+  if ((rc = accp_getm66(L, 0, -1, dp, delta, m))) return rc;
+  double t0[7];
+  const double psix = std::acos((m[0] + m[7]) * 0.5), psiy = std::acos((m[14] + m[21]) * 0.5);
+  t0[0] = 0;
+  t0[1] = psix;
+  t0[2] = psiy;
+  t0[3] = m[1] / std::sin(psix);
+  t0[4] = (m[0] - m[7]) / std::sin(psix) / 2.0;
+  t0[5] = m[15] / std::sin(psiy);
+  t0[6] = (m[14] - m[21]) / std::sin(psiy) / 2.0;
+  if (n == 0) {
+    for (int k = 0; k < 7; ++k) tw[k] = t0[k];
+    return ACCP_OK;
+  }
+  // aline.cpp:1057-1093: carried from point to point
+  t0[1] = t0[2] = 0;
+  auto length = [&](int st, int ed) {
+    double l = 0;
+    for (int k = st; k < ed; ++k) l += R.e[k].len;
+    return l;
+  };
+  int id = idx[0];
+  double T[36];
+  if ((rc = accp_getm66(L, 0, id, dp, delta, T))) return rc;
+  double len01 = length(0, id);
+  for (int i = 0; i < n; ++i) {
+    const double bx = t0[3], ax = t0[4], by = t0[5], ay = t0[6];
+    double *t1 = tw + 7 * i;
+    auto sq = [](double v) { return v * v; };
+    t1[0] = t0[0] + len01;
+    t1[3] = 1.0 / bx * (sq(T[0] * bx - T[1] * ax) + sq(T[1]));
+    t1[5] = 1.0 / by * (sq(T[14] * by - T[15] * ay) + sq(T[15]));
+    t1[4] = -1.0 / bx * ((T[0] * bx - T[1] * ax) * (T[6] * bx - T[7] * ax) + T[1] * T[7]);
+    t1[6] = -1.0 / by * ((T[14] * by - T[15] * ay) * (T[20] * by - T[21] * ay) + T[15] * T[21]);
+    t1[1] = t0[1] + std::atan2(T[1], T[0] * bx - T[1] * ax);
+    t1[2] = t0[2] + std::atan2(T[15], T[14] * by - T[15] * ay);
+    for (int k = 0; k < 7; ++k) t0[k] = t1[k];
+    if (i < n - 1) {
+      const int id2 = idx[i + 1];
+      if ((rc = accp_getm66(L, id, id2, dp, delta, T))) return rc;
+      len01 = length(id, id2);
+      id = id2;
+    }
+  }
+  return ACCP_OK;
+}
+
+extern "C" int accp_calcchrom2(accp_lattice *L, double c[4]) {
+  if (!L || !c) return fail(ACCP_E_ARG, "accp_calcchrom2: null argument");
+  // aline.cpp:1133-1157
+  double cx[11], cy[11], dp[11], v[2];
+  for (int i = 0; i < 11; ++i) {
+    dp[i] = 1.0e-3 * (i - 6.0);
+    const int rc = accp_calcchrom(L, dp[i], ACCP_DELTA_DIFF, v);
+    if (rc) return rc;
+    cx[i] = v[0];
+    cy[i] = v[1];
+  }
+  c[2] = lsq_slope(11, dp, cx) / 2.0;
+  c[3] = lsq_slope(11, dp, cy) / 2.0;
+  const int rc = accp_calcchrom(L, 0.0, ACCP_DELTA_DIFF, v);
+  c[0] = v[0];
+  c[1] = v[1];
+  return rc;
+}
+
+extern "C" int accp_trackma(accp_lattice *L, double dppmax, double dppmin, int nturn, int nstep,
+                            const int32_t *locs, int nloc, double *dpp_plus, double *dpp_minus, double *avg) {
+  if (!L || !locs || nloc <= 0 || !dpp_plus || !dpp_minus || !avg)
+    return fail(ACCP_E_ARG, "accp_trackma: bad arguments");
+  if (!(dppmax > 0) || !(dppmin > 0) || nstep < 1 || nturn < 0)  // aline.cpp:783-784
+    return fail(ACCP_E_ARG, "accp_trackma: dppmax and dppmin must be positive, nstep at least 1");
+  Records R;
+  int rc = R.load(L);
+  if (rc) return rc;
+  const int ne = (int)R.e.size();
+  for (int k = 0; k < nloc; ++k)
+    if (locs[k] < 0 || locs[k] >= ne) return fail(ACCP_E_ARG, "accp_trackma: location out of range");
+  // One lattice per location, the ring started there (resetfirstelement,
+  // aline.cpp:557-606), all tracked in one batched launch.
+  std::vector<accp_lattice *> lats(nloc, nullptr);
+  accp_batch *b = nullptr;
+  double *d_r = nullptr;
+  const int np = 2 * (nstep + 1);
+  const size_t bytes = sizeof(double) * 6 * (size_t)np * nloc;
+  std::vector<double> r((size_t)6 * np * nloc, 0.0);
+  std::vector<int64_t> offs(nloc + 1);
+  const double stepsize = (dppmax - dppmin) / nstep;
+  for (int k = 0; k < nloc && !rc; ++k) {
+    std::vector<accp_elem> rot(ne);
+    for (int i = 0; i < ne; ++i) rot[i] = R.e[(locs[k] + i) % ne];
+    rc = accp_lattice_build(rot.data(), ne, &lats[k]);
+    double *q = r.data() + (size_t)6 * np * k;
+    for (int j = 0; j < nstep + 1; ++j) {  // aline.cpp:797-801
+      q[j * 12 + 5] = dppmin + stepsize * j;
+      q[j * 12 + 6 + 5] = -(dppmin + stepsize * j);
+    }
+    offs[k] = (int64_t)np * k;
+  }
+  offs[nloc] = (int64_t)np * nloc;
+  if (!rc) rc = accp_batch_create(lats.data(), nloc, &b);
+  if (!rc && hipMalloc(&d_r, bytes) != hipSuccess) rc = fail(ACCP_E_NOMEM, "accp_trackma: hipMalloc failed");
+  if (!rc && hipMemcpy(d_r, r.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(ACCP_E_HIP, "accp_trackma: upload failed");
+  if (!rc) rc = accp_track_batched(b, d_r, offs.data(), nturn, nullptr, nullptr, nullptr);
+  if (!rc && (hipDeviceSynchronize() != hipSuccess ||
+              hipMemcpy(r.data(), d_r, bytes, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = fail(ACCP_E_HIP, "accp_trackma: tracking or download failed");
+  if (d_r) (void)hipFree(d_r);
+  accp_batch_free(b);
+  for (accp_lattice *l : lats) accp_lattice_free(l);
+  if (rc) return rc;
+  // aline.cpp:818-831: the largest offset that survives, on each side
+  double sum = 0;
+  for (int k = 0; k < nloc; ++k) {
+    const double *q = r.data() + (size_t)6 * np * k;
+    dpp_plus[k] = dppmin;
+    dpp_minus[k] = -dppmin;
+    for (int j = 1; j < nstep + 1; ++j) {
+      if (!std::isnan(q[j * 12])) dpp_plus[k] = dppmin + stepsize * j;
+      if (!std::isnan(q[j * 12 + 6])) dpp_minus[k] = -(dppmin + stepsize * j);
+    }
+    sum += std::fmin(dpp_plus[k], -dpp_minus[k]);
+  }
+  *avg = sum / nloc;
   return ACCP_OK;
 }

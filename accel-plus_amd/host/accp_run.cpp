@@ -16,14 +16,16 @@
 //            modify_element: type=, famname=, index=, field=, value=|delta=;
 //            correct_tune / correct_chrom: type=, famname=(F1 F2) [a, b]
 //            [REPEAT] [METHOD=FD|ADJOINT]; set_max_num_threads=N (accepted)
-//   Optics:  print_tune / print_chrom [dpp=, delta=]; print_matrix: type=RING
+//   Optics:  print_tune / print_chrom [dpp=, delta=]; print_2nd_chrom;
+//            print_matrix: type=RING; print_optics: type=, famname=, index=
 //   Track:   Distribution: Zero NPar= | Load f[, format=plain|accel-export] |
 //            Coordinate r=(...;...) | Gaussian NPar= sigx= ... | Linear X=a:d:b
 //            [shift=(...)] [print]; Monitor: interval=, fraction=,
 //            printflag=Coordinate|Stat1|Stat2; Do_Track: NTurn=,
 //            radiation=, RFCavity=, PrintFinal=No|Yes|Full [, seed=]
 //   DAMA:    TRACK_DA: xmin=, xmax=, ymax=, nray=, nstep=, nturn=
-//            [, print=loss_turn] [, weightminusside=true]
+//            [, print=loss_turn] [, weightminusside=true];
+//            TRACK_MA: type=, [famname=,] nmoni=, dppmin=, dppmax=, nstep=, nturn=
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -477,18 +479,53 @@ void proc_lattice(std::istream &in, Line &L) {
 
 // ---- Optics block (process.cpp:1107-1395): tunes, chromaticities, one-turn matrix
 void proc_optics(std::istream &in, Line &L) {
-  const std::vector<string> keys = {"PRINT_MATRIX", "PRINT_TUNE", "PRINT_CHROM", "END"};
+  const std::vector<string> keys = {"PRINT_MATRIX", "PRINT_TUNE", "PRINT_2ND_CHROM", "PRINT_CHROM",
+                                    "PRINT_OPTICS", "END"};
   string line;
   for (;;) {
     const int k = look_for(in, line, keys);
     if (k < 0 || keys[k] == "END") return;
     double dpp = 0, delta = ACCP_DELTA_DIFF;
+    string type, fam;
+    unsigned index = 0;
     for (const auto &kv : key_values(line)) {
       if (kv.first == "DPP") dpp = std::stod(kv.second);
       else if (kv.first == "DELTA") delta = std::stod(kv.second);
+      else if (kv.first == "TYPE") type = kv.second;
+      else if (kv.first == "FAMNAME") fam = kv.second;
+      else if (kv.first == "INDEX") index = (unsigned)std::stoul(kv.second);
     }
     if (L.parse_only) continue;
+    if (k == 4) {
+      // PRINT_OPTICS (process.cpp:1122-1226): Twiss at the selected elements
+      std::vector<int32_t> list;
+      if (type.empty()) list = {(int32_t)index};
+      else if (ieq(type, "ALL"))
+        for (size_t i = 0; i < L.e.size(); ++i) list.push_back((int32_t)i);
+      else list = L.find(type, fam, index);
+      std::cout << "nlist=" << list.size() << std::endl;
+      if (list.empty()) continue;
+      std::vector<double> tw(7 * list.size());
+      check(accp_calctwiss(L.device(), list.data(), (int)list.size(), dpp, delta, tw.data()), "accp_calctwiss");
+      std::cout << "$$$Optics-Twiss, dp=" << fmt(dpp) << "*************************** " << list.size() << " points"
+                << std::endl;
+      std::cout << "index\ts (m)\tbetx (m)\tbety (m)\talfx\talfy\tpsix\tpsiy\n";
+      for (size_t i = 0; i < list.size(); ++i) {
+        const double *t = &tw[7 * i];
+        std::cout << list[i] << "\t" << fmt(t[0]) << "\t" << fmt(t[3]) << "\t" << fmt(t[5]) << "\t" << fmt(t[4])
+                  << "\t" << fmt(t[6]) << "\t" << fmt(t[1]) << "\t" << fmt(t[2]) << "\t" << std::endl;
+      }
+      continue;
+    }
     L.set_rad(ACCP_RAD_OFF);
+    if (k == 2) {
+      // PRINT_2ND_CHROM (process.cpp:1380-1388)
+      double c[4];
+      check(accp_calcchrom2(L.device(), c), "accp_calcchrom2");
+      std::cout << "Chromaticity: [" << fmt(c[0]) << "\t" << fmt(c[1]) << "]" << std::endl;
+      std::cout << "2nd-order Chromaticity: [" << fmt(c[2]) << "\t" << fmt(c[3]) << "], " << std::endl;
+      continue;
+    }
     if (k == 0) {
       double m[36];
       check(accp_getm66(L.device(), 0, -1, dpp, delta, m), "accp_getm66");
@@ -790,11 +827,45 @@ void proc_track(std::istream &in, Line &L) {
 
 // ---- DAMA block (process.cpp:2393-2500) --------------------------------------------
 void proc_dama(std::istream &in, Line &L) {
-  const std::vector<string> keys = {"TRACK_DA", "END"};
+  const std::vector<string> keys = {"TRACK_DA", "TRACK_MA", "END"};
   string line;
   for (;;) {
     const int k = look_for(in, line, keys);
     if (k < 0 || keys[k] == "END") return;
+    if (k == 1) {
+      // TRACK_MA (process.cpp:2503-2580): locations = elements of type
+      // [and famname], the first nmoni of them
+      string type, fam;
+      unsigned nmoni = 0;
+      int nstep = 0, nturn = 0;
+      double dppmin = 0, dppmax = 0;
+      for (const auto &kv : key_values(line)) {
+        if (kv.first == "TYPE") type = kv.second;
+        else if (kv.first == "FAMNAME") fam = kv.second;
+        else if (kv.first == "DPPMIN") dppmin = std::stod(kv.second);
+        else if (kv.first == "DPPMAX") dppmax = std::stod(kv.second);
+        else if (kv.first == "NMONI") nmoni = (unsigned)std::stoul(kv.second);
+        else if (kv.first == "NSTEP") nstep = std::stoi(kv.second);
+        else if (kv.first == "NTURN") nturn = std::stoi(kv.second);
+      }
+      std::cout << "#" << nmoni << " " << nstep << " " << nturn << " " << fmt(dppmin) << " " << fmt(dppmax)
+                << std::endl;
+      std::vector<int32_t> list = L.find(type, fam);
+      if (nmoni > list.size()) nmoni = (unsigned)list.size();
+      list.resize(nmoni);
+      if (L.parse_only || list.empty()) continue;
+      L.set_rad(ACCP_RAD_DAMP);
+      L.set_on(ACCP_CAVITY, 1);
+      std::vector<double> plus(nmoni), minus(nmoni);
+      double avg = 0;
+      check(accp_trackma(L.device(), dppmax, dppmin, nturn, nstep, list.data(), (int)nmoni, plus.data(),
+                         minus.data(), &avg),
+            "accp_trackma");
+      std::cout << "$$$MA " << nturn << " turns, average MA=" << fmt(avg) << std::endl;
+      for (unsigned i = 0; i < nmoni; ++i) std::cout << fmt(plus[i]) << " \t" << fmt(minus[i]) << " \t" << std::endl;
+      std::cout << "******************************" << std::endl;
+      continue;
+    }
     int nray = 0, nstep = 0, nturn = 0;
     double xmin = 0, xmax = 0, ymax = 0;
     bool loss = false, neg = false;

@@ -59,7 +59,8 @@ EXPORTS = (
     "accp_adjoint_create", "accp_adjoint_forward", "accp_adjoint_backward", "accp_adjoint_set_reverse",
     "accp_adjoint_free",
     "accp_lattice_get", "accp_getm66", "accp_calctune", "accp_calcchrom", "accp_calcresp", "accp_correct",
-    "accp_setrfphase", "accp_trackda",
+    "accp_setrfphase", "accp_trackda", "accp_getco4", "accp_getco6", "accp_calctwiss", "accp_calcchrom2",
+    "accp_trackma",
 )
 
 DP = POINTER(c_double)
@@ -210,6 +211,40 @@ class Lattice:
             "accp_trackda")
         out = (area.value, rx[:nr], ry[:nr])
         return out + (lt,) if loss_turn else out
+
+    def getco4(self, dp=0.0):
+        rc = np.zeros(4)
+        self.L.check(self.L.lib.accp_getco4(self.h, c_double(dp), _dp(rc)), "accp_getco4")
+        self.arr = self.get()
+        return rc
+
+    def getco6(self):
+        rc = np.zeros(6)
+        self.L.check(self.L.lib.accp_getco6(self.h, _dp(rc)), "accp_getco6")
+        self.arr = self.get()
+        return rc
+
+    def calctwiss(self, idx=(), dp=0.0, delta=1e-8):
+        """Rows {s, psix, psiy, betx, alfx, bety, alfy}: one at the ring start
+        when idx is empty, else one per listed element."""
+        ix = np.ascontiguousarray(idx, dtype=np.int32)
+        tw = np.zeros((max(len(ix), 1), 7))
+        self.L.check(self.L.lib.accp_calctwiss(self.h, ix.ctypes.data_as(POINTER(c_int32)), len(ix), c_double(dp),
+                                               c_double(delta), _dp(tw)), "accp_calctwiss")
+        return tw[0] if len(ix) == 0 else tw
+
+    def calcchrom2(self):
+        c = np.zeros(4)
+        self.L.check(self.L.lib.accp_calcchrom2(self.h, _dp(c)), "accp_calcchrom2")
+        return c
+
+    def trackma(self, dppmax, dppmin, nturn, nstep, locs):
+        lc = np.ascontiguousarray(locs, dtype=np.int32)
+        p, m, avg = np.zeros(len(lc)), np.zeros(len(lc)), c_double(0.0)
+        self.L.check(self.L.lib.accp_trackma(self.h, c_double(dppmax), c_double(dppmin), nturn, nstep,
+                                             lc.ctypes.data_as(POINTER(c_int32)), len(lc), _dp(p), _dp(m),
+                                             byref(avg)), "accp_trackma")
+        return avg.value, p, m
 
     def track(self, r, nturn=1, st=0, ed=-1, seed=0, turn0=0, pid0=0, tuning=None,
               mon_interval=0, mon_max=0, loss_turn=False):

@@ -105,7 +105,7 @@ typedef struct accp_elem {
 } accp_elem;
 
 typedef struct accp_lattice accp_lattice; /* device-resident element table   */
-typedef struct accp_batch accp_batch;     /* many seeds, one topology         */
+typedef struct accp_batch accp_batch;     /* many lattices, one element count */
 typedef struct accp_adjoint accp_adjoint; /* reverse-mode turn map context    */
 
 /* Element-table staging inside the tracking kernel. */
@@ -174,7 +174,8 @@ int accp_track_device(accp_lattice *lat, double *d_r, int n, int nturn, int st,
 int accp_beam_stat(const double *r, int n, double *stat /* ACCP_STAT_LEN */);
 
 /* ---- batched multi-seed tracking ---------------------------------------- */
-/* All seeds must share one topology (types, slices, orders). */
+/* All seeds must have the same number of elements (error seeds of one
+ * lattice, or one ring started at different elements) and no wake element. */
 int accp_batch_create(accp_lattice *const *seeds, int nseed, accp_batch **out);
 /* Track all seeds in one launch sequence.  d_r is one device AoS buffer;
  * particles of seed s are [offsets[s], offsets[s+1]) (offsets: host array of
@@ -287,6 +288,35 @@ int accp_setrfphase(accp_lattice *lat, double *phis);
 int accp_trackda(accp_lattice *lat, double xmax, double xmin, double ymax, int nturn,
                  int nray, int nstep, int negweight, double *rx, double *ry,
                  double *area, int32_t *loss_turn);
+/* ALine::getco4 (src/lib/accel/aline.cpp:838-870): 4D closed orbit at dp,
+ * rc = (I - M44)^-1 r1 with M44 the transverse block of getm66 at dp and r1
+ * the one-turn image of (0,0,0,0,0,dp); radiation and cavities are switched
+ * off, as the reference does.  (The reference's own M44 is not the block of
+ * getm66: its m66.submatrix call re-points m66 at the uninitialised m44, so
+ * its result is r1 alone; this computes the closed orbit.) */
+int accp_getco4(accp_lattice *lat, double dp, double rc[4]);
+/* ALine::getco6 (src/lib/accel/aline.cpp:871-889): 6D closed orbit with
+ * classical radiation and the cavities on (the flags are left so). */
+int accp_getco6(accp_lattice *lat, double rc[6]);
+/* ALine::calctwiss (src/lib/accel/aline.cpp:1044-1107): with n = 0 the
+ * periodic Twiss functions at the ring start, else carried to elements
+ * idx[0..n) (increasing) by the transfer matrices between them; per point
+ * tw[7*i..] = {s, psix, psiy, betx, alfx, bety, alfy}. */
+int accp_calctwiss(accp_lattice *lat, const int32_t *idx, int n, double dp, double delta,
+                   double *tw);
+/* ALine::calcchrom2 (src/lib/accel/aline.cpp:1127-1158): c = {chromx,
+ * chromy, chromx2, chromy2}, the second-order terms from the least-squares
+ * slope of the chromaticity over dpp = 1e-3 (i - 6), i = 0..10. */
+int accp_calcchrom2(accp_lattice *lat, double c[4]);
+/* ALine::trackMA (src/lib/accel/aline.cpp:781-836): for each location
+ * locs[k], the ring started there, 2 (nstep+1) particles at
+ * +-(dppmin + j (dppmax-dppmin)/nstep), nturn turns; dpp_plus[k] and
+ * dpp_minus[k] receive the largest surviving offsets on each side, *avg the
+ * mean of min(dpp_plus, -dpp_minus).  All locations are tracked in one
+ * batched launch. */
+int accp_trackma(accp_lattice *lat, double dppmax, double dppmin, int nturn, int nstep,
+                 const int32_t *locs, int nloc, double *dpp_plus, double *dpp_minus,
+                 double *avg);
 
 #ifdef __cplusplus
 }

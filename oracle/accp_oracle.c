@@ -684,6 +684,174 @@ double orc_trackda(const accp_elem *elems, int nelem, double xmax, double xmin, 
   return s;
 }
 
+/* Solve A x = b (n <= 6) by Gaussian elimination with partial pivoting. */
+static void solve_small(int n, double *A, double *b) {
+  for (int c = 0; c < n; c++) {
+    int piv = c;
+    for (int r = c + 1; r < n; r++)
+      if (fabs(A[r * n + c]) > fabs(A[piv * n + c])) piv = r;
+    if (piv != c) {
+      for (int k = 0; k < n; k++) {
+        double t = A[c * n + k];
+        A[c * n + k] = A[piv * n + k];
+        A[piv * n + k] = t;
+      }
+      double t = b[c];
+      b[c] = b[piv];
+      b[piv] = t;
+    }
+    for (int r = c + 1; r < n; r++) {
+      double f = A[r * n + c] / A[c * n + c];
+      for (int k = c; k < n; k++) A[r * n + k] -= f * A[c * n + k];
+      b[r] -= f * b[c];
+    }
+  }
+  for (int r = n - 1; r >= 0; r--) {
+    double s = b[r];
+    for (int k = r + 1; k < n; k++) s -= A[r * n + k] * b[k];
+    b[r] = s / A[r * n + r];
+  }
+}
+
+static void set_flags(accp_elem *elems, int nelem, int rad, int cav) {
+  for (int k = 0; k < nelem; k++) {
+    if (elems[k].type == ACCP_BENDMPOLE) elems[k].rad = rad;
+    if (elems[k].type == ACCP_CAVITY) elems[k].on = cav;
+  }
+}
+
+/* aline.cpp:838-870 */
+void orc_getco4(accp_elem *elems, int nelem, double dp, double rc[4]) {
+  set_flags(elems, nelem, 0, 0);
+  double m[36], r[6] = {0, 0, 0, 0, 0, dp}, A[16];
+  orc_getm66(elems, nelem, 0, nelem, dp, ORC_DELTA_DIFF, m);
+  orc_line_pass(elems, nelem, r, 1, 0, nelem, 1, 0, 0, 0);
+  for (int i = 0; i < 4; i++) {
+    for (int j = 0; j < 4; j++) A[i * 4 + j] = (i == j ? 1.0 : 0.0) - m[i * 6 + j];
+    rc[i] = r[i];
+  }
+  solve_small(4, A, rc);
+}
+
+/* aline.cpp:871-889 */
+void orc_getco6(accp_elem *elems, int nelem, double rc[6]) {
+  set_flags(elems, nelem, 1, 1);
+  double m[36], r[6] = {0, 0, 0, 0, 0, 0}, A[36];
+  orc_getm66(elems, nelem, 0, nelem, 0.0, ORC_DELTA_DIFF, m);
+  orc_line_pass(elems, nelem, r, 1, 0, nelem, 1, 0, 0, 0);
+  for (int i = 0; i < 6; i++) {
+    for (int j = 0; j < 6; j++) A[i * 6 + j] = (i == j ? 1.0 : 0.0) - m[i * 6 + j];
+    rc[i] = r[i];
+  }
+  solve_small(6, A, rc);
+}
+
+/* aline.cpp:1095-1107 */
+void orc_calctwiss0(const accp_elem *elems, int nelem, double dp, double tw[7]) {
+  double m[36];
+  orc_getm66(elems, nelem, 0, nelem, dp, ORC_DELTA_DIFF, m);
+  double psix = acos((m[0] + m[7]) * 0.5), psiy = acos((m[14] + m[21]) * 0.5);
+  tw[0] = 0;
+  tw[1] = psix;
+  tw[2] = psiy;
+  tw[3] = m[1] / sin(psix);
+  tw[4] = (m[0] - m[7]) / sin(psix) / 2.0;
+  tw[5] = m[15] / sin(psiy);
+  tw[6] = (m[14] - m[21]) / sin(psiy) / 2.0;
+}
+
+static double line_length(const accp_elem *elems, int st, int ed) {
+  double s = 0;
+  for (int k = st; k < ed; k++) s += elems[k].len;
+  return s;
+}
+
+/* aline.cpp:1044-1094: Twiss at the start, then carried point to point by
+ * the transfer matrices between the listed elements. */
+void orc_calctwiss(const accp_elem *elems, int nelem, const int *idx, int n, double dp,
+                   double *tw) {
+  double t0[7], T[36];
+  orc_calctwiss0(elems, nelem, dp, t0);
+  t0[1] = t0[2] = 0;
+  int id = idx[0];
+  orc_getm66(elems, nelem, 0, id, dp, ORC_DELTA_DIFF, T);
+  double len01 = line_length(elems, 0, id);
+  for (int i = 0; i < n; i++) {
+    const double bx = t0[3], ax = t0[4], by = t0[5], ay = t0[6];
+    double *t1 = tw + 7 * i;
+    t1[0] = t0[0] + len01;
+    t1[3] = 1.0 / bx * (sq(T[0] * bx - T[1] * ax) + sq(T[1]));
+    t1[5] = 1.0 / by * (sq(T[14] * by - T[15] * ay) + sq(T[15]));
+    t1[4] = -1.0 / bx * ((T[0] * bx - T[1] * ax) * (T[6] * bx - T[7] * ax) + T[1] * T[7]);
+    t1[6] = -1.0 / by * ((T[14] * by - T[15] * ay) * (T[20] * by - T[21] * ay) + T[15] * T[21]);
+    t1[1] = t0[1] + atan2(T[1], T[0] * bx - T[1] * ax);
+    t1[2] = t0[2] + atan2(T[15], T[14] * by - T[15] * ay);
+    for (int k = 0; k < 7; k++) t0[k] = t1[k];
+    if (i < n - 1) {
+      int id2 = idx[i + 1];
+      orc_getm66(elems, nelem, id, id2, dp, ORC_DELTA_DIFF, T);
+      len01 = line_length(elems, id, id2);
+      id = id2;
+    }
+  }
+}
+
+/* Least-squares slope of y on x, centred on the mean of x. */
+static double lsq_slope(int n, const double *x, const double *y) {
+  double sx = 0;
+  for (int i = 0; i < n; i++) sx += x[i];
+  const double mx = sx / n;
+  double stt = 0, sty = 0;
+  for (int i = 0; i < n; i++) {
+    const double t = x[i] - mx;
+    stt += t * t;
+    sty += t * y[i];
+  }
+  return sty / stt;
+}
+
+/* aline.cpp:1127-1158 */
+void orc_calcchrom2(const accp_elem *elems, int nelem, double c[4]) {
+  double cx[11], cy[11], dp[11];
+  for (int i = 0; i < 11; i++) {
+    dp[i] = 1.0e-3 * (i - 6.0);
+    orc_calcchrom(elems, nelem, dp[i], ORC_DELTA_DIFF, &cx[i], &cy[i]);
+  }
+  c[2] = lsq_slope(11, dp, cx) / 2.0;
+  c[3] = lsq_slope(11, dp, cy) / 2.0;
+  orc_calcchrom(elems, nelem, 0.0, ORC_DELTA_DIFF, &c[0], &c[1]);
+}
+
+/* aline.cpp:781-836; resetfirstelement (aline.cpp:557-606) as a rotated copy */
+double orc_trackma(const accp_elem *elems, int nelem, double dppmax, double dppmin, int nturn,
+                   int nstep, const int *locs, int nloc, double *dpp_plus, double *dpp_minus) {
+  const int np = 2 * (nstep + 1);
+  double *r = (double *)malloc(sizeof(double) * 6 * (size_t)np);
+  accp_elem *rot = (accp_elem *)malloc(sizeof(accp_elem) * (size_t)nelem);
+  const double stepsize = (dppmax - dppmin) / nstep;
+  double avg = 0;
+  for (int i = 0; i < nloc; i++) {
+    const int e0 = locs[i];
+    for (int k = 0; k < nelem; k++) rot[k] = elems[(e0 + k) % nelem];
+    memset(r, 0, sizeof(double) * 6 * (size_t)np);
+    for (int j = 0; j < nstep + 1; j++) {
+      r[j * 12 + 5] = dppmin + stepsize * j;
+      r[j * 12 + 6 + 5] = -(dppmin + stepsize * j);
+    }
+    orc_line_pass(rot, nelem, r, np, 0, nelem, nturn, 0, 0, 0);
+    dpp_plus[i] = dppmin;
+    dpp_minus[i] = -dppmin;
+    for (int j = 1; j < nstep + 1; j++) {
+      if (!isnan(r[j * 12])) dpp_plus[i] = dppmin + stepsize * j;
+      if (!isnan(r[j * 12 + 6])) dpp_minus[i] = -(dppmin + stepsize * j);
+    }
+    avg += fmin(dpp_plus[i], -dpp_minus[i]);
+  }
+  free(r);
+  free(rot);
+  return avg / nloc;
+}
+
 /* utility.cpp:322-373 */
 int orc_beam_stat(const double *r, int n, double turn, double *stat) {
   double ar[6] = {0}, sg[6] = {0}, cr[6] = {0};

@@ -89,6 +89,33 @@ double orc_setrfphase(accp_elem *elems, int nelem);
 double orc_trackda(const accp_elem *elems, int nelem, double xmax, double xmin,
                    double ymax, int nturn, int nray, int nstep, double *rx,
                    double *ry, int negweight);
+/* ALine::getco4 (aline.cpp:838-870): 4D closed orbit at dp with radiation
+ * and cavities off (the flags are left so, as the reference leaves them):
+ * rc = (I - M44)^-1 r1 with r1 the one-turn image of (0,0,0,0,0,dp).  The
+ * reference inverts by SVD (matvec.cpp:269-284); this solves the same
+ * system by Gaussian elimination with partial pivoting. */
+void orc_getco4(accp_elem *elems, int nelem, double dp, double rc[4]);
+/* ALine::getco6 (aline.cpp:871-889): 6D, radiation (classical) and
+ * cavities on, rc = (I - M66)^-1 r1 from the origin. */
+void orc_getco6(accp_elem *elems, int nelem, double rc[6]);
+/* ALine::calctwiss at the ring start (aline.cpp:1095-1107) and propagated
+ * to elements idx[0..n) (aline.cpp:1044-1094); per point
+ * {s, psix, psiy, betx, alfx, bety, alfy}. */
+void orc_calctwiss0(const accp_elem *elems, int nelem, double dp, double tw[7]);
+void orc_calctwiss(const accp_elem *elems, int nelem, const int *idx, int n, double dp,
+                   double *tw);
+/* ALine::calcchrom2 (aline.cpp:1127-1158): chromaticities at
+ * dpp = 1e-3 (i - 6), i = 0..10, a least-squares line through them
+ * (slope / 2 = second-order chromaticity); c = {chromx, chromy, chromx2,
+ * chromy2}.  The line is the textbook centred least-squares slope. */
+void orc_calcchrom2(const accp_elem *elems, int nelem, double c[4]);
+/* ALine::trackMA (aline.cpp:781-836): for each location, the ring started
+ * there, 2 (nstep+1) particles at +-(dppmin + j step), nturn turns; the
+ * largest surviving offset on each side; returns the average of
+ * min(dpp+, -dpp-). */
+double orc_trackma(const accp_elem *elems, int nelem, double dppmax, double dppmin, int nturn,
+                   int nstep, const int *locs, int nloc, double *dpp_plus, double *dpp_minus);
+
 /* calc_beam_stat (utility.cpp:322-373) into an ACCP_STAT_LEN record. */
 int orc_beam_stat(const double *r, int n, double turn, double *stat);
 

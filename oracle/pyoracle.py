@@ -56,6 +56,13 @@ class Oracle:
         L.orc_wake_w.argtypes = [c_double, c_double, c_double, c_double]
         L.orc_wake_w.restype = c_double
         L.orc_wake_pairwise.argtypes = [E, DP, c_int, DP]
+        L.orc_getco4.argtypes = [E, c_int, c_double, DP]
+        L.orc_getco6.argtypes = [E, c_int, DP]
+        L.orc_calctwiss0.argtypes = [E, c_int, c_double, DP]
+        L.orc_calctwiss.argtypes = [E, c_int, IP, c_int, c_double, DP]
+        L.orc_calcchrom2.argtypes = [E, c_int, DP]
+        L.orc_trackma.argtypes = [E, c_int, c_double, c_double, c_int, c_int, IP, c_int, DP, DP]
+        L.orc_trackma.restype = c_double
         for nm in ("orc_quad6", "orc_drift_kzero", "orc_fastdrift"):
             getattr(L, nm).argtypes = [DP, c_double] + ([c_double] if nm == "orc_quad6" else [])
         self.lib = L
@@ -128,6 +135,44 @@ class Oracle:
         self.lib.orc_beam_stat(_dp(r), r.size // 6, turn, _dp(out))
         return out
 
+    def getco4(self, elems_arr, dp=0.0):
+        """Modifies the flags of elems_arr (radiation and cavities off)."""
+        rc = np.zeros(4)
+        self.lib.orc_getco4(elems_arr, len(elems_arr), dp, _dp(rc))
+        return rc
+
+    def getco6(self, elems_arr):
+        """Modifies the flags of elems_arr (radiation and cavities on)."""
+        rc = np.zeros(6)
+        self.lib.orc_getco6(elems_arr, len(elems_arr), _dp(rc))
+        return rc
+
+    def calctwiss0(self, elems, dp=0.0):
+        a = self._arr(elems)
+        tw = np.zeros(7)
+        self.lib.orc_calctwiss0(a, len(a), dp, _dp(tw))
+        return tw
+
+    def calctwiss(self, elems, idx, dp=0.0):
+        a = self._arr(elems)
+        ix = np.ascontiguousarray(idx, dtype=np.int32)
+        tw = np.zeros((len(ix), 7))
+        self.lib.orc_calctwiss(a, len(a), _ip(ix), len(ix), dp, _dp(tw))
+        return tw
+
+    def calcchrom2(self, elems):
+        a = self._arr(elems)
+        c = np.zeros(4)
+        self.lib.orc_calcchrom2(a, len(a), _dp(c))
+        return c
+
+    def trackma(self, elems, dppmax, dppmin, nturn, nstep, locs):
+        a = self._arr(elems)
+        lc = np.ascontiguousarray(locs, dtype=np.int32)
+        p, m = np.zeros(len(lc)), np.zeros(len(lc))
+        avg = self.lib.orc_trackma(a, len(a), dppmax, dppmin, nturn, nstep, _ip(lc), len(lc), _dp(p), _dp(m))
+        return avg, p, m
+
     def philox(self, ctr, key):
         c = (c_uint32 * 4)(*ctr)
         k = (c_uint32 * 2)(*key)
@@ -184,6 +229,14 @@ class Reference:
         L.ref_getfield.restype = c_double
         L.ref_get_mpole.argtypes = [c_void_p, c_int, c_char_p, c_int]
         L.ref_get_mpole.restype = c_double
+        IPT = POINTER(c_int)
+        L.ref_getco4.argtypes = [c_void_p, c_double, DP]
+        L.ref_getco6.argtypes = [c_void_p, DP]
+        L.ref_calctwiss0.argtypes = [c_void_p, c_double, DP]
+        L.ref_calctwiss.argtypes = [c_void_p, IPT, c_int, c_double, DP]
+        L.ref_calcchrom2.argtypes = [c_void_p, DP]
+        L.ref_trackma.argtypes = [c_void_p, c_double, c_double, c_int, c_int, IPT, c_int, DP, DP]
+        L.ref_trackma.restype = c_double
         self.lib = L
 
     def load(self, elems):
@@ -291,6 +344,38 @@ class RefLine:
         s = self.lib.ref_trackda(self.h, xmax, xmin, ymax, nturn, nray, nstep, _dp(rx), _dp(ry), int(negweight))
         nr = min(max(nray, 3), 51)
         return s, rx[:nr], ry[:nr]
+
+    def getco4(self, dp=0.0):
+        rc = np.zeros(4)
+        self.lib.ref_getco4(self.h, dp, _dp(rc))
+        return rc
+
+    def getco6(self):
+        rc = np.zeros(6)
+        self.lib.ref_getco6(self.h, _dp(rc))
+        return rc
+
+    def calctwiss0(self, dp=0.0):
+        tw = np.zeros(7)
+        self.lib.ref_calctwiss0(self.h, dp, _dp(tw))
+        return tw
+
+    def calctwiss(self, idx, dp=0.0):
+        ix = np.ascontiguousarray(idx, dtype=np.int32)
+        tw = np.zeros((len(ix), 7))
+        self.lib.ref_calctwiss(self.h, _ip(ix), len(ix), dp, _dp(tw))
+        return tw
+
+    def calcchrom2(self):
+        c = np.zeros(4)
+        self.lib.ref_calcchrom2(self.h, _dp(c))
+        return c
+
+    def trackma(self, dppmax, dppmin, nturn, nstep, locs):
+        lc = np.ascontiguousarray(locs, dtype=np.int32)
+        p, m = np.zeros(len(lc)), np.zeros(len(lc))
+        avg = self.lib.ref_trackma(self.h, dppmax, dppmin, nturn, nstep, _ip(lc), len(lc), _dp(p), _dp(m))
+        return avg, p, m
 
     def getfield(self, idx, name):
         return self.lib.ref_getfield(self.h, idx, name.encode())

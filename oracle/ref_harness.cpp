@@ -148,4 +148,54 @@ double ref_get_mpole(void *h, int idx, const char *name, int index) {
   else if (e->gettype() == "BendMPole") static_cast<ABendMPole *>(e)->getfield(&v, std::string(name), index);
   return v;
 }
+
+// ALine::getco4 / getco6 (aline.cpp:838-889): closed orbit; both set the
+// radiation and cavity flags as the reference does.
+void ref_getco4(void *h, double dp, double *rc) {
+  Vec v(4);
+  static_cast<ALine *>(h)->getco4(v, dp);
+  for (int i = 0; i < 4; i++) rc[i] = v[i + 1];
+}
+void ref_getco6(void *h, double *rc) {
+  Vec v(6);
+  static_cast<ALine *>(h)->getco6(v);
+  for (int i = 0; i < 6; i++) rc[i] = v[i + 1];
+}
+
+// ALine::calctwiss at the ring start (aline.cpp:1095-1107) and at element
+// indices (aline.cpp:1044-1094): per point {s, psix, psiy, betx, alfx, bety, alfy}.
+void ref_calctwiss0(void *h, double dp, double *tw) {
+  Twiss t;
+  static_cast<ALine *>(h)->calctwiss(t, dp);
+  const double v[7] = {t.s, t.psix, t.psiy, t.betx, t.alfx, t.bety, t.alfy};
+  for (int i = 0; i < 7; i++) tw[i] = v[i];
+}
+void ref_calctwiss(void *h, const int *idx, int n, double dp, double *tw) {
+  Twiss *t = new Twiss[n];
+  unsigned *u = new unsigned[n];
+  for (int i = 0; i < n; i++) u[i] = (unsigned)idx[i];
+  static_cast<ALine *>(h)->calctwiss(t, u, n, dp);
+  for (int i = 0; i < n; i++) {
+    const double v[7] = {t[i].s, t[i].psix, t[i].psiy, t[i].betx, t[i].alfx, t[i].bety, t[i].alfy};
+    for (int k = 0; k < 7; k++) tw[7 * i + k] = v[k];
+  }
+  delete[] t;
+  delete[] u;
+}
+
+// ALine::calcchrom2 (aline.cpp:1127-1158)
+void ref_calcchrom2(void *h, double *c) {
+  static_cast<ALine *>(h)->calcchrom2(c[0], c[1], c[2], c[3]);
+}
+
+// ALine::trackMA (aline.cpp:781-836)
+double ref_trackma(void *h, double dppmax, double dppmin, int nturn, int nstep, const int *locs, int n,
+                   double *dpp_plus, double *dpp_minus) {
+  unsigned *u = new unsigned[n];
+  for (int i = 0; i < n; i++) u[i] = (unsigned)locs[i];
+  const double avg = static_cast<ALine *>(h)->trackMA(dppmax, dppmin, nturn, (unsigned)nstep, dpp_plus,
+                                                      dpp_minus, (unsigned)n, u);
+  delete[] u;
+  return avg;
+}
 }

@@ -2,12 +2,17 @@
 
 The reference draws its quantum-excitation noise from rand() (grandn,
 utility.cpp:194-217), the oracle from the counter-based Philox stream, so
-the two runs agree only statistically.  Both track the same 16384-particle
+the two runs agree only statistically.  Both track the same 8192-particle
 bunch through the RADQE ring with the cavity at the synchronous phase for
-10000 turns, then record the beam every 100 turns for 2000 more.  The
-reference runs on one thread (its rand() is shared between threads).
-Writes tests/golden/equilibrium_ref.npz.  Takes about half an hour:
-python tests/golden/make_equilibrium_ref.py"""
+10000 turns, then record the beam every 100 turns for 2000 more.  Each code
+runs as 8 worker processes of 1024 particles: the reference workers are
+single-threaded (its rand() state is shared between threads) and seeded
+apart with srand, the oracle workers draw disjoint Philox streams (particle
+ids offset by the worker's first particle).  The statistics are those of
+the whole bunch at each record.  Writes tests/golden/equilibrium_ref.npz.
+Takes about 40 minutes on 8 cores: python tests/golden/make_equilibrium_ref.py"""
+import ctypes
+import multiprocessing as mp
 import os
 import sys
 
@@ -23,37 +28,57 @@ import pyaccp.lattices as lat  # noqa: E402
 from make_equilibrium import equilibrium_ring  # noqa: E402
 from pyoracle import Oracle, Reference  # noqa: E402
 
-N, SEED, WARM, BLOCK, NBLOCK = 16384, 5, 10000, 100, 20
+N, SEED, WARM, BLOCK, NBLOCK, WORKERS = 8192, 5, 10000, 100, 20, 8
 
 
-def stats(s):
-    """sigma_delta, sigma_x and the projected rms emittance
-    sqrt(<x x><x' x'> - <x x'>^2) of one beam_stat record."""
-    return s[13], s[8], np.sqrt(max(s[8] ** 2 * s[9] ** 2 - s[14] ** 2, 0.0))
+def beam():
+    return lat.gaussian_beam(N, seed=21) * np.array([1, 1, 1, 1, 1, 0.5])
 
 
-def run(step, r):
+def worker(args):
+    """One slice of the bunch: its coordinates at every record turn."""
+    code, k = args
+    per = N // WORKERS
+    r = beam()[k * per:(k + 1) * per].copy()
+    O = Oracle()
+    ring = equilibrium_ring(O)
+    if code == "oracle":
+        def step(r, n, t0):
+            return O.track_omp(ring, r, n, 1, seed=SEED, turn0=t0, pid0=k * per)
+    else:
+        ctypes.CDLL("libc.so.6").srand(1000 + k)
+        line = Reference().load(ring)
+        line.set_threads(1)
+
+        def step(r, n, t0):
+            return line.track(r, n)
     r = step(r, WARM, 0)
-    rows = []
-    for k in range(NBLOCK):
-        rows.append(stats(Oracle().beam_stat(r)))
-        r = step(r, BLOCK, WARM + BLOCK * k)
-    return np.array(rows), int(np.isnan(r[:, 0]).sum())
+    snaps = []
+    for b in range(NBLOCK):
+        snaps.append(r.copy())
+        r = step(r, BLOCK, WARM + BLOCK * b)
+    return np.array(snaps)
+
+
+def stats(O, r):
+    """sigma_delta, sigma_x and the projected rms emittance
+    sqrt(<x x><x' x'> - <x x'>^2) of the whole bunch."""
+    s = O.beam_stat(r)
+    return s[13], s[8], np.sqrt(max(s[8] ** 2 * s[9] ** 2 - s[14] ** 2, 0.0)), s[1]
 
 
 def main():
     O = Oracle()
-    ring = equilibrium_ring(O)
-    beam = lat.gaussian_beam(N, seed=21) * np.array([1, 1, 1, 1, 1, 0.5])
-    threads = os.cpu_count() or 8
-    orc, orc_lost = run(lambda r, n, t0: O.track_omp(ring, r, n, threads, seed=SEED, turn0=t0), beam)
-    print("oracle", orc.mean(axis=0), "lost", orc_lost, flush=True)
-    line = Reference().load(ring)
-    line.set_threads(1)
-    ref, ref_lost = run(lambda r, n, t0: line.track(r, n), beam)
-    print("reference", ref.mean(axis=0), "lost", ref_lost, flush=True)
-    np.savez_compressed(os.path.join(HERE, "equilibrium_ref.npz"), oracle=orc, reference=ref,
-                        n=np.array([N]), warm=np.array([WARM]), block=np.array([BLOCK]))
+    out = {}
+    with mp.get_context("fork").Pool(WORKERS) as pool:
+        for code in ("oracle", "reference"):
+            parts = pool.map(worker, [(code, k) for k in range(WORKERS)])
+            snaps = np.concatenate(parts, axis=1)  # (NBLOCK, N, 6)
+            out[code] = np.array([stats(O, snaps[b]) for b in range(NBLOCK)])
+            print(code, out[code][:, :3].mean(axis=0), "live", out[code][-1, 3], flush=True)
+    np.savez_compressed(os.path.join(HERE, "equilibrium_ref.npz"), oracle=out["oracle"],
+                        reference=out["reference"], n=np.array([N]), warm=np.array([WARM]),
+                        block=np.array([BLOCK]))
 
 
 if __name__ == "__main__":

@@ -4,6 +4,7 @@ Runs in the build container only (it needs oracle/_ref/libaccel_ref.so,
 built by `make -C oracle` from the Accel+ sources).  Every fixture is data:
 inputs and the reference's outputs for them, written with numpy.savez
 (no pickles).  Regenerate with:  python tests/golden/make_golden.py
+(or only the optics-services fixture: python tests/golden/make_golden.py optics2)
 """
 import os
 import sys
@@ -70,8 +71,53 @@ def element_cases():
     return c
 
 
+def corrected_fodo():
+    """The FODO ring with its chromaticity corrected to about (2, 1) by two
+    correct_chrom steps of the oracle (bit-exact with the reference's)."""
+    from pyoracle import Oracle
+    O = Oracle()
+    arr = el.to_array(lat.fodo_ring())
+    sf, sd = lat.family(lat.fodo_ring(), "SF"), lat.family(lat.fodo_ring(), "SD")
+    for _ in range(2):
+        O.correct(arr, 2, sf, sd, 2.0, 1.0)
+    return [arr[i] for i in range(len(arr))]
+
+
+def optics2(R, out):
+    """Twiss functions, second-order chromaticity, 6D closed orbit and
+    momentum aperture (aline.cpp:781-889, :1044-1158).  getco4 has no
+    fixture: the reference's returns the one-turn image of the off-momentum
+    origin, not a closed orbit (its 4x4 block is uninitialised memory)."""
+    fodo = lat.fodo_ring()
+    idx = sorted(lat.family(fodo, "QF") + lat.family(fodo, "SD"))
+    line = R.load(fodo)
+    tw0 = line.calctwiss0(0.0)
+    tw = line.calctwiss(idx, 1e-4)
+    cf = corrected_fodo()
+    chrom2 = R.load(cf).calcchrom2()
+    b2 = np.array([e.polyB[2] for e in cf if e.type == el.STRMPOLE])
+    da = lat.da_ring(0)
+    l6 = R.load(da)
+    phis = l6.setrfphase()
+    co6 = l6.getco6()
+    locs = lat.family(da, "QF")[:4]
+    lm = R.load(da)
+    lm.setrfphase()
+    lm.set_rad(1)
+    lm.set_cavity(1)
+    ma = lm.trackma(0.01, 0.0005, 64, 19, locs)
+    np.savez_compressed(os.path.join(out, "optics2.npz"), twiss0=tw0, twiss_idx=np.array(idx, dtype=np.int32),
+                        twiss=tw, chrom2_b2=b2, chrom2=chrom2, phis=np.array([phis]), co6=co6,
+                        ma_locs=np.array(locs, dtype=np.int32), ma_avg=np.array([ma[0]]), ma_plus=ma[1],
+                        ma_minus=ma[2])
+    print("optics2:", tw0, chrom2, co6, ma)
+
+
 def main():
     R = Reference()
+    if sys.argv[1:] == ["optics2"]:
+        optics2(R, HERE)
+        return
     out = os.path.join(HERE)
     P = probe_particles()
 
@@ -147,6 +193,7 @@ def main():
     nl, ar, sg, sr = beam_stat_ref(R, b)
     np.savez_compressed(os.path.join(out, "beamstat.npz"), beam=b, nlive=np.array([nl]), mean=ar,
                         sigma=sg, cross=sr)
+    optics2(R, out)
     print("fixtures written to", out)
 
 

@@ -126,3 +126,35 @@ End
     # adjoint responses: two Newton steps reach the targets
     assert np.allclose(chroms[-1], [2.0, 1.0], rtol=0, atol=1e-5)
     assert np.allclose(tunes[-1], [0.27, 0.25], rtol=0, atol=1e-8)
+
+
+def test_optics_and_momentum_aperture_commands(tmp_path, oracle):
+    """PRINT_OPTICS on the QFs at dp = 0 against the oracle's calctwiss
+    (bit-exact with the reference's); TRACK_MA against the reference's
+    momentum apertures (optics2.npz) within one step."""
+    g = golden("optics2.npz")
+    el.write_lattice(lat.fodo_ring(), str(tmp_path / "fodo.lat"))
+    el.write_lattice(lat.da_ring(0), str(tmp_path / "da.lat"))
+    out = run(tmp_path, """Lattice
+  file_name = fodo.lat
+End
+Optics
+  print_optics: type=Quad, famname=QF
+End
+Lattice
+  file_name = da.lat, type=ERING
+End
+DAMA
+  TRACK_MA: type=Quad, famname=QF, nmoni=4, dppmin=0.0005, dppmax=0.01, nstep=19, nturn=64
+End
+""")
+    qf = lat.family(lat.fodo_ring(), "QF")
+    rows = np.array([numbers(s) for s in after(out, "$$$Optics-Twiss", len(qf) + 1)[1:]])
+    assert np.array_equal(rows[:, 0], qf)
+    ref = oracle.calctwiss(lat.fodo_ring(), qf, 0.0)  # {s, psix, psiy, betx, alfx, bety, alfy}
+    mine = rows[:, [1, 6, 7, 2, 4, 3, 5]]
+    assert np.allclose(mine, ref, rtol=1e-10, atol=1e-12)
+    ma = np.array([numbers(s) for s in after(out, "$$$MA", 4)])
+    step = (0.01 - 0.0005) / 19
+    assert np.all(np.abs(ma[:, 0] - g["ma_plus"]) <= step * (1 + 1e-9))
+    assert np.all(np.abs(ma[:, 1] - g["ma_minus"]) <= step * (1 + 1e-9))

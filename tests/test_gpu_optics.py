@@ -208,3 +208,73 @@ def test_trackda_argument_checks(gpu_lib, oracle):
         L.trackda(0.001, 0.02, 0.01, 8, 11, 10)
     area, rx, ry = L.trackda(0.02, 0.001, 0.01, 4, 99, 1)  # clamped to 51 rays, 2 steps
     assert len(rx) == 51
+
+
+def test_twiss_and_second_order_chromaticity(gpu_lib):
+    """Against the reference (optics2.npz): Twiss functions from the
+    one-turn matrix (3.5e-15 from the reference's) to 1e-11.  The
+    second-order chromaticity is the slope of chromaticities taken at
+    dpp up to 6e-3, where the 1e-8 rays ride on orbits of 6e-3 m: each
+    chromaticity keeps eps * 6e-3 / 1e-8 / 2e-5 ~ 3e-6 of round-off, and the
+    slope over a 1e-2 span carries ~1e-4 of it."""
+    from tests.golden.make_golden import corrected_fodo
+    g = golden("optics2.npz")
+    L = gpu_lib.lattice(lat.fodo_ring())
+    assert np.allclose(L.calctwiss(), g["twiss0"], rtol=1e-11, atol=1e-13)
+    tw = L.calctwiss(g["twiss_idx"], dp=1e-4)
+    assert np.allclose(tw, g["twiss"], rtol=1e-11, atol=1e-12)
+    c = gpu_lib.lattice(corrected_fodo()).calcchrom2()
+    print("chrom2", c - g["chrom2"])
+    assert np.allclose(c[:2], g["chrom2"][:2], rtol=0, atol=1e-7)
+    assert np.allclose(c[2:], g["chrom2"][2:], rtol=1e-4, atol=0)
+
+
+def test_closed_orbits(gpu_lib, oracle):
+    """getco6 against the reference (it inverts I - M by SVD, this by
+    elimination; (I - M) has a condition number of a few hundred from the
+    slow synchrotron motion, so 1e-10 of the orbit); getco4 against the
+    oracle's same computation and by closure (the reference's getco4 returns
+    the one-turn image of the off-momentum origin, not a closed orbit)."""
+    g = golden("optics2.npz")
+    L = gpu_lib.lattice(lat.da_ring(0))
+    L.setrfphase()
+    co6 = L.getco6()
+    assert np.all(np.abs(co6 - g["co6"]) <= 1e-10 * np.abs(g["co6"]).max()), co6 - g["co6"]
+    arr = L.get()
+    assert all(e.rad == 1 for e in arr if e.type == el.BENDMPOLE)
+    assert all(e.on == 1 for e in arr if e.type == el.CAVITY)
+    fodo = lat.fodo_ring()
+    L = gpu_lib.lattice(fodo)
+    for dp in (1e-4, 1e-3):
+        co = L.getco4(dp)
+        ref = oracle.getco4(el.to_array(fodo), dp)
+        # the difference matrix of 1e-8 rays on a 1e-4..1e-3 orbit keeps
+        # eps * orbit / 1e-8 ~ 1e-12..1e-11 relative
+        assert np.all(np.abs(co - ref) <= 1e-9 * np.abs(ref).max())
+        r = np.array([[co[0], co[1], co[2], co[3], 0.0, dp]])
+        assert np.abs(L.track(r, 1)[0, :4] - r[0, :4]).max() <= 50 * co[0] ** 2
+
+
+def test_momentum_aperture_matches_reference(gpu_lib):
+    """All locations in one batched launch of the ring rotated to start at
+    each; the apertures agree with the reference's except where a particle
+    at the edge of the bucket survives on one side of round-off (one step)."""
+    g = golden("optics2.npz")
+    L = gpu_lib.lattice(lat.da_ring(0))
+    L.setrfphase()
+    arr = L.get()
+    for i in range(len(arr)):
+        if arr[i].type == el.BENDMPOLE:
+            arr[i].rad = 1
+        if arr[i].type == el.CAVITY:
+            arr[i].on = 1
+    L.update(arr)
+    avg, p, m = L.trackma(0.01, 0.0005, 64, 19, g["ma_locs"])
+    step = (0.01 - 0.0005) / 19
+    print("MA", avg, p, m, "reference", g["ma_avg"], g["ma_plus"], g["ma_minus"])
+    assert np.all(np.abs(p - g["ma_plus"]) <= step * (1 + 1e-9))
+    assert np.all(np.abs(m - g["ma_minus"]) <= step * (1 + 1e-9))
+    if np.array_equal(p, g["ma_plus"]) and np.array_equal(m, g["ma_minus"]):
+        assert avg == pytest.approx(g["ma_avg"][0], rel=1e-15)
+    with pytest.raises(abi.AccpError, match="location out of range"):
+        L.trackma(0.01, 0.0005, 8, 19, [len(arr)])

@@ -112,3 +112,32 @@ def test_oracle_matches_live_reference(oracle, reference):
     line = reference.load(ring)
     # ALine::pass(r, N, st, ed): one pass over an element range
     assert same_bits(oracle.track(ring, beam, 1, 17, 100), line.track_range(beam, 17, 100))
+
+
+def test_optics_services_oracle(oracle):
+    """Twiss functions, second-order chromaticity, 6D closed orbit and
+    momentum aperture against the reference (optics2.npz).  Twiss and the
+    chromaticity fit are bit-exact; the closed orbit solves (I - M) x = r1
+    by elimination where the reference inverts by SVD, so it agrees to
+    round-off; the momentum aperture follows bit-exact tracking."""
+    from tests.golden.make_golden import corrected_fodo
+    g = golden("optics2.npz")
+    fodo = lat.fodo_ring()
+    assert same_bits(oracle.calctwiss0(fodo, 0.0), g["twiss0"])
+    assert same_bits(oracle.calctwiss(fodo, g["twiss_idx"], 1e-4), g["twiss"])
+    cf = corrected_fodo()
+    assert same_bits([e.polyB[2] for e in cf if e.type == el.STRMPOLE], g["chrom2_b2"])
+    assert same_bits(oracle.calcchrom2(cf), g["chrom2"])
+    arr = el.to_array(lat.da_ring(0))
+    assert same_bits([oracle.setrfphase(arr)], g["phis"])
+    co6 = oracle.getco6(arr)
+    assert np.all(np.abs(co6 - g["co6"]) <= 1e-12 * np.abs(g["co6"]).max())
+    arr = el.to_array(lat.da_ring(0))
+    oracle.setrfphase(arr)
+    for i in range(len(arr)):
+        if arr[i].type == el.BENDMPOLE:
+            arr[i].rad = 1
+        if arr[i].type == el.CAVITY:
+            arr[i].on = 1
+    avg, p, m = oracle.trackma(arr, 0.01, 0.0005, 64, 19, g["ma_locs"])
+    assert avg == g["ma_avg"][0] and same_bits(p, g["ma_plus"]) and same_bits(m, g["ma_minus"])

@@ -108,3 +108,22 @@ def test_beam_stat_closed_form(oracle):
     assert s[1] == 499
     np.testing.assert_allclose(s[2:8], live.mean(0), rtol=1e-12, atol=1e-22)
     np.testing.assert_allclose(s[8:14], live.std(0), rtol=1e-12)
+
+
+def test_closed_orbit_closes(oracle):
+    """getco4's closed orbit maps onto itself after one turn up to the
+    second-order terms the linear solve leaves out (residual ~ orbit^2), and
+    at dp = 0 it is the origin.  The reference's getco4 cannot pin this: its
+    4x4 block is uninitialised memory (m66.submatrix re-points m66 at m44,
+    matvec.cpp:221-233), so it returns the one-turn image r1 of the
+    off-momentum origin, which the closure test below rejects."""
+    fodo = lat.fodo_ring()
+    assert np.all(oracle.getco4(el.to_array(fodo), 0.0) == 0)
+    for dp in (1e-5, 1e-4, 1e-3):
+        co = oracle.getco4(el.to_array(fodo), dp)
+        r = np.array([[co[0], co[1], co[2], co[3], 0.0, dp]])
+        res = np.abs(oracle.track(fodo, r, 1)[0, :4] - r[0, :4]).max()
+        assert res <= 50 * co[0] ** 2, (dp, res, co)
+        r1 = oracle.track(fodo, np.array([[0, 0, 0, 0, 0, dp]]), 1)[0, :4]
+        bad = np.array([[r1[0], r1[1], r1[2], r1[3], 0.0, dp]])
+        assert np.abs(oracle.track(fodo, bad, 1)[0, :4] - bad[0, :4]).max() > 10 * res
