@@ -13,6 +13,7 @@
 //
 // Commands handled (the path's callers in the reference):
 //   Lattice: file_name=F[, type=ERING]; set_flags: radiation=, cavity=, wake=;
+//            export_element: type=RING|T[, famname=F]; apply_setpt=file;
 //            modify_element: type=, famname=, index=, field=, value=|delta=;
 //            correct_tune / correct_chrom: type=, famname=(F1 F2) [a, b]
 //            [REPEAT] [METHOD=FD|ADJOINT]; set_max_num_threads=N (accepted)
@@ -24,7 +25,7 @@
 //            printflag=Coordinate|Stat1|Stat2; Do_Track: NTurn=,
 //            radiation=, RFCavity=, PrintFinal=No|Yes|Full [, seed=]
 //   DAMA:    TRACK_DA: xmin=, xmax=, ymax=, nray=, nstep=, nturn=
-//            [, print=loss_turn] [, weightminusside=true];
+//            [, print=loss_turn] [, weightminusside=true]; apply_setpt=file;
 //            TRACK_MA: type=, [famname=,] nmoni=, dppmin=, dppmax=, nstep=, nturn=
 #include <algorithm>
 #include <cctype>
@@ -307,6 +308,121 @@ double *field_ptr(accp_elem &x, const string &f, unsigned index) {
   return nullptr;
 }
 
+// AccelElement::serialize of each type (accel.cpp:143-858): the lattice-file
+// grammar, at 17 significant digits so that a file written here reads back
+// to the same doubles.
+string lossless(double v) {
+  std::ostringstream o;
+  o << std::setprecision(17) << v;
+  return o.str();
+}
+
+void serialize(const accp_elem &x, std::ostream &os) {
+  const string head = string(x.famname) + "  : \t" + Line::type_name(x.type) + "\t" + lossless(x.len);
+  auto vals = [&](std::initializer_list<double> v) {
+    string t;
+    for (double d : v) t += "\t" + lossless(d);
+    return t;
+  };
+  switch (x.type) {
+    case ACCP_DRIFT: os << head << "\t" << std::endl; break;
+    case ACCP_QUAD: os << head << vals({x.k}) << std::endl; break;
+    case ACCP_CORRECTOR: os << head << vals({x.kickx, x.kicky}) << std::endl; break;
+    case ACCP_APERTURE: os << head << vals({x.xlim[0], x.xlim[1], x.ylim[0], x.ylim[1]}) << std::endl; break;
+    case ACCP_MONITOR: os << head << "\t1\t1\t0" << std::endl; break;
+    case ACCP_BEND: os << head << vals({x.theta, x.e1, x.e2, x.k}) << std::endl; break;
+    case ACCP_BENDMPOLE:
+      os << head << vals({x.theta, x.e1, x.e2}) << "\t" << x.max_npole << std::endl << "\t\t\t";
+      for (int i = 0; i < x.max_npole; ++i) os << lossless(x.polyB[i]) << "\t";
+      os << std::endl;
+      break;
+    case ACCP_STRMPOLE:
+      os << head << "\t" << x.max_npole << std::endl << "\t\t\t";
+      for (int i = 0; i < x.max_npole; ++i) os << lossless(x.polyB[i]) << "\t";
+      os << std::endl << "\t\t\t";
+      for (int i = 0; i < x.max_npole; ++i) os << lossless(x.polyA[i]) << "\t";
+      os << std::endl;
+      break;
+    case ACCP_CAVITY: os << head << vals({x.volt, x.freq, x.phis, x.energy}) << std::endl; break;
+    case ACCP_LONGWAKE:
+      os << head << vals({x.wake_rs, x.wake_q, x.wake_fr, x.wake_charge, x.wake_zmin, x.wake_zmax}) << "\t"
+         << x.nbins << "\t" << lossless(x.energy) << std::endl;
+      break;
+  }
+}
+
+// type=..., famname=... of a command (process.cpp:2221-2268).
+void type_famname(const string &line, string &type, string &fam) {
+  for (const auto &kv : key_values(line)) {
+    if (kv.first == "TYPE") type = kv.second;
+    else if (kv.first == "FAMNAME") fam = kv.second;
+  }
+}
+
+// EXPORT_ELEMENT (process.cpp:505-547)
+void cmd_export(Line &L, const string &line) {
+  string type, fam;
+  type_famname(line, type, fam);
+  std::cout << "$$$Export-Elements***********************" << std::endl;
+  if (ieq(type, "RING") || ieq(type, "LINE")) {
+    for (const auto &x : L.e) serialize(x, std::cout);
+  } else {
+    for (int32_t i : L.find(type, fam)) serialize(L.e[i], std::cout);
+    std::cout << "******************************************" << std::endl;
+  }
+}
+
+// apply_line_setpt (process.cpp:2273-2392): a "$FirstElement: type=, famname="
+// line that must name the line's first element (zero-length Monitors
+// before it are skipped), then "$SetValue: type=StrMPole, famname=F,
+// PolyB[n]" (or PolyA[n]) lines, each followed by one value per family
+// member.  Only StrMPole set points are applied, as in the reference.
+void cmd_apply_setpt(Line &L, const string &line) {
+  string file = trim(line.substr(line.find_first_of("=:") + 1));
+  size_t e = file.find('=');
+  if (e != string::npos) file = trim(file.substr(e + 1));
+  std::cout << file << std::endl;
+  std::ifstream in(file);
+  if (!in) throw std::runtime_error("cannot open set-point file " + file);
+  string sl;
+  bool first_ok = false;
+  while (get_logical_line(in, sl)) {
+    if (!ieq(sl.substr(0, 13), "$FirstElement")) continue;
+    string type, fam;
+    type_famname(sl, type, fam);
+    for (const auto &x : L.e) {
+      if (type == Line::type_name(x.type) && fam == x.famname) {
+        first_ok = true;
+        break;
+      }
+      if (x.len > 0 || x.type != ACCP_MONITOR) break;
+    }
+    break;
+  }
+  if (!first_ok) {
+    std::cout << "#first element check: 0" << std::endl;
+    return;
+  }
+  std::cout << "#first element matches" << std::endl;
+  while (get_logical_line(in, sl)) {
+    if (!ieq(sl.substr(0, 9), "$SetValue")) continue;
+    string type, fam;
+    type_famname(sl, type, fam);
+    const std::vector<int32_t> list = L.find(type, fam);
+    std::vector<double> val(list.size());
+    for (double &v : val) in >> v;
+    if (type != "StrMPole") continue;
+    const size_t p = sl.find("Poly");
+    if (p == string::npos) continue;
+    const bool b = sl.compare(p, 5, "PolyB") == 0;
+    const size_t a = sl.find('[', p), z = sl.find(']', p);
+    const int index = std::stoi(sl.substr(a + 1, z - a - 1));
+    if (index < 0 || index >= ACCP_MAX_MPOLE) throw std::runtime_error("set point: multipole index out of range");
+    for (size_t i = 0; i < list.size(); ++i) (b ? L.e[list[i]].polyB : L.e[list[i]].polyA)[index] = val[i];
+    L.dirty = true;
+  }
+}
+
 // ---- Lattice block (process.cpp:173-988) ------------------------------------------
 void cmd_file_name(Line &L, const string &line) {
   string s = line;
@@ -456,6 +572,14 @@ void proc_lattice(std::istream &in, Line &L) {
     const int k = look_for(in, line, keys);
     if (k < 0 || keys[k] == "END") return;
     if (k != 0 && L.e.empty()) throw std::runtime_error("Lattice file name needs to be given first");
+    if (keys[k] == "EXPORT_ELEMENT") {
+      cmd_export(L, line);
+      continue;
+    }
+    if (keys[k] == "APPLY_SETPT") {
+      cmd_apply_setpt(L, line);
+      continue;
+    }
     if (k >= 6) {
       std::cerr << "accp-run: " << keys[k] << " is not carried by this backend; line ignored" << std::endl;
       continue;
@@ -762,9 +886,9 @@ void do_track(Line &L, std::vector<double> &r, const Monitor &mon, const string 
     std::cout << "*************************************" << std::endl;
     std::cout << "$$$Track-Data: Full export ************************" << std::endl;
     std::cout << "Np=" << np << ", Nprint=" << np << std::endl;
-    for (int i = 0; i < np; ++i) {
+    for (int i = 0; i < np; ++i) {  // lossless: Distribution Load format=accel-export reads it back
       std::cout << nturn << "\t";
-      for (int j = 0; j < 6; ++j) std::cout << fmt(r[6 * i + j]) << "\t";
+      for (int j = 0; j < 6; ++j) std::cout << lossless(r[6 * i + j]) << "\t";
       std::cout << std::endl;
     }
   } else if (ieq(printfinal, "Yes") || ieq(printfinal, "same")) {
@@ -827,11 +951,15 @@ void proc_track(std::istream &in, Line &L) {
 
 // ---- DAMA block (process.cpp:2393-2500) --------------------------------------------
 void proc_dama(std::istream &in, Line &L) {
-  const std::vector<string> keys = {"TRACK_DA", "TRACK_MA", "END"};
+  const std::vector<string> keys = {"TRACK_DA", "TRACK_MA", "APPLY_SETPT", "END"};
   string line;
   for (;;) {
     const int k = look_for(in, line, keys);
     if (k < 0 || keys[k] == "END") return;
+    if (k == 2) {
+      cmd_apply_setpt(L, line);
+      continue;
+    }
     if (k == 1) {
       // TRACK_MA (process.cpp:2503-2580): locations = elements of type
       // [and famname], the first nmoni of them

@@ -27,7 +27,7 @@ Lattice   % the first block
   set_flags: radiation=radQE, cavity=on, wake=off
   modify_element: type=Quad, famname=QF, field=K, delta=0.001
   correct_tune: type=Quad, famname=(QF QD) [0.27, 0.26] REPEAT
-  export_element: type=Quad
+  print_lat_summary
 End
 Optics
   print_tune
@@ -55,7 +55,7 @@ def test_parse_only_reads_every_block(tmp_path):
     assert "Total elements: 322" in out
     assert "radiation turned on with quantum excitation" in out
     assert "1 cavity(ies) turned on" in out
-    assert "EXPORT_ELEMENT is not carried by this backend" in p.stderr
+    assert "PRINT_LAT_SUMMARY is not carried by this backend" in p.stderr
     for block in ("OPTICS", "TRACK", "DAMA"):
         assert f"\n{block}\n" in out
     assert "#0.001 0.02 0.01 11 10 64 0" in out
@@ -75,3 +75,59 @@ def test_tracking_without_a_device_is_an_error(tmp_path, lib):
     el.write_lattice(lat.fodo_ring(), str(tmp_path / "fodo.lat"))
     p = run(tmp_path, "Lattice\n file_name = fodo.lat\nEnd\nOptics\n print_tune\nEnd\n")
     assert p.returncode == 1 and "no HIP device" in p.stderr
+
+
+def export_block(out):
+    lines = out.splitlines()
+    i = lines.index("$$$Export-Elements***********************")
+    body = []
+    for s in lines[i + 1:]:
+        if s.startswith("***") or s in ("OPTICS", "TRACK", "DAMA", "LATTICE") or s.startswith("Total elements"):
+            break
+        body.append(s)
+    return body
+
+
+def test_export_is_lossless_and_reads_back(tmp_path):
+    """EXPORT_ELEMENT writes the reference's lattice grammar at 17
+    significant digits: read back, the ring is the same to the last bit."""
+    ring = lat.da_ring(3, extra_quads=True)
+    el.write_lattice(ring, str(tmp_path / "da.lat"))
+    p = run(tmp_path, "Lattice\n file_name = da.lat\n export_element: type=RING\nEnd\n", "--parse-only")
+    assert p.returncode == 0, p.stderr
+    body = export_block(p.stdout)
+    (tmp_path / "again.lat").write_text("\n".join(body) + "\nRETURN\n")
+    q = run(tmp_path, "Lattice\n file_name = again.lat\n export_element: type=RING\nEnd\n", "--parse-only")
+    assert q.returncode == 0, q.stderr
+    assert export_block(q.stdout) == body
+    # the numbers themselves: every K and multipole coefficient survives
+    vals = [float(t) for s in body for t in s.replace(":", " ").split()[2:] if t.replace(".", "").replace("-", "")
+            .replace("e", "").replace("+", "").isdigit()]
+    ref = []
+    for e in ring:
+        ref.append(e.len)
+        if e.type == el.QUAD:
+            ref.append(e.k)
+        if e.type == el.STRMPOLE:
+            ref += list(e.polyB[:e.max_npole]) + list(e.polyA[:e.max_npole])
+    assert set(ref) <= set(vals)
+
+
+def test_apply_setpt(tmp_path):
+    """Set points (process.cpp:2273-2392): the first element must match, then
+    one PolyB[2] value per SF member."""
+    el.write_lattice(lat.da_ring(0), str(tmp_path / "da.lat"))
+    vals = [20.0 + 0.125 * i for i in range(20)]
+    (tmp_path / "sp.txt").write_text("$FirstElement: type=Drift, famname=D1\n"
+                                     "$SetValue: type=StrMPole, famname=SF, PolyB[2]\n" +
+                                     " ".join(repr(v) for v in vals) + "\n")
+    p = run(tmp_path, "Lattice\n file_name = da.lat\n apply_setpt = sp.txt\n"
+                      " export_element: type=StrMPole, famname=SF\nEnd\n", "--parse-only")
+    assert p.returncode == 0, p.stderr
+    assert "#first element matches" in p.stdout
+    body = export_block(p.stdout)
+    b2 = [float(body[3 * k + 1].split()[2]) for k in range(20)]
+    assert b2 == vals
+    (tmp_path / "bad.txt").write_text("$FirstElement: type=Quad, famname=QF\n")
+    p = run(tmp_path, "Lattice\n file_name = da.lat\n apply_setpt = bad.txt\nEnd\n", "--parse-only")
+    assert "#first element check: 0" in p.stdout

@@ -158,3 +158,40 @@ End
     step = (0.01 - 0.0005) / 19
     assert np.all(np.abs(ma[:, 0] - g["ma_plus"]) <= step * (1 + 1e-9))
     assert np.all(np.abs(ma[:, 1] - g["ma_minus"]) <= step * (1 + 1e-9))
+
+
+def test_accel_export_reload_is_lossless(tmp_path):
+    """PrintFinal=Full writes 17 significant digits, and Distribution Load
+    format=accel-export reads them back (process.cpp:1499-1545): 20 turns,
+    dump, reload, 5 more turns equals 25 turns in one run, bit for bit."""
+    el.write_lattice(lat.fodo_ring(), str(tmp_path / "fodo.lat"))
+    el.write_plain_beam(lat.gaussian_beam(64, seed=9), str(tmp_path / "beam.txt"))
+    first = run(tmp_path, """Lattice
+  file_name = fodo.lat
+End
+Track
+  Distribution: Load beam.txt
+  Do_Track: NTurn=20, PrintFinal=Full
+End
+""")
+    (tmp_path / "dump.txt").write_text("\n".join(first) + "\n")
+    cont = run(tmp_path, """Lattice
+  file_name = fodo.lat
+End
+Track
+  Distribution: Load dump.txt, format=accel-export
+  Do_Track: NTurn=5, PrintFinal=Full
+End
+""")
+    once = run(tmp_path, """Lattice
+  file_name = fodo.lat
+End
+Track
+  Distribution: Load beam.txt
+  Do_Track: NTurn=25, PrintFinal=Full
+End
+""")
+    a = after(cont, "$$$Track-Data: Full export", 65)[1:]
+    b = after(once, "$$$Track-Data: Full export", 65)[1:]
+    # rows are "turn<TAB>x<TAB>...": the coordinates must agree to the last digit
+    assert len(a) == 64 and [r.split("\t")[1:] for r in a] == [r.split("\t")[1:] for r in b]
