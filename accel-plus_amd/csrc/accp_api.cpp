@@ -109,6 +109,9 @@ struct Compiled {
 // Elements the lean kernel variant does not carry.
 bool needs_full(const Hdr &h) {
   switch (h.op) {
+    case OP_MATRIX:
+    case OP_SYNCRAD:
+    case OP_CRABCAV:
     case OP_BEND:
     case OP_BENDMP_RAD:
     case OP_BENDMP_QE:
@@ -226,6 +229,35 @@ int compile_lattice(const accp_elem *E, int n, Compiled &out) {
         h.op = OP_CAVITY;
         h.a = e.on ? 1 : 0;
         P.insert(P.end(), {e.len, e.volt / e.energy, TWOPI * e.freq / C0, e.phis});
+        break;
+      case ACCP_MATRIX:  // AMatrix::pass (accel.cpp:1697-1729)
+        h.op = OP_MATRIX;
+        P.push_back(e.len);
+        P.insert(P.end(), e.mat, e.mat + 36);
+        break;
+      case ACCP_SYNCRAD: {
+        // ASyncRad::pass (accel.cpp:1813-1854): constants in its order;
+        // Jz = 4 - Jx - Jy (accel.h:551); flags from set_rad_onoff
+        // (aline.cpp:273-288)
+        if (e.energy == 0) return fail(ACCP_E_LATTICE, "element %d: SyncRad energy is zero", i);
+        const double jz = 4 - e.jx - e.jy;
+        const double alfx = e.u0 / 2. / e.energy * e.jx, alfy = e.u0 / 2. / e.energy * e.jy;
+        const double alfz = e.u0 / 2. / e.energy * jz;
+        const double sx = std::sqrt(2. * alfx), sy = std::sqrt(2. * alfy), sz = std::sqrt(2. * alfz);
+        const double sigx = std::sqrt(e.emitx * e.betax), sigy = std::sqrt(e.emity * e.betay);
+        const bool damp = e.rad == ACCP_RAD_LUMP || e.rad == ACCP_RAD_LUMP_DAMP;
+        const bool qe = e.rad == ACCP_RAD_LUMP || e.rad == ACCP_RAD_LUMP_QE;
+        h.op = OP_SYNCRAD;
+        h.a = (damp ? 1 : 0) | (qe ? 2 : 0);
+        P.insert(P.end(), {e.len, 1 - alfx, 1 - alfy, 1 - alfz, sigx * sx, sigy * sy, e.sigdpp * sz});
+        break;
+      }
+      case ACCP_CRABCAV:  // ACrabCav::pass (accel.cpp:968-999)
+        if (e.energy == 0) return fail(ACCP_E_LATTICE, "element %d: CrabCav energy is zero", i);
+        h.op = OP_CRABCAV;
+        h.a = (e.on ? 1 : 0) | (e.sigphi > 0 ? 2 : 0) | (e.sigdvv > 0 ? 4 : 0);
+        P.insert(P.end(), {e.len, e.devoltx / e.energy, e.devolty / e.energy, TWOPI * e.freq / C0, e.phis,
+                           e.sigphi, e.sigdvv});
         break;
       case ACCP_LONGWAKE:
         if (e.nbins < 1 || e.nbins > WAKE_MAX_BINS || !(e.wake_zmax > e.wake_zmin) ||

@@ -330,6 +330,42 @@ __device__ __forceinline__ void element_body(Part (&p)[PPL], const Hdr h, const 
       for (int k = 0; k < PPL; ++k) corrector(p[k], L, kx, ky);
       break;
     }
+    case OP_MATRIX: {
+      if (!FULL) break;
+      const double L = P[0];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) matrix_map(p[k], L, P + 1);
+      break;
+    }
+    case OP_SYNCRAD: {
+      if (!FULL) break;
+      const bool damp = h.a & 1, qe = h.a & 2;
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        double g0 = 0, g1 = 0, g2 = 0;
+        if (qe) {
+          g0 = gauss_kick(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, 0);
+          g1 = gauss_kick(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, 1);
+          g2 = gauss_kick(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, 2);
+        }
+        syncrad(p[k], P, damp, qe, g0, g1, g2);
+      }
+      break;
+    }
+    case OP_CRABCAV: {
+      if (!FULL) break;
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        double g0 = 0, g1 = 0, g2 = 0;
+        if ((h.a & 1) && (h.a & 2)) g0 = gauss_kick(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, 0);
+        if ((h.a & 1) && (h.a & 4)) {
+          g1 = gauss_kick(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, 1);
+          g2 = gauss_kick(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, 2);
+        }
+        crabcav(p[k], P, h.a, g0, g1, g2);
+      }
+      break;
+    }
     default:
       break;
   }

@@ -410,6 +410,69 @@ ACCP_HD void quad_series(Part &p, const PT *C, double L, double L2, bool focusin
   p.z -= (x * xp * m12 * m21 + y * yp * m34 * m43) * 0.5;
 }
 
+// ---- AMatrix: 6x6 map between half drifts (accel.cpp:1697-1729) -------------
+template <class PT>
+ACCP_HD void matrix_map(Part &p, double L, const PT *m) {
+  if (L != 0) drift_nl(p, 0.5 * L * p.ipd);
+  const double r[6] = {p.x, p.px, p.y, p.py, p.z, p.d};
+  double t[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+    t[j] = m[j * 6 + 0] * r[0] + m[j * 6 + 1] * r[1] + m[j * 6 + 2] * r[2] + m[j * 6 + 3] * r[3] +
+           m[j * 6 + 4] * r[4] + m[j * 6 + 5] * r[5];
+  p.x = t[0];
+  p.px = t[1];
+  p.y = t[2];
+  p.py = t[3];
+  p.z = t[4];
+  set_delta(p, t[5]);
+  if (L != 0) drift_nl(p, 0.5 * L * p.ipd);
+}
+
+// ---- ASyncRad: lumped damping and excitation (accel.cpp:1813-1854) ----------
+// C = [L, 1-ax, 1-ay, 1-az, sigx sx, sigy sy, sigdpp sz]; g = three normals.
+template <class PT>
+ACCP_HD void syncrad(Part &p, const PT *C, bool damp, bool qe, double g0, double g1, double g2) {
+  const double L = C[0];
+  if (L != 0) drift_nl(p, 0.5 * L * p.ipd);
+  double d = p.d;
+  if (damp) {
+    p.x *= C[1];
+    p.y *= C[2];
+    d *= C[3];
+  }
+  if (qe) {
+    p.x += C[4] * g0;
+    p.y += C[5] * g1;
+    d += C[6] * g2;
+  }
+  set_delta(p, d);
+  if (L != 0) drift_nl(p, 0.5 * L * p.ipd);
+}
+
+// ---- ACrabCav: transverse deflecting cavity (accel.cpp:968-999) -------------
+// C = [L, Vx/E, Vy/E, k, phis, sigphi, sigdVV]; g = three normals.
+template <class PT>
+ACCP_HD void crabcav(Part &p, const PT *C, int flags, double g0, double g1, double g2) {
+  const double L = C[0];
+  if (L != 0) drift_nl(p, 0.5 * L * p.ipd);
+  if (flags & 1) {
+    const double k = C[3];
+    double om = -k * p.z + C[4];
+    if (flags & 2) om = -k * p.z + C[4] + C[5] * g0;
+    double nvxa = C[1], nvya = C[2];
+    if (flags & 4) {
+      nvxa = C[1] * (1.0 + C[6] * g1);
+      nvya = C[2] * (1.0 + C[6] * g2);
+    }
+    const double so = sin(om), co = cos(om);
+    p.px += nvxa * so;
+    p.py += nvya * so;
+    set_delta(p, p.d + -(nvya * p.y + nvxa * p.x) * k * co / (1 + p.d));
+  }
+  if (L != 0) drift_nl(p, 0.5 * L * p.ipd);
+}
+
 // ---- counter-based normal variates (quantum excitation) -------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
 // easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies with a

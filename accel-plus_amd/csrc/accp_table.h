@@ -28,6 +28,9 @@ enum Op : int32_t {
   OP_APERTURE,    // [L, xmin, xmax, ymin, ymax]
   OP_CORRECTOR,   // [L, kickx, kicky]
   OP_WAKE,        // collective: the host splits the turn here; no-op inside a pass
+  OP_MATRIX,      // [L, M00..M55]                          (row-major 6x6)
+  OP_SYNCRAD,     // [L, 1-ax, 1-ay, 1-az, sigx sx, sigy sy, sigdpp sz]   a = damping | excitation<<1
+  OP_CRABCAV,     // [L, Vx/E, Vy/E, 2 pi f / c, phis, sigphi, sigdVV]    a = on | phase noise<<1 | amplitude noise<<2
   OP_COUNT
 };
 

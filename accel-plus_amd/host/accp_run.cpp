@@ -205,6 +205,24 @@ std::vector<accp_elem> read_lattice(const string &path) {
     } else if (type == "Cavity") {
       e = new_elem(ACCP_CAVITY, fam);
       ps >> e.len >> e.volt >> e.freq >> e.phis >> e.energy;
+    } else if (type == "Matrix") {  // accel.cpp:1673-1687: L, then six rows
+      e = new_elem(ACCP_MATRIX, fam);
+      ps >> e.len;
+      for (int i = 0; i < 6; ++i) {
+        string row;
+        std::getline(in, row);
+        std::istringstream rs(row);
+        for (int j = 0; j < 6; ++j) rs >> e.mat[6 * i + j];
+      }
+    } else if (type == "SyncRad") {  // accel.cpp:1743-1753: L E U0 Jx Jy / ex ey sigdpp bx by
+      e = new_elem(ACCP_SYNCRAD, fam);
+      ps >> e.len >> e.energy >> e.u0 >> e.jx >> e.jy;
+      string row;
+      std::getline(in, row);
+      std::istringstream(row) >> e.emitx >> e.emity >> e.sigdpp >> e.betax >> e.betay;
+    } else if (type == "CrabCav") {  // accel.cpp:937-943
+      e = new_elem(ACCP_CRABCAV, fam);
+      ps >> e.len >> e.devoltx >> e.devolty >> e.freq >> e.phis >> e.energy >> e.sigphi >> e.sigdvv;
     } else if (type == "LongWake") {  // new element: L Rs Q fr charge zmin zmax nbins E
       e = new_elem(ACCP_LONGWAKE, fam);
       ps >> e.len >> e.wake_rs >> e.wake_q >> e.wake_fr >> e.wake_charge >> e.wake_zmin >> e.wake_zmax >>
@@ -249,6 +267,9 @@ struct Line {
       case ACCP_STRMPOLE: return "StrMPole";
       case ACCP_CAVITY: return "Cavity";
       case ACCP_LONGWAKE: return "LongWake";
+      case ACCP_MATRIX: return "Matrix";
+      case ACCP_SYNCRAD: return "SyncRad";
+      case ACCP_CRABCAV: return "CrabCav";
     }
     return "?";
   }
@@ -269,7 +290,8 @@ struct Line {
   int set_rad(int flag) {
     int n = 0;
     for (auto &x : e)
-      if (x.type == ACCP_BENDMPOLE) x.rad = flag, ++n;
+      if (x.type == ACCP_BENDMPOLE) x.rad = flag <= ACCP_RAD_QE ? flag : ACCP_RAD_OFF, ++n;
+      else if (x.type == ACCP_SYNCRAD) x.rad = flag >= ACCP_RAD_LUMP ? flag : ACCP_RAD_OFF, ++n;
     dirty = true;
     return n;
   }
@@ -344,6 +366,21 @@ void serialize(const accp_elem &x, std::ostream &os) {
       os << std::endl;
       break;
     case ACCP_CAVITY: os << head << vals({x.volt, x.freq, x.phis, x.energy}) << std::endl; break;
+    case ACCP_MATRIX:
+      os << head << "\t" << std::endl;
+      for (int i = 0; i < 6; ++i) {
+        os << "\t";
+        for (int j = 0; j < 6; ++j) os << lossless(x.mat[6 * i + j]) << "\t";
+        os << std::endl;
+      }
+      break;
+    case ACCP_SYNCRAD:
+      os << head << vals({x.energy, x.u0, x.jx, x.jy}) << "\t" << std::endl
+         << vals({x.emitx, x.emity, x.sigdpp, x.betax, x.betay}) << "\t" << std::endl;
+      break;
+    case ACCP_CRABCAV:
+      os << head << vals({x.devoltx, x.devolty, x.freq, x.phis, x.energy, x.sigphi, x.sigdvv}) << std::endl;
+      break;
     case ACCP_LONGWAKE:
       os << head << vals({x.wake_rs, x.wake_q, x.wake_fr, x.wake_charge, x.wake_zmin, x.wake_zmax}) << "\t"
          << x.nbins << "\t" << lossless(x.energy) << std::endl;
@@ -451,6 +488,8 @@ void cmd_file_name(Line &L, const string &line) {
   if (ncav > 0) std::cout << ncav << " cavity(ies) turned off" << std::endl;
   const int nwake = L.set_on(ACCP_LONGWAKE, 0);
   if (nwake > 0) std::cout << nwake << " wake element(s) turned off" << std::endl;
+  const int ncrab = L.set_on(ACCP_CRABCAV, 0);
+  if (ncrab > 0) std::cout << ncrab << " crab cavity(ies) turned off" << std::endl;
 }
 
 void cmd_set_flags(Line &L, const string &line) {
@@ -462,9 +501,11 @@ void cmd_set_flags(Line &L, const string &line) {
         L.set_rad(ACCP_RAD_DAMP), std::cout << "radiation turned on without quantum excitation" << std::endl;
       else if (ieq(v, "radQE"))
         L.set_rad(ACCP_RAD_QE), std::cout << "radiation turned on with quantum excitation" << std::endl;
-    } else if (kv.first == "CAVITY" || kv.first == "WAKE") {
-      const int type = kv.first == "CAVITY" ? ACCP_CAVITY : ACCP_LONGWAKE;
-      const char *what = kv.first == "CAVITY" ? " cavity(ies) turned " : " wake element(s) turned ";
+    } else if (kv.first == "CAVITY" || kv.first == "WAKE" || kv.first == "CRABCAV") {
+      const int type = kv.first == "CAVITY" ? ACCP_CAVITY : kv.first == "WAKE" ? ACCP_LONGWAKE : ACCP_CRABCAV;
+      const char *what = kv.first == "CAVITY"   ? " cavity(ies) turned "
+                         : kv.first == "WAKE" ? " wake element(s) turned "
+                                              : " crab cavity(ies) turned ";
       if (ieq(v, "off") || ieq(v, "on")) {
         const int on = ieq(v, "on");
         const int n = L.set_on(type, on);
@@ -838,8 +879,9 @@ void do_track(Line &L, std::vector<double> &r, const Monitor &mon, const string 
   if (ieq(str_rad, "OFF")) L.set_rad(ACCP_RAD_OFF);
   else if (ieq(str_rad, "RAD")) L.set_rad(ACCP_RAD_DAMP);
   else if (ieq(str_rad, "RADQE")) L.set_rad(ACCP_RAD_QE);
-  else if (!str_rad.empty() && !ieq(str_rad, "NO"))
-    throw std::runtime_error("radiation=" + str_rad + " (lumped radiation) is not carried by this backend");
+  else if (ieq(str_rad, "LUMP")) L.set_rad(ACCP_RAD_LUMP);
+  else if (ieq(str_rad, "LUMPDAMP")) L.set_rad(ACCP_RAD_LUMP_DAMP);
+  else if (ieq(str_rad, "LUMPQE")) L.set_rad(ACCP_RAD_LUMP_QE);
   if (ieq(str_rf, "OFF")) L.set_on(ACCP_CAVITY, 0);
   else if (ieq(str_rf, "ON")) L.set_on(ACCP_CAVITY, 1);
   const int np = (int)(r.size() / 6);

@@ -13,13 +13,14 @@ import numpy as np
 
 MAX_MPOLE = 10
 
-DRIFT, QUAD, CORRECTOR, APERTURE, MONITOR, BEND, BENDMPOLE, STRMPOLE, CAVITY, LONGWAKE = range(1, 11)
+DRIFT, QUAD, CORRECTOR, APERTURE, MONITOR, BEND, BENDMPOLE, STRMPOLE, CAVITY, LONGWAKE, MATRIX, SYNCRAD, CRABCAV = \
+    range(1, 14)
 TYPE_NAMES = {
     DRIFT: "Drift", QUAD: "Quad", CORRECTOR: "Corrector", APERTURE: "Aperture",
     MONITOR: "Monitor", BEND: "Bend", BENDMPOLE: "BendMPole", STRMPOLE: "StrMPole",
-    CAVITY: "Cavity", LONGWAKE: "LongWake",
+    CAVITY: "Cavity", LONGWAKE: "LongWake", MATRIX: "Matrix", SYNCRAD: "SyncRad", CRABCAV: "CrabCav",
 }
-RAD_OFF, RAD_DAMP, RAD_QE = 0, 1, 2
+RAD_OFF, RAD_DAMP, RAD_QE, RAD_LUMP, RAD_LUMP_DAMP, RAD_LUMP_QE = 0, 1, 2, 3, 4, 5
 FIELD_K, FIELD_POLYB, FIELD_POLYA = 1, 2, 3
 DEFAULT_E0 = 3000.0  # MeV, accel.h:30
 C0 = 2.99792458e8  # accel.h:29
@@ -37,6 +38,10 @@ class Elem(ctypes.Structure):
         ("polyB", c_double * MAX_MPOLE), ("polyA", c_double * MAX_MPOLE),
         ("wake_rs", c_double), ("wake_q", c_double), ("wake_fr", c_double),
         ("wake_charge", c_double), ("wake_zmin", c_double), ("wake_zmax", c_double),
+        ("mat", c_double * 36),
+        ("u0", c_double), ("jx", c_double), ("jy", c_double),
+        ("emitx", c_double), ("emity", c_double), ("sigdpp", c_double), ("betax", c_double), ("betay", c_double),
+        ("devoltx", c_double), ("devolty", c_double), ("sigphi", c_double), ("sigdvv", c_double),
         ("famname", c_char * 32),
     ]
 
@@ -66,6 +71,10 @@ def _mk(t, name, L, **kw):
         elif k in ("xlim", "ylim"):
             arr = getattr(e, k)
             arr[0], arr[1] = float(v[0]), float(v[1])
+        elif k == "mat":
+            arr = getattr(e, k)
+            for i, x in enumerate(v):
+                arr[i] = float(x)
         else:
             setattr(e, k, v)
     return e
@@ -109,6 +118,24 @@ def strmpole(L, polyB, polyA=None, name="SM", nslice=10, max_npole=None):
 def cavity(L, volt, freq, phis=0.0, energy=DEFAULT_E0, on=1, name="RF"):
     return _mk(CAVITY, name, L, volt=float(volt), freq=float(freq), phis=float(phis),
                energy=float(energy), on=on)
+
+
+def matrix(L, m, name="MAT"):
+    """AMatrix: a 6x6 map (row-major, 36 numbers) between two half drifts."""
+    return _mk(MATRIX, name, L, mat=[float(x) for x in m])
+
+
+def syncrad(L, energy, u0, jx, jy, emitx, emity, sigdpp, betax, betay, rad=RAD_OFF, name="SR"):
+    """ASyncRad: lumped radiation damping and excitation."""
+    return _mk(SYNCRAD, name, L, energy=float(energy), u0=float(u0), jx=float(jx), jy=float(jy),
+               emitx=float(emitx), emity=float(emity), sigdpp=float(sigdpp), betax=float(betax),
+               betay=float(betay), rad=rad)
+
+
+def crabcav(L, devoltx, devolty, freq, phis, energy=DEFAULT_E0, sigphi=0.0, sigdvv=0.0, on=1, name="CRAB"):
+    """ACrabCav: transverse deflecting cavity with phase and amplitude noise."""
+    return _mk(CRABCAV, name, L, devoltx=float(devoltx), devolty=float(devolty), freq=float(freq),
+               phis=float(phis), energy=float(energy), sigphi=float(sigphi), sigdvv=float(sigdvv), on=on)
 
 
 def longwake(rs, q, fr, charge, nbins, zmin, zmax, energy=DEFAULT_E0, on=1, name="WAKE"):
@@ -162,6 +189,16 @@ def write_lattice(elems, path):
             lines.append("   " + " ".join(_g(e.polyA[i]) for i in range(e.max_npole)))
         elif t == CAVITY:
             lines.append(f"{n} : Cavity {_g(e.len)} {_g(e.volt)} {_g(e.freq)} {_g(e.phis)} {_g(e.energy)}")
+        elif t == MATRIX:
+            lines.append(f"{n} : Matrix {_g(e.len)}")
+            for i in range(6):
+                lines.append("   " + " ".join(_g(e.mat[6 * i + j]) for j in range(6)))
+        elif t == SYNCRAD:
+            lines.append(f"{n} : SyncRad {_g(e.len)} {_g(e.energy)} {_g(e.u0)} {_g(e.jx)} {_g(e.jy)}")
+            lines.append(f"   {_g(e.emitx)} {_g(e.emity)} {_g(e.sigdpp)} {_g(e.betax)} {_g(e.betay)}")
+        elif t == CRABCAV:
+            lines.append(f"{n} : CrabCav {_g(e.len)} {_g(e.devoltx)} {_g(e.devolty)} {_g(e.freq)} {_g(e.phis)} "
+                         f"{_g(e.energy)} {_g(e.sigphi)} {_g(e.sigdvv)}")
         else:
             raise ValueError(f"element type {t} has no reference lattice record")
     lines.append("RETURN")

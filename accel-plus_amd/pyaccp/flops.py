@@ -75,6 +75,15 @@ def element_flops(e, mode="useful"):
         return e.nslice * (4 + 4 * FASTDRIFT + 3 * kick)
     if t == el.CAVITY:
         return (7 if e.on else 0) + (2 * (2 + FASTDRIFT) if e.len != 0 else 0)
+    drifts = 2 * (2 + FASTDRIFT) if e.len != 0 else 0
+    if t == el.MATRIX:  # AMatrix::pass (accel.cpp:1715-1719): 36 multiplies, 30 adds
+        return 66 + drifts
+    if t == el.SYNCRAD:  # ASyncRad::pass (accel.cpp:1830-1848)
+        damp = e.rad in (el.RAD_LUMP, el.RAD_LUMP_DAMP)
+        qe = e.rad in (el.RAD_LUMP, el.RAD_LUMP_QE)
+        return (3 if damp else 0) + (6 if qe else 0) + drifts
+    if t == el.CRABCAV:  # ACrabCav::pass (accel.cpp:977-995)
+        return (15 if e.on else 0) + drifts
     return 0
 
 

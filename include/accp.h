@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define ACCP_ABI_VERSION 1
+#define ACCP_ABI_VERSION 2
 
 /* ---- error codes ------------------------------------------------------- */
 #define ACCP_OK 0
@@ -62,14 +62,23 @@ extern "C" {
 #define ACCP_STRMPOLE 8  /* AStrMPole   src/lib/accel/accel.cpp:800-847 */
 #define ACCP_CAVITY 9    /* ACavity     src/lib/accel/accel.cpp:903-923 */
 #define ACCP_LONGWAKE 10 /* longitudinal resonator wake: new, no reference class */
+#define ACCP_MATRIX 11   /* AMatrix     src/lib/accel/accel.cpp:1661-1730 */
+#define ACCP_SYNCRAD 12  /* ASyncRad    src/lib/accel/accel.cpp:1733-1854 (lumped radiation) */
+#define ACCP_CRABCAV 13  /* ACrabCav    src/lib/accel/accel.cpp:927-1000 */
 
 #define ACCP_MAX_MPOLE 10 /* MAX_MPOLE, src/lib/accel/accel.h:48 */
 #define ACCP_NAME_LEN 32
 
-/* Radiation flag values of ABendMPole::set_rad_onoff (src/lib/accel/accel.h:577-579). */
+/* Radiation flag values of ALine::set_rad_onoff (src/lib/accel/accel.h:577-582,
+ * src/lib/accel/aline.cpp:258-293).  A BendMPole radiates with 1 and 2; a
+ * SyncRad element acts with 3 (damping and excitation), 4 (damping only) and
+ * 5 (excitation only); any other value leaves it (or the bend) off. */
 #define ACCP_RAD_OFF 0
 #define ACCP_RAD_DAMP 1
 #define ACCP_RAD_QE 2
+#define ACCP_RAD_LUMP 3
+#define ACCP_RAD_LUMP_DAMP 4
+#define ACCP_RAD_LUMP_QE 5
 
 /*
  * One element, in the units of the reference lattice file (metres, radians,
@@ -81,8 +90,8 @@ typedef struct accp_elem {
   int32_t type;      /* ACCP_* type code                                      */
   int32_t nslice;    /* BendMPole/StrMPole slices (reference default 10)      */
   int32_t max_npole; /* BendMPole/StrMPole MaxNPole (1..10)                   */
-  int32_t rad;       /* BendMPole radiation flag ACCP_RAD_*                   */
-  int32_t on;        /* Cavity flag_onoff, LongWake on/off                    */
+  int32_t rad;       /* BendMPole / SyncRad radiation flag ACCP_RAD_*         */
+  int32_t on;        /* Cavity / CrabCav flag_onoff, LongWake on/off          */
   int32_t nbins;     /* LongWake: number of longitudinal bins                 */
   double len;        /* length [m]                                            */
   double k;          /* Quad K [m^-2]; Bend gradient K                        */
@@ -101,6 +110,11 @@ typedef struct accp_elem {
   double wake_fr;    /* LongWake resonant frequency [Hz]                      */
   double wake_charge;/* LongWake bunch charge [C]                             */
   double wake_zmin, wake_zmax; /* LongWake binning window in z [m]            */
+  double mat[36];    /* Matrix: row-major 6x6 map                             */
+  double u0, jx, jy; /* SyncRad: energy loss per turn (units of energy), damping partitions */
+  double emitx, emity, sigdpp, betax, betay; /* SyncRad equilibrium beam      */
+  double devoltx, devolty; /* CrabCav deflecting voltages (units of energy)  */
+  double sigphi, sigdvv;   /* CrabCav phase and relative amplitude noise      */
   char famname[ACCP_NAME_LEN]; /* family name (host-side lookups only)       */
 } accp_elem;
 

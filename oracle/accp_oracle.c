@@ -401,20 +401,22 @@ void orc_elem_pass(const accp_elem *e, double *r, int n, const orc_ctx *ctx) {
         break;
       case ACCP_BENDMPOLE: { /* accel.cpp:649-705 */
         int mo = e->max_npole - 1;
+        /* ABendMPole::set_rad_onoff keeps 0..2 and turns anything else off (accel.h:216-218) */
+        const int rad = (e->rad == 1 || e->rad == 2) ? e->rad : 0;
         orc_edge(p, irho, e->e1);
         for (int m = 0; m < e->nslice; m++) {
-          if (e->rad <= 1) {
+          if (rad <= 1) {
             orc_hxdrift6(p, l1, irho);
-            if (e->rad) orc_bndthinkickrad(p, e->polyB, k1, irho, e->energy, mo, e->rad, 0);
+            if (rad) orc_bndthinkickrad(p, e->polyB, k1, irho, e->energy, mo, rad, 0);
             else orc_bndthinkick(p, e->polyB, k1, irho, mo);
             orc_hxdrift6(p, l2, irho);
-            if (e->rad) orc_bndthinkickrad(p, e->polyB, k2, irho, e->energy, mo, e->rad, 0);
+            if (rad) orc_bndthinkickrad(p, e->polyB, k2, irho, e->energy, mo, rad, 0);
             else orc_bndthinkick(p, e->polyB, k2, irho, mo);
             orc_hxdrift6(p, l2, irho);
-            if (e->rad) orc_bndthinkickrad(p, e->polyB, k1, irho, e->energy, mo, e->rad, 0);
+            if (rad) orc_bndthinkickrad(p, e->polyB, k1, irho, e->energy, mo, rad, 0);
             else orc_bndthinkick(p, e->polyB, k1, irho, mo);
             orc_hxdrift6(p, l1, irho);
-          } else if (e->rad == 2) {
+          } else {
             orc_hxdrift6(p, 0.5 * sl, irho);
             double gs = orc_gauss(ctx->seed, ctx->pid0 + (uint64_t)i, ctx->turn,
                                   (uint32_t)ctx->eidx, (uint32_t)m);
@@ -446,6 +448,61 @@ void orc_elem_pass(const accp_elem *e, double *r, int n, const orc_ctx *ctx) {
         double nv = e->volt / e->energy;
         if (e->len != 0) orc_fastdrift(p, 0.5 * e->len / (1 + p[5]));
         if (e->on) p[5] += nv * sin(ORC_TWOPI * e->freq * p[4] / ORC_C0 + e->phis);
+        if (e->len != 0) orc_fastdrift(p, 0.5 * e->len / (1 + p[5]));
+        break;
+      }
+      case ACCP_MATRIX: { /* accel.cpp:1697-1729 */
+        const double *m = e->mat;
+        if (e->len != 0) orc_fastdrift(p, 0.5 * e->len / (1 + p[5]));
+        double tr[6];
+        for (int j = 0; j < 6; j++)
+          tr[j] = m[j * 6 + 0] * p[0] + m[j * 6 + 1] * p[1] + m[j * 6 + 2] * p[2] + m[j * 6 + 3] * p[3] +
+                  m[j * 6 + 4] * p[4] + m[j * 6 + 5] * p[5];
+        for (int j = 0; j < 6; j++) p[j] = tr[j];
+        if (e->len != 0) orc_fastdrift(p, 0.5 * e->len / (1 + p[5]));
+        break;
+      }
+      case ACCP_SYNCRAD: { /* accel.cpp:1813-1854, flags from set_rad_onoff (aline.cpp:273-288) */
+        const int damp = e->rad == 3 || e->rad == 4, qe = e->rad == 3 || e->rad == 5;
+        const double jz = 4 - e->jx - e->jy; /* accel.h:551 */
+        const double alfx = e->u0 / 2. / e->energy * e->jx, alfy = e->u0 / 2. / e->energy * e->jy;
+        const double alfz = e->u0 / 2. / e->energy * jz;
+        const double sx = sqrt(2. * alfx), sy = sqrt(2. * alfy), sz = sqrt(2. * alfz);
+        const double sigx = sqrt(e->emitx * e->betax), sigy = sqrt(e->emity * e->betay);
+        if (e->len != 0) orc_fastdrift(p, 0.5 * e->len / (1 + p[5]));
+        if (damp) {
+          p[0] *= (1 - alfx);
+          p[2] *= (1 - alfy);
+          p[5] *= (1 - alfz);
+        }
+        if (qe) {
+          const uint64_t pid = ctx->pid0 + (uint64_t)i;
+          const uint32_t k = (uint32_t)ctx->eidx;
+          p[0] += sigx * sx * orc_gauss(ctx->seed, pid, ctx->turn, k, 0);
+          p[2] += sigy * sy * orc_gauss(ctx->seed, pid, ctx->turn, k, 1);
+          p[5] += e->sigdpp * sz * orc_gauss(ctx->seed, pid, ctx->turn, k, 2);
+        }
+        if (e->len != 0) orc_fastdrift(p, 0.5 * e->len / (1 + p[5]));
+        break;
+      }
+      case ACCP_CRABCAV: { /* accel.cpp:968-999 */
+        const double nvx = e->devoltx / e->energy, nvy = e->devolty / e->energy;
+        const double k = ORC_TWOPI * e->freq / ORC_C0;
+        if (e->len != 0) orc_fastdrift(p, 0.5 * e->len / (1 + p[5]));
+        if (e->on) {
+          const uint64_t pid = ctx->pid0 + (uint64_t)i;
+          const uint32_t ei = (uint32_t)ctx->eidx;
+          double om, nvxa = nvx, nvya = nvy;
+          if (e->sigphi > 0) om = -k * p[4] + e->phis + e->sigphi * orc_gauss(ctx->seed, pid, ctx->turn, ei, 0);
+          else om = -k * p[4] + e->phis;
+          if (e->sigdvv > 0) {
+            nvxa = nvx * (1.0 + e->sigdvv * orc_gauss(ctx->seed, pid, ctx->turn, ei, 1));
+            nvya = nvy * (1.0 + e->sigdvv * orc_gauss(ctx->seed, pid, ctx->turn, ei, 2));
+          }
+          p[1] += nvxa * sin(om);
+          p[3] += nvya * sin(om);
+          p[5] += -(nvya * p[2] + nvxa * p[0]) * k * cos(om) / (1 + p[5]);
+        }
         if (e->len != 0) orc_fastdrift(p, 0.5 * e->len / (1 + p[5]));
         break;
       }

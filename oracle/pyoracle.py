@@ -207,6 +207,7 @@ class Reference:
         L.ref_nelem.argtypes = [c_void_p]
         L.ref_set_rad.argtypes = [c_void_p, c_int]
         L.ref_set_cavity.argtypes = [c_void_p, c_int]
+        L.ref_set_crab.argtypes = [c_void_p, c_int]
         L.ref_set_threads.argtypes = [c_int]
         L.ref_pass.argtypes = [c_void_p, DP, c_int, c_int]
         L.ref_pass_range.argtypes = [c_void_p, DP, c_int, c_int, c_int]
@@ -261,10 +262,15 @@ class Reference:
                 self.lib.ref_setfield_int(h, i, b"NSlice", e.nslice)
             if e.type == el.BENDMPOLE and e.energy != el.DEFAULT_E0:
                 self.lib.ref_setfield(h, i, b"energy", e.energy)
-        rads = {e.rad for e in elems if e.type == el.BENDMPOLE}
+        rads = {e.rad for e in elems if e.type in (el.BENDMPOLE, el.SYNCRAD)}
         if len(rads) > 1:
             raise ValueError("reference sets one radiation flag for the whole line")
         line.set_rad(rads.pop() if rads else 0)
+        crabs = [e.on for e in elems if e.type == el.CRABCAV]
+        if crabs:
+            if len(set(crabs)) > 1:
+                raise ValueError("reference sets one crab-cavity flag for the whole line")
+            line.lib.ref_set_crab(h, int(crabs[0]))
         cav = [e.on for e in elems if e.type == el.CAVITY]
         if cav:
             if len(set(cav)) > 1:

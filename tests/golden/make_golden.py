@@ -4,7 +4,7 @@ Runs in the build container only (it needs oracle/_ref/libaccel_ref.so,
 built by `make -C oracle` from the Accel+ sources).  Every fixture is data:
 inputs and the reference's outputs for them, written with numpy.savez
 (no pickles).  Regenerate with:  python tests/golden/make_golden.py
-(or only the optics-services fixture: python tests/golden/make_golden.py optics2)
+(or one of the later fixtures alone: python tests/golden/make_golden.py optics2|elements2)
 """
 import os
 import sys
@@ -113,10 +113,42 @@ def optics2(R, out):
     print("optics2:", tw0, chrom2, co6, ma)
 
 
+def elements2_cases():
+    """Matrix, SyncRad (damping only: the excitation draws from rand()) and
+    CrabCav (no noise), plus a BendMPole under a lumped-radiation flag (the
+    bend then runs without radiation)."""
+    rng = np.random.default_rng(7)
+    m = np.eye(6) + 0.05 * rng.standard_normal((6, 6))
+    bend = el.bendmpole(1.5, 2 * np.pi / 40, polyB=[0, 0.1, 2.0], rad=el.RAD_LUMP, name="BL")
+    return [
+        ("matrix_thin", el.matrix(0.0, m.ravel())),
+        ("matrix_thick", el.matrix(0.5, m.ravel())),
+        ("syncrad_damp", el.syncrad(0.0, 3000.0, 0.75, 1.3, 1.0, 5e-9, 5e-11, 8e-4, 10.0, 5.0,
+                                    rad=el.RAD_LUMP_DAMP)),
+        ("syncrad_damp_thick", el.syncrad(0.4, 3000.0, 0.75, 1.3, 1.0, 5e-9, 5e-11, 8e-4, 10.0, 5.0,
+                                          rad=el.RAD_LUMP_DAMP)),
+        ("crab", el.crabcav(0.0, 1.5, 0.5, 1.5e9, 0.3, 3000.0)),
+        ("crab_thick", el.crabcav(0.3, 1.5, 0.5, 1.5e9, 0.3, 3000.0)),
+        ("bend_lump", bend),
+    ]
+
+
+def elements2(R, out):
+    P = probe_particles()
+    res = {"particles": P}
+    for name, e in elements2_cases():
+        res[name] = R.load([e]).elem_pass(0, P)
+    np.savez_compressed(os.path.join(out, "elements2.npz"), **res)
+    print("elements2:", sorted(res))
+
+
 def main():
     R = Reference()
     if sys.argv[1:] == ["optics2"]:
         optics2(R, HERE)
+        return
+    if sys.argv[1:] == ["elements2"]:
+        elements2(R, HERE)
         return
     out = os.path.join(HERE)
     P = probe_particles()
@@ -194,6 +226,7 @@ def main():
     np.savez_compressed(os.path.join(out, "beamstat.npz"), beam=b, nlive=np.array([nl]), mean=ar,
                         sigma=sg, cross=sr)
     optics2(R, out)
+    elements2(R, out)
     print("fixtures written to", out)
 
 

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol(lib):
 
 
 def test_abi_version(lib):
-    assert lib.lib.accp_abi_version() == 1
+    assert lib.lib.accp_abi_version() == 2  # accp_elem gained the Matrix, SyncRad and CrabCav fields
 
 
 def test_elem_layout_matches_c():

@@ -216,3 +216,32 @@ def test_compaction_and_loss_turns(gpu_lib, oracle):
         newly = np.isnan(r[:, 0]) & (want == 64)
         want[newly] = t
     assert np.array_equal(lt, want)
+
+
+def test_matrix_syncrad_crabcav(gpu_lib, oracle):
+    """The elements of the full kernel beyond the tracking set: against the
+    reference fixtures (elements2.npz) at 1e-12, and with quantum excitation
+    and crab-cavity noise against the oracle on the same Philox stream."""
+    from conftest import golden
+    from tests.golden.make_golden import elements2_cases
+    g = golden("elements2.npz")
+    P = g["particles"]
+    for name, e in elements2_cases():
+        assert_coords_close(gpu_lib.lattice([e]).track(P, 1), g[name], 1e-12, what=name)
+    beam = lat.gaussian_beam(500, seed=31)
+    noisy = [el.syncrad(0.2, 3000.0, 0.75, 1.3, 1.0, 5e-9, 5e-11, 8e-4, 10.0, 5.0, rad=el.RAD_LUMP),
+             el.syncrad(0.0, 3000.0, 0.75, 1.3, 1.0, 5e-9, 5e-11, 8e-4, 10.0, 5.0, rad=el.RAD_LUMP_QE),
+             el.crabcav(0.3, 1.5, 0.5, 1.5e9, 0.3, 3000.0, sigphi=1e-3, sigdvv=1e-2)]
+    for e in noisy:
+        out = gpu_lib.lattice([e]).track(beam, 3, seed=17, turn0=5, pid0=100)
+        ref = oracle.track([e], beam, 3, seed=17, turn0=5, pid0=100)
+        assert_coords_close(out, ref, 1e-12, what=e.name)
+        assert not np.array_equal(out, oracle.track([e], beam, 3, seed=18, turn0=5, pid0=100))
+    # a lumped ring: linear one-turn matrix, crab cavity, lumped radiation
+    m = np.eye(6)
+    mu = 2 * np.pi * 0.31
+    m[0, 0], m[0, 1], m[1, 0], m[1, 1] = np.cos(mu), 10 * np.sin(mu), -np.sin(mu) / 10, np.cos(mu)
+    ring = [el.matrix(1.0, m.ravel()), el.crabcav(0.0, 0.5, 0.0, 1.5e9, 0.0, 3000.0),
+            el.syncrad(0.0, 3000.0, 0.75, 1.3, 1.0, 5e-9, 5e-11, 8e-4, 10.0, 5.0, rad=el.RAD_LUMP)]
+    assert_coords_close(gpu_lib.lattice(ring).track(beam, 50, seed=3), oracle.track(ring, beam, 50, seed=3),
+                        1e-10, what="lumped ring, 50 turns")

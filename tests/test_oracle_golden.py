@@ -141,3 +141,12 @@ def test_optics_services_oracle(oracle):
             arr[i].on = 1
     avg, p, m = oracle.trackma(arr, 0.01, 0.0005, 64, 19, g["ma_locs"])
     assert avg == g["ma_avg"][0] and same_bits(p, g["ma_plus"]) and same_bits(m, g["ma_minus"])
+
+
+def test_element_fixtures2_bit_exact(oracle):
+    """Matrix, SyncRad (damping), CrabCav and a bend under a lumped-radiation
+    flag: one pass, bit for bit with the reference (elements2.npz)."""
+    from tests.golden.make_golden import elements2_cases
+    g = golden("elements2.npz")
+    for name, e in elements2_cases():
+        assert same_bits(oracle.track([e], g["particles"], 1), g[name]), name
