@@ -112,6 +112,8 @@ bool needs_full(const Hdr &h) {
     case OP_MATRIX:
     case OP_SYNCRAD:
     case OP_CRABCAV:
+    case OP_RFMULT:
+    case OP_IDEALWIG:
     case OP_BEND:
     case OP_BENDMP_RAD:
     case OP_BENDMP_QE:
@@ -259,6 +261,29 @@ int compile_lattice(const accp_elem *E, int n, Compiled &out) {
         P.insert(P.end(), {e.len, e.devoltx / e.energy, e.devolty / e.energy, TWOPI * e.freq / C0, e.phis,
                            e.sigphi, e.sigdvv});
         break;
+      case ACCP_RFMULT:  // ARFMult::pass (accel.cpp:1146-1201)
+        if (e.max_npole < 0 || e.max_npole > ACCP_MAX_MPOLE)
+          return fail(ACCP_E_LATTICE, "element %d: RFMult MaxNPole out of range", i);
+        h.op = OP_RFMULT;
+        h.a = e.sigphi > 0 ? 1 : 0;
+        h.b = e.max_npole;
+        P.insert(P.end(), {e.len, TWOPI * e.freq / C0, e.phis, e.sigphi});
+        P.insert(P.end(), e.polyB, e.polyB + e.max_npole);
+        P.insert(P.end(), e.polyA, e.polyA + e.max_npole);
+        break;
+      case ACCP_IDEALWIG: {  // AIdealWig::pass (accel.cpp:1463-1537)
+        if (e.len != 0 && (e.nslice < 1 || !(e.wig_lambda > 0)))
+          return fail(ACCP_E_LATTICE, "element %d: IdealWig needs nslice>=1 and lambda>0", i);
+        const double k = 2.0 * PI / (e.wig_lambda > 0 ? e.wig_lambda : 1.0);
+        const double hh = e.wig_h * e.wig_h, lns = e.len / (e.nslice > 0 ? e.nslice : 1);
+        const double crad = CGAMMA * e.energy * e.energy * e.energy / (TWOPI * 1e9);
+        const bool rad = e.rad == ACCP_RAD_DAMP || e.rad == ACCP_RAD_QE;  // accel.h:448-452
+        h.op = e.len == 0 ? OP_NOP : OP_IDEALWIG;
+        h.a = e.nslice;
+        h.b = (e.wig_type & 0xff) | (rad ? 1 << 8 : 0);
+        P.insert(P.end(), {e.len, k * k, hh, lns, 0.5 * lns, crad * hh * lns / 2.0});
+        break;
+      }
       case ACCP_LONGWAKE:
         if (e.nbins < 1 || e.nbins > WAKE_MAX_BINS || !(e.wake_zmax > e.wake_zmin) ||
             e.wake_q <= 0.5 || e.energy == 0)

@@ -352,6 +352,21 @@ __device__ __forceinline__ void element_body(Part (&p)[PPL], const Hdr h, const 
       }
       break;
     }
+    case OP_RFMULT: {
+      if (!FULL) break;
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        const double g = h.a ? gauss_kick(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, 0) : 0.0;
+        rfmult(p[k], P, h.b, h.a != 0, g);
+      }
+      break;
+    }
+    case OP_IDEALWIG: {
+      if (!FULL) break;
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) idealwig(p[k], P, h.a, h.b & 0xff, (h.b >> 8) & 1);
+      break;
+    }
     case OP_CRABCAV: {
       if (!FULL) break;
 #pragma unroll

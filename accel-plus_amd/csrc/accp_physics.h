@@ -473,6 +473,66 @@ ACCP_HD void crabcav(Part &p, const PT *C, int flags, double g0, double g1, doub
   if (L != 0) drift_nl(p, 0.5 * L * p.ipd);
 }
 
+// ---- ARFMult: RF multipole (accel.cpp:1146-1201) ---------------------------
+// C = [L, k, phis, sigphi, B0..Bn-1, A0..An-1]; g = phase normal.
+template <class PT>
+ACCP_HD void rfmult(Part &p, const PT *C, int n, bool noise, double g) {
+  const double L = C[0], k = C[1];
+  if (L != 0) drift_nl(p, 0.5 * L * p.ipd);
+  const double om = noise ? -k * p.z + C[2] + C[3] * g : -k * p.z + C[2];
+  const double rho = sqrt(p.x * p.x + p.y * p.y);
+  const double ph = atan2(p.y, p.x);
+  const PT *B = C + 4, *A = C + 4 + n;
+  double AsL = 0, ArhoL = 0, AphiL = 0, nrho = 1.0;
+  for (int m = 0; m < n; ++m) {
+    double sn, cs;
+    sincos((m + 1) * ph, &sn, &cs);
+    const double td = B[m] * cs + A[m] * sn;
+    ArhoL += nrho * td;
+    AphiL += nrho * (-B[m] * sn + A[m] * cs);
+    nrho *= rho;
+    AsL += 1.0 / (m + 1) * nrho * td;
+  }
+  double sph, cph, so, co;
+  sincos(ph, &sph, &cph);
+  sincos(om, &so, &co);
+  p.px += (ArhoL * cph - AphiL * sph) * co;
+  p.py += (ArhoL * sph + AphiL * cph) * co;
+  set_delta(p, p.d + k * AsL * so / (1. + p.d));
+  if (L != 0) drift_nl(p, 0.5 * L * p.ipd);
+}
+
+// ---- AIdealWig: ideal wiggler (accel.cpp:1463-1537) -------------------------
+// C = [L, k^2, h^2, L/nslice, L/(2 nslice), crad h^2 L/nslice/2].
+template <class PT>
+ACCP_HD void idealwig(Part &p, const PT *C, int nslice, int type, bool rad) {
+  const double k2 = C[1], hh = C[2], lns = C[3], half = C[4], rq = C[5];
+  for (int j = 0; j < nslice; ++j) {
+    drift_nl(p, half * p.ipd);
+    const double y = p.y, x = p.x;
+    if (rad) {  // half of the slice's classical loss, angles kept
+      const double xp = p.px * p.ipd, yp = p.py * p.ipd;
+      set_delta(p, p.d - rq * (1 + p.d) * (1 + p.d));
+      p.px = xp * (1 + p.d);
+      p.py = yp * (1 + p.d);
+    }
+    if (type == 0) {
+      const double y3 = y * y * y;
+      p.py -= hh * (0.5 * y + 1.0 / 3.0 * y3 * k2 + 1.0 / 15.0 * y3 * y * y * k2 * k2) * lns;
+    } else if (type == 1) {
+      const double x3 = x * x * x;
+      p.px -= hh * (0.5 * x + 1.0 / 3.0 * x3 * k2 + 1.0 / 15.0 * x3 * x * x * k2 * k2) * lns;
+    }
+    if (rad) {
+      const double xp = p.px * p.ipd, yp = p.py * p.ipd;
+      set_delta(p, p.d - rq * (1 + p.d) * (1 + p.d));
+      p.px = xp * (1 + p.d);
+      p.py = yp * (1 + p.d);
+    }
+    drift_nl(p, half * p.ipd);
+  }
+}
+
 // ---- counter-based normal variates (quantum excitation) -------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
 // easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies with a

@@ -31,6 +31,8 @@ enum Op : int32_t {
   OP_MATRIX,      // [L, M00..M55]                          (row-major 6x6)
   OP_SYNCRAD,     // [L, 1-ax, 1-ay, 1-az, sigx sx, sigy sy, sigdpp sz]   a = damping | excitation<<1
   OP_CRABCAV,     // [L, Vx/E, Vy/E, 2 pi f / c, phis, sigphi, sigdVV]    a = on | phase noise<<1 | amplitude noise<<2
+  OP_RFMULT,      // [L, 2 pi f / c, phis, sigphi, B0..Bn-1, A0..An-1]     a = phase noise, b = n
+  OP_IDEALWIG,    // [L, k^2, h^2, L/nslice, L/(2 nslice), radiation coef] a = nslice, b = type | radiation<<8
   OP_COUNT
 };
 

@@ -220,6 +220,18 @@ std::vector<accp_elem> read_lattice(const string &path) {
       string row;
       std::getline(in, row);
       std::istringstream(row) >> e.emitx >> e.emity >> e.sigdpp >> e.betax >> e.betay;
+    } else if (type == "RFMult") {  // accel.cpp:1058-1068: L f phis E sigphi sigdVV MaxNPole / PolyB / PolyA
+      e = new_elem(ACCP_RFMULT, fam);
+      ps >> e.len >> e.freq >> e.phis >> e.energy >> e.sigphi >> e.sigdvv >> e.max_npole;
+      if (e.max_npole < 0 || e.max_npole > ACCP_MAX_MPOLE)
+        throw std::runtime_error("RFMult " + fam + ": MaxNPole out of range");
+      for (int i = 0; i < e.max_npole; ++i) in >> e.polyB[i];
+      std::getline(in, rest);
+      for (int i = 0; i < e.max_npole; ++i) in >> e.polyA[i];
+      std::getline(in, rest);
+    } else if (type == "IdealWig") {  // accel.cpp:1449-1456: L nslice lambda h type
+      e = new_elem(ACCP_IDEALWIG, fam);
+      ps >> e.len >> e.nslice >> e.wig_lambda >> e.wig_h >> e.wig_type;
     } else if (type == "CrabCav") {  // accel.cpp:937-943
       e = new_elem(ACCP_CRABCAV, fam);
       ps >> e.len >> e.devoltx >> e.devolty >> e.freq >> e.phis >> e.energy >> e.sigphi >> e.sigdvv;
@@ -270,6 +282,8 @@ struct Line {
       case ACCP_MATRIX: return "Matrix";
       case ACCP_SYNCRAD: return "SyncRad";
       case ACCP_CRABCAV: return "CrabCav";
+      case ACCP_RFMULT: return "RFMult";
+      case ACCP_IDEALWIG: return "IdealWig";
     }
     return "?";
   }
@@ -290,7 +304,7 @@ struct Line {
   int set_rad(int flag) {
     int n = 0;
     for (auto &x : e)
-      if (x.type == ACCP_BENDMPOLE) x.rad = flag <= ACCP_RAD_QE ? flag : ACCP_RAD_OFF, ++n;
+      if (x.type == ACCP_BENDMPOLE || x.type == ACCP_IDEALWIG) x.rad = flag <= ACCP_RAD_QE ? flag : ACCP_RAD_OFF, ++n;
       else if (x.type == ACCP_SYNCRAD) x.rad = flag >= ACCP_RAD_LUMP ? flag : ACCP_RAD_OFF, ++n;
     dirty = true;
     return n;
@@ -377,6 +391,17 @@ void serialize(const accp_elem &x, std::ostream &os) {
     case ACCP_SYNCRAD:
       os << head << vals({x.energy, x.u0, x.jx, x.jy}) << "\t" << std::endl
          << vals({x.emitx, x.emity, x.sigdpp, x.betax, x.betay}) << "\t" << std::endl;
+      break;
+    case ACCP_RFMULT:
+      os << head << vals({x.freq, x.phis, x.energy, x.sigphi, x.sigdvv}) << "\t" << x.max_npole << std::endl
+         << "\t\t\t";
+      for (int i = 0; i < x.max_npole; ++i) os << lossless(x.polyB[i]) << "\t";
+      os << std::endl << "\t\t\t";
+      for (int i = 0; i < x.max_npole; ++i) os << lossless(x.polyA[i]) << "\t";
+      os << std::endl;
+      break;
+    case ACCP_IDEALWIG:
+      os << head << "\t" << x.nslice << vals({x.wig_lambda, x.wig_h}) << "\t" << x.wig_type << std::endl;
       break;
     case ACCP_CRABCAV:
       os << head << vals({x.devoltx, x.devolty, x.freq, x.phis, x.energy, x.sigphi, x.sigdvv}) << std::endl;

@@ -13,12 +13,13 @@ import numpy as np
 
 MAX_MPOLE = 10
 
-DRIFT, QUAD, CORRECTOR, APERTURE, MONITOR, BEND, BENDMPOLE, STRMPOLE, CAVITY, LONGWAKE, MATRIX, SYNCRAD, CRABCAV = \
-    range(1, 14)
+(DRIFT, QUAD, CORRECTOR, APERTURE, MONITOR, BEND, BENDMPOLE, STRMPOLE, CAVITY, LONGWAKE, MATRIX, SYNCRAD, CRABCAV,
+ RFMULT, IDEALWIG) = range(1, 16)
 TYPE_NAMES = {
     DRIFT: "Drift", QUAD: "Quad", CORRECTOR: "Corrector", APERTURE: "Aperture",
     MONITOR: "Monitor", BEND: "Bend", BENDMPOLE: "BendMPole", STRMPOLE: "StrMPole",
     CAVITY: "Cavity", LONGWAKE: "LongWake", MATRIX: "Matrix", SYNCRAD: "SyncRad", CRABCAV: "CrabCav",
+    RFMULT: "RFMult", IDEALWIG: "IdealWig",
 }
 RAD_OFF, RAD_DAMP, RAD_QE, RAD_LUMP, RAD_LUMP_DAMP, RAD_LUMP_QE = 0, 1, 2, 3, 4, 5
 FIELD_K, FIELD_POLYB, FIELD_POLYA = 1, 2, 3
@@ -42,6 +43,7 @@ class Elem(ctypes.Structure):
         ("u0", c_double), ("jx", c_double), ("jy", c_double),
         ("emitx", c_double), ("emity", c_double), ("sigdpp", c_double), ("betax", c_double), ("betay", c_double),
         ("devoltx", c_double), ("devolty", c_double), ("sigphi", c_double), ("sigdvv", c_double),
+        ("wig_lambda", c_double), ("wig_h", c_double), ("wig_type", c_int32), ("reserved", c_int32),
         ("famname", c_char * 32),
     ]
 
@@ -138,6 +140,19 @@ def crabcav(L, devoltx, devolty, freq, phis, energy=DEFAULT_E0, sigphi=0.0, sigd
                phis=float(phis), energy=float(energy), sigphi=float(sigphi), sigdvv=float(sigdvv), on=on)
 
 
+def rfmult(L, freq, phis, polyB, polyA=None, energy=DEFAULT_E0, sigphi=0.0, sigdvv=0.0, name="RFM"):
+    """ARFMult: RF multipole, fields B_n, A_n modulated by cos(-k z + phis)."""
+    n = len(polyB)
+    return _mk(RFMULT, name, L, freq=float(freq), phis=float(phis), energy=float(energy), sigphi=float(sigphi),
+               sigdvv=float(sigdvv), polyB=polyB, polyA=polyA if polyA is not None else [0.0] * n, max_npole=n)
+
+
+def idealwig(L, nslice, lam, h, wig_type=0, energy=DEFAULT_E0, rad=RAD_OFF, name="WIG"):
+    """AIdealWig: wiggler of period lam and peak curvature h, horizontal (0) or vertical (1)."""
+    return _mk(IDEALWIG, name, L, nslice=int(nslice), wig_lambda=float(lam), wig_h=float(h), wig_type=int(wig_type),
+               energy=float(energy), rad=rad)
+
+
 def longwake(rs, q, fr, charge, nbins, zmin, zmax, energy=DEFAULT_E0, on=1, name="WAKE"):
     return _mk(LONGWAKE, name, 0.0, wake_rs=float(rs), wake_q=float(q), wake_fr=float(fr),
                wake_charge=float(charge), nbins=int(nbins), wake_zmin=float(zmin),
@@ -196,6 +211,13 @@ def write_lattice(elems, path):
         elif t == SYNCRAD:
             lines.append(f"{n} : SyncRad {_g(e.len)} {_g(e.energy)} {_g(e.u0)} {_g(e.jx)} {_g(e.jy)}")
             lines.append(f"   {_g(e.emitx)} {_g(e.emity)} {_g(e.sigdpp)} {_g(e.betax)} {_g(e.betay)}")
+        elif t == RFMULT:
+            lines.append(f"{n} : RFMult {_g(e.len)} {_g(e.freq)} {_g(e.phis)} {_g(e.energy)} {_g(e.sigphi)} "
+                         f"{_g(e.sigdvv)} {e.max_npole}")
+            lines.append("   " + " ".join(_g(e.polyB[i]) for i in range(e.max_npole)))
+            lines.append("   " + " ".join(_g(e.polyA[i]) for i in range(e.max_npole)))
+        elif t == IDEALWIG:
+            lines.append(f"{n} : IdealWig {_g(e.len)} {e.nslice} {_g(e.wig_lambda)} {_g(e.wig_h)} {e.wig_type}")
         elif t == CRABCAV:
             lines.append(f"{n} : CrabCav {_g(e.len)} {_g(e.devoltx)} {_g(e.devolty)} {_g(e.freq)} {_g(e.phis)} "
                          f"{_g(e.energy)} {_g(e.sigphi)} {_g(e.sigdvv)}")

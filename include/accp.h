@@ -65,6 +65,8 @@ extern "C" {
 #define ACCP_MATRIX 11   /* AMatrix     src/lib/accel/accel.cpp:1661-1730 */
 #define ACCP_SYNCRAD 12  /* ASyncRad    src/lib/accel/accel.cpp:1733-1854 (lumped radiation) */
 #define ACCP_CRABCAV 13  /* ACrabCav    src/lib/accel/accel.cpp:927-1000 */
+#define ACCP_RFMULT 14   /* ARFMult     src/lib/accel/accel.cpp:1043-1202 (RF multipole) */
+#define ACCP_IDEALWIG 15 /* AIdealWig   src/lib/accel/accel.cpp:1386-1537 (ideal wiggler) */
 
 #define ACCP_MAX_MPOLE 10 /* MAX_MPOLE, src/lib/accel/accel.h:48 */
 #define ACCP_NAME_LEN 32
@@ -90,7 +92,7 @@ typedef struct accp_elem {
   int32_t type;      /* ACCP_* type code                                      */
   int32_t nslice;    /* BendMPole/StrMPole slices (reference default 10)      */
   int32_t max_npole; /* BendMPole/StrMPole MaxNPole (1..10)                   */
-  int32_t rad;       /* BendMPole / SyncRad radiation flag ACCP_RAD_*         */
+  int32_t rad;       /* BendMPole / IdealWig / SyncRad radiation flag ACCP_RAD_* */
   int32_t on;        /* Cavity / CrabCav flag_onoff, LongWake on/off          */
   int32_t nbins;     /* LongWake: number of longitudinal bins                 */
   double len;        /* length [m]                                            */
@@ -114,7 +116,10 @@ typedef struct accp_elem {
   double u0, jx, jy; /* SyncRad: energy loss per turn (units of energy), damping partitions */
   double emitx, emity, sigdpp, betax, betay; /* SyncRad equilibrium beam      */
   double devoltx, devolty; /* CrabCav deflecting voltages (units of energy)  */
-  double sigphi, sigdvv;   /* CrabCav phase and relative amplitude noise      */
+  double sigphi, sigdvv;   /* CrabCav / RFMult phase and amplitude noise      */
+  double wig_lambda, wig_h; /* IdealWig period [m] and peak curvature [1/m]  */
+  int32_t wig_type;         /* IdealWig: 0 horizontal, 1 vertical wiggler     */
+  int32_t reserved;
   char famname[ACCP_NAME_LEN]; /* family name (host-side lookups only)       */
 } accp_elem;
 

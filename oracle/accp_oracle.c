@@ -346,6 +346,17 @@ static int outside(const double *p, const accp_elem *e) {
   return p[0] <= e->xlim[0] || p[0] >= e->xlim[1] || p[2] <= e->ylim[0] || p[2] >= e->ylim[1];
 }
 
+/* Half of the ideal wiggler's classical radiation loss per slice, angles
+ * kept (accel.cpp:1495-1504). */
+static void wig_rad_half(double *p, const accp_elem *e) {
+  const double crad = ORC_CGAMMA * e->energy * e->energy * e->energy / (ORC_TWOPI * 1e9);
+  const double h = e->wig_h, dp = p[5];
+  const double yp = p[3] / (1 + dp), xp = p[1] / (1 + dp);
+  p[5] = p[5] - crad * sq(1 + p[5]) * h * h * e->len / e->nslice / 2.0;
+  p[1] = xp * (1 + p[5]);
+  p[3] = yp * (1 + p[5]);
+}
+
 void orc_elem_pass(const accp_elem *e, double *r, int n, const orc_ctx *ctx) {
   if (e->type == ACCP_LONGWAKE) {
     orc_longwake(e, r, n);
@@ -483,6 +494,52 @@ void orc_elem_pass(const accp_elem *e, double *r, int n, const orc_ctx *ctx) {
           p[5] += e->sigdpp * sz * orc_gauss(ctx->seed, pid, ctx->turn, k, 2);
         }
         if (e->len != 0) orc_fastdrift(p, 0.5 * e->len / (1 + p[5]));
+        break;
+      }
+      case ACCP_RFMULT: { /* accel.cpp:1146-1201 */
+        const double k = ORC_TWOPI * e->freq / ORC_C0;
+        if (e->len != 0) orc_fastdrift(p, 0.5 * e->len / (1 + p[5]));
+        double om;
+        if (e->sigphi > 0)
+          om = -k * p[4] + e->phis +
+               e->sigphi * orc_gauss(ctx->seed, ctx->pid0 + (uint64_t)i, ctx->turn, (uint32_t)ctx->eidx, 0);
+        else
+          om = -k * p[4] + e->phis;
+        const double rho = sqrt(sq(p[0]) + sq(p[2]));
+        const double ph = atan2(p[2], p[0]);
+        double AsL = 0, ArhoL = 0, AphiL = 0, nrho = 1.0;
+        for (int m = 0; m < e->max_npole; m++) {
+          const double nphi = (m + 1) * ph;
+          const double td = (e->polyB[m] * cos(nphi) + e->polyA[m] * sin(nphi));
+          ArhoL += nrho * td;
+          AphiL += nrho * (-e->polyB[m] * sin(nphi) + e->polyA[m] * cos(nphi));
+          nrho *= rho;
+          AsL += 1.0 / (m + 1) * nrho * td;
+        }
+        p[1] += (ArhoL * cos(ph) - AphiL * sin(ph)) * cos(om);
+        p[3] += (ArhoL * sin(ph) + AphiL * cos(ph)) * cos(om);
+        p[5] += k * AsL * sin(om) / (1. + p[5]);
+        if (e->len != 0) orc_fastdrift(p, 0.5 * e->len / (1 + p[5]));
+        break;
+      }
+      case ACCP_IDEALWIG: { /* accel.cpp:1463-1537 */
+        if (e->len == 0) break;
+        const double k = 2.0 * ORC_PI / e->wig_lambda, h = e->wig_h;
+        const int rad = e->rad == 1 || e->rad == 2; /* AIdealWig::set_rad_onoff (accel.h:448-452) */
+        for (int j = 0; j < e->nslice; j++) {
+          orc_fastdrift(p, 0.5 * e->len / (1 + p[5]) / e->nslice);
+          const double y = p[2], x = p[0];
+          if (rad) wig_rad_half(p, e);
+          if (e->wig_type == 0) {
+            const double y3 = y * y * y, k2 = k * k;
+            p[3] -= h * h * (0.5 * y + 1.0 / 3.0 * y3 * k2 + 1.0 / 15.0 * y3 * y * y * k2 * k2) * e->len / e->nslice;
+          } else if (e->wig_type == 1) {
+            const double x3 = x * x * x, k2 = k * k;
+            p[1] -= h * h * (0.5 * x + 1.0 / 3.0 * x3 * k2 + 1.0 / 15.0 * x3 * x * x * k2 * k2) * e->len / e->nslice;
+          }
+          if (rad) wig_rad_half(p, e);
+          orc_fastdrift(p, 0.5 * e->len / (1 + p[5]) / e->nslice);
+        }
         break;
       }
       case ACCP_CRABCAV: { /* accel.cpp:968-999 */

@@ -262,7 +262,9 @@ class Reference:
                 self.lib.ref_setfield_int(h, i, b"NSlice", e.nslice)
             if e.type == el.BENDMPOLE and e.energy != el.DEFAULT_E0:
                 self.lib.ref_setfield(h, i, b"energy", e.energy)
-        rads = {e.rad for e in elems if e.type in (el.BENDMPOLE, el.SYNCRAD)}
+            if e.type == el.IDEALWIG and e.energy != el.DEFAULT_E0:
+                raise ValueError("the reference's IdealWig energy is not settable from the lattice file")
+        rads = {e.rad for e in elems if e.type in (el.BENDMPOLE, el.SYNCRAD, el.IDEALWIG)}
         if len(rads) > 1:
             raise ValueError("reference sets one radiation flag for the whole line")
         line.set_rad(rads.pop() if rads else 0)

@@ -114,8 +114,9 @@ def optics2(R, out):
 
 
 def elements2_cases():
-    """Matrix, SyncRad (damping only: the excitation draws from rand()) and
-    CrabCav (no noise), plus a BendMPole under a lumped-radiation flag (the
+    """Matrix, SyncRad (damping only: the excitation draws from rand()),
+    CrabCav and RFMult (no noise), ideal wigglers (horizontal, vertical, with
+    classical radiation), plus a BendMPole under a lumped-radiation flag (the
     bend then runs without radiation)."""
     rng = np.random.default_rng(7)
     m = np.eye(6) + 0.05 * rng.standard_normal((6, 6))
@@ -130,6 +131,11 @@ def elements2_cases():
         ("crab", el.crabcav(0.0, 1.5, 0.5, 1.5e9, 0.3, 3000.0)),
         ("crab_thick", el.crabcav(0.3, 1.5, 0.5, 1.5e9, 0.3, 3000.0)),
         ("bend_lump", bend),
+        ("rfmult", el.rfmult(0.0, 5e8, 0.2, [1e-4, 2e-3, 0.5, 3.0], [0.0, 1e-3, 0.2, 0.0])),
+        ("rfmult_thick", el.rfmult(0.3, 5e8, 0.2, [1e-4, 2e-3, 0.5], [0.0, 1e-3, 0.2])),
+        ("wiggler_h", el.idealwig(2.0, 20, 0.1, 0.5, 0)),
+        ("wiggler_v_rad", el.idealwig(2.0, 20, 0.1, 0.5, 1, rad=el.RAD_DAMP)),
+        ("wiggler_h_rad2", el.idealwig(1.0, 7, 0.08, 0.3, 0, rad=el.RAD_QE)),
     ]
 
 

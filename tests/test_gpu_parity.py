@@ -218,7 +218,7 @@ def test_compaction_and_loss_turns(gpu_lib, oracle):
     assert np.array_equal(lt, want)
 
 
-def test_matrix_syncrad_crabcav(gpu_lib, oracle):
+def test_matrix_syncrad_crabcav_rfmult_wiggler(gpu_lib, oracle):
     """The elements of the full kernel beyond the tracking set: against the
     reference fixtures (elements2.npz) at 1e-12, and with quantum excitation
     and crab-cavity noise against the oracle on the same Philox stream."""
@@ -229,7 +229,8 @@ def test_matrix_syncrad_crabcav(gpu_lib, oracle):
     for name, e in elements2_cases():
         assert_coords_close(gpu_lib.lattice([e]).track(P, 1), g[name], 1e-12, what=name)
     beam = lat.gaussian_beam(500, seed=31)
-    noisy = [el.syncrad(0.2, 3000.0, 0.75, 1.3, 1.0, 5e-9, 5e-11, 8e-4, 10.0, 5.0, rad=el.RAD_LUMP),
+    noisy = [el.rfmult(0.1, 5e8, 0.2, [1e-4, 2e-3, 0.5], [0.0, 1e-3, 0.2], sigphi=2e-3),
+             el.syncrad(0.2, 3000.0, 0.75, 1.3, 1.0, 5e-9, 5e-11, 8e-4, 10.0, 5.0, rad=el.RAD_LUMP),
              el.syncrad(0.0, 3000.0, 0.75, 1.3, 1.0, 5e-9, 5e-11, 8e-4, 10.0, 5.0, rad=el.RAD_LUMP_QE),
              el.crabcav(0.3, 1.5, 0.5, 1.5e9, 0.3, 3000.0, sigphi=1e-3, sigdvv=1e-2)]
     for e in noisy:
