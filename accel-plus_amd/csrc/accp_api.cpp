@@ -284,6 +284,19 @@ int compile_lattice(const accp_elem *E, int n, Compiled &out) {
         P.insert(P.end(), {e.len, k * k, hh, lns, 0.5 * lns, crad * hh * lns / 2.0});
         break;
       }
+      case ACCP_TRANSWAKE:  // ATransWake (accel.cpp:1546-1657), whole beam
+        if (e.energy == 0 || !(e.tw_q[0] > 0.5) || !(e.tw_q[1] > 0.5))
+          return fail(ACCP_E_LATTICE, "element %d: TransWake needs energy>0 and Q>1/2 in both planes", i);
+        if (e.on) {
+          h.op = OP_WAKE;
+          out.wakes.push_back(i);
+        } else if (e.len != 0) {
+          h.op = OP_DRIFT;  // two half drifts of the element with the wake off
+          P.push_back(e.len);
+        } else {
+          h.op = OP_NOP;
+        }
+        break;
       case ACCP_LONGWAKE:
         if (e.nbins < 1 || e.nbins > WAKE_MAX_BINS || !(e.wake_zmax > e.wake_zmin) ||
             e.wake_q <= 0.5 || e.energy == 0)

@@ -88,6 +88,24 @@ int wake_prepare(WakeState &w, const accp_elem *E, int n, const std::vector<int>
   for (int idx : wakes) {
     if (idx < 0 || idx >= n) continue;
     const accp_elem &e = E[idx];
+    if (e.type == ACCP_TRANSWAKE) {
+      // ATransWake::pass constants (accel.cpp:1605-1615)
+      WakeElem we;
+      we.eidx = idx;
+      we.kind = 1;
+      we.len = e.len;
+      we.qm_charge = e.wake_charge;
+      we.e0ev = e.energy * 1e6;
+      we.wx = e.tw_w[0];
+      we.wy = e.tw_w[1];
+      const double kRx = 2 * PI * e.tw_fr[0] * 1.0e9 / C0, kRy = 2 * PI * e.tw_fr[1] * 1.0e9 / C0;
+      we.ax = kRx / 2 / e.tw_q[0];
+      we.ay = kRy / 2 / e.tw_q[1];
+      we.kbx = kRx * std::sqrt(1.0 - 0.25 / (e.tw_q[0] * e.tw_q[0]));
+      we.kby = kRy * std::sqrt(1.0 - 0.25 / (e.tw_q[1] * e.tw_q[1]));
+      w.elems.push_back(we);
+      continue;
+    }
     WakeElem we;
     we.eidx = idx;
     we.nbins = e.nbins;
@@ -117,12 +135,18 @@ int wake_prepare(WakeState &w, const accp_elem *E, int n, const std::vector<int>
 }
 
 int wake_apply(WakeState &w, int eidx, double *d_r, int64_t n, hipStream_t s) {
-  const WakeElem *we = nullptr;
-  for (const WakeElem &x : w.elems)
+  WakeElem *we = nullptr;
+  for (WakeElem &x : w.elems)
     if (x.eidx == eidx) we = &x;
   if (!we) {
     g_wake_err = "no prepared wake at this element";
     return ACCP_E_STATE;
+  }
+  if (we->kind == 1) {
+    const char *err = nullptr;
+    const int rc = transwake_apply(*we, d_r, n, s, &err);
+    if (rc) g_wake_err = err ? err : "TransWake failed";
+    return rc;
   }
   if (n <= 0) return ACCP_OK;
   const int nb = we->nbins;
@@ -145,6 +169,7 @@ int wake_apply(WakeState &w, int eidx, double *d_r, int64_t n, hipStream_t s) {
 
 void wake_release(WakeState &w) {
   for (WakeElem &x : w.elems) {
+    transwake_release(x);
     if (x.d_wt) (void)hipFree(x.d_wt);
     if (x.d_cnt) (void)hipFree(x.d_cnt);
     if (x.d_v) (void)hipFree(x.d_v);

@@ -232,6 +232,10 @@ std::vector<accp_elem> read_lattice(const string &path) {
     } else if (type == "IdealWig") {  // accel.cpp:1449-1456: L nslice lambda h type
       e = new_elem(ACCP_IDEALWIG, fam);
       ps >> e.len >> e.nslice >> e.wig_lambda >> e.wig_h >> e.wig_type;
+    } else if (type == "TransWake") {  // accel.cpp:1546-1554: L E Ne Wx fRx Qx Wy fRy Qy
+      e = new_elem(ACCP_TRANSWAKE, fam);
+      ps >> e.len >> e.energy >> e.wake_charge >> e.tw_w[0] >> e.tw_fr[0] >> e.tw_q[0] >> e.tw_w[1] >> e.tw_fr[1] >>
+          e.tw_q[1];
     } else if (type == "CrabCav") {  // accel.cpp:937-943
       e = new_elem(ACCP_CRABCAV, fam);
       ps >> e.len >> e.devoltx >> e.devolty >> e.freq >> e.phis >> e.energy >> e.sigphi >> e.sigdvv;
@@ -284,6 +288,7 @@ struct Line {
       case ACCP_CRABCAV: return "CrabCav";
       case ACCP_RFMULT: return "RFMult";
       case ACCP_IDEALWIG: return "IdealWig";
+      case ACCP_TRANSWAKE: return "TransWake";
     }
     return "?";
   }
@@ -312,7 +317,7 @@ struct Line {
   int set_on(int type, int on) {
     int n = 0;
     for (auto &x : e)
-      if (x.type == type) x.on = on, ++n;
+      if (x.type == type || (type == ACCP_LONGWAKE && x.type == ACCP_TRANSWAKE)) x.on = on, ++n;
     dirty = true;
     return n;
   }
@@ -402,6 +407,10 @@ void serialize(const accp_elem &x, std::ostream &os) {
       break;
     case ACCP_IDEALWIG:
       os << head << "\t" << x.nslice << vals({x.wig_lambda, x.wig_h}) << "\t" << x.wig_type << std::endl;
+      break;
+    case ACCP_TRANSWAKE:
+      os << head << vals({x.energy, x.wake_charge, x.tw_w[0], x.tw_fr[0], x.tw_q[0], x.tw_w[1], x.tw_fr[1], x.tw_q[1]})
+         << std::endl;
       break;
     case ACCP_CRABCAV:
       os << head << vals({x.devoltx, x.devolty, x.freq, x.phis, x.energy, x.sigphi, x.sigdvv}) << std::endl;

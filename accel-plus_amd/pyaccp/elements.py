@@ -14,12 +14,12 @@ import numpy as np
 MAX_MPOLE = 10
 
 (DRIFT, QUAD, CORRECTOR, APERTURE, MONITOR, BEND, BENDMPOLE, STRMPOLE, CAVITY, LONGWAKE, MATRIX, SYNCRAD, CRABCAV,
- RFMULT, IDEALWIG) = range(1, 16)
+ RFMULT, IDEALWIG, TRANSWAKE) = range(1, 17)
 TYPE_NAMES = {
     DRIFT: "Drift", QUAD: "Quad", CORRECTOR: "Corrector", APERTURE: "Aperture",
     MONITOR: "Monitor", BEND: "Bend", BENDMPOLE: "BendMPole", STRMPOLE: "StrMPole",
     CAVITY: "Cavity", LONGWAKE: "LongWake", MATRIX: "Matrix", SYNCRAD: "SyncRad", CRABCAV: "CrabCav",
-    RFMULT: "RFMult", IDEALWIG: "IdealWig",
+    RFMULT: "RFMult", IDEALWIG: "IdealWig", TRANSWAKE: "TransWake",
 }
 RAD_OFF, RAD_DAMP, RAD_QE, RAD_LUMP, RAD_LUMP_DAMP, RAD_LUMP_QE = 0, 1, 2, 3, 4, 5
 FIELD_K, FIELD_POLYB, FIELD_POLYA = 1, 2, 3
@@ -43,7 +43,9 @@ class Elem(ctypes.Structure):
         ("u0", c_double), ("jx", c_double), ("jy", c_double),
         ("emitx", c_double), ("emity", c_double), ("sigdpp", c_double), ("betax", c_double), ("betay", c_double),
         ("devoltx", c_double), ("devolty", c_double), ("sigphi", c_double), ("sigdvv", c_double),
-        ("wig_lambda", c_double), ("wig_h", c_double), ("wig_type", c_int32), ("reserved", c_int32),
+        ("wig_lambda", c_double), ("wig_h", c_double),
+        ("tw_w", c_double * 2), ("tw_fr", c_double * 2), ("tw_q", c_double * 2),
+        ("wig_type", c_int32), ("reserved", c_int32),
         ("famname", c_char * 32),
     ]
 
@@ -70,7 +72,7 @@ def _mk(t, name, L, **kw):
             arr = getattr(e, k)
             for i, x in enumerate(v):
                 arr[i] = float(x)
-        elif k in ("xlim", "ylim"):
+        elif k in ("xlim", "ylim", "tw_w", "tw_fr", "tw_q"):
             arr = getattr(e, k)
             arr[0], arr[1] = float(v[0]), float(v[1])
         elif k == "mat":
@@ -153,6 +155,13 @@ def idealwig(L, nslice, lam, h, wig_type=0, energy=DEFAULT_E0, rad=RAD_OFF, name
                energy=float(energy), rad=rad)
 
 
+def transwake(L, ne, wx, frx, qx, wy, fry, qy, energy=DEFAULT_E0, on=1, name="TWAKE"):
+    """ATransWake: transverse resonator wake of the whole beam; ne = bunch
+    charge [nC], W per plane, resonant frequency [GHz], quality factor."""
+    return _mk(TRANSWAKE, name, L, wake_charge=float(ne), tw_w=(wx, wy), tw_fr=(frx, fry), tw_q=(qx, qy),
+               energy=float(energy), on=on)
+
+
 def longwake(rs, q, fr, charge, nbins, zmin, zmax, energy=DEFAULT_E0, on=1, name="WAKE"):
     return _mk(LONGWAKE, name, 0.0, wake_rs=float(rs), wake_q=float(q), wake_fr=float(fr),
                wake_charge=float(charge), nbins=int(nbins), wake_zmin=float(zmin),
@@ -218,6 +227,9 @@ def write_lattice(elems, path):
             lines.append("   " + " ".join(_g(e.polyA[i]) for i in range(e.max_npole)))
         elif t == IDEALWIG:
             lines.append(f"{n} : IdealWig {_g(e.len)} {e.nslice} {_g(e.wig_lambda)} {_g(e.wig_h)} {e.wig_type}")
+        elif t == TRANSWAKE:
+            lines.append(f"{n} : TransWake {_g(e.len)} {_g(e.energy)} {_g(e.wake_charge)} {_g(e.tw_w[0])} "
+                         f"{_g(e.tw_fr[0])} {_g(e.tw_q[0])} {_g(e.tw_w[1])} {_g(e.tw_fr[1])} {_g(e.tw_q[1])}")
         elif t == CRABCAV:
             lines.append(f"{n} : CrabCav {_g(e.len)} {_g(e.devoltx)} {_g(e.devolty)} {_g(e.freq)} {_g(e.phis)} "
                          f"{_g(e.energy)} {_g(e.sigphi)} {_g(e.sigdvv)}")

@@ -67,6 +67,7 @@ extern "C" {
 #define ACCP_CRABCAV 13  /* ACrabCav    src/lib/accel/accel.cpp:927-1000 */
 #define ACCP_RFMULT 14   /* ARFMult     src/lib/accel/accel.cpp:1043-1202 (RF multipole) */
 #define ACCP_IDEALWIG 15 /* AIdealWig   src/lib/accel/accel.cpp:1386-1537 (ideal wiggler) */
+#define ACCP_TRANSWAKE 16 /* ATransWake src/lib/accel/accel.cpp:1546-1657, over the whole beam */
 
 #define ACCP_MAX_MPOLE 10 /* MAX_MPOLE, src/lib/accel/accel.h:48 */
 #define ACCP_NAME_LEN 32
@@ -93,7 +94,7 @@ typedef struct accp_elem {
   int32_t nslice;    /* BendMPole/StrMPole slices (reference default 10)      */
   int32_t max_npole; /* BendMPole/StrMPole MaxNPole (1..10)                   */
   int32_t rad;       /* BendMPole / IdealWig / SyncRad radiation flag ACCP_RAD_* */
-  int32_t on;        /* Cavity / CrabCav flag_onoff, LongWake on/off          */
+  int32_t on;        /* Cavity / CrabCav flag_onoff, LongWake / TransWake on  */
   int32_t nbins;     /* LongWake: number of longitudinal bins                 */
   double len;        /* length [m]                                            */
   double k;          /* Quad K [m^-2]; Bend gradient K                        */
@@ -110,7 +111,7 @@ typedef struct accp_elem {
   double wake_rs;    /* LongWake shunt impedance [Ohm]                        */
   double wake_q;     /* LongWake quality factor                               */
   double wake_fr;    /* LongWake resonant frequency [Hz]                      */
-  double wake_charge;/* LongWake bunch charge [C]                             */
+  double wake_charge;/* LongWake bunch charge [C]; TransWake Ne [nC]          */
   double wake_zmin, wake_zmax; /* LongWake binning window in z [m]            */
   double mat[36];    /* Matrix: row-major 6x6 map                             */
   double u0, jx, jy; /* SyncRad: energy loss per turn (units of energy), damping partitions */
@@ -118,6 +119,9 @@ typedef struct accp_elem {
   double devoltx, devolty; /* CrabCav deflecting voltages (units of energy)  */
   double sigphi, sigdvv;   /* CrabCav / RFMult phase and amplitude noise      */
   double wig_lambda, wig_h; /* IdealWig period [m] and peak curvature [1/m]  */
+  double tw_w[2], tw_fr[2], tw_q[2]; /* TransWake x/y: strength W, resonant
+                                      * frequency [GHz], quality factor; the
+                                      * bunch charge [nC] is wake_charge    */
   int32_t wig_type;         /* IdealWig: 0 horizontal, 1 vertical wiggler     */
   int32_t reserved;
   char famname[ACCP_NAME_LEN]; /* family name (host-side lookups only)       */

@@ -357,9 +357,43 @@ static void wig_rad_half(double *p, const accp_elem *e) {
   p[3] = yp * (1 + p[5]);
 }
 
+/* ATransWake::pass (accel.cpp:1603-1657) with the whole beam as its slice
+ * (one thread): particle by particle, a half drift, the pairwise kick from
+ * every other live particle that is not behind it, a half drift.  q is the
+ * charge per macro-particle in pC from Ne [nC] over all n particles. */
+static void orc_transwake(const accp_elem *e, double *r, int n) {
+  const double q = e->wake_charge / n * 1000;
+  const double kRx = 2 * ORC_PI * e->tw_fr[0] * 1.0e9 / ORC_C0, kRy = 2 * ORC_PI * e->tw_fr[1] * 1.0e9 / ORC_C0;
+  const double ax = kRx / 2 / e->tw_q[0], ay = kRy / 2 / e->tw_q[1];
+  const double kbx = kRx * sqrt(1.0 - 0.25 / sq(e->tw_q[0])), kby = kRy * sqrt(1.0 - 0.25 / sq(e->tw_q[1]));
+  for (int i = 0; i < n; i++) {
+    double *p = r + 6 * i;
+    if (isnan(p[0])) continue;
+    if (e->len != 0) orc_fastdrift(p, 0.5 * e->len / (1 + p[5]));
+    if (e->on) {
+      double dxp = 0, dyp = 0;
+      for (int j = 0; j < n; j++) {
+        const double *pj = r + 6 * j;
+        if (isnan(pj[0]) || i == j) continue;
+        const double zij = p[4] - pj[4];
+        if (zij > 0) continue;
+        dxp = dxp - q * e->tw_w[0] * exp(ax * zij) * sin(kbx * zij) * pj[0] / e->energy / 1.0e6;
+        dyp = dyp - q * e->tw_w[1] * exp(ay * zij) * sin(kby * zij) * pj[2] / e->energy / 1.0e6;
+      }
+      p[1] += dxp * (1 + p[5]);
+      p[3] += dyp * (1 + p[5]);
+    }
+    if (e->len != 0) orc_fastdrift(p, 0.5 * e->len / (1 + p[5]));
+  }
+}
+
 void orc_elem_pass(const accp_elem *e, double *r, int n, const orc_ctx *ctx) {
   if (e->type == ACCP_LONGWAKE) {
     orc_longwake(e, r, n);
+    return;
+  }
+  if (e->type == ACCP_TRANSWAKE) {
+    orc_transwake(e, r, n);
     return;
   }
   if (e->type == ACCP_MONITOR) return; /* coordinates untouched (accel.cpp:437-475) */
@@ -589,7 +623,7 @@ void orc_line_pass(const accp_elem *elems, int nelem, double *r, int n, int st, 
 int orc_line_pass_omp(const accp_elem *elems, int nelem, double *r, int n, int nturn,
                       int nthreads, uint64_t seed, uint64_t turn0, uint64_t pid0) {
   for (int k = 0; k < nelem; k++)
-    if (elems[k].type == ACCP_LONGWAKE && elems[k].on) return -1;
+    if ((elems[k].type == ACCP_LONGWAKE || elems[k].type == ACCP_TRANSWAKE) && elems[k].on) return -1;
 #ifdef _OPENMP
   if (nthreads < 1) nthreads = 1;
   omp_set_num_threads(nthreads);

@@ -208,6 +208,7 @@ class Reference:
         L.ref_set_rad.argtypes = [c_void_p, c_int]
         L.ref_set_cavity.argtypes = [c_void_p, c_int]
         L.ref_set_crab.argtypes = [c_void_p, c_int]
+        L.ref_set_wake.argtypes = [c_void_p, c_int]
         L.ref_set_threads.argtypes = [c_int]
         L.ref_pass.argtypes = [c_void_p, DP, c_int, c_int]
         L.ref_pass_range.argtypes = [c_void_p, DP, c_int, c_int, c_int]
@@ -273,6 +274,11 @@ class Reference:
             if len(set(crabs)) > 1:
                 raise ValueError("reference sets one crab-cavity flag for the whole line")
             line.lib.ref_set_crab(h, int(crabs[0]))
+        wakes = [e.on for e in elems if e.type == el.TRANSWAKE]
+        if wakes:
+            if len(set(wakes)) > 1:
+                raise ValueError("reference sets one wake flag for the whole line")
+            line.lib.ref_set_wake(h, int(wakes[0]))
         cav = [e.on for e in elems if e.type == el.CAVITY]
         if cav:
             if len(set(cav)) > 1:

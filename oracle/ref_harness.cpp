@@ -45,6 +45,8 @@ void ref_set_rad(void *h, int flag) { static_cast<ALine *>(h)->set_rad_onoff((un
 
 void ref_set_cavity(void *h, int on) { static_cast<ALine *>(h)->set_cavity_onoff(on != 0); }
 
+void ref_set_wake(void *h, int on) { static_cast<ALine *>(h)->set_wake_onoff(on != 0); }
+
 void ref_set_crab(void *h, int on) { static_cast<ALine *>(h)->set_crab_onoff(on != 0); }
 
 void ref_set_threads(int n) { omp_set_num_threads(n < 1 ? 1 : n); }

@@ -4,7 +4,7 @@ Runs in the build container only (it needs oracle/_ref/libaccel_ref.so,
 built by `make -C oracle` from the Accel+ sources).  Every fixture is data:
 inputs and the reference's outputs for them, written with numpy.savez
 (no pickles).  Regenerate with:  python tests/golden/make_golden.py
-(or one of the later fixtures alone: python tests/golden/make_golden.py optics2|elements2)
+(or one of the later fixtures alone: python tests/golden/make_golden.py optics2|elements2|transwake)
 """
 import os
 import sys
@@ -148,8 +148,29 @@ def elements2(R, out):
     print("elements2:", sorted(res))
 
 
+def transwake_case():
+    """A 2 nC bunch of 1000 particles (one lost, a few sharing z) through a
+    transverse resonator wake (x: 1e6, 1.5 GHz, Q 5; y: 2e6, 2 GHz, Q 3)."""
+    b = lat.gaussian_beam(1000, seed=5)
+    b[7, 0] = np.nan
+    b[10:14, 4] = b[9, 4]
+    return b, el.transwake(0.0, 2.0, 1.0e6, 1.5, 5.0, 2.0e6, 2.0, 3.0, on=1)
+
+
+def transwake(R, out):
+    """ATransWake on one thread (the whole beam is its slice)."""
+    b, w = transwake_case()
+    line = R.load([w])
+    line.set_threads(1)
+    np.savez_compressed(os.path.join(out, "transwake.npz"), beam=b, pass1=line.track(b, 1))
+    print("transwake written")
+
+
 def main():
     R = Reference()
+    if sys.argv[1:] == ["transwake"]:
+        transwake(R, HERE)
+        return
     if sys.argv[1:] == ["optics2"]:
         optics2(R, HERE)
         return
@@ -233,6 +254,7 @@ def main():
                         sigma=sg, cross=sr)
     optics2(R, out)
     elements2(R, out)
+    transwake(R, out)
     print("fixtures written to", out)
 
 

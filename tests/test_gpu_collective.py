@@ -100,3 +100,46 @@ def test_radiation_equilibrium_energy_spread(gpu_lib, oracle):
           f"sigma_x gpu {sx:.4e} oracle {sx_ref:.4e}")
     assert abs(sd / sd_ref - 1) < 0.02
     assert abs(sx / sx_ref - 1) < 0.02
+
+
+def test_transwake_matches_reference(gpu_lib):
+    """The sorted-scan transverse wake against the reference's pairwise sum
+    (one thread, transwake.npz): the kicks agree to 1e-10 of the largest
+    (same sum, other order: round-off grows with the chain of exp(c dz)
+    factors); everything else is untouched."""
+    from conftest import golden
+    from tests.golden.make_golden import transwake_case
+    g = golden("transwake.npz")
+    b, w = transwake_case()
+    out = gpu_lib.lattice([w]).track(b, 1)
+    live = ~np.isnan(b[:, 0])
+    assert np.array_equal(np.isnan(out[:, 0]), ~live)
+    for c in (0, 2, 4, 5):
+        assert np.array_equal(out[live, c], b[live, c])
+    for c in (1, 3):
+        kick, ref = out[live, c] - b[live, c], g["pass1"][live, c] - b[live, c]
+        assert np.abs(ref).max() > 1e-8
+        assert np.abs(kick - ref).max() <= 1e-10 * np.abs(ref).max(), c
+
+
+def test_transwake_ring_and_length(gpu_lib, oracle):
+    """20 turns of a ring with the wake at its end against the oracle; a
+    wake with length acts as half drift, kick of the whole beam, half drift;
+    switched off it is a drift."""
+    beam = lat.gaussian_beam(400, seed=23)
+    # strong enough to move px by 2e-4 over the run, below the threshold where
+    # it drives the undamped ring unstable within 20 turns (3e6 does)
+    w = el.transwake(0.0, 2.0, 2.0e5, 1.5, 5.0, 2.0e5, 2.0, 3.0, on=1)
+    ring = lat.fodo_ring() + [w]
+    assert_coords_close(gpu_lib.lattice(ring).track(beam, 20), oracle.track(ring, beam, 20), 1e-9,
+                        what="ring with a transverse wake, 20 turns")
+    wl = el.transwake(0.6, 2.0, 2.0e5, 1.5, 5.0, 2.0e5, 2.0, 3.0, on=1)
+    split = [el.drift(0.3), w, el.drift(0.3)]
+    # the scanned kick differs from the pairwise sum by round-off (<= 1e-10 of
+    # the kick, as against the reference); the second half drift carries it
+    # into x
+    assert_coords_close(gpu_lib.lattice([wl]).track(beam, 1), oracle.track(split, beam, 1), 1e-10,
+                        what="wake with length")
+    off = el.transwake(0.6, 2.0, 2.0e5, 1.5, 5.0, 2.0e5, 2.0, 3.0, on=0)
+    assert_coords_close(gpu_lib.lattice([off]).track(beam, 1), oracle.track([el.drift(0.6)], beam, 1), 1e-14,
+                        what="wake off")

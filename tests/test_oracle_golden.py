@@ -165,3 +165,15 @@ def test_radiation_equilibrium_oracle_vs_reference():
         mo, mr = o[:, k].mean(), r[:, k].mean()
         assert abs(mo / mr - 1) < 0.02, (name, mo, mr)
     assert abs(o[-1, 3] - r[-1, 3]) <= 0.01 * r[-1, 3]
+
+
+def test_transwake_bit_exact(oracle):
+    """The transverse wake over the whole beam (one reference thread),
+    pairwise as the reference sums it: bit for bit."""
+    from tests.golden.make_golden import transwake_case
+    g = golden("transwake.npz")
+    b, w = transwake_case()
+    assert same_bits(b, g["beam"])
+    out = oracle.track([w], b, 1)
+    assert same_bits(out, g["pass1"])
+    assert np.nanmax(np.abs(out[:, 1] - b[:, 1])) > 1e-8  # the kick is there
