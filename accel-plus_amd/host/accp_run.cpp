@@ -20,7 +20,8 @@
 //   Optics:  print_tune / print_chrom [dpp=, delta=]; print_2nd_chrom;
 //            print_matrix: type=RING; print_optics: type=, famname=, index=
 //   Track:   Distribution: Zero NPar= | Load f[, format=plain|accel-export] |
-//            Coordinate r=(...;...) | Gaussian NPar= sigx= ... | Linear X=a:d:b
+//            Coordinate r=(...;...) | Gaussian NPar= sigx=|emit_x=[,betx=] ... |
+//            Linear X=a:d:b
 //            [shift=(...)] [print]; Monitor: interval=, fraction=,
 //            printflag=Coordinate|Stat1|Stat2; Do_Track: NTurn=,
 //            radiation=, RFCavity=, PrintFinal=No|Yes|Full [, seed=]
@@ -638,6 +639,115 @@ void cmd_correct(Line &L, const string &line, int quantity) {
   std::cout << "*********************************" << std::endl;
 }
 
+// ALine::resetfirstelement (aline.cpp:557-605): rotate the line so that
+// element i comes first.
+void rotate_first(Line &L, int32_t i) {
+  if (i <= 0) return;
+  std::rotate(L.e.begin(), L.e.begin() + i, L.e.end());
+  L.dirty = true;
+}
+
+// RESET_FIRST: type=, famname=[, index=] (process.cpp:254-340).  index picks
+// the index-th member (the reference's findelements leaves the result of
+// index > 1 unset; the intended member is taken).  With famname alone, each
+// element of that name met while walking the (rotating) line moves the
+// family's first member to the front, as the reference loops.
+void cmd_reset_first(Line &L, const string &line) {
+  string type, fam;
+  unsigned index = 1;
+  for (const auto &kv : key_values(line)) {
+    if (kv.first == "TYPE") type = kv.second;
+    else if (kv.first == "FAMNAME") fam = kv.second;
+    else if (kv.first == "INDEX") index = (unsigned)std::stoul(kv.second);
+  }
+  if (!type.empty() && !fam.empty()) {
+    std::cout << "First element moved to: type=" << type << ", famname=" << fam << ", index= " << index << std::endl;
+    if (index == 0) std::cerr << "resetfirstelement: element index cannot be zero" << std::endl;
+    const std::vector<int32_t> m = index ? L.find(type, fam, index) : std::vector<int32_t>{};
+    if (m.empty())
+      std::cout << "resetfirstelement: element not found (type =" << type << ", famname =" << fam
+                << ", index =" << index << ")" << std::endl;
+    else rotate_first(L, m[0]);
+  } else if (type.empty() && !fam.empty()) {
+    for (size_t i = 0; i < L.e.size(); ++i)
+      if (fam == L.e[i].famname) {
+        const string t = Line::type_name(L.e[i].type);
+        std::cout << "First element moved to: type=" << t << ", famname=" << fam << ", index= " << 1 << std::endl;
+        rotate_first(L, L.find(t, fam, 1)[0]);
+      }
+  }
+}
+
+// PRINT_ELEM_VALUE: type=[, famname=] then the names to print, for Quad
+// (Index, Length, K) and StrMPole (Index, Length, PolyB[1..3], PolyA[1..3])
+// (process.cpp:344-480).  Other types print an empty row per element.
+void cmd_print_elem_value(Line &L, const string &line) {
+  string s = line.substr(line.find_first_of("=:") + 1);
+  std::replace(s.begin(), s.end(), ',', ' ');
+  s = trim(s);
+  // get_type_famname_str (process.cpp:2221-2267): two passes over
+  // "key=value"; each pass drops up to the first blank after the '=' (or,
+  // with no '=' left, the first word), so with type= alone the first name to
+  // print is consumed, as in the reference
+  string type, fam;
+  int flag = -1;
+  for (int pass = 0; pass < 2; ++pass) {
+    const size_t e = s.find('=');
+    const string key = trim(s.substr(0, e));
+    s.erase(0, e + 1);  // e == npos: npos + 1 == 0, nothing erased
+    s.erase(0, std::min(s.size(), s.find_first_not_of(" \t")));
+    const size_t b = s.find_first_of(" \t");
+    if (ieq(key, "type")) type = trim(s.substr(0, b)), flag = pass ? flag + 1 : 1;
+    else if (ieq(key, "famname")) fam = trim(s.substr(0, b)), flag = pass ? flag + 10 : 10;
+    s.erase(0, b + 1);
+  }
+  if (flag < 1) return;
+  const std::vector<int32_t> list = L.find(type, flag == 11 ? fam : string());
+  std::cout << "$$$Print-Element_Values***********************" << std::endl;
+  for (int32_t i : list) {
+    const accp_elem &x = L.e[i];
+    if (type == "StrMPole") {
+      if (s.find("Index") != string::npos) std::cout << i << "\t";
+      if (s.find("Length") != string::npos) std::cout << fmt(x.len) << "\t";
+      for (int k = 1; k <= 3; ++k)
+        if (s.find("PolyB[" + std::to_string(k) + "]") != string::npos) std::cout << fmt(x.polyB[k]) << "\t";
+      for (int k = 1; k <= 3; ++k)
+        if (s.find("PolyA[" + std::to_string(k) + "]") != string::npos) std::cout << fmt(x.polyA[k]) << "\t";
+    } else if (type == "Quad") {
+      if (s.find("Index") != string::npos) std::cout << i << "\t";
+      if (s.find("Length") != string::npos) std::cout << fmt(x.len) << "\t";
+      if (s.find("K") != string::npos) std::cout << fmt(x.k) << "\t";
+    }
+    std::cout << std::endl;
+  }
+  std::cout << "******************************************" << std::endl;
+}
+
+// PRINT_LAT_SUMMARY (process.cpp:482-512): radiation off, tunes and
+// chromaticities, then the 6D closed orbit (which leaves classical radiation
+// and the cavities on, as ALine::getco6 does) and the ring-start Twiss
+// functions of that state.
+void cmd_lat_summary(Line &L) {
+  std::cout << "$$$Lattice-Summary***********************" << std::endl;
+  std::cout << "Total " << L.e.size() << " elements, total length = " << fmt(L.length()) << " m" << std::endl;
+  std::cout << "First element is " << Line::type_name(L.e[0].type) << "." << L.e[0].famname << std::endl;
+  L.set_rad(ACCP_RAD_OFF);
+  double t[2], c[2], co[6], tw[7];
+  check(accp_calctune(L.device(), 0.0, ACCP_DELTA_DIFF, t), "accp_calctune");
+  std::cout << "Tunes: " << fmt(t[0]) << "\t" << fmt(t[1]) << "\t" << std::endl;
+  check(accp_calcchrom(L.device(), 0.0, ACCP_DELTA_DIFF, c), "accp_calcchrom");
+  std::cout << "Chromaticity: " << fmt(c[0]) << "\t" << fmt(c[1]) << "\t" << std::endl;
+  check(accp_getco6(L.device(), co), "accp_getco6");
+  L.pull();
+  std::cout << "Closed orbit 6D " << std::endl;
+  for (double v : co) std::cout << fmt(v) << "\t";
+  std::cout << std::endl;
+  check(accp_calctwiss(L.device(), nullptr, 0, 0.0, ACCP_DELTA_DIFF, tw), "accp_calctwiss");
+  std::cout << "betax\tbetay\talphax\talphay" << std::endl;
+  std::cout << fmt(tw[3]) << " \t" << fmt(tw[5]) << " \t" << fmt(tw[4]) << " \t" << fmt(tw[6]) << std::endl;
+  std::cout << "******************************************" << std::endl;
+}
+
 void proc_lattice(std::istream &in, Line &L) {
   const std::vector<string> keys = {"FILE_NAME",      "CORRECT_TUNE", "CORRECT_CHROM", "SET_MAX_NUM_THREADS",
                                     "MODIFY_ELEMENT", "SET_FLAGS",    "RESET_FIRST",   "PRINT_ELEM_VALUE",
@@ -655,11 +765,19 @@ void proc_lattice(std::istream &in, Line &L) {
       cmd_apply_setpt(L, line);
       continue;
     }
-    if (k >= 6) {
-      std::cerr << "accp-run: " << keys[k] << " is not carried by this backend; line ignored" << std::endl;
+    if (keys[k] == "RESET_FIRST") {
+      cmd_reset_first(L, line);
       continue;
     }
-    if (L.parse_only && (k == 1 || k == 2)) continue;
+    if (keys[k] == "PRINT_ELEM_VALUE") {
+      cmd_print_elem_value(L, line);
+      continue;
+    }
+    if (L.parse_only && (k == 1 || k == 2 || k == 8)) continue;
+    if (keys[k] == "PRINT_LAT_SUMMARY") {
+      cmd_lat_summary(L);
+      continue;
+    }
     switch (k) {
       case 0: cmd_file_name(L, line); break;
       case 1: cmd_correct(L, line, ACCP_TUNE); break;
@@ -778,7 +896,38 @@ struct Normals {
   }
 };
 
-std::vector<double> gen_distri(const string &line, uint64_t seed) {
+// sigma and divergence of one plane of a Gaussian beam, given as emit_x/betx
+// (betx from the ring-start Twiss when absent, 1 m when not positive or not
+// finite) or as sigx/sigxp (process.cpp:1619-1735).  The reference reads betx
+// at an offset taken from the wrong string (:1629-1633); its intended value
+// is the number after "betx=", which is what is read here.
+void gauss_plane(const string &sel, Line &L, int plane, double &sig, double &sigp) {
+  const string emit = plane ? "emit_y" : "emit_x", bet = plane ? "bety" : "betx";
+  bool f;
+  const double e = find_num(sel, emit, f);
+  if (!f) {
+    sig = find_num(sel, plane ? "sigy" : "sigx", f);
+    sigp = find_num(sel, plane ? "sigyp" : "sigxp", f);
+    return;
+  }
+  double b = find_num(sel, bet, f);
+  if (!f) {
+    double tw[7];
+    if (L.parse_only) b = 1.0;
+    else {
+      check(accp_calctwiss(L.device(), nullptr, 0, 0.0, 1e-8, tw), "accp_calctwiss");
+      b = tw[3 + 2 * plane];
+    }
+  }
+  if (!(b > 0) || !std::isfinite(b)) {
+    b = 1.0;
+    std::cout << "#" << bet << "=1.0 m is assumed" << std::endl;
+  }
+  sig = std::sqrt(b * e);
+  sigp = std::sqrt(e / b);
+}
+
+std::vector<double> gen_distri(const string &line, uint64_t seed, Line &L) {
   std::cout << line << std::endl;
   string sel = trim(line.substr(line.find_first_of("=:") + 1));
   size_t pos = sel.find_first_of(" \t");
@@ -840,12 +989,11 @@ std::vector<double> gen_distri(const string &line, uint64_t seed) {
     bool f;
     int np = (int)find_num(sel, "NPar", f);
     if (!f) np = 1;
-    // sigx/sigxp/... as the reference reads them (process.cpp:1634-1750);
-    // emit_x/betx forms need twiss functions this backend does not carry
-    if (sel.find("emit_x") != string::npos || sel.find("emit_y") != string::npos)
-      throw std::runtime_error("Gaussian: emit_x/emit_y need calctwiss; give sigx/sigxp/sigy/sigyp");
-    const double sg[6] = {find_num(sel, "sigx", f),  find_num(sel, "sigxp", f), find_num(sel, "sigy", f),
-                          find_num(sel, "sigyp", f), find_num(sel, "sigz", f),  find_num(sel, "sigdpp", f)};
+    double sg[6];
+    gauss_plane(sel, L, 0, sg[0], sg[1]);
+    gauss_plane(sel, L, 1, sg[2], sg[3]);
+    sg[4] = find_num(sel, "sigz", f);
+    sg[5] = find_num(sel, "sigdpp", f);
     Normals N(seed);
     r.resize(6 * (size_t)std::max(np, 0));
     for (int i = 0; i < np; ++i)
@@ -989,7 +1137,7 @@ void proc_track(std::istream &in, Line &L) {
     const int k = look_for(in, line, keys);
     if (k < 0 || keys[k] == "END") return;
     if (k == 0) {
-      r = gen_distri(line, seed);
+      r = gen_distri(line, seed, L);
       const string up = upper(line);
       size_t p = up.find("SHIFT");
       if (p != string::npos && !r.empty()) {

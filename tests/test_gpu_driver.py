@@ -195,3 +195,62 @@ End
     b = after(once, "$$$Track-Data: Full export", 65)[1:]
     # rows are "turn<TAB>x<TAB>...": the coordinates must agree to the last digit
     assert len(a) == 64 and [r.split("\t")[1:] for r in a] == [r.split("\t")[1:] for r in b]
+
+
+def test_gaussian_emittance_uses_ring_twiss(tmp_path, oracle):
+    """Distribution: Gaussian emit_x= without betx takes betx (and bety) from
+    the periodic Twiss functions at the ring start (process.cpp:1640-1650):
+    the beam equals the sigx/sigxp form with the oracle's calctwiss."""
+    el.write_lattice(lat.fodo_ring(), str(tmp_path / "fodo.lat"))
+    tw = oracle.calctwiss0(lat.fodo_ring())
+    ex, ey = 3.0e-9, 5.0e-10
+    bx, by = float(tw[3]), float(tw[5])
+    sig = [(ex * bx) ** 0.5, (ex / bx) ** 0.5, (ey * by) ** 0.5, (ey / by) ** 0.5]
+
+    def beam(spec):
+        out = run(tmp_path, f"""Lattice
+  file_name = fodo.lat
+End
+Track
+  Distribution: Gaussian NPar=40, {spec}, print
+End
+""")
+        i = out.index("$$$Init-Distribution*************")
+        return np.array([numbers(s) for s in out[i + 1:i + 41]])
+
+    a = beam(f"emit_x={ex!r}, emit_y={ey!r}")
+    b = beam("sigx={!r}, sigxp={!r}, sigy={!r}, sigyp={!r}".format(*sig))
+    assert a.shape == (40, 6) and np.abs(a).max() > 0
+    assert np.allclose(a, b, rtol=1e-12, atol=0)
+
+
+def test_lattice_summary(tmp_path, oracle):
+    """PRINT_LAT_SUMMARY (process.cpp:482-512) against the oracle: tunes and
+    chromaticities with radiation off, the 6D closed orbit (radiation and
+    cavities then stay on, as getco6 leaves them) and the ring-start Twiss
+    functions of that state."""
+    ring = lat.da_ring(0)
+    el.write_lattice(ring, str(tmp_path / "da.lat"))
+    out = run(tmp_path, """Lattice
+  file_name = da.lat, type=ERING
+  print_lat_summary
+End
+""")
+    arr = el.to_array(ring)
+    oracle.setrfphase(arr)
+    for i in range(len(arr)):
+        if arr[i].type == el.CAVITY:
+            arr[i].on = 0
+        if arr[i].type == el.BENDMPOLE:
+            arr[i].rad = 0
+    head = after(out, "$$$Lattice-Summary", 9)
+    assert head[0].startswith(f"Total {len(arr)} elements, total length = ")
+    assert float(head[0].split("=")[1].split()[0]) == pytest.approx(sum(e.len for e in ring), rel=1e-15)
+    assert np.allclose(numbers(head[2])[:2], oracle.tune(arr), rtol=0, atol=1e-12)
+    assert np.allclose(numbers(head[3])[:2], oracle.chrom(arr), rtol=1e-6, atol=1e-6)
+    # (I - M) has a condition number of a few hundred (slow synchrotron
+    # motion): 1e-10 of the orbit, as for getco6 itself
+    co = oracle.getco6(arr)
+    assert np.all(np.abs(np.array(numbers(head[5])) - co) <= 1e-10 * np.abs(co).max())
+    tw = oracle.calctwiss0(arr)
+    assert np.allclose(numbers(head[7]), [tw[3], tw[5], tw[4], tw[6]], rtol=1e-9, atol=1e-12)
