@@ -114,6 +114,7 @@ bool needs_full(const Hdr &h) {
     case OP_CRABCAV:
     case OP_RFMULT:
     case OP_IDEALWIG:
+    case OP_WIGTABLE:
     case OP_BEND:
     case OP_BENDMP_RAD:
     case OP_BENDMP_QE:
@@ -282,6 +283,19 @@ int compile_lattice(const accp_elem *E, int n, Compiled &out) {
         h.a = e.nslice;
         h.b = (e.wig_type & 0xff) | (rad ? 1 << 8 : 0);
         P.insert(P.end(), {e.len, k * k, hh, lns, 0.5 * lns, crad * hh * lns / 2.0});
+        break;
+      }
+      case ACCP_WIGTABLE: {  // AWigTable::pass (accel.cpp:1338-1382)
+        const int mx = e.wt_mx, my = e.wt_my;
+        if (mx < 2 || mx > ACCP_WIGTAB_MAX_X || my < 2 || my > ACCP_WIGTAB_MAX_Y || !e.wt_table ||
+            (e.len != 0 && e.nslice < 1))
+          return fail(ACCP_E_LATTICE, "element %d: WigTable needs 2<=mx<=%d, 2<=my<=%d, a table and nslice>=1", i,
+                      ACCP_WIGTAB_MAX_X, ACCP_WIGTAB_MAX_Y);
+        h.op = OP_WIGTABLE;
+        h.a = e.len != 0 ? e.nslice : 0;
+        h.b = mx | (my << 16);
+        P.insert(P.end(), {0.5 * e.len, (double)(e.len != 0 ? e.nslice : 1)});
+        P.insert(P.end(), e.wt_table, e.wt_table + mx + my + 2 * mx * my);
         break;
       }
       case ACCP_TRANSWAKE:  // ATransWake (accel.cpp:1546-1657), whole beam
@@ -490,6 +504,7 @@ int enqueue_run(const TableRef &T, WorkSet &ws, int max_tiles, double *d_r, int6
 // ---- lattice -------------------------------------------------------------------
 struct accp_lattice {
   std::vector<accp_elem> elems;
+  std::vector<std::vector<double>> tables;  // WigTable data the records point at
   Compiled tab;
   DevBuf d_hdr, d_par, d_meta;  // meta: lat_hdr0, lat_par0, lat_npar of one lattice
   DevBuf d_beam, d_snap, d_stat, d_loss;
@@ -500,6 +515,22 @@ struct accp_lattice {
                     (int)tab.hdr.size(), lds_bytes_for(tab.hdr.size(), tab.par.size()), tab.full};
   }
 };
+
+// Records kept by the lattice: WigTable pointers re-aimed at the lattice's
+// own copies of the tables (made before the old copies are released, since
+// the caller's records may point at them).
+static void keep_records(accp_lattice *L, const accp_elem *elems, int n) {
+  std::vector<accp_elem> recs(elems, elems + n);
+  std::vector<std::vector<double>> tabs;
+  for (accp_elem &e : recs)
+    if (e.type == ACCP_WIGTABLE && e.wt_table) {
+      const size_t m = (size_t)e.wt_mx + e.wt_my + 2 * (size_t)e.wt_mx * e.wt_my;
+      tabs.emplace_back(e.wt_table, e.wt_table + m);
+      e.wt_table = tabs.back().data();
+    }
+  L->elems = std::move(recs);
+  L->tables = std::move(tabs);
+}
 
 static int upload_lattice(accp_lattice *L) {
   const size_t nh = L->tab.hdr.size(), np = L->tab.par.size();
@@ -521,8 +552,8 @@ extern "C" int accp_lattice_build(const accp_elem *elems, int n, accp_lattice **
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail(ACCP_E_HIP, "no HIP device available");
   accp_lattice *L = new accp_lattice;
-  L->elems.assign(elems, elems + n);
   int rc = compile_lattice(elems, n, L->tab);
+  if (!rc) keep_records(L, elems, n);
   if (!rc) rc = upload_lattice(L);
   if (rc) {
     accp_lattice_free(L);
@@ -541,7 +572,7 @@ extern "C" int accp_lattice_update(accp_lattice *L, const accp_elem *elems, int 
   Compiled c;
   int rc = compile_lattice(elems, n, c);
   if (rc) return rc;
-  L->elems.assign(elems, elems + n);
+  keep_records(L, elems, n);
   L->tab = std::move(c);
   return upload_lattice(L);
 }

@@ -367,6 +367,12 @@ __device__ __forceinline__ void element_body(Part (&p)[PPL], const Hdr h, const 
       for (int k = 0; k < PPL; ++k) idealwig(p[k], P, h.a, h.b & 0xff, (h.b >> 8) & 1);
       break;
     }
+    case OP_WIGTABLE: {
+      if (!FULL) break;
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) wigtable(p[k], P, h.a, h.b & 0xffff, h.b >> 16);
+      break;
+    }
     case OP_CRABCAV: {
       if (!FULL) break;
 #pragma unroll

@@ -533,6 +533,56 @@ ACCP_HD void idealwig(Part &p, const PT *C, int nslice, int type, bool rad) {
   }
 }
 
+// ---- AWigTable: kick table with bilinear interpolation (accel.cpp:1338-1382)
+// Interval of a monotonic grid by bisection, the lower index clamped to
+// [0, n-2] so that points outside the grid extrapolate from the edge cell
+// (interp2, utility.cpp:174-187, through a fresh bilinear interpolator per
+// call, whose first lookup is always the bisection).
+template <class PT>
+ACCP_HD int wt_cell(const PT *g, int n, double v) {
+  const bool asc = g[n - 1] >= g[0];
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    const int mid = (hi + lo) >> 1;
+    if ((v >= g[mid]) == asc) lo = mid;
+    else hi = mid;
+  }
+  return lo < n - 2 ? lo : n - 2;
+}
+
+// Bilinear value of tab (my rows of mx) at (x, y); t and u are the cell
+// fractions, the four corners summed in the interpolator's order.
+template <class PT>
+ACCP_HD double wt_interp(const PT *xl, int mx, const PT *yl, int my, const PT *tab, double x, double y) {
+  const int i = wt_cell(xl, mx, x), j = wt_cell(yl, my, y);
+  const double t = (x - xl[i]) / (xl[i + 1] - xl[i]);
+  const double u = (y - yl[j]) / (yl[j + 1] - yl[j]);
+  const PT *r0 = tab + j * mx + i, *r1 = r0 + mx;
+  return (1. - t) * (1. - u) * r0[0] + t * (1. - u) * r0[1] + (1. - t) * u * r1[0] + t * u * r1[1];
+}
+
+// C = [L/2, nslice, xl[mx], yl[my], xkick[my][mx], ykick[my][mx]];
+// nslice = 0: one full kick (zero length).
+template <class PT>
+ACCP_HD void wigtable(Part &p, const PT *C, int nslice, int mx, int my) {
+  const PT *xl = C + 2, *yl = xl + mx, *xk = yl + my, *yk = xk + mx * my;
+  if (nslice == 0) {
+    const double x = p.x, y = p.y;
+    p.px += wt_interp(xl, mx, yl, my, xk, x, y);
+    p.py += wt_interp(xl, mx, yl, my, yk, x, y);
+    return;
+  }
+  const double ns = C[1];
+  const double nl = C[0] * p.ipd / ns;  // 0.5 L/(1+delta)/nslice
+  for (int s = 0; s < nslice; ++s) {
+    drift_nl(p, nl);
+    const double x = p.x, y = p.y;
+    p.px += wt_interp(xl, mx, yl, my, xk, x, y) / ns;
+    p.py += wt_interp(xl, mx, yl, my, yk, x, y) / ns;
+    drift_nl(p, nl);
+  }
+}
+
 // ---- counter-based normal variates (quantum excitation) -------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
 // easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies with a

@@ -33,6 +33,7 @@ enum Op : int32_t {
   OP_CRABCAV,     // [L, Vx/E, Vy/E, 2 pi f / c, phis, sigphi, sigdVV]    a = on | phase noise<<1 | amplitude noise<<2
   OP_RFMULT,      // [L, 2 pi f / c, phis, sigphi, B0..Bn-1, A0..An-1]     a = phase noise, b = n
   OP_IDEALWIG,    // [L, k^2, h^2, L/nslice, L/(2 nslice), radiation coef] a = nslice, b = type | radiation<<8
+  OP_WIGTABLE,    // [L/2, nslice, xl[mx], yl[my], xkick[my][mx], ykick[my][mx]] a = nslice (0: thin), b = mx | my<<16
   OP_COUNT
 };
 

@@ -34,6 +34,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -147,7 +148,20 @@ accp_elem new_elem(int type, const string &fam) {
   return e;
 }
 
-std::vector<accp_elem> read_lattice(const string &path) {
+// WigTable rows: n numbers from one line of the file.  The reference appends
+// each line to one stream and reads across the joins (accel.cpp:1298-1335),
+// so a line that does not end in a blank fuses with the next one; the values
+// are read line by line here, which is the same for files written by
+// serialize (every value followed by a tab).
+void read_row(std::istream &in, double *out, int n, const string &fam) {
+  string row;
+  if (!std::getline(in, row)) throw std::runtime_error("WigTable " + fam + ": table ends early");
+  std::istringstream rs(row);
+  for (int i = 0; i < n; ++i)
+    if (!(rs >> out[i])) throw std::runtime_error("WigTable " + fam + ": short table row");
+}
+
+std::vector<accp_elem> read_lattice(const string &path, std::deque<std::vector<double>> &tables) {
   std::ifstream in(path);
   if (!in) throw std::runtime_error("cannot open lattice file " + path);
   std::vector<accp_elem> v;
@@ -237,6 +251,20 @@ std::vector<accp_elem> read_lattice(const string &path) {
       e = new_elem(ACCP_TRANSWAKE, fam);
       ps >> e.len >> e.energy >> e.wake_charge >> e.tw_w[0] >> e.tw_fr[0] >> e.tw_q[0] >> e.tw_w[1] >> e.tw_fr[1] >>
           e.tw_q[1];
+    } else if (type == "WigTable") {  // accel.cpp:1298-1335: L nslice mx my / xl / yl / xkick rows / blank / ykick rows
+      e = new_elem(ACCP_WIGTABLE, fam);
+      ps >> e.len >> e.nslice >> e.wt_mx >> e.wt_my;
+      const int mx = e.wt_mx, my = e.wt_my;
+      if (mx < 2 || mx > ACCP_WIGTAB_MAX_X || my < 2 || my > ACCP_WIGTAB_MAX_Y)
+        throw std::runtime_error("WigTable " + fam + ": mx must be 2..101 and my 2..21");
+      tables.emplace_back((size_t)mx + my + 2 * (size_t)mx * my);
+      double *t = tables.back().data();
+      read_row(in, t, mx, fam);
+      read_row(in, t + mx, my, fam);
+      for (int i = 0; i < my; ++i) read_row(in, t + mx + my + (size_t)i * mx, mx, fam);
+      std::getline(in, rest);  // the blank line between the two tables
+      for (int i = 0; i < my; ++i) read_row(in, t + mx + my + (size_t)(my + i) * mx, mx, fam);
+      e.wt_table = t;
     } else if (type == "CrabCav") {  // accel.cpp:937-943
       e = new_elem(ACCP_CRABCAV, fam);
       ps >> e.len >> e.devoltx >> e.devolty >> e.freq >> e.phis >> e.energy >> e.sigphi >> e.sigdvv;
@@ -256,6 +284,7 @@ std::vector<accp_elem> read_lattice(const string &path) {
 // ---- the line: host records plus the device table ------------------------------
 struct Line {
   std::vector<accp_elem> e;
+  std::deque<std::vector<double>> tables;  // WigTable data read from the lattice file
   accp_lattice *lat = nullptr;
   bool dirty = true;
   bool parse_only = false;
@@ -290,6 +319,7 @@ struct Line {
       case ACCP_RFMULT: return "RFMult";
       case ACCP_IDEALWIG: return "IdealWig";
       case ACCP_TRANSWAKE: return "TransWake";
+      case ACCP_WIGTABLE: return "WigTable";
     }
     return "?";
   }
@@ -416,6 +446,21 @@ void serialize(const accp_elem &x, std::ostream &os) {
     case ACCP_CRABCAV:
       os << head << vals({x.devoltx, x.devolty, x.freq, x.phis, x.energy, x.sigphi, x.sigdvv}) << std::endl;
       break;
+    case ACCP_WIGTABLE: {  // AWigTable::serialize (accel.cpp:1246-1270)
+      const int mx = x.wt_mx, my = x.wt_my;
+      const double *t = x.wt_table;
+      os << head << "\t" << x.nslice << "\t" << mx << "\t" << my << std::endl;
+      auto row = [&](const double *v, int n) {
+        for (int i = 0; i < n; ++i) os << lossless(v[i]) << "\t";
+        os << std::endl;
+      };
+      row(t, mx);
+      row(t + mx, my);
+      for (int i = 0; i < my; ++i) row(t + mx + my + (size_t)i * mx, mx);
+      os << std::endl;
+      for (int i = 0; i < my; ++i) row(t + mx + my + (size_t)(my + i) * mx, mx);
+      break;
+    }
     case ACCP_LONGWAKE:
       os << head << vals({x.wake_rs, x.wake_q, x.wake_fr, x.wake_charge, x.wake_zmin, x.wake_zmax}) << "\t"
          << x.nbins << "\t" << lossless(x.energy) << std::endl;
@@ -503,7 +548,8 @@ void cmd_file_name(Line &L, const string &line) {
   s = trim(s.substr(p + 1));
   size_t q = s.find_first_of(" \t");
   const string file = s.substr(0, q);
-  L.e = read_lattice(file);
+  L.tables.clear();
+  L.e = read_lattice(file, L.tables);
   if (L.lat) {
     accp_lattice_free(L.lat);
     L.lat = nullptr;

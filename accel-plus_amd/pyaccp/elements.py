@@ -14,13 +14,14 @@ import numpy as np
 MAX_MPOLE = 10
 
 (DRIFT, QUAD, CORRECTOR, APERTURE, MONITOR, BEND, BENDMPOLE, STRMPOLE, CAVITY, LONGWAKE, MATRIX, SYNCRAD, CRABCAV,
- RFMULT, IDEALWIG, TRANSWAKE) = range(1, 17)
+ RFMULT, IDEALWIG, TRANSWAKE, WIGTABLE) = range(1, 18)
 TYPE_NAMES = {
     DRIFT: "Drift", QUAD: "Quad", CORRECTOR: "Corrector", APERTURE: "Aperture",
     MONITOR: "Monitor", BEND: "Bend", BENDMPOLE: "BendMPole", STRMPOLE: "StrMPole",
     CAVITY: "Cavity", LONGWAKE: "LongWake", MATRIX: "Matrix", SYNCRAD: "SyncRad", CRABCAV: "CrabCav",
-    RFMULT: "RFMult", IDEALWIG: "IdealWig", TRANSWAKE: "TransWake",
+    RFMULT: "RFMult", IDEALWIG: "IdealWig", TRANSWAKE: "TransWake", WIGTABLE: "WigTable",
 }
+WIGTAB_MAX_X, WIGTAB_MAX_Y = 101, 21  # accel.h:401-402
 RAD_OFF, RAD_DAMP, RAD_QE, RAD_LUMP, RAD_LUMP_DAMP, RAD_LUMP_QE = 0, 1, 2, 3, 4, 5
 FIELD_K, FIELD_POLYB, FIELD_POLYA = 1, 2, 3
 DEFAULT_E0 = 3000.0  # MeV, accel.h:30
@@ -45,7 +46,8 @@ class Elem(ctypes.Structure):
         ("devoltx", c_double), ("devolty", c_double), ("sigphi", c_double), ("sigdvv", c_double),
         ("wig_lambda", c_double), ("wig_h", c_double),
         ("tw_w", c_double * 2), ("tw_fr", c_double * 2), ("tw_q", c_double * 2),
-        ("wig_type", c_int32), ("reserved", c_int32),
+        ("wig_type", c_int32), ("wt_mx", c_int32), ("wt_my", c_int32), ("reserved", c_int32),
+        ("wt_table", ctypes.POINTER(c_double)),
         ("famname", c_char * 32),
     ]
 
@@ -183,6 +185,33 @@ def _g(v):
     return "%.17g" % v
 
 
+# WigTable buffers stay alive as long as the process: records are copied by
+# value into ctypes arrays, which do not keep Python references.
+_TABLES = []
+
+
+def wigtable(L, nslice, xl, yl, xkick, ykick, name="WT"):
+    """AWigTable: kicks [rad] tabulated on the grid xl (mx points) by yl (my
+    points), xkick[iy][ix] and ykick[iy][ix], interpolated bilinearly;
+    nslice drift-kick-drift slices for L > 0, one full kick for L = 0."""
+    xl, yl = np.asarray(xl, float), np.asarray(yl, float)
+    xk, yk = np.asarray(xkick, float), np.asarray(ykick, float)
+    mx, my = len(xl), len(yl)
+    if not (2 <= mx <= WIGTAB_MAX_X and 2 <= my <= WIGTAB_MAX_Y) or xk.shape != (my, mx) or yk.shape != (my, mx):
+        raise ValueError("WigTable: need 2<=mx<=101, 2<=my<=21 and (my, mx) kick tables")
+    buf = np.ascontiguousarray(np.concatenate([xl, yl, xk.ravel(), yk.ravel()]))
+    _TABLES.append(buf)
+    return _mk(WIGTABLE, name, L, nslice=int(nslice), wt_mx=mx, wt_my=my,
+               wt_table=buf.ctypes.data_as(ctypes.POINTER(c_double)))
+
+
+def wigtable_arrays(e):
+    """(xl, yl, xkick, ykick) of a WigTable record, as copies."""
+    mx, my = e.wt_mx, e.wt_my
+    v = np.ctypeslib.as_array(e.wt_table, shape=(mx + my + 2 * mx * my,)).copy()
+    return v[:mx], v[mx:mx + my], v[mx + my:mx + my + mx * my].reshape(my, mx), v[mx + my + mx * my:].reshape(my, mx)
+
+
 def write_lattice(elems, path):
     """Write `elems` in the Accel+ lattice-file grammar (one record per
     element, multipole coefficient lines after StrMPole/BendMPole headers,
@@ -230,6 +259,16 @@ def write_lattice(elems, path):
         elif t == TRANSWAKE:
             lines.append(f"{n} : TransWake {_g(e.len)} {_g(e.energy)} {_g(e.wake_charge)} {_g(e.tw_w[0])} "
                          f"{_g(e.tw_fr[0])} {_g(e.tw_q[0])} {_g(e.tw_w[1])} {_g(e.tw_fr[1])} {_g(e.tw_q[1])}")
+        elif t == WIGTABLE:
+            # AWigTable::create (accel.cpp:1298-1335) appends each line to one
+            # stream, so every line ends in a blank, as serialize writes them
+            xl, yl, xk, yk = wigtable_arrays(e)
+            lines.append(f"{n} : WigTable {_g(e.len)} {e.nslice} {e.wt_mx} {e.wt_my}")
+            lines.append("".join(_g(v) + "\t" for v in xl))
+            lines.append("".join(_g(v) + "\t" for v in yl))
+            lines += ["".join(_g(v) + "\t" for v in row) for row in xk]
+            lines.append("")
+            lines += ["".join(_g(v) + "\t" for v in row) for row in yk]
         elif t == CRABCAV:
             lines.append(f"{n} : CrabCav {_g(e.len)} {_g(e.devoltx)} {_g(e.devolty)} {_g(e.freq)} {_g(e.phis)} "
                          f"{_g(e.energy)} {_g(e.sigphi)} {_g(e.sigdvv)}")

@@ -84,6 +84,12 @@ def element_flops(e, mode="useful"):
         return (3 if damp else 0) + (6 if qe else 0) + drifts
     if t == el.CRABCAV:  # ACrabCav::pass (accel.cpp:977-995)
         return (15 if e.on else 0) + drifts
+    if t == el.WIGTABLE:  # AWigTable::pass (accel.cpp:1338-1382)
+        # interp2 (interp_2d.h:10-20): cell fractions 6, weights 2, corners 8, sum 3
+        kick = 2 * 19
+        if e.len == 0:
+            return kick + 2
+        return e.nslice * (2 * (3 + FASTDRIFT) + kick + 4)
     return 0
 
 

@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define ACCP_ABI_VERSION 2
+#define ACCP_ABI_VERSION 3
 
 /* ---- error codes ------------------------------------------------------- */
 #define ACCP_OK 0
@@ -68,8 +68,11 @@ extern "C" {
 #define ACCP_RFMULT 14   /* ARFMult     src/lib/accel/accel.cpp:1043-1202 (RF multipole) */
 #define ACCP_IDEALWIG 15 /* AIdealWig   src/lib/accel/accel.cpp:1386-1537 (ideal wiggler) */
 #define ACCP_TRANSWAKE 16 /* ATransWake src/lib/accel/accel.cpp:1546-1657, over the whole beam */
+#define ACCP_WIGTABLE 17 /* AWigTable   src/lib/accel/accel.cpp:1205-1382 (kick-table wiggler) */
 
 #define ACCP_MAX_MPOLE 10 /* MAX_MPOLE, src/lib/accel/accel.h:48 */
+#define ACCP_WIGTAB_MAX_X 101 /* MAX_WIGTAB_X, src/lib/accel/accel.h:401 */
+#define ACCP_WIGTAB_MAX_Y 21  /* MAX_WIGTAB_Y, src/lib/accel/accel.h:402 */
 #define ACCP_NAME_LEN 32
 
 /* Radiation flag values of ALine::set_rad_onoff (src/lib/accel/accel.h:577-582,
@@ -123,7 +126,15 @@ typedef struct accp_elem {
                                       * frequency [GHz], quality factor; the
                                       * bunch charge [nC] is wake_charge    */
   int32_t wig_type;         /* IdealWig: 0 horizontal, 1 vertical wiggler     */
+  int32_t wt_mx, wt_my;     /* WigTable: grid points in x (2..101) and y (2..21) */
   int32_t reserved;
+  const double *wt_table;   /* WigTable: xl[mx], yl[my], then the x kicks and
+                             * the y kicks [rad], each my rows of mx values
+                             * (row iy, column ix), as in the lattice file.
+                             * Read when a lattice is built or updated (the
+                             * library keeps its own copy); in the records
+                             * accp_lattice_get returns it points at that
+                             * copy, valid until the next update or free.   */
   char famname[ACCP_NAME_LEN]; /* family name (host-side lookups only)       */
 } accp_elem;
 

@@ -357,6 +357,31 @@ static void wig_rad_half(double *p, const accp_elem *e) {
   p[3] = yp * (1 + p[5]);
 }
 
+/* Lower index of the grid cell holding v: bisection over a monotonic grid,
+ * clamped to [0, n-2] (Base_interp::locate, interp_1d.h:21-37, with mm = 2;
+ * interp2 builds a fresh interpolator per call, so hunt() is never used). */
+static int wt_locate(const double *xx, int n, double v) {
+  const int ascnd = xx[n - 1] >= xx[0];
+  int jl = 0, ju = n - 1;
+  while (ju - jl > 1) {
+    const int jm = (ju + jl) >> 1;
+    if ((v >= xx[jm]) == ascnd) jl = jm;
+    else ju = jm;
+  }
+  return jl < 0 ? 0 : (jl > n - 2 ? n - 2 : jl);
+}
+
+/* interp2 (utility.cpp:174-187) through Bilin_interp::interp
+ * (interp_2d.h:10-20): tab holds my rows of mx values, tab[iy*mx + ix]. */
+static double wt_interp2(const double *tab, const double *xl, int mx, const double *yl, int my, double x,
+                         double y) {
+  const int i = wt_locate(xl, mx, x), j = wt_locate(yl, my, y);
+  const double t = (x - xl[i]) / (xl[i + 1] - xl[i]);
+  const double u = (y - yl[j]) / (yl[j + 1] - yl[j]);
+  return (1. - t) * (1. - u) * tab[j * mx + i] + t * (1. - u) * tab[j * mx + i + 1] +
+         (1. - t) * u * tab[(j + 1) * mx + i] + t * u * tab[(j + 1) * mx + i + 1];
+}
+
 /* ATransWake::pass (accel.cpp:1603-1657) with the whole beam as its slice
  * (one thread): particle by particle, a half drift, the pairwise kick from
  * every other live particle that is not behind it, a half drift.  q is the
@@ -573,6 +598,24 @@ void orc_elem_pass(const accp_elem *e, double *r, int n, const orc_ctx *ctx) {
           }
           if (rad) wig_rad_half(p, e);
           orc_fastdrift(p, 0.5 * e->len / (1 + p[5]) / e->nslice);
+        }
+        break;
+      }
+      case ACCP_WIGTABLE: { /* accel.cpp:1338-1382 */
+        const int mx = e->wt_mx, my = e->wt_my;
+        const double *xl = e->wt_table, *yl = xl + mx, *xk = yl + my, *yk = xk + mx * my;
+        if (e->len == 0) {
+          const double x = p[0], y = p[2];
+          p[1] += wt_interp2(xk, xl, mx, yl, my, x, y);
+          p[3] += wt_interp2(yk, xl, mx, yl, my, x, y);
+        } else {
+          for (int j = 0; j < e->nslice; j++) {
+            orc_fastdrift(p, 0.5 * e->len / (1 + p[5]) / e->nslice);
+            const double x = p[0], y = p[2];
+            p[1] += wt_interp2(xk, xl, mx, yl, my, x, y) / e->nslice;
+            p[3] += wt_interp2(yk, xl, mx, yl, my, x, y) / e->nslice;
+            orc_fastdrift(p, 0.5 * e->len / (1 + p[5]) / e->nslice);
+          }
         }
         break;
       }

@@ -4,7 +4,7 @@ Runs in the build container only (it needs oracle/_ref/libaccel_ref.so,
 built by `make -C oracle` from the Accel+ sources).  Every fixture is data:
 inputs and the reference's outputs for them, written with numpy.savez
 (no pickles).  Regenerate with:  python tests/golden/make_golden.py
-(or one of the later fixtures alone: python tests/golden/make_golden.py optics2|elements2|transwake)
+(or one of the later fixtures alone: python tests/golden/make_golden.py optics2|elements2|transwake|wigtable)
 """
 import os
 import sys
@@ -148,6 +148,54 @@ def elements2(R, out):
     print("elements2:", sorted(res))
 
 
+def wigtable_particles():
+    """The probe particles plus 300 in a cloud over and beyond the tables'
+    grids, 20 of them exactly on grid nodes."""
+    rng = np.random.default_rng(11)
+    c = np.zeros((300, 6))
+    c[:, 0] = rng.uniform(-6e-3, 6e-3, 300)
+    c[:, 1] = rng.normal(0, 1e-4, 300)
+    c[:, 2] = rng.uniform(-3e-3, 3e-3, 300)
+    c[:, 3] = rng.normal(0, 5e-5, 300)
+    c[:, 4] = rng.normal(0, 3e-3, 300)
+    c[:, 5] = rng.normal(0, 1e-2, 300)
+    c[:20, 0] = np.linspace(-4e-3, 4e-3, 17)[rng.integers(0, 17, 20)]
+    c[:20, 2] = np.linspace(-2e-3, 2e-3, 9)[rng.integers(0, 9, 20)]
+    return np.concatenate([probe_particles(), c])
+
+
+def wigtable_cases():
+    """Thin and thick tables on uniform grids, a table on descending grids,
+    and the largest table (101 x 21) on non-uniform grids."""
+    rng = np.random.default_rng(13)
+
+    def tables(xl, yl, a):
+        X, Y = np.meshgrid(xl, yl)
+        xk = a * (np.sin(300 * X) * np.cosh(400 * Y) + 0.1 * rng.standard_normal(X.shape))
+        yk = a * (np.cos(250 * X) * np.sinh(500 * Y) + 0.1 * rng.standard_normal(X.shape))
+        return xk, yk
+    xl, yl = np.linspace(-4e-3, 4e-3, 17), np.linspace(-2e-3, 2e-3, 9)
+    xk, yk = tables(xl, yl, 2e-5)
+    xn = np.sort(np.concatenate([[-5e-3, 5e-3], rng.uniform(-5e-3, 5e-3, 99)]))
+    yn = np.sort(np.concatenate([[-2.5e-3, 2.5e-3], rng.uniform(-2.5e-3, 2.5e-3, 19)]))
+    xkn, ykn = tables(xn, yn, 1e-5)
+    return [
+        ("wt_thin", el.wigtable(0.0, 1, xl, yl, xk, yk)),
+        ("wt_thick", el.wigtable(1.5, 10, xl, yl, xk, yk)),
+        ("wt_desc", el.wigtable(0.0, 1, xl[::-1], yl[::-1], xk[::-1, ::-1], yk[::-1, ::-1])),
+        ("wt_max", el.wigtable(0.8, 4, xn, yn, xkn, ykn)),
+    ]
+
+
+def wigtable(R, out):
+    P = wigtable_particles()
+    res = {"particles": P}
+    for name, e in wigtable_cases():
+        res[name] = R.load([e]).elem_pass(0, P)
+    np.savez_compressed(os.path.join(out, "wigtable.npz"), **res)
+    print("wigtable:", sorted(res))
+
+
 def transwake_case():
     """A 2 nC bunch of 1000 particles (one lost, a few sharing z) through a
     transverse resonator wake (x: 1e6, 1.5 GHz, Q 5; y: 2e6, 2 GHz, Q 3)."""
@@ -170,6 +218,9 @@ def main():
     R = Reference()
     if sys.argv[1:] == ["transwake"]:
         transwake(R, HERE)
+        return
+    if sys.argv[1:] == ["wigtable"]:
+        wigtable(R, HERE)
         return
     if sys.argv[1:] == ["optics2"]:
         optics2(R, HERE)
@@ -255,6 +306,7 @@ def main():
     optics2(R, out)
     elements2(R, out)
     transwake(R, out)
+    wigtable(R, out)
     print("fixtures written to", out)
 
 

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol(lib):
 
 
 def test_abi_version(lib):
-    assert lib.lib.accp_abi_version() == 2  # accp_elem gained the Matrix, SyncRad and CrabCav fields
+    assert lib.lib.accp_abi_version() == 3  # accp_elem gained the WigTable grid sizes and table pointer
 
 
 def test_elem_layout_matches_c():
@@ -43,7 +43,7 @@ def test_elem_layout_matches_c():
 #include <stddef.h>
 #include "accp.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(accp_elem), offsetof(accp_elem, len),
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(accp_elem), offsetof(accp_elem, len), offsetof(accp_elem, wt_table),
          offsetof(accp_elem, polyB), offsetof(accp_elem, wake_rs), offsetof(accp_elem, famname),
          sizeof(accp_track_opts));
   printf("%zu %zu\n", sizeof(accp_tuning), sizeof(accp_param));
@@ -58,7 +58,7 @@ int main(void) {
         a, b = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")[:2]
     vals = [int(v) for v in a.split()]
     E = el.Elem
-    assert vals == [ctypes.sizeof(E), E.len.offset, E.polyB.offset, E.wake_rs.offset, E.famname.offset,
+    assert vals == [ctypes.sizeof(E), E.len.offset, E.wt_table.offset, E.polyB.offset, E.wake_rs.offset, E.famname.offset,
                     ctypes.sizeof(abi.TrackOpts)]
     assert [int(v) for v in b.split()] == [ctypes.sizeof(abi.Tuning), ctypes.sizeof(abi.Param)]
 

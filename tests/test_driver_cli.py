@@ -5,6 +5,7 @@ with an error instead of falling back to the CPU."""
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 import pyaccp.elements as el
@@ -189,3 +190,37 @@ End
     # to the front again (20 messages, one rotation)
     assert p.stdout.count("First element moved to: type=StrMPole, famname=SD, index= 1") == 20
     assert rows[2] == [f"{i}\t-25\t0\t" for i in range(0, 240, 12)]
+
+
+def test_wigtable_lattice_reads_and_exports_losslessly(tmp_path):
+    """A WigTable record in the reference grammar (accel.cpp:1298-1335:
+    header, x grid, y grid, x-kick rows, blank line, y-kick rows) reads in,
+    and EXPORT_ELEMENT writes it back (AWigTable::serialize layout) so that
+    every grid value and kick survives to the last bit."""
+    from tests.golden.make_golden import wigtable_cases
+    cases = [e for _, e in wigtable_cases()]
+    ring = [el.drift(1.0)] + cases + [el.quad(0.3, 1.2)]
+    el.write_lattice(ring, str(tmp_path / "wt.lat"))
+    p = run(tmp_path, "Lattice\n file_name = wt.lat\n export_element: type=RING\nEnd\n", "--parse-only")
+    assert p.returncode == 0, p.stderr
+    assert f"Total elements: {len(ring)}" in p.stdout
+    body = export_block(p.stdout)
+    (tmp_path / "again.lat").write_text("\n".join(body) + "\nRETURN\n")
+    q = run(tmp_path, "Lattice\n file_name = again.lat\n export_element: type=RING\nEnd\n", "--parse-only")
+    assert q.returncode == 0, q.stderr
+    assert export_block(q.stdout) == body
+    # the tables themselves, row by row
+    k = 0
+    for e in cases:
+        while "WigTable" not in body[k]:
+            k += 1
+        head = body[k].split()
+        assert [int(head[4]), int(head[5]), int(head[6])] == [e.nslice, e.wt_mx, e.wt_my]
+        assert float(head[3]) == e.len
+        xl, yl, xk, yk = el.wigtable_arrays(e)
+        rows = [np.array([float(t) for t in s.split()]) for s in body[k + 1:k + 3 + 2 * e.wt_my + 1]]
+        assert np.array_equal(rows[0], xl) and np.array_equal(rows[1], yl)
+        assert np.array_equal(np.array(rows[2:2 + e.wt_my]), xk)
+        assert rows[2 + e.wt_my].size == 0
+        assert np.array_equal(np.array(rows[3 + e.wt_my:]), yk)
+        k += 1

@@ -177,3 +177,14 @@ def test_transwake_bit_exact(oracle):
     out = oracle.track([w], b, 1)
     assert same_bits(out, g["pass1"])
     assert np.nanmax(np.abs(out[:, 1] - b[:, 1])) > 1e-8  # the kick is there
+
+
+def test_wigtable_fixtures_bit_exact(oracle):
+    """WigTable, thin and sliced, ascending, descending and non-uniform
+    grids up to the largest table (101 x 21), particles inside, on the nodes
+    of and beyond the grid: one pass, bit for bit with the reference
+    (wigtable.npz)."""
+    from tests.golden.make_golden import wigtable_cases
+    g = golden("wigtable.npz")
+    for name, e in wigtable_cases():
+        assert same_bits(oracle.track([e], g["particles"], 1), g[name]), name

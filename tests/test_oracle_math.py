@@ -127,3 +127,34 @@ def test_closed_orbit_closes(oracle):
         r1 = oracle.track(fodo, np.array([[0, 0, 0, 0, 0, dp]]), 1)[0, :4]
         bad = np.array([[r1[0], r1[1], r1[2], r1[3], 0.0, dp]])
         assert np.abs(oracle.track(fodo, bad, 1)[0, :4] - bad[0, :4]).max() > 10 * res
+
+
+def test_wigtable_reproduces_bilinear_fields(oracle):
+    """Bilinear interpolation is exact for f = a + b x + c y + d x y, inside
+    the grid and (extrapolating from the edge cells) outside it, on any
+    grid; at the nodes it returns the tabulated value bit for bit."""
+    rng = np.random.default_rng(3)
+    xl = np.sort(rng.uniform(-4e-3, 4e-3, 12))
+    yl = np.sort(rng.uniform(-2e-3, 2e-3, 7))[::-1]  # descending
+    X, Y = np.meshgrid(xl, yl)
+
+    def f(c, x, y):
+        return c[0] + c[1] * x + c[2] * y + c[3] * x * y
+    cx, cy = (1e-6, 2e-3, -1e-3, 0.5), (-2e-6, 1e-3, 3e-3, -0.8)
+    w = el.wigtable(0.0, 1, xl, yl, f(cx, X, Y), f(cy, X, Y))
+    r = np.zeros((400, 6))
+    r[:, 0] = rng.uniform(-8e-3, 8e-3, 400)
+    r[:, 2] = rng.uniform(-4e-3, 4e-3, 400)
+    out = oracle.track([w], r, 1)
+    assert np.allclose(out[:, 1], f(cx, r[:, 0], r[:, 2]), rtol=0, atol=1e-17)
+    assert np.allclose(out[:, 3], f(cy, r[:, 0], r[:, 2]), rtol=0, atol=1e-17)
+    nodes = np.zeros((X.size, 6))
+    nodes[:, 0], nodes[:, 2] = X.ravel(), Y.ravel()
+    out = oracle.track([w], nodes, 1)
+    assert np.array_equal(out[:, 1], f(cx, X, Y).ravel()) and np.array_equal(out[:, 3], f(cy, X, Y).ravel())
+    # sliced: the kick is split into nslice parts between drifts; as L -> 0
+    # the parts add up to the thin kick (to the round-off of five additions
+    # on kicks up to 2e-5)
+    ws = el.wigtable(1e-9, 5, xl, yl, f(cx, X, Y), f(cy, X, Y))
+    out = oracle.track([ws], r, 1)
+    assert np.allclose(out[:, 1], f(cx, r[:, 0], r[:, 2]), rtol=0, atol=1e-16)
