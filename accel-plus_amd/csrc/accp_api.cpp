@@ -17,6 +17,7 @@
 #include "../../include/accp.h"
 #include "accp_adjoint_kernels.h"
 #include "accp_error.h"
+#include "accp_internal.h"
 #include "accp_kernels.h"
 #include "accp_table.h"
 #include "accp_wake.h"
@@ -679,6 +680,23 @@ extern "C" int accp_track_device(accp_lattice *L, double *d_r, int n, int nturn,
   RunSpec R = spec_from(opts, nturn, st, ed);
   R.d_loss_turn = opts ? opts->loss_turn : nullptr;
   return track_device_impl(L, d_r, n, R, opts ? &opts->tuning : nullptr, (hipStream_t)stream);
+}
+
+int accp::track_record(accp_lattice *L, double *r, int n, int nturn, const double **d_rec) {
+  if (!L || !r || n <= 0 || nturn <= 0 || !d_rec) return fail(ACCP_E_ARG, "track_record: bad arguments");
+  const size_t bytes = (size_t)n * 6 * sizeof(double);
+  int rc;
+  if ((rc = L->d_beam.ensure(bytes)) || (rc = L->d_snap.ensure((size_t)nturn * bytes))) return rc;
+  HIPCHK(hipMemcpy(L->d_beam.p, r, bytes, hipMemcpyHostToDevice));
+  RunSpec R = spec_from(nullptr, nturn, 0, (int)L->elems.size());
+  R.snap = (double *)L->d_snap.p;
+  R.nsnap = nturn;
+  R.mon_interval = 1;
+  if ((rc = track_device_impl(L, (double *)L->d_beam.p, n, R, nullptr, nullptr))) return rc;
+  HIPCHK(hipMemcpy(r, L->d_beam.p, bytes, hipMemcpyDeviceToHost));
+  HIPCHK(hipDeviceSynchronize());
+  *d_rec = (const double *)L->d_snap.p;
+  return ACCP_OK;
 }
 
 extern "C" int accp_track(accp_lattice *L, double *r, int n, int nturn, int st, int ed,

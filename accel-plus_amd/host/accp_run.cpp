@@ -27,7 +27,8 @@
 //            radiation=, RFCavity=, PrintFinal=No|Yes|Full [, seed=]
 //   DAMA:    TRACK_DA: xmin=, xmax=, ymax=, nray=, nstep=, nturn=
 //            [, print=loss_turn] [, weightminusside=true]; apply_setpt=file;
-//            TRACK_MA: type=, [famname=,] nmoni=, dppmin=, dppmax=, nstep=, nturn=
+//            TRACK_MA: type=, [famname=,] nmoni=, dppmin=, dppmax=, nstep=, nturn=;
+//            TRACK_DNUDJ: xmax=, ymax=, nturn=
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -1221,13 +1222,38 @@ void proc_track(std::istream &in, Line &L) {
 
 // ---- DAMA block (process.cpp:2393-2500) --------------------------------------------
 void proc_dama(std::istream &in, Line &L) {
-  const std::vector<string> keys = {"TRACK_DA", "TRACK_MA", "APPLY_SETPT", "END"};
+  const std::vector<string> keys = {"TRACK_DA", "TRACK_MA", "APPLY_SETPT", "TRACK_DNUDJ", "END"};
   string line;
   for (;;) {
     const int k = look_for(in, line, keys);
     if (k < 0 || keys[k] == "END") return;
     if (k == 2) {
       cmd_apply_setpt(L, line);
+      continue;
+    }
+    if (k == 3) {
+      // TRACK_DNUDJ: xmax=, ymax=, nturn= (process.cpp:2587-2640): radiation
+      // and cavities off, then the amplitude-dependent tune shift.  A key
+      // left out takes ALine::trackdnudexy's default (accel.h:651) where the
+      // reference reads an uninitialised variable.
+      double xm = 0.004, ym = 0.003;
+      int nturn = 512;
+      for (const auto &kv : key_values(line)) {
+        if (kv.first == "XMAX") xm = std::stod(kv.second);
+        else if (kv.first == "YMAX") ym = std::stod(kv.second);
+        else if (kv.first == "NTURN") nturn = std::stoi(kv.second);
+      }
+      std::cout << "#" << fmt(xm) << " " << fmt(ym) << " " << nturn << std::endl;
+      if (L.parse_only) continue;
+      L.set_rad(ACCP_RAD_OFF);
+      L.set_on(ACCP_CAVITY, 0);
+      double d[4], rows[36];
+      check(accp_trackdnudexy(L.device(), nturn, xm, ym, d, rows), "accp_trackdnudexy");
+      for (int i = 0; i < 12; ++i)  // the lines trackdnudexy prints (aline.cpp:744, :768)
+        std::cout << fmt(rows[3 * i]) << "\t" << fmt(rows[3 * i + 1]) << "\t" << fmt(rows[3 * i + 2]) << std::endl;
+      std::cout << "$$$dtuning coefficients" << std::endl;
+      std::cout << fmt(d[0]) << " \t" << fmt(d[1]) << " \t" << fmt(d[2]) << " \t" << fmt(d[3]) << " \t" << std::endl;
+      std::cout << "*************************" << std::endl;
       continue;
     }
     if (k == 1) {

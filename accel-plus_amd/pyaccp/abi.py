@@ -60,7 +60,7 @@ EXPORTS = (
     "accp_adjoint_free",
     "accp_lattice_get", "accp_getm66", "accp_calctune", "accp_calcchrom", "accp_calcresp", "accp_correct",
     "accp_setrfphase", "accp_trackda", "accp_getco4", "accp_getco6", "accp_calctwiss", "accp_calcchrom2",
-    "accp_trackma",
+    "accp_trackma", "accp_naff", "accp_tracktune", "accp_trackdiffusion", "accp_trackdnudexy",
 )
 
 DP = POINTER(c_double)
@@ -106,6 +106,14 @@ class Library:
     def check(self, rc, what=""):
         if rc != 0:
             raise AccpError(f"{what} failed ({rc}): {self.lib.accp_last_error().decode()}")
+
+    def naff(self, signals, xlow=0.0, xhi=0.5):
+        """naff of each row of `signals` (rows of a power-of-two length)."""
+        d = np.ascontiguousarray(np.atleast_2d(signals), dtype=np.float64)
+        out = np.zeros(d.shape[0])
+        self.check(self.lib.accp_naff(_dp(d), d.shape[0], d.shape[1], c_double(xlow), c_double(xhi), _dp(out)),
+                   "accp_naff")
+        return out
 
     def device_count(self):
         c = c_int(0)
@@ -245,6 +253,28 @@ class Lattice:
                                              lc.ctypes.data_as(POINTER(c_int32)), len(lc), _dp(p), _dp(m),
                                              byref(avg)), "accp_trackma")
         return avg.value, p, m
+
+    def tracktune(self, nturn, x0, y0, xlow=0.0, xhi=0.5, ylow=0.0, yhi=0.5):
+        t = np.zeros(2)
+        self.L.check(self.L.lib.accp_tracktune(self.h, nturn, c_double(x0), c_double(y0), c_double(xlow),
+                                               c_double(xhi), c_double(ylow), c_double(yhi), _dp(t)),
+                     "accp_tracktune")
+        return t
+
+    def trackdiffusion(self, nturn, x0, y0, xlow=0.0, xhi=0.5, ylow=0.0, yhi=0.5):
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        y0 = np.ascontiguousarray(y0, dtype=np.float64)
+        nux, nuy, dif, tot = np.zeros(len(x0)), np.zeros(len(x0)), np.zeros(len(x0)), c_double(0.0)
+        self.L.check(self.L.lib.accp_trackdiffusion(self.h, nturn, len(x0), _dp(x0), _dp(y0), _dp(nux), _dp(nuy),
+                                                    _dp(dif), c_double(xlow), c_double(xhi), c_double(ylow),
+                                                    c_double(yhi), byref(tot)), "accp_trackdiffusion")
+        return tot.value, nux, nuy, dif
+
+    def trackdnudexy(self, nturn, xm, ym, rows=False):
+        d, rw = np.zeros(4), np.zeros(36)
+        self.L.check(self.L.lib.accp_trackdnudexy(self.h, nturn, c_double(xm), c_double(ym), _dp(d), _dp(rw)),
+                     "accp_trackdnudexy")
+        return (d, rw.reshape(12, 3)) if rows else d
 
     def track(self, r, nturn=1, st=0, ed=-1, seed=0, turn0=0, pid0=0, tuning=None,
               mon_interval=0, mon_max=0, loss_turn=False):

@@ -285,6 +285,71 @@ def adjoint_leg(lib, stream):
     }
 
 
+def diffusion_leg(lib, nx=128, ny=64, nturn=1024, cpu_sample=32):
+    """Frequency map (trackDiffusion, aline.cpp:1421-1484) of the seed-0 DA
+    lattice with radiation and cavities off: a 128 x 64 grid of starting
+    points (x0 to 8 mm, y0 to 4 mm), 1024 turns, NAFF of both halves of
+    each plane; the whole call (one tracking launch sequence recording every
+    turn, one NAFF launch).  The reference's own trackDiffusion, one particle
+    after another on one core, on the first cpu_sample points of the grid."""
+    import numpy as np
+
+    import pyaccp.elements as el
+    import pyaccp.lattices as lat
+
+    log("diffusion leg")
+    ring = []
+    for e in lat.da_ring(0):
+        e = e.copy()
+        if e.type == el.CAVITY:
+            e.on = 0
+        if e.type == el.BENDMPOLE:
+            e.rad = 0
+        ring.append(e)
+    L = lib.lattice(ring)
+    X, Y = np.meshgrid(np.linspace(1e-4, 8e-3, nx), np.linspace(1e-4, 4e-3, ny))
+    x0, y0 = X.ravel(), Y.ravel()
+    L.trackdiffusion(64, x0[:8], y0[:8], 0.05, 0.45, 0.05, 0.45)  # warmup
+    t0 = time.perf_counter()
+    tot, nux, nuy, dif = L.trackdiffusion(nturn, x0, y0, 0.05, 0.45, 0.05, 0.45)
+    gpu_s = time.perf_counter() - t0
+    live = int((~np.isnan(dif)).sum())
+    sig = np.cos(2 * np.pi * np.outer(np.linspace(0.1, 0.4, 4 * live), np.arange(nturn // 2)))
+    t0 = time.perf_counter()
+    lib.naff(sig, 0.05, 0.45)
+    naff_s = time.perf_counter() - t0
+    res = {
+        "workload": f"trackDiffusion of a {nx} x {ny} grid, {nturn} turns, seed-0 DA lattice "
+                    f"({len(ring)} elements), {live} of {nx * ny} points survive",
+        "ms": round(gpu_s * 1e3, 3),
+        "points_per_s": nx * ny / gpu_s,
+        "naff_ms": round(naff_s * 1e3, 3),
+        "naff_signals": 4 * live,
+        "naff_samples": nturn // 2,
+    }
+    try:
+        from pyoracle import Reference
+
+        line = Reference().load(ring)
+        line.set_threads(1)
+        t0 = time.perf_counter()
+        _, rnux, rnuy, _ = line.trackdiffusion(nturn, x0[:cpu_sample], y0[:cpu_sample], 0.05, 0.45, 0.05, 0.45)
+        cpu_s = time.perf_counter() - t0
+        ok = ~np.isnan(rnux)
+        res["cpu_reference"] = {
+            "points_per_s": cpu_sample / cpu_s,
+            "cores": 1,
+            "sample": f"the first {cpu_sample} grid points, {cpu_s:.2f} s",
+            "max_tune_difference": float(np.max(np.abs(np.concatenate([nux[:cpu_sample][ok] - rnux[ok],
+                                                                       nuy[:cpu_sample][ok] - rnuy[ok]]))))
+            if ok.any() else None,
+        }
+        res["speedup_vs_reference_core"] = round((nx * ny / gpu_s) / (cpu_sample / cpu_s), 1)
+    except FileNotFoundError:
+        pass
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,12 +361,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--tune", default="", help="block,ppl,staging(scalar|lds) instead of autotuning")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
-    ap.add_argument("--legs", default="saturated,collective,adjoint,plain,cpu",
+    ap.add_argument("--legs", default="saturated,collective,adjoint,diffusion,plain,cpu",
                     help="comma list of the legs run after the batched one "
-                         "(saturated, collective, adjoint, plain, cpu); 'none' for the batched leg alone")
+                         "(saturated, collective, adjoint, diffusion, plain, cpu); 'none' for the batched "
+                         "leg alone")
     args = ap.parse_args()
     legs = set() if args.legs == "none" else set(args.legs.split(","))
-    unknown = legs - {"saturated", "collective", "adjoint", "plain", "cpu"}
+    unknown = legs - {"saturated", "collective", "adjoint", "diffusion", "plain", "cpu"}
     if unknown:
         ap.error(f"unknown legs {sorted(unknown)}")
     for flag, leg in ((args.no_plain, "plain"), (args.no_cpu, "cpu"), (args.no_adjoint, "adjoint")):
@@ -434,6 +500,9 @@ def main():
     if "adjoint" in legs:
         log("adjoint leg")
         result["adjoint"] = adjoint_leg(lib, stream)
+
+    if "diffusion" in legs:
+        result["diffusion"] = diffusion_leg(lib)
 
     if "plain" in legs:
         log("plain per-seed GPU path")

@@ -352,6 +352,40 @@ int accp_trackma(accp_lattice *lat, double dppmax, double dppmin, int nturn, int
                  const int32_t *locs, int nloc, double *dpp_plus, double *dpp_minus,
                  double *avg);
 
+/* ---- tune diagnostics (NAFF) ---------------------------------------------
+ * naff (src/lib/accel/utility.cpp:48-96): the frequency, in tune units
+ * (0..1/2), of the strongest spectral line of a signal within [xlow, xhi]
+ * (clamped to [0.9/m, 1/2]), refined by a golden-section search of the
+ * sin^4-windowed projection to a relative bracket of 3e-8.  On the GPU the
+ * spectrum and the search run in one workgroup per signal. */
+#define ACCP_NAFF_MAX_LEN 4096 /* longest signal; lengths are powers of two >= 4 */
+/* nsig host signals of m samples each, data[s*m + i]; tunes[nsig]. */
+int accp_naff(const double *data, int nsig, int m, double xlow, double xhi, double *tunes);
+/* ALine::tracktune (src/lib/accel/aline.cpp:1159-1186): one particle from
+ * (x0, 0, y0, 0, 0, 0); the tunes of its positions at the start and after
+ * each of nt - 1 turns, nt the power of two >= nturn (nt <= 4096). */
+int accp_tracktune(accp_lattice *lat, int nturn, double x0, double y0, double xlow, double xhi,
+                   double ylow, double yhi, double tune[2]);
+/* ALine::trackDiffusion (src/lib/accel/aline.cpp:1421-1484): np particles
+ * from (x0[i], 0, y0[i], 0, 0, 0), all tracked in one launch; nux, nuy the
+ * tunes of the first half of nt positions (nt the power of two >= nturn,
+ * nt/2 <= 4096), dif = max(-4 ln nt, ln(2 |dnu| / nt)) with dnu the tune
+ * change to the second half; a particle lost within nt - 1 turns gets NaN
+ * and adds -3 to *total (the sum of dif, optional). */
+int accp_trackdiffusion(accp_lattice *lat, int nturn, int np, const double *x0, const double *y0,
+                        double *nux, double *nuy, double *dif, double xlow, double xhi, double ylow,
+                        double yhi, double *total);
+/* ALine::trackdnudexy (src/lib/accel/aline.cpp:716-779): amplitude-dependent
+ * tune shift.  Six particles at x = 1e-5 + k xm/5 (y = 1e-5 + 0.02 x) and six
+ * at y = 1e-5 + k ym/5 (x = 1e-5 + 0.02 y), nt turns each (one launch), tunes
+ * within 0.1 of the ring's linear tunes; dnudexy = least-squares slopes
+ * {dnux/dex, dnuy/dex, dnux/dey, dnuy/dey} against ex = x^2/betx, ey =
+ * y^2/bety (betx, bety at the ring start).  rows (optional, 36 doubles)
+ * receives {e, nux, nuy} of the twelve particles, the lines the reference
+ * prints. */
+int accp_trackdnudexy(accp_lattice *lat, int nturn, double xm, double ym, double dnudexy[4],
+                      double *rows);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1079,3 +1079,240 @@ int orc_beam_stat(const double *r, int n, double turn, double *stat) {
   }
   return nlive;
 }
+
+/* ---- tune diagnostics: NAFF (utility.cpp:21-96) ---------------------------
+ * The spectrum only locates the strongest line; any correct DFT gives the
+ * same power up to round-off, so the restatement uses its own iterative
+ * radix-2 transform (Cooley-Tukey, complex, imaginary part zero) where the
+ * reference calls realft.  The refinement is the golden-section search of
+ * the windowed projection, with the reference's constants and steps: its
+ * result depends on them bit for bit. */
+static void orc_fft(double *re, double *im, int n) {
+  for (int i = 1, j = 0; i < n; i++) { /* bit-reversal permutation */
+    int bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      double t = re[i]; re[i] = re[j]; re[j] = t;
+      t = im[i]; im[i] = im[j]; im[j] = t;
+    }
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    const int h = len >> 1;
+    for (int k = 0; k < h; k++) {
+      const double a = -2 * ORC_PI * k / len, wr = cos(a), wi = sin(a);
+      for (int s = 0; s < n; s += len) {
+        const int p = s + k, q = p + h;
+        const double tr = re[q] * wr - im[q] * wi, ti = re[q] * wi + im[q] * wr;
+        re[q] = re[p] - tr;
+        im[q] = im[p] - ti;
+        re[p] += tr;
+        im[p] += ti;
+      }
+    }
+  }
+}
+
+/* fprojection (utility.cpp:21-41): minus the modulus of the projection of
+ * the signal, windowed by sin^4(pi i / N), on exp(-2 pi i nv j). */
+double orc_fprojection(double nv, int n, const double *x) {
+  double ry = 0, iy = 0;
+  for (int i = 0; i < n; i++) {
+    const double rz = cos(2 * ORC_PI * nv * i), iz = -sin(2 * ORC_PI * nv * i);
+    const double w = pow(sin(ORC_PI * i / n / 1.0), 4);
+    ry += rz * x[i] * w;
+    iy += iz * x[i] * w;
+  }
+  return -sqrt(ry * ry + iy * iy);
+}
+
+/* Golden-section search (Kiefer 1953) for a minimum of fprojection in the
+ * bracket (ax, bx, cx), as mins.h:72-110 runs it: section ratio 0.61803399,
+ * the new point placed in the larger part, stop when the bracket is below
+ * 3e-8 (|x1| + |x2|). */
+static double orc_golden(double ax, double bx, double cx, int n, const double *x) {
+  const double R = 0.61803399, C = 1.0 - R, tol = 3.0e-8;
+  double x0 = ax, x3 = cx, x1, x2;
+  if (fabs(cx - bx) > fabs(bx - ax)) {
+    x1 = bx;
+    x2 = bx + C * (cx - bx);
+  } else {
+    x2 = bx;
+    x1 = bx - C * (bx - ax);
+  }
+  double f1 = orc_fprojection(x1, n, x), f2 = orc_fprojection(x2, n, x);
+  while (fabs(x3 - x0) > tol * (fabs(x1) + fabs(x2))) {
+    if (f2 < f1) {
+      x0 = x1;
+      x1 = x2;
+      x2 = R * x2 + C * x3;
+      f1 = f2;
+      f2 = orc_fprojection(x2, n, x);
+    } else {
+      x3 = x2;
+      x2 = x1;
+      x1 = R * x1 + C * x0;
+      f2 = f1;
+      f1 = orc_fprojection(x1, n, x);
+    }
+  }
+  return f1 < f2 ? x1 : x2;
+}
+
+/* naff (utility.cpp:48-96): the strongest line of the spectrum between xlow
+ * and xhi (at least 0.9/N, at most 1/2) brackets the golden-section search
+ * to one bin either side.  The spectrum is read as the reference's packed
+ * real transform: pair 0 holds F_0 and F_N/2, pair k > 0 holds F_k. */
+double orc_naff(int n, const double *data, double xlow, double xhi) {
+  double *re = (double *)malloc(sizeof(double) * (size_t)n), *im = (double *)malloc(sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; i++) {
+    re[i] = data[i];
+    im[i] = 0;
+  }
+  orc_fft(re, im, n);
+  xlow = xlow > 0.9 / n ? xlow : 0.9 / n;
+  xhi = 0.5 < xhi ? 0.5 : xhi;
+  const unsigned ilow = ((int)(xlow * n * 2) >> 1) << 1, ihi = ((int)(xhi * n * 2) >> 1) << 1;
+#define ORC_PAIR_POWER(i) \
+  ((i) == 0 ? sq(re[0]) + sq(re[n / 2]) : sq(re[(i) / 2]) + sq(im[(i) / 2]))
+  double imx = ilow, fm = ORC_PAIR_POWER(ilow);
+  for (unsigned i = ilow + 2; i < ihi; i += 2) {
+    const double fnew = ORC_PAIR_POWER(i);
+    if (fnew > fm) {
+      imx = i;
+      fm = fnew;
+    }
+  }
+#undef ORC_PAIR_POWER
+  free(re);
+  free(im);
+  const double lo = (imx - 2) * 1.0 / n / 2.0, hi = (imx + 2) * 1.0 / n / 2.0;
+  xlow = xlow > lo ? xlow : lo;
+  xhi = xhi < hi ? xhi : hi;
+  return orc_golden(xlow, (xlow + xhi) / 2.0, xhi, n, data);
+}
+
+static unsigned orc_pow2_at_least(unsigned nturn) {
+  unsigned nt = 1;
+  while (nt < nturn) nt <<= 1;
+  return nt;
+}
+
+/* ALine::tracktune (aline.cpp:1159-1186): one particle from (x0, 0, y0, 0,
+ * 0, 0), nt - 1 turns (nt the power of two >= nturn), NAFF of the nt
+ * positions (the start included). */
+void orc_tracktune(const accp_elem *elems, int nelem, unsigned nturn, double x0, double y0, double xlow,
+                   double xhi, double ylow, double yhi, double tune[2]) {
+  const unsigned nt = orc_pow2_at_least(nturn);
+  double *xl = (double *)malloc(sizeof(double) * nt), *yl = (double *)malloc(sizeof(double) * nt);
+  double r[6] = {x0, 0, y0, 0, 0, 0};
+  xl[0] = r[0];
+  yl[0] = r[2];
+  for (unsigned i = 1; i < nt; i++) {
+    orc_line_pass(elems, nelem, r, 1, 0, nelem, 1, 0, 0, 0);
+    xl[i] = r[0];
+    yl[i] = r[2];
+  }
+  tune[0] = orc_naff((int)nt, xl, xlow, xhi);
+  tune[1] = orc_naff((int)nt, yl, ylow, yhi);
+  free(xl);
+  free(yl);
+}
+
+/* ALine::trackDiffusion (aline.cpp:1421-1484): per particle the tunes of
+ * the two halves of nt - 1 turns; dif = max(-4 ln nt, ln(2 |dnu| / nt));
+ * lost particles give NaN and -3 to the returned sum. */
+double orc_trackdiffusion(const accp_elem *elems, int nelem, int nturn, int np, const double *x0,
+                          const double *y0, double *nux, double *nuy, double *dif, double xlow, double xhi,
+                          double ylow, double yhi) {
+  const unsigned nt = orc_pow2_at_least((unsigned)nturn);
+  const double mdiffu = -4 * log(nt * 1.0);
+  double *xl = (double *)malloc(sizeof(double) * nt), *yl = (double *)malloc(sizeof(double) * nt);
+  double tdif = 0;
+  for (int ip = 0; ip < np; ip++) {
+    double r[6] = {x0[ip], 0, y0[ip], 0, 0, 0};
+    xl[0] = r[0];
+    yl[0] = r[2];
+    for (unsigned j = 1; j < nt; j++) {
+      orc_line_pass(elems, nelem, r, 1, 0, nelem, 1, 0, 0, 0);
+      xl[j] = r[0];
+      yl[j] = r[2];
+    }
+    if (isnan(r[0])) {
+      nux[ip] = nuy[ip] = dif[ip] = NAN;
+      tdif += -3.0;
+    } else {
+      const int h = (int)(nt >> 1);
+      const double nux0 = orc_naff(h, xl, xlow, xhi), nuy0 = orc_naff(h, yl, ylow, yhi);
+      const double nux1 = orc_naff(h, xl + h, xlow, xhi), nuy1 = orc_naff(h, yl + h, ylow, yhi);
+      nux[ip] = nux0;
+      nuy[ip] = nuy0;
+      const double d = log(sqrt(sq(nux1 - nux0) + sq(nuy1 - nuy0)) * 2.0 / nt);
+      dif[ip] = mdiffu > d ? mdiffu : d;
+      tdif += dif[ip];
+    }
+  }
+  free(xl);
+  free(yl);
+  return tdif;
+}
+
+/* Least-squares slope of y on x (the textbook centred form that Fitab
+ * evaluates, fitab.h:37-57). */
+static double orc_slope(int n, const double *x, const double *y) {
+  double sx = 0, sy = 0, st2 = 0, b = 0;
+  for (int i = 0; i < n; i++) {
+    sx += x[i];
+    sy += y[i];
+  }
+  const double sxoss = sx / n;
+  for (int i = 0; i < n; i++) {
+    const double t = x[i] - sxoss;
+    st2 += t * t;
+    b += t * y[i];
+  }
+  (void)sy;
+  return b / st2;
+}
+
+/* ALine::trackdnudexy (aline.cpp:716-779): tune shift with amplitude.  Six
+ * particles at x = 1e-5 + k xm / 5 (y = 1e-5 + 0.02 x), nt turns each, tunes
+ * by NAFF within 0.1 of the linear tunes; slopes of nux and nuy against
+ * x^2 / betx; then the same in y.  d = {dnux/dex, dnuy/dex, dnux/dey,
+ * dnuy/dey}. */
+void orc_trackdnudexy(const accp_elem *elems, int nelem, unsigned nturn, double xm, double ym, double d[4]) {
+  double tw[7];
+  orc_calctwiss0(elems, nelem, 0.0, tw);
+  const double tunex = tw[1] / 2 / ORC_PI, tuney = tw[2] / 2 / ORC_PI;
+  const unsigned np = 6, nt = orc_pow2_at_least(nturn);
+  const double tiny = 1.0e-5;
+  double xt[6], yt[6], de[6];
+  double *xl = (double *)malloc(sizeof(double) * nt), *yl = (double *)malloc(sizeof(double) * nt);
+  const double xlo = 0 > tunex - 0.1 ? 0 : tunex - 0.1, xhi = 0.5 < tunex + 0.1 ? 0.5 : tunex + 0.1;
+  const double ylo = 0 > tuney - 0.1 ? 0 : tuney - 0.1, yhi = 0.5 < tuney + 0.1 ? 0.5 : tuney + 0.1;
+  for (int plane = 0; plane < 2; plane++) {
+    for (unsigned k = 0; k < np; k++) {
+      double r[6] = {0, 0, 0, 0, 0, 0};
+      if (plane == 0) {
+        r[0] = tiny + k * 1.0 / (np - 1) * xm;
+        r[2] = tiny + r[0] * .02;
+        de[k] = r[0] * r[0] / tw[3];
+      } else {
+        r[2] = tiny + k * 1.0 / (np - 1) * ym;
+        r[0] = tiny + r[2] * .02;
+        de[k] = r[2] * r[2] / tw[5];
+      }
+      for (unsigned i = 0; i < nt; i++) {
+        xl[i] = r[0];
+        yl[i] = r[2];
+        orc_line_pass(elems, nelem, r, 1, 0, nelem, 1, 0, 0, 0);
+      }
+      xt[k] = orc_naff((int)nt, xl, xlo, xhi);
+      yt[k] = orc_naff((int)nt, yl, ylo, yhi);
+    }
+    d[2 * plane] = orc_slope((int)np, de, xt);
+    d[2 * plane + 1] = orc_slope((int)np, de, yt);
+  }
+  free(xl);
+  free(yl);
+}

@@ -116,6 +116,22 @@ void orc_calcchrom2(const accp_elem *elems, int nelem, double c[4]);
 double orc_trackma(const accp_elem *elems, int nelem, double dppmax, double dppmin, int nturn,
                    int nstep, const int *locs, int nloc, double *dpp_plus, double *dpp_minus);
 
+/* ---- tune diagnostics ------------------------------------------------- */
+/* naff (utility.cpp:48-96): the frequency (tune, 0..0.5) of the strongest
+ * line of data[0..n) between xlow and xhi; n a power of two. */
+double orc_naff(int n, const double *data, double xlow, double xhi);
+/* fprojection (utility.cpp:21-41). */
+double orc_fprojection(double nv, int n, const double *x);
+/* ALine::tracktune (aline.cpp:1159-1186). */
+void orc_tracktune(const accp_elem *elems, int nelem, unsigned nturn, double x0, double y0, double xlow,
+                   double xhi, double ylow, double yhi, double tune[2]);
+/* ALine::trackDiffusion (aline.cpp:1421-1484); returns the sum of dif. */
+double orc_trackdiffusion(const accp_elem *elems, int nelem, int nturn, int np, const double *x0,
+                          const double *y0, double *nux, double *nuy, double *dif, double xlow, double xhi,
+                          double ylow, double yhi);
+/* ALine::trackdnudexy (aline.cpp:716-779). */
+void orc_trackdnudexy(const accp_elem *elems, int nelem, unsigned nturn, double xm, double ym, double d[4]);
+
 /* calc_beam_stat (utility.cpp:322-373) into an ACCP_STAT_LEN record. */
 int orc_beam_stat(const double *r, int n, double turn, double *stat);
 

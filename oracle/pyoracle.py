@@ -8,7 +8,7 @@ cpu_baseline leg import this module.
 import ctypes
 import os
 import tempfile
-from ctypes import POINTER, c_char_p, c_double, c_int, c_uint32, c_uint64, c_void_p
+from ctypes import POINTER, c_char_p, c_double, c_int, c_uint, c_uint32, c_uint64, c_void_p
 
 import numpy as np
 
@@ -63,6 +63,16 @@ class Oracle:
         L.orc_calcchrom2.argtypes = [E, c_int, DP]
         L.orc_trackma.argtypes = [E, c_int, c_double, c_double, c_int, c_int, IP, c_int, DP, DP]
         L.orc_trackma.restype = c_double
+        L.orc_naff.argtypes = [c_int, DP, c_double, c_double]
+        L.orc_naff.restype = c_double
+        L.orc_fprojection.argtypes = [c_double, c_int, DP]
+        L.orc_fprojection.restype = c_double
+        L.orc_tracktune.argtypes = [E, c_int, c_uint, c_double, c_double, c_double, c_double, c_double, c_double,
+                                    DP]
+        L.orc_trackdiffusion.argtypes = [E, c_int, c_int, c_int, DP, DP, DP, DP, DP, c_double, c_double,
+                                         c_double, c_double]
+        L.orc_trackdiffusion.restype = c_double
+        L.orc_trackdnudexy.argtypes = [E, c_int, c_uint, c_double, c_double, DP]
         for nm in ("orc_quad6", "orc_drift_kzero", "orc_fastdrift"):
             getattr(L, nm).argtypes = [DP, c_double] + ([c_double] if nm == "orc_quad6" else [])
         self.lib = L
@@ -173,6 +183,35 @@ class Oracle:
         avg = self.lib.orc_trackma(a, len(a), dppmax, dppmin, nturn, nstep, _ip(lc), len(lc), _dp(p), _dp(m))
         return avg, p, m
 
+    def naff(self, data, xlow=0.0, xhi=0.5):
+        d = np.ascontiguousarray(data, dtype=np.float64)
+        return self.lib.orc_naff(len(d), _dp(d), xlow, xhi)
+
+    def fprojection(self, nv, data):
+        d = np.ascontiguousarray(data, dtype=np.float64)
+        return self.lib.orc_fprojection(nv, len(d), _dp(d))
+
+    def tracktune(self, elems, nturn, x0, y0, xlow=0.0, xhi=0.5, ylow=0.0, yhi=0.5):
+        a = self._arr(elems)
+        t = np.zeros(2)
+        self.lib.orc_tracktune(a, len(a), nturn, x0, y0, xlow, xhi, ylow, yhi, _dp(t))
+        return t
+
+    def trackdiffusion(self, elems, nturn, x0, y0, xlow=0.0, xhi=0.5, ylow=0.0, yhi=0.5):
+        a = self._arr(elems)
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        y0 = np.ascontiguousarray(y0, dtype=np.float64)
+        nux, nuy, dif = np.zeros(len(x0)), np.zeros(len(x0)), np.zeros(len(x0))
+        tot = self.lib.orc_trackdiffusion(a, len(a), nturn, len(x0), _dp(x0), _dp(y0), _dp(nux), _dp(nuy),
+                                          _dp(dif), xlow, xhi, ylow, yhi)
+        return tot, nux, nuy, dif
+
+    def trackdnudexy(self, elems, nturn, xm, ym):
+        a = self._arr(elems)
+        d = np.zeros(4)
+        self.lib.orc_trackdnudexy(a, len(a), nturn, xm, ym, _dp(d))
+        return d
+
     def philox(self, ctr, key):
         c = (c_uint32 * 4)(*ctr)
         k = (c_uint32 * 2)(*key)
@@ -239,7 +278,19 @@ class Reference:
         L.ref_calcchrom2.argtypes = [c_void_p, DP]
         L.ref_trackma.argtypes = [c_void_p, c_double, c_double, c_int, c_int, IPT, c_int, DP, DP]
         L.ref_trackma.restype = c_double
+        L.ref_naff.argtypes = [c_int, DP, c_double, c_double]
+        L.ref_naff.restype = c_double
+        L.ref_tracktune.argtypes = [c_void_p, c_uint, c_double, c_double, c_double, c_double, c_double, c_double,
+                                    DP]
+        L.ref_trackdiffusion.argtypes = [c_void_p, c_int, c_int, DP, DP, DP, DP, DP, c_double, c_double,
+                                         c_double, c_double]
+        L.ref_trackdiffusion.restype = c_double
+        L.ref_trackdnudexy.argtypes = [c_void_p, c_uint, c_double, c_double, DP]
         self.lib = L
+
+    def naff(self, data, xlow=0.0, xhi=0.5):
+        d = np.ascontiguousarray(data, dtype=np.float64)
+        return self.lib.ref_naff(len(d), _dp(d), xlow, xhi)
 
     def load(self, elems):
         """Write `elems` as a lattice file, load it, then apply the state the
@@ -390,6 +441,24 @@ class RefLine:
         p, m = np.zeros(len(lc)), np.zeros(len(lc))
         avg = self.lib.ref_trackma(self.h, dppmax, dppmin, nturn, nstep, _ip(lc), len(lc), _dp(p), _dp(m))
         return avg, p, m
+
+    def tracktune(self, nturn, x0, y0, xlow=0.0, xhi=0.5, ylow=0.0, yhi=0.5):
+        t = np.zeros(2)
+        self.lib.ref_tracktune(self.h, nturn, x0, y0, xlow, xhi, ylow, yhi, _dp(t))
+        return t
+
+    def trackdiffusion(self, nturn, x0, y0, xlow=0.0, xhi=0.5, ylow=0.0, yhi=0.5):
+        x0 = np.array(x0, dtype=np.float64)
+        y0 = np.array(y0, dtype=np.float64)
+        nux, nuy, dif = np.zeros(len(x0)), np.zeros(len(x0)), np.zeros(len(x0))
+        tot = self.lib.ref_trackdiffusion(self.h, nturn, len(x0), _dp(x0), _dp(y0), _dp(nux), _dp(nuy), _dp(dif),
+                                          xlow, xhi, ylow, yhi)
+        return tot, nux, nuy, dif
+
+    def trackdnudexy(self, nturn, xm, ym):
+        d = np.zeros(4)
+        self.lib.ref_trackdnudexy(self.h, nturn, xm, ym, _dp(d))
+        return d
 
     def getfield(self, idx, name):
         return self.lib.ref_getfield(self.h, idx, name.encode())

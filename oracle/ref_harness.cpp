@@ -202,4 +202,31 @@ double ref_trackma(void *h, double dppmax, double dppmin, int nturn, int nstep, 
   delete[] u;
   return avg;
 }
+
+// naff (utility.cpp:48-96) on a copy of the signal
+double ref_naff(int n, const double *data, double xlow, double xhi) {
+  double *d = new double[n];
+  std::memcpy(d, data, sizeof(double) * n);
+  const double v = naff(n, d, xlow, xhi);
+  delete[] d;
+  return v;
+}
+
+// ALine::tracktune (aline.cpp:1159-1186)
+void ref_tracktune(void *h, unsigned nturn, double x0, double y0, double xlow, double xhi, double ylow,
+                   double yhi, double *tune) {
+  static_cast<ALine *>(h)->tracktune(tune[0], tune[1], nturn, x0, y0, xlow, xhi, ylow, yhi);
+}
+
+// ALine::trackDiffusion (aline.cpp:1421-1484)
+double ref_trackdiffusion(void *h, int nturn, int np, double *x0, double *y0, double *nux, double *nuy,
+                          double *dif, double xlow, double xhi, double ylow, double yhi) {
+  return static_cast<ALine *>(h)->trackDiffusion(nturn, (unsigned)np, x0, y0, nux, nuy, dif, xlow, xhi, ylow,
+                                                 yhi);
+}
+
+// ALine::trackdnudexy (aline.cpp:716-779)
+void ref_trackdnudexy(void *h, unsigned nturn, double xm, double ym, double *d) {
+  static_cast<ALine *>(h)->trackdnudexy(d, nturn, xm, ym);
+}
 }

@@ -4,7 +4,7 @@ Runs in the build container only (it needs oracle/_ref/libaccel_ref.so,
 built by `make -C oracle` from the Accel+ sources).  Every fixture is data:
 inputs and the reference's outputs for them, written with numpy.savez
 (no pickles).  Regenerate with:  python tests/golden/make_golden.py
-(or one of the later fixtures alone: python tests/golden/make_golden.py optics2|elements2|transwake|wigtable)
+(or one of the later fixtures alone: python tests/golden/make_golden.py optics2|elements2|transwake|wigtable|naff)
 """
 import os
 import sys
@@ -196,6 +196,54 @@ def wigtable(R, out):
     print("wigtable:", sorted(res))
 
 
+def naff_signals():
+    """Signals for naff: (data, xlow, xhi) with lines anywhere in (0, 1/2),
+    two lines of similar strength, noise, a constant, a line at the lowest
+    bins, a restricted window, and a NaN sample."""
+    rng = np.random.default_rng(17)
+    out = []
+    for n in (16, 64, 256, 1024, 4096):
+        for _ in range(4):
+            nu = rng.uniform(0.02, 0.48)
+            t = np.arange(n)
+            x = np.cos(2 * np.pi * nu * t + rng.uniform(0, 6)) + 0.3 * np.sin(2 * np.pi * rng.uniform(0, 0.5) * t)
+            out.append((x + 1e-3 * rng.standard_normal(n), 0.0, 0.5))
+    t = np.arange(512)
+    two = np.cos(2 * np.pi * 0.21 * t) + 0.98 * np.cos(2 * np.pi * 0.3 * t)
+    out.append((two, 0.0, 0.5))
+    out.append((two, 0.25, 0.35))
+    out.append((np.full(128, 0.7), 0.0, 0.5))
+    out.append((np.cos(2 * np.pi * 1.3 / 256 * np.arange(256)), 0.0, 0.5))
+    out.append((np.cos(2 * np.pi * 0.4999 * np.arange(256)), 0.0, 0.5))
+    noisy = rng.standard_normal(256)
+    noisy[100] = np.nan
+    out.append((noisy, 0.1, 0.4))
+    return out
+
+
+def naff(R, out):
+    """naff on synthetic signals; tracktune, trackDiffusion and trackdnudexy
+    on the FODO ring and on a DA ring with errors (radiation and cavities
+    off, as TRACK_DNUDJ sets them, process.cpp:2619-2621)."""
+    sig = naff_signals()
+    res = {"naff": np.array([R.naff(d, lo, hi) for d, lo, hi in sig])}
+    rings = {"fodo": lat.fodo_ring(), "da": lat.da_ring(2)}
+    for name, ring in rings.items():
+        line = R.load(ring)
+        line.set_rad(0)
+        line.set_cavity(0)
+        res[f"tune_{name}"] = np.array([line.tracktune(300, 1e-3, 5e-4), line.tracktune(128, 4e-3, 2e-3, 0.2, 0.35,
+                                                                                          0.2, 0.3)])
+        res[f"dnudexy_{name}"] = line.trackdnudexy(256, 0.004, 0.003)
+        x0 = np.array([1e-4, 2e-3, 5e-3, 8e-3, 1.2e-2, 0.05])
+        y0 = np.array([1e-4, 1e-3, 3e-3, 2e-3, 4e-3, 0.02])
+        tot, nux, nuy, dif = line.trackdiffusion(512, x0, y0, 0.05, 0.45, 0.05, 0.45)
+        res[f"diffusion_{name}"] = np.array([nux, nuy, dif])
+        res[f"diffusion_total_{name}"] = np.array([tot])
+    np.savez_compressed(os.path.join(out, "naff.npz"), **res)
+    print("naff:", {k: v.shape for k, v in res.items()})
+
+
 def transwake_case():
     """A 2 nC bunch of 1000 particles (one lost, a few sharing z) through a
     transverse resonator wake (x: 1e6, 1.5 GHz, Q 5; y: 2e6, 2 GHz, Q 3)."""
@@ -218,6 +266,9 @@ def main():
     R = Reference()
     if sys.argv[1:] == ["transwake"]:
         transwake(R, HERE)
+        return
+    if sys.argv[1:] == ["naff"]:
+        naff(R, HERE)
         return
     if sys.argv[1:] == ["wigtable"]:
         wigtable(R, HERE)
@@ -307,6 +358,7 @@ def main():
     elements2(R, out)
     transwake(R, out)
     wigtable(R, out)
+    naff(R, out)
     print("fixtures written to", out)
 
 

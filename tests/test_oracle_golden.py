@@ -188,3 +188,36 @@ def test_wigtable_fixtures_bit_exact(oracle):
     g = golden("wigtable.npz")
     for name, e in wigtable_cases():
         assert same_bits(oracle.track([e], g["particles"], 1), g[name]), name
+
+
+def _quiet(ring):
+    """The ring with radiation and cavities off (TRACK_DNUDJ, process.cpp:2619-2621)."""
+    arr = el.to_array(ring)
+    for i in range(len(arr)):
+        if arr[i].type == el.CAVITY:
+            arr[i].on = 0
+        if arr[i].type in (el.BENDMPOLE, el.IDEALWIG):
+            arr[i].rad = 0
+    return arr
+
+
+def test_naff_and_tune_diagnostics_bit_exact(oracle):
+    """naff (its own transform, the reference's golden-section steps) on
+    synthetic signals, and tracktune, trackdnudexy, trackDiffusion on two
+    rings: bit for bit with the reference (naff.npz)."""
+    from tests.golden.make_golden import naff_signals
+    g = golden("naff.npz")
+    got = np.array([oracle.naff(d, lo, hi) for d, lo, hi in naff_signals()])
+    assert same_bits(got, g["naff"])
+    for name, ring in (("fodo", lat.fodo_ring()), ("da", lat.da_ring(2))):
+        arr = _quiet(ring)
+        t = np.array([oracle.tracktune(arr, 300, 1e-3, 5e-4), oracle.tracktune(arr, 128, 4e-3, 2e-3, 0.2, 0.35,
+                                                                                0.2, 0.3)])
+        assert same_bits(t, g[f"tune_{name}"]), name
+        assert same_bits(oracle.trackdnudexy(arr, 256, 0.004, 0.003), g[f"dnudexy_{name}"]), name
+        x0 = np.array([1e-4, 2e-3, 5e-3, 8e-3, 1.2e-2, 0.05])
+        y0 = np.array([1e-4, 1e-3, 3e-3, 2e-3, 4e-3, 0.02])
+        tot, nux, nuy, dif = oracle.trackdiffusion(arr, 512, x0, y0, 0.05, 0.45, 0.05, 0.45)
+        assert same_bits(np.array([nux, nuy, dif]), g[f"diffusion_{name}"]), name
+        assert same_bits(np.array([tot]), g[f"diffusion_total_{name}"]), name
+        assert np.isnan(dif[-1]) and not np.isnan(dif[:2]).any()  # small amplitudes survive, 5 cm is lost
