@@ -24,7 +24,8 @@
 //            Linear X=a:d:b
 //            [shift=(...)] [print]; Monitor: interval=, fraction=,
 //            printflag=Coordinate|Stat1|Stat2; Do_Track: NTurn=,
-//            radiation=, RFCavity=, PrintFinal=No|Yes|Full [, seed=]
+//            radiation=, RFCavity=, PrintFinal=No|Yes|Full [, seed=];
+//            Transport: type=, famname=, index=n|[i j ...]
 //   DAMA:    TRACK_DA: xmin=, xmax=, ymax=, nray=, nstep=, nturn=
 //            [, print=loss_turn] [, weightminusside=true]; apply_setpt=file;
 //            TRACK_MA: type=, [famname=,] nmoni=, dppmin=, dppmax=, nstep=, nturn=;
@@ -1174,8 +1175,98 @@ void do_track(Line &L, std::vector<double> &r, const Monitor &mon, const string 
   std::cout << "*************************************" << std::endl;
 }
 
+// get_elem_list (process.cpp:1050-1099): the watch points of a command.
+// type empty: element `index`; ALL: every element; no famname: every
+// element of the type, or its index-th (index > 0); "[i j ...]": the family
+// members i, j, ... (1-based, kept only below the family size and written in
+// place, as the reference does); else the family, or its index-th member
+// (the reference's findelements leaves that result unset; the intended
+// member is taken).
+std::vector<int32_t> elem_list(const Line &L, const string &type, const string &fam, unsigned index,
+                               const string &indices) {
+  if (type.empty()) return {(int32_t)index};
+  if (ieq(type, "ALL")) {
+    std::vector<int32_t> v(L.e.size());
+    for (size_t i = 0; i < v.size(); ++i) v[i] = (int32_t)i;
+    return v;
+  }
+  if (fam.empty()) {
+    std::vector<int32_t> v = L.find(type, "");
+    if (index > 0) v = index <= v.size() ? std::vector<int32_t>{v[index - 1]} : std::vector<int32_t>{};
+    return v;
+  }
+  if (!indices.empty()) {
+    std::vector<int32_t> v = L.find(type, fam);
+    std::istringstream is(indices);
+    int t, i = 0;
+    while (is >> t)
+      if (t > 0 && t < (int)v.size()) v[i++] = v[t - 1];
+    v.resize(i);
+    return v;
+  }
+  return L.find(type, fam, index);
+}
+
+// TRANSPORT: type=, famname=, index=n|[i j ...] (process.cpp:2108-2214):
+// a copy of the beam passes elements [0, w0), then [w(k-1), wk) for each
+// further watch point w, printing the particles at each.  The reference
+// prints the untransported distribution (it writes r, having tracked its
+// copy nr); the transported coordinates are printed here.
+void cmd_transport(Line &L, const std::vector<double> &r, const string &line) {
+  string s = line.substr(line.find_first_of("=:") + 1);
+  std::replace(s.begin(), s.end(), ',', ' ');
+  s = trim(s);
+  string type, fam, indices;
+  unsigned index = 0;
+  for (int cnt = 0; cnt < 3 && !s.empty(); ++cnt) {
+    size_t e = s.find('=');
+    const string key = upper(trim(s.substr(0, e)));
+    s = e == string::npos ? "" : trim(s.substr(e + 1));
+    size_t b = s.find_first_of(" \t");
+    if (key == "TYPE") type = s.substr(0, b);
+    else if (key == "FAMNAME") fam = s.substr(0, b);
+    else if (key == "INDEX") {
+      if (!s.empty() && s.front() == '[') {
+        b = s.find(']');
+        indices = trim(s.substr(1, b - 1));
+      } else {
+        std::istringstream(s) >> index;
+      }
+    }
+    s = b == string::npos ? "" : trim(s.substr(b + 1));
+  }
+  if (indices.empty()) std::cout << type << " " << fam << " " << index << std::endl;
+  else std::cout << type << " " << fam << " " << indices << " " << std::endl;
+  const std::vector<int32_t> list = elem_list(L, type, fam, index, indices);
+  if (list.empty()) {
+    std::cout << "No watch point found" << std::endl;
+    return;
+  }
+  if (L.parse_only) return;
+  const int np = (int)(r.size() / 6);
+  std::vector<double> nr = r;
+  std::cout << "$$$Track-Transport: ************************" << std::endl;
+  int from = 0;
+  for (int32_t w : list) {
+    if (w < 0 || w > (int)L.e.size()) throw std::runtime_error("TRANSPORT: ring index out of range");
+    if (w > from && np > 0) check(accp_track(L.device(), nr.data(), np, 1, from, w, nullptr), "accp_track");
+    double spos = 0;
+    for (int i = 0; i < w; ++i) spos += L.e[i].len;
+    std::cout << "type=" << type << ", famname=" << fam << ", ring index=" << w << ", spos=" << fmt(spos)
+              << std::endl;
+    for (int i = 0; i < np; ++i) {
+      std::cout << w << "\t";
+      for (int j = 0; j < 6; ++j) std::cout << fmt(nr[6 * (size_t)i + j]) << "\t";
+      std::cout << std::endl;
+    }
+    std::cout << "#------end of ring index=" << w << std::endl;
+    from = w;
+  }
+  std::cout << "******************************" << std::endl;
+}
+
 void proc_track(std::istream &in, Line &L) {
-  const std::vector<string> keys = {"DISTRIBUTION", "MONITOR", "DO_TRACK", "END"};
+  const std::vector<string> keys = {"DISTRIBUTION", "MONITOR", "DO_TRACK", "TRANSPORT", "END"};
   std::vector<double> r;
   Monitor mon;
   uint64_t seed = 0;
@@ -1216,6 +1307,8 @@ void proc_track(std::istream &in, Line &L) {
     } else if (k == 2) {
       if (L.parse_only) continue;
       do_track(L, r, mon, line, seed);
+    } else if (k == 3) {
+      cmd_transport(L, r, line);
     }
   }
 }

@@ -276,3 +276,37 @@ End
     assert np.allclose(d, g["dnudexy_da"], rtol=1e-4, atol=0), (d, g["dnudexy_da"])
     # amplitudes grow in each scan; tunes stay within 0.1 of the linear ones
     assert np.all(np.diff(rows[:6, 0]) > 0) and np.all(np.diff(rows[6:, 0]) > 0)
+
+
+def test_transport_to_watch_points(tmp_path, oracle):
+    """TRANSPORT (process.cpp:2108-2214): the beam passes [0, w0) and then
+    [w(k-1), wk) for QF members 1, 3 and 5 ([1,3,5], 1-based); the printed
+    coordinates at each watch point against the oracle over the same
+    ranges (the transported copy, where the reference prints its
+    untransported input)."""
+    ring = lat.fodo_ring()
+    el.write_lattice(ring, str(tmp_path / "fodo.lat"))
+    beam = lat.gaussian_beam(10, seed=8)
+    el.write_plain_beam(beam, str(tmp_path / "beam.txt"))
+    out = run(tmp_path, """Lattice
+  file_name = fodo.lat
+End
+Track
+  Distribution: Load beam.txt
+  Transport: type=Quad, famname=QF, index=[1,3,5]
+End
+""")
+    qf = lat.family(ring, "QF")
+    watch = [qf[0], qf[2], qf[4]]
+    r = beam.copy()
+    start = 0
+    arr = el.to_array(ring)
+    for w in watch:
+        i = out.index(next(s for s in out if s.startswith(f"type=Quad, famname=QF, ring index={w},")))
+        spos = float(out[i].split("spos=")[1])
+        assert spos == pytest.approx(sum(e.len for e in ring[:w]), rel=1e-14)
+        got = np.array([numbers(s) for s in out[i + 1:i + 11]])
+        assert np.all(got[:, 0] == w)
+        r = oracle.track(arr, r, 1, st=start, ed=w)
+        assert_coords_close(got[:, 1:], r, 1e-12, what=f"transport to {w}")
+        start = w
