@@ -285,9 +285,9 @@ def adjoint_leg(lib, stream):
     }
 
 
-def diffusion_leg(lib, nx=128, ny=64, nturn=1024, cpu_sample=32):
+def diffusion_leg(lib, nx=256, ny=128, nturn=1024, cpu_sample=32):
     """Frequency map (trackDiffusion, aline.cpp:1421-1484) of the seed-0 DA
-    lattice with radiation and cavities off: a 128 x 64 grid of starting
+    lattice with radiation and cavities off: a 256 x 128 grid of starting
     points (x0 to 8 mm, y0 to 4 mm), 1024 turns, NAFF of both halves of
     each plane; the whole call (one tracking launch sequence recording every
     turn, one NAFF launch).  The reference's own trackDiffusion, one particle
