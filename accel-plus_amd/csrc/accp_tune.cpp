@@ -81,6 +81,8 @@ extern "C" int accp_naff(const double *data, int nsig, int m, double xlow, doubl
   int rc;
   if ((rc = check_len(m)) || (rc = check_window(xlow, xhi))) return rc;
   if (nsig == 0) return ACCP_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(ACCP_E_HIP, "no HIP device available");
   const size_t bytes = sizeof(double) * (size_t)nsig * m;
   double *d = nullptr;
   if (hipMalloc(&d, bytes) != hipSuccess) return fail(ACCP_E_NOMEM, "accp_naff: hipMalloc failed");

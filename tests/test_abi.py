@@ -71,6 +71,16 @@ def test_bad_arguments_report_errors(lib):
     assert L.accp_track(None, None, 0, 1, 0, -1, None) == -1
     assert L.accp_batch_replay(None, None) == -5
     assert L.accp_lattice_free(None) == 0
+    # NAFF: power-of-two lengths 4..4096 and an ordered window, checked
+    # before any device work
+    sig = (ctypes.c_double * 24)()
+    out = (ctypes.c_double * 2)()
+    assert L.accp_naff(sig, 2, 12, ctypes.c_double(0.0), ctypes.c_double(0.5), out) == -1
+    assert b"power-of-two length" in L.accp_last_error()
+    assert L.accp_naff(sig, 1, 8, ctypes.c_double(0.3), ctypes.c_double(0.2), out) == -1
+    assert b"window" in L.accp_last_error()
+    assert L.accp_tracktune(None, 100, ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0),
+                            ctypes.c_double(0.5), ctypes.c_double(0), ctypes.c_double(0.5), out) == -1
 
 
 def test_no_device_is_an_error_not_a_fallback(lib):
@@ -78,6 +88,8 @@ def test_no_device_is_an_error_not_a_fallback(lib):
         pytest.skip("a GPU is visible; covered by the gpu tests")
     with pytest.raises(pyaccp.AccpError, match="no HIP device"):
         lib.lattice([el.drift(1.0)])
+    with pytest.raises(pyaccp.AccpError, match="no HIP device"):
+        lib.naff([[1.0, 0.0, -1.0, 0.0]])
 
 
 def test_missing_library_raises():
