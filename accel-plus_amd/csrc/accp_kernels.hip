@@ -396,17 +396,19 @@ __device__ __forceinline__ void element_body(Part (&p)[PPL], const Hdr h, const 
 // (e.g. accel.cpp:166-167): every lane computes (the PPL instruction streams
 // stay interleaved, no divergent branch per element) and lost particles get
 // their state back afterwards.  (An exec-masked branch per element measured
-// slower on the latency-bound tail of a dynamic-aperture run.)
+// slower on the latency-bound tail of a dynamic-aperture run.)  Padding
+// lanes of a partly filled tile are never written back, so they compute
+// freely: the restore runs only in waves that hold a lost particle.
 template <int PPL, bool FULL, class PT>
 __device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const PT *P,
                                               const KernelCtx &ctx, const uint64_t (&pid)[PPL],
-                                              uint64_t turn, int eidx) {
+                                              const bool (&inb)[PPL], uint64_t turn, int eidx) {
   Part keep[PPL];
   bool live[PPL];
 #pragma unroll
   for (int k = 0; k < PPL; ++k) {
     keep[k] = p[k];
-    live[k] = is_live(p[k]);
+    live[k] = is_live(p[k]) || !inb[k];
   }
   element_body<PPL, FULL>(p, h, P, ctx, pid, turn, eidx);
 #pragma unroll
@@ -482,7 +484,7 @@ __global__ __launch_bounds__(BLOCK) void k_track(KernelCtx ctx) {
     // unless this is the first launch of the call (lost on input: -1)
     record[k] = inb[k] && ctx.loss_turn && (is_live(p[k]) || ctx.turn_rel0 == 0);
     lost_at[k] = 0x7fffffff;
-    if (!inb[k]) p[k].x = __builtin_nan("");  // padding lanes behave as lost
+    if (!inb[k]) p[k].x = __builtin_nan("");  // padding lanes start lost, never written back
   }
 
   int t = 0;
@@ -501,7 +503,7 @@ __global__ __launch_bounds__(BLOCK) void k_track(KernelCtx ctx) {
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
       const bool lv = is_live(p[k]);
-      any |= lv;
+      any |= lv && inb[k];
       if (!lv && lost_at[k] == 0x7fffffff) lost_at[k] = ctx.turn_rel0 + t - 1;
     }
     if (!__any(any)) break;  // every particle of this wave is lost: frozen from here on
@@ -510,7 +512,7 @@ __global__ __launch_bounds__(BLOCK) void k_track(KernelCtx ctx) {
     Hdr h = load_hdr<LDS>(hdr, ctx.st);
     for (int e = ctx.st; e < ctx.ed; ++e) {
       const Hdr hn = load_hdr<LDS>(hdr, e + 1 < ctx.ed ? e + 1 : e);
-      apply_element<PPL, FULL>(p, h, par + h.off, ctx, pid, turn, e);
+      apply_element<PPL, FULL>(p, h, par + h.off, ctx, pid, inb, turn, e);
       h = hn;
     }
   }
