@@ -137,12 +137,14 @@ def saturated_leg(lib, glat, elems, tuning, stream):
     }
 
 
-def collective_leg(lib, stream, nturn=200):
+def collective_leg(lib, stream, nturn=200, eq_turns=20000):
     """Collective tracking (radiation with quantum excitation, cavity at the
     synchronous phase from the GPU's setrfphase, longitudinal resonator wake
-    of 5 kOhm, Q = 1, 10 GHz, 2 nC in 256 bins) of 1e5 Gaussian particles on
+    of 5 kOhm, Q = 1, 10 GHz, 2 nC in 256 bins over +-15 cm) of 1e5 Gaussian particles on
     the seed-0 DA lattice; every turn splits at the wake for the histogram,
-    convolution and kick."""
+    convolution and kick.  Timed over nturn turns; then the whole workload
+    (eq_turns turns) for the equilibrium energy spread, bunch length and
+    emittance."""
     import numpy as np
 
     import pyaccp.elements as el
@@ -151,7 +153,8 @@ def collective_leg(lib, stream, nturn=200):
     from pyaccp.hipmem import DeviceArray, Event
 
     log("collective leg")
-    ring = lat.da_ring(0) + [el.longwake(5e3, 1.0, 10e9, 2e-9, 256, -0.02, 0.02, energy=3000.0, on=1)]
+    # the window spans +-6 equilibrium bunch lengths (sigma_z ~ 25 mm on this lattice)
+    ring = lat.da_ring(0) + [el.longwake(5e3, 1.0, 10e9, 2e-9, 256, -0.15, 0.15, energy=3000.0, on=1)]
     L = lib.lattice(ring)
     phis = L.setrfphase()
     arr = L.get()
@@ -175,7 +178,7 @@ def collective_leg(lib, stream, nturn=200):
     out = d.numpy()
     alive = int((~np.isnan(out[:, 0])).sum())
     fl = n * nturn * flops.flops_per_turn([arr[i] for i in range(len(arr))], "useful")
-    return {
+    res = {
         "workload": f"{n} particles, RADQE + cavity (phi_s = {phis:.6f} from setrfphase) + LongWake "
                     f"(256 bins), {len(ring)} elements, {nturn} turns ({alive} survive)",
         "ms": round(ms, 3),
@@ -183,6 +186,35 @@ def collective_leg(lib, stream, nturn=200):
         "particle_turns_per_s": n * nturn / (ms * 1e-3),
         "achieved_tflops": round(fl / (ms * 1e-3) / 1e12, 3),
     }
+    if eq_turns:
+        # the whole collective workload: 20000 turns (five longitudinal
+        # damping times, U0 = 0.75 MeV at 3 GeV), in pieces of 1000 turns
+        # that continue one RNG stream; statistics of the last five pieces
+        log("collective leg: equilibrium run")
+        d.copy_from(d0, stream.handle)
+        piece = 1000
+        recs = []
+        t0 = time.perf_counter()
+        for k in range(eq_turns // piece):
+            L.track_device(d.ptr, n, piece, seed=3, turn0=k * piece, stream=stream.handle)
+            if k >= eq_turns // piece - 5:
+                s6 = lib.beam_stat(d.numpy())
+                emit = np.sqrt(max(s6[8] ** 2 * s6[9] ** 2 - s6[14] ** 2, 0.0))
+                recs.append((s6[1], s6[13], s6[12], emit))
+        sec = time.perf_counter() - t0
+        recs = np.array(recs)
+        res["equilibrium"] = {
+            "turns": eq_turns,
+            "seconds": round(sec, 2),
+            "particle_turns_per_s": n * eq_turns / sec,
+            "survivors": int(recs[-1, 0]),
+            "sigma_delta": float(recs[:, 1].mean()),
+            "sigma_z_m": float(recs[:, 2].mean()),
+            "emittance_x_m_rad": float(recs[:, 3].mean()),
+            "note": "means over turns 16000..20000 (5 records); sigma_delta of the ring without the wake "
+                    "is 8.3e-4 (Sands, 3 GeV, rho = 9.55 m)",
+        }
+    return res
 
 
 def adjoint_leg(lib, stream):
