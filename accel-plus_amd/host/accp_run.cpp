@@ -18,7 +18,8 @@
 //            correct_tune / correct_chrom: type=, famname=(F1 F2) [a, b]
 //            [REPEAT] [METHOD=FD|ADJOINT]; set_max_num_threads=N (accepted)
 //   Optics:  print_tune / print_chrom [dpp=, delta=]; print_2nd_chrom;
-//            print_matrix: type=RING; print_optics: type=, famname=, index=
+//            print_matrix: type=RING|T, famname=, index=n|[i j ...], line_style=;
+//            print_optics: type=, famname=, index=n|[i j ...]
 //   Track:   Distribution: Zero NPar= | Load f[, format=plain|accel-export] |
 //            Coordinate r=(...;...) | Gaussian NPar= sigx=|emit_x=[,betx=] ... |
 //            Linear X=a:d:b
@@ -842,6 +843,78 @@ void proc_lattice(std::istream &in, Line &L) {
   }
 }
 
+// get_elem_list (process.cpp:1050-1099): the elements a command selects.
+// type empty: element `index`; ALL: every element; no famname: every
+// element of the type, or its index-th (index > 0); "[i j ...]": the family
+// members i, j, ... (1-based, kept only below the family size and written in
+// place, as the reference does); else the family, or its index-th member
+// (the reference's findelements leaves that result unset; the intended
+// member is taken).
+std::vector<int32_t> elem_list(const Line &L, const string &type, const string &fam, unsigned index,
+                               const string &indices) {
+  if (type.empty()) return {(int32_t)index};
+  if (ieq(type, "ALL")) {
+    std::vector<int32_t> v(L.e.size());
+    for (size_t i = 0; i < v.size(); ++i) v[i] = (int32_t)i;
+    return v;
+  }
+  if (fam.empty()) {
+    std::vector<int32_t> v = L.find(type, "");
+    if (index > 0) v = index <= v.size() ? std::vector<int32_t>{v[index - 1]} : std::vector<int32_t>{};
+    return v;
+  }
+  if (!indices.empty()) {
+    std::vector<int32_t> v = L.find(type, fam);
+    std::istringstream is(indices);
+    int t, i = 0;
+    while (is >> t)
+      if (t > 0 && t < (int)v.size()) v[i++] = v[t - 1];
+    v.resize(i);
+    return v;
+  }
+  return L.find(type, fam, index);
+}
+
+// print_m66_to_line (process.cpp:1397-1425): "oneline" (36 values on one
+// line), "4by4" (the transverse block) or "6by6"; every value followed by a
+// tab.
+void print_m66(const double *m, const string &style) {
+  if (ieq(style, "oneline")) {
+    for (int i = 0; i < 36; ++i) std::cout << fmt(m[i]) << "\t";
+    std::cout << std::endl;
+    return;
+  }
+  const int n = ieq(style, "4by4") ? 4 : ieq(style, "6by6") ? 6 : 0;
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) std::cout << fmt(m[6 * i + j]) << "\t";
+    std::cout << std::endl;
+  }
+}
+
+// "index=[i j ...]" of a command: the list inside the brackets (commas as
+// blanks), or "" when index is a single number or absent.
+string index_list(const string &line) {
+  const string up = upper(line);
+  size_t p = up.find("INDEX");
+  if (p == string::npos) return "";
+  p = line.find('=', p);
+  if (p == string::npos) return "";
+  size_t a = line.find_first_not_of(" \t", p + 1);
+  if (a == string::npos || line[a] != '[') return "";
+  size_t b = line.find(']', a);
+  string v = line.substr(a + 1, b == string::npos ? string::npos : b - a - 1);
+  std::replace(v.begin(), v.end(), ',', ' ');
+  return trim(v);
+}
+
+// The line with its bracketed index list replaced by 0 (so key=value
+// parsing sees "index=0"; the list itself comes from index_list).
+string strip_index_list(const string &line) {
+  size_t a = line.find('['), b = line.find(']');
+  if (a == string::npos || b == string::npos || b < a) return line;
+  return line.substr(0, a) + "0" + line.substr(b + 1);
+}
+
 // ---- Optics block (process.cpp:1107-1395): tunes, chromaticities, one-turn matrix
 void proc_optics(std::istream &in, Line &L) {
   const std::vector<string> keys = {"PRINT_MATRIX", "PRINT_TUNE", "PRINT_2ND_CHROM", "PRINT_CHROM",
@@ -853,21 +926,21 @@ void proc_optics(std::istream &in, Line &L) {
     double dpp = 0, delta = ACCP_DELTA_DIFF;
     string type, fam;
     unsigned index = 0;
-    for (const auto &kv : key_values(line)) {
+    string indices, style = "6by6";
+    const string bracketed = index_list(line);
+    for (const auto &kv : key_values(bracketed.empty() ? line : strip_index_list(line))) {
       if (kv.first == "DPP") dpp = std::stod(kv.second);
       else if (kv.first == "DELTA") delta = std::stod(kv.second);
       else if (kv.first == "TYPE") type = kv.second;
       else if (kv.first == "FAMNAME") fam = kv.second;
       else if (kv.first == "INDEX") index = (unsigned)std::stoul(kv.second);
+      else if (kv.first == "LINE_STYLE") style = kv.second;
     }
+    indices = bracketed;
     if (L.parse_only) continue;
     if (k == 4) {
       // PRINT_OPTICS (process.cpp:1122-1226): Twiss at the selected elements
-      std::vector<int32_t> list;
-      if (type.empty()) list = {(int32_t)index};
-      else if (ieq(type, "ALL"))
-        for (size_t i = 0; i < L.e.size(); ++i) list.push_back((int32_t)i);
-      else list = L.find(type, fam, index);
+      std::vector<int32_t> list = elem_list(L, type, fam, index, indices);
       std::cout << "nlist=" << list.size() << std::endl;
       if (list.empty()) continue;
       std::vector<double> tw(7 * list.size());
@@ -892,16 +965,36 @@ void proc_optics(std::istream &in, Line &L) {
       continue;
     }
     if (k == 0) {
+      // PRINT_MATRIX (process.cpp:1228-1325): the one-turn matrix, or the
+      // matrices from the ring start to each selected element
       double m[36];
-      check(accp_getm66(L.device(), 0, -1, dpp, delta, m), "accp_getm66");
-      std::cout << "$$$OPTICS-MATRIX*******************ONE-TURN" << std::endl;
-      std::cout << "type=RING at " << Line::type_name(L.e[0].type) << ", famname=" << L.e[0].famname
-                << ", ring index=0, spos=" << fmt(L.length()) << std::endl;
-      for (int i = 0; i < 6; ++i) {
-        for (int j = 0; j < 6; ++j) std::cout << fmt(m[i * 6 + j]) << (j < 5 ? "\t" : "");
-        std::cout << std::endl;
+      if (ieq(type, "RING") || ieq(type, "ONETURN")) {
+        check(accp_getm66(L.device(), 0, -1, dpp, delta, m), "accp_getm66");
+        std::cout << "$$$OPTICS-MATRIX*******************ONE-TURN" << std::endl;
+        std::cout << "type=RING at " << Line::type_name(L.e[0].type) << ", famname=" << L.e[0].famname
+                  << ", ring index=0, spos=" << fmt(L.length()) << std::endl;
+        print_m66(m, style);
+        std::cout << "****************************" << std::endl;
+      } else {
+        const std::vector<int32_t> list = elem_list(L, type, fam, index, indices);
+        if (list.empty()) {
+          std::cout << "#no element found for matrix print out, type=" << type << ", famname=" << fam << ",index"
+                    << index << std::endl;
+          continue;
+        }
+        std::cout << "$$$OPTICS-MATRIX, dpp=" << fmt(dpp) << "******************* " << list.size() << " points"
+                  << std::endl;
+        for (int32_t e : list) {
+          if (e < 0 || e > (int)L.e.size()) throw std::runtime_error("PRINT_MATRIX: ring index out of range");
+          check(accp_getm66(L.device(), 0, e, dpp, delta, m), "accp_getm66");
+          double spos = 0;
+          for (int i = 0; i < e; ++i) spos += L.e[i].len;
+          std::cout << "type=" << type << ", famname=" << fam << ", ring index=" << e << ", spos=" << fmt(spos)
+                    << std::endl;
+          print_m66(m, style);
+        }
+        std::cout << "**********************" << std::endl;
       }
-      std::cout << "****************************" << std::endl;
     } else if (k == 1) {
       double t[2];
       check(accp_calctune(L.device(), dpp, delta, t), "accp_calctune");
@@ -1173,38 +1266,6 @@ void do_track(Line &L, std::vector<double> &r, const Monitor &mon, const string 
     }
   }
   std::cout << "*************************************" << std::endl;
-}
-
-// get_elem_list (process.cpp:1050-1099): the watch points of a command.
-// type empty: element `index`; ALL: every element; no famname: every
-// element of the type, or its index-th (index > 0); "[i j ...]": the family
-// members i, j, ... (1-based, kept only below the family size and written in
-// place, as the reference does); else the family, or its index-th member
-// (the reference's findelements leaves that result unset; the intended
-// member is taken).
-std::vector<int32_t> elem_list(const Line &L, const string &type, const string &fam, unsigned index,
-                               const string &indices) {
-  if (type.empty()) return {(int32_t)index};
-  if (ieq(type, "ALL")) {
-    std::vector<int32_t> v(L.e.size());
-    for (size_t i = 0; i < v.size(); ++i) v[i] = (int32_t)i;
-    return v;
-  }
-  if (fam.empty()) {
-    std::vector<int32_t> v = L.find(type, "");
-    if (index > 0) v = index <= v.size() ? std::vector<int32_t>{v[index - 1]} : std::vector<int32_t>{};
-    return v;
-  }
-  if (!indices.empty()) {
-    std::vector<int32_t> v = L.find(type, fam);
-    std::istringstream is(indices);
-    int t, i = 0;
-    while (is >> t)
-      if (t > 0 && t < (int)v.size()) v[i++] = v[t - 1];
-    v.resize(i);
-    return v;
-  }
-  return L.find(type, fam, index);
 }
 
 // TRANSPORT: type=, famname=, index=n|[i j ...] (process.cpp:2108-2214):

@@ -310,3 +310,33 @@ End
         r = oracle.track(arr, r, 1, st=start, ed=w)
         assert_coords_close(got[:, 1:], r, 1e-12, what=f"transport to {w}")
         start = w
+
+
+def test_print_matrix_selections_and_styles(tmp_path, oracle):
+    """PRINT_MATRIX (process.cpp:1228-1325, print_m66_to_line :1397-1425):
+    the one-turn matrix in "oneline" style and the matrices from the ring
+    start to QF members 1, 2 and 3 ("[1,2,3]") in "4by4" style, against the
+    oracle's getm66 over the same element ranges."""
+    ring = lat.fodo_ring()
+    el.write_lattice(ring, str(tmp_path / "fodo.lat"))
+    out = run(tmp_path, """Lattice
+  file_name = fodo.lat
+End
+Optics
+  print_matrix: type=RING, line_style=oneline
+  print_matrix: type=Quad, famname=QF, index=[1,2,3], line_style=4by4
+End
+""")
+    i = out.index("$$$OPTICS-MATRIX*******************ONE-TURN")
+    one = np.array(numbers(out[i + 2]))
+    assert one.shape == (36,)
+    assert np.allclose(one.reshape(6, 6), oracle.getm66(ring), rtol=1e-9, atol=1e-12)
+    qf = lat.family(ring, "QF")
+    j = next(k for k, s in enumerate(out) if s.startswith("$$$OPTICS-MATRIX, dpp=0"))
+    assert out[j].endswith(" 3 points")
+    for n, e in enumerate(qf[:3]):
+        head = out[j + 1 + 5 * n]
+        assert head.startswith(f"type=Quad, famname=QF, ring index={e},")
+        m4 = np.array([numbers(s) for s in out[j + 2 + 5 * n:j + 6 + 5 * n]])
+        assert m4.shape == (4, 4)
+        assert np.allclose(m4, oracle.getm66(ring, st=0, ed=e)[:4, :4], rtol=1e-9, atol=1e-12), e
