@@ -17,6 +17,31 @@
 
 namespace accp {
 
+// Two standard normals for (seed, particle, turn, element, pair): two 53-bit
+// uniforms in (0,1) from the four Philox words, Box-Muller with both
+// branches (the angle through sincospi: no range reduction on [0, 2)).
+__device__ inline void gauss_pair(uint64_t seed, uint64_t pid, uint64_t turn, uint32_t eidx, uint32_t pair,
+                                  double &g0, double &g1) {
+  uint32_t c[4] = {(uint32_t)pid, (uint32_t)turn, eidx, pair};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const uint64_t k1 = ((uint64_t)(c[0] >> 5) << 26) | (uint64_t)(c[1] >> 6);
+  const uint64_t k2 = ((uint64_t)(c[2] >> 5) << 26) | (uint64_t)(c[3] >> 6);
+  const double u1 = ((double)k1 + 0.5) * (1.0 / 9007199254740992.0);
+  const double u2 = ((double)k2 + 0.5) * (1.0 / 9007199254740992.0);
+  const double rr = sqrt(-2.0 * log(u1));
+  double sn, cs;
+  sincospi(2.0 * u2, &sn, &cs);
+  g0 = rr * cs;
+  g1 = rr * sn;
+}
+
+// The normal of slice s: pair s/2, cosine branch for even s, sine for odd.
+__device__ inline double gauss_kick(uint64_t seed, uint64_t pid, uint64_t turn, uint32_t eidx, uint32_t slice) {
+  double g0, g1;
+  gauss_pair(seed, pid, turn, eidx, slice >> 1, g0, g1);
+  return (slice & 1) ? g1 : g0;
+}
+
 // ABendMPole::pass (accel.cpp:649-705): edge, nslice slices of the 4th-order
 // integrator (radiation off or classical, flag <= 1) or of the 2nd-order one
 // with quantum excitation (flag 2, accel.cpp:693-700), edge.
@@ -36,12 +61,16 @@ __device__ __forceinline__ void bendmp(Part (&p)[PPL], const Hdr h, const PT *P,
   if (h.op == OP_BENDMP_QE) {
     const double sl = P[BM_SL], crad = P[BM_CRAD], cq = P[BM_CQ];
     const double half = 0.5 * sl, hs = P[BM_HSH], hq = P[BM_HQH];
+    // slices 2j and 2j+1 take the two normals of one Philox block
+    double godd[PPL];
     for (int m = 0; m < ns; ++m) {
 #pragma unroll
       for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
-        const double g = gauss_kick(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, (uint32_t)m);
+        double g;
+        if ((m & 1) == 0) gauss_pair(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, (uint32_t)(m >> 1), g, godd[k]);
+        else g = godd[k];
         bend_kick_rad(p[k], B, mo, sl, h0, k1c, k2c, crad, cq, 2, g);
       }
 #pragma unroll

@@ -603,17 +603,4 @@ ACCP_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
   }
 }
 
-// Standard normal for (seed, particle, turn, element, slice): two 53-bit
-// uniforms in (0,1) from the four Philox words, Box-Muller cosine branch.
-ACCP_HD double gauss_kick(uint64_t seed, uint64_t pid, uint64_t turn, uint32_t eidx,
-                          uint32_t slice) {
-  uint32_t c[4] = {(uint32_t)pid, (uint32_t)turn, eidx, slice};
-  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-  const uint64_t k1 = ((uint64_t)(c[0] >> 5) << 26) | (uint64_t)(c[1] >> 6);
-  const uint64_t k2 = ((uint64_t)(c[2] >> 5) << 26) | (uint64_t)(c[3] >> 6);
-  const double u1 = ((double)k1 + 0.5) * (1.0 / 9007199254740992.0);
-  const double u2 = ((double)k2 + 0.5) * (1.0 / 9007199254740992.0);
-  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
-}
-
 }  // namespace accp

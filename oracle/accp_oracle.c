@@ -267,13 +267,14 @@ static double u53(uint32_t a, uint32_t b) {
 }
 
 double orc_gauss(uint64_t seed, uint64_t pid, uint64_t turn, uint32_t eidx, uint32_t slice) {
-  uint32_t ctr[4] = {(uint32_t)pid, (uint32_t)turn, eidx, slice};
+  uint32_t ctr[4] = {(uint32_t)pid, (uint32_t)turn, eidx, slice >> 1};
   uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   uint32_t w[4];
   orc_philox4x32_10(ctr, key, w);
   double u1 = u53(w[0], w[1]);
   double u2 = u53(w[2], w[3]);
-  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+  const double a = ORC_PI * (2.0 * u2);
+  return sqrt(-2.0 * log(u1)) * ((slice & 1) ? sin(a) : cos(a));
 }
 
 /* ---- longitudinal resonator wake ----------------------------------------- */

@@ -38,8 +38,10 @@ void orc_fastdrift(double *r, double norm_l);                         /* :416-42
 /* ---- counter-based Gaussian stream (quantum excitation) ----------------- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* Standard normal for one radiation kick.  Counter = (particle id, turn,
- * element index, slice), key = seed; Philox4x32-10 -> two 53-bit uniforms
- * in (0,1) -> Box-Muller cosine branch. */
+ * element index, slice / 2), key = seed; Philox4x32-10 -> two 53-bit
+ * uniforms u1, u2 in (0,1) -> Box-Muller, sqrt(-2 ln u1) times cos(2 pi u2)
+ * for an even slice and sin(2 pi u2) for an odd one (two slices share one
+ * Philox block). */
 double orc_gauss(uint64_t seed, uint64_t pid, uint64_t turn, uint32_t eidx,
                  uint32_t slice);
 

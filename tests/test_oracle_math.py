@@ -25,6 +25,14 @@ def test_gauss_stream_moments(oracle):
     assert abs(np.mean(g ** 4) - 3) < 0.1
     assert oracle.gauss(5, 1, 3, 7, 1) != oracle.gauss(5, 1, 3, 7, 2)
     assert oracle.gauss(5, 1, 3, 7, 1) == oracle.gauss(5, 1, 3, 7, 1)
+    # slices 2j and 2j+1 are the two Box-Muller branches of one Philox block:
+    # uncorrelated, each standard normal, and r^2 = g0^2 + g1^2 is
+    # exponential with mean 2
+    g0 = np.array([oracle.gauss(5, pid, 3, 7, 0) for pid in range(40000)])
+    assert abs(g0.mean()) < 0.02 and abs(g0.std() - 1) < 0.02
+    assert abs(np.corrcoef(g0, np.array([oracle.gauss(5, pid, 3, 7, 1) for pid in range(40000)]))[0, 1]) < 0.02
+    r2 = g0 ** 2 + np.array([oracle.gauss(5, pid, 3, 7, 1) for pid in range(40000)]) ** 2
+    assert abs(r2.mean() - 2) < 0.05 and abs(np.mean(r2 > 2 * np.log(2)) - 0.5) < 0.01
 
 
 def test_one_turn_matrix_is_symplectic(oracle):
