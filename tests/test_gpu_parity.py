@@ -283,3 +283,38 @@ def test_wigtable(gpu_lib, oracle):
             assert np.array_equal(a, b)
     assert_coords_close(gpu_lib.lattice([arr[i] for i in range(len(ring))]).track(beam, 5),
                         oracle.track(ring, beam, 5), 1e-11, what="rebuilt from the returned records")
+
+
+def test_da_scan_at_full_size(gpu_lib, oracle):
+    """The DA scan at its full size (101 x 51 x 5 = 25755 points on the
+    seed-0 lattice with errors): the survival map after 256 turns against the
+    oracle (identical except next to the survival boundary), then all 2048
+    turns on the GPU, where two size-independent properties hold: the
+    survivors of 2048 turns are a subset of those of 256 turns, and the
+    batched path (one seed, captured graph) gives the plain path's map and
+    coordinates bit for bit."""
+    import pyaccp.abi as abi
+    from pyaccp.hipmem import DeviceArray, synchronize
+
+    da = lat.da_ring(0)
+    grid = lat.da_grid(101, 51)
+    L = gpu_lib.lattice(da)
+    g256 = L.track(grid, 256)
+    o256 = oracle.track_omp(da, grid, 256, 16)
+    sg = ~np.isnan(g256[:, 0]).reshape(5, 51, 101)
+    so = ~np.isnan(o256[:, 0]).reshape(5, 51, 101)
+    print("survivors after 256 turns: gpu", sg.sum(), "oracle", so.sum(), "differ", int((sg != so).sum()))
+    for d, iy, ix in np.argwhere(sg != so):
+        nb = so[d, max(iy - 1, 0):iy + 2, max(ix - 1, 0):ix + 2]
+        assert nb.any() and not nb.all(), f"survival differs away from the boundary at {(d, iy, ix)}"
+    g2048 = L.track(grid, 2048)
+    s2048 = ~np.isnan(g2048[:, 0]).reshape(5, 51, 101)
+    assert not (s2048 & ~sg).any() and s2048.sum() > 0
+    B = abi.Batch(gpu_lib, [L])
+    d = DeviceArray(grid)
+    B.capture(d.ptr, np.array([0, len(grid)], dtype=np.int64), 2048)
+    B.replay()
+    synchronize()
+    out = d.numpy()
+    assert np.array_equal(np.isnan(out), np.isnan(g2048))
+    assert np.array_equal(np.nan_to_num(out), np.nan_to_num(g2048))
