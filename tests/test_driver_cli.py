@@ -32,14 +32,18 @@ Lattice   % the first block
 End
 Optics
   print_tune
+  print_matrix: type=Quad, famname=QF, index=[1,2], line_style=oneline
+  print_optics: type=Quad, famname=QD
 End
 Track
   Distribution: Zero NPar=4
   Monitor: interval=10, printflag=Stat2
   Do_Track: NTurn=100, radiation=off, RFCavity=on
+  Transport: type=Quad, famname=QF, index=[1,3]
 End
 DAMA
   TRACK_DA: xmin=0.001, xmax=0.02, ymax=0.01, nray=11, nstep=10, nturn=64
+  TRACK_DNUDJ: xmax=0.004, ymax=0.003, nturn=256
 End
 STOP
 Lattice
@@ -59,6 +63,8 @@ def test_parse_only_reads_every_block(tmp_path):
     assert "not carried" not in p.stderr
     for block in ("OPTICS", "TRACK", "DAMA"):
         assert f"\n{block}\n" in out
+    assert "Quad QF 1 3 " in out  # the transport watch list as the reference echoes it
+    assert "#0.004 0.003 256" in out  # TRACK_DNUDJ echo
     assert "#0.001 0.02 0.01 11 10 64 0" in out
 
 
