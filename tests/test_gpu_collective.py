@@ -143,3 +143,21 @@ def test_transwake_ring_and_length(gpu_lib, oracle):
     off = el.transwake(0.6, 2.0, 2.0e5, 1.5, 5.0, 2.0e5, 2.0, 3.0, on=0)
     assert_coords_close(gpu_lib.lattice([off]).track(beam, 1), oracle.track([el.drift(0.6)], beam, 1), 1e-14,
                         what="wake off")
+
+
+def test_transwake_degenerate_beams(gpu_lib, oracle):
+    """TransWake on an all-lost beam (nothing moves), a single particle (no
+    other particle, no kick; only the half drifts) and, for a wake of zero
+    length, particles sharing one z (sin 0 = 0: no kick between them, bit
+    for bit)."""
+    w = el.transwake(0.4, 2.0, 1.0e6, 1.5, 5.0, 2.0e6, 2.0, 3.0, on=1)
+    L = gpu_lib.lattice([w])
+    lost = lat.gaussian_beam(5, seed=3)
+    lost[:, 0] = np.nan
+    assert np.array_equal(np.isnan(L.track(lost, 3)), np.isnan(lost))
+    one = lat.gaussian_beam(1, seed=4)
+    assert_coords_close(L.track(one, 2), oracle.track([el.drift(0.4)], one, 2), 1e-14, what="one particle")
+    same = lat.gaussian_beam(6, seed=5)
+    same[:, 4] = 1e-3
+    thin = el.transwake(0.0, 2.0, 1.0e6, 1.5, 5.0, 2.0e6, 2.0, 3.0, on=1)
+    assert np.array_equal(gpu_lib.lattice([thin]).track(same, 1), same)
