@@ -682,13 +682,14 @@ extern "C" int accp_track_device(accp_lattice *L, double *d_r, int n, int nturn,
   return track_device_impl(L, d_r, n, R, opts ? &opts->tuning : nullptr, (hipStream_t)stream);
 }
 
-int accp::track_record(accp_lattice *L, double *r, int n, int nturn, const double **d_rec) {
+int accp::track_record(accp_lattice *L, double *r, int n, int nturn, const double **d_rec, uint64_t pid0) {
   if (!L || !r || n <= 0 || nturn <= 0 || !d_rec) return fail(ACCP_E_ARG, "track_record: bad arguments");
   const size_t bytes = (size_t)n * 6 * sizeof(double);
   int rc;
   if ((rc = L->d_beam.ensure(bytes)) || (rc = L->d_snap.ensure((size_t)nturn * bytes))) return rc;
   HIPCHK(hipMemcpy(L->d_beam.p, r, bytes, hipMemcpyHostToDevice));
   RunSpec R = spec_from(nullptr, nturn, 0, (int)L->elems.size());
+  R.pid0 = pid0;
   R.snap = (double *)L->d_snap.p;
   R.nsnap = nturn;
   R.mon_interval = 1;

@@ -8,6 +8,7 @@
 // least-squares slopes), built with -ffp-contract=off.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -20,6 +21,10 @@
 namespace {
 
 constexpr double PI = 3.1415926535897932384626;  // accel.h:23
+
+// Cap on the turn-by-turn record of one tracking launch (trackDiffusion
+// tracks its particles in groups below it).
+constexpr int64_t RECORD_BYTES = int64_t(8) << 30;
 
 int fail(int code, const std::string &msg) { return accp::set_error(code, msg.c_str()); }
 
@@ -125,48 +130,56 @@ extern "C" int accp_trackdiffusion(accp_lattice *L, int nturn, int np, const dou
   if (total) *total = 0;
   if (np == 0) return ACCP_OK;
   // aline.cpp:1421-1484: each particle from (x0, 0, y0, 0, 0, 0); the tunes
-  // of the first and second halves of nt positions
-  std::vector<double> r(6 * (size_t)np, 0.0);
-  for (int i = 0; i < np; ++i) {
-    r[6 * (size_t)i] = x0[i];
-    r[6 * (size_t)i + 2] = y0[i];
-  }
-  const double *rec = nullptr;
-  if ((rc = accp::track_record(L, r.data(), np, nt, &rec))) return rc;
-  // lost after nt - 1 turns: x of the last record
-  std::vector<double> last(6 * (size_t)np);
-  if (hipMemcpy(last.data(), rec + (size_t)(nt - 1) * np * 6, sizeof(double) * 6 * np, hipMemcpyDeviceToHost) !=
-      hipSuccess)
-    return fail(ACCP_E_HIP, "accp_trackdiffusion: copy");
-  std::vector<accp::NaffJob> jobs;
-  std::vector<int> live;
-  const int64_t stride = 6 * (int64_t)np;
-  for (int i = 0; i < np; ++i) {
-    if (std::isnan(last[6 * (size_t)i])) continue;
-    live.push_back(i);
-    for (int half = 0; half < 2; ++half) {
-      const int64_t b = ((int64_t)half * h * np + i) * 6;
-      jobs.push_back(job(b, stride, h, xlow, xhi));
-      jobs.push_back(job(b + 2, stride, h, ylow, yhi));
-    }
-  }
-  std::vector<double> out;
-  if ((rc = run_jobs(rec, jobs, out))) return rc;
+  // of the first and second halves of nt positions.  Particles go in groups
+  // whose turn-by-turn record (48 B per particle and turn) stays below
+  // RECORD_BYTES; the results do not depend on the grouping.
+  const int group = (int)std::max<int64_t>(1, std::min<int64_t>(np, RECORD_BYTES / (48 * (int64_t)nt)));
   const double mdiffu = -4 * std::log(nt * 1.0);
   double tdif = 0;
-  size_t k = 0;
-  for (int i = 0; i < np; ++i) {
-    if (k < live.size() && live[k] == i) {
-      const double nux0 = out[4 * k], nuy0 = out[4 * k + 1], nux1 = out[4 * k + 2], nuy1 = out[4 * k + 3];
-      nux[i] = nux0;
-      nuy[i] = nuy0;
-      const double d = std::log(std::sqrt((nux1 - nux0) * (nux1 - nux0) + (nuy1 - nuy0) * (nuy1 - nuy0)) * 2.0 / nt);
-      dif[i] = mdiffu > d ? mdiffu : d;
-      tdif += dif[i];
-      ++k;
-    } else {
-      nux[i] = nuy[i] = dif[i] = NAN;
-      tdif += -3.0;
+  for (int g0 = 0; g0 < np; g0 += group) {
+    const int n = std::min(group, np - g0);
+    std::vector<double> r(6 * (size_t)n, 0.0);
+    for (int i = 0; i < n; ++i) {
+      r[6 * (size_t)i] = x0[g0 + i];
+      r[6 * (size_t)i + 2] = y0[g0 + i];
+    }
+    const double *rec = nullptr;
+    if ((rc = accp::track_record(L, r.data(), n, nt, &rec, (uint64_t)g0))) return rc;
+    // lost after nt - 1 turns: x of the last record
+    std::vector<double> last(6 * (size_t)n);
+    if (hipMemcpy(last.data(), rec + (size_t)(nt - 1) * n * 6, sizeof(double) * 6 * n, hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      return fail(ACCP_E_HIP, "accp_trackdiffusion: copy");
+    std::vector<accp::NaffJob> jobs;
+    std::vector<int> live;
+    const int64_t stride = 6 * (int64_t)n;
+    for (int i = 0; i < n; ++i) {
+      if (std::isnan(last[6 * (size_t)i])) continue;
+      live.push_back(i);
+      for (int half = 0; half < 2; ++half) {
+        const int64_t b = ((int64_t)half * h * n + i) * 6;
+        jobs.push_back(job(b, stride, h, xlow, xhi));
+        jobs.push_back(job(b + 2, stride, h, ylow, yhi));
+      }
+    }
+    std::vector<double> out;
+    if ((rc = run_jobs(rec, jobs, out))) return rc;
+    size_t k = 0;
+    for (int i = 0; i < n; ++i) {
+      const int ip = g0 + i;
+      if (k < live.size() && live[k] == i) {
+        const double nux0 = out[4 * k], nuy0 = out[4 * k + 1], nux1 = out[4 * k + 2], nuy1 = out[4 * k + 3];
+        nux[ip] = nux0;
+        nuy[ip] = nuy0;
+        const double d =
+            std::log(std::sqrt((nux1 - nux0) * (nux1 - nux0) + (nuy1 - nuy0) * (nuy1 - nuy0)) * 2.0 / nt);
+        dif[ip] = mdiffu > d ? mdiffu : d;
+        tdif += dif[ip];
+        ++k;
+      } else {
+        nux[ip] = nuy[ip] = dif[ip] = NAN;
+        tdif += -3.0;
+      }
     }
   }
   if (total) *total = tdif;
