@@ -148,23 +148,45 @@ ACCP_HD void hxdrift_c_rev(Part &p, Adj &l, const HxC &c, double h, double c4) {
   p.z = z;
 }
 
-// ---- bndthinkick, normal multipoles only, delta fixed (bend_kick_c<MO,true>) ----
-template <int MO>
+// ---- bndthinkick, normal multipoles only, delta fixed (bend_kick_c<MO,PLAIN>) ----
+// PLAIN: PolyB[1] = PolyB[2] = 0, so the curvature terms vanish.  Otherwise
+// (atpass.cpp:231-233) the px decrement F gains h (k1c (x^2 - y^2/2) +
+// k2c (x^3 - 4/3 x y^2)) and the py increment G gains h (k1c x y +
+// 4/3 k2c x^2 y + c3 y^3), c3 = h/6 k1c - k2c/3; their x and y derivatives
+// join those of the multipole sum (Cauchy-Riemann: d re/dy = -Im P',
+// d im/dx = Im P').
+template <int MO, bool PLAIN>
 ACCP_HD void bend_kick_rev(Part &p, Adj &l, const double (&b)[MO + 1], double L, double h,
-                           double lhd, double lh) {
+                           double lhd, double lh, double k1c = 0.0, double k2c = 0.0) {
   const double x = p.x, y = p.y;
   double a0[MO + 1];
 #pragma unroll
   for (int i = 0; i <= MO; ++i) a0[i] = 0.0;
   double re, im, dre, dim;
   horner_d<MO>(b, a0, x, y, re, im, dre, dim);
-  p.px += L * (re + h * (h * x)) - lhd;
-  if (MO > 0) p.py -= L * im;
-  p.z += lh * x;
-  const double lpx = l.px, lpy = MO > 0 ? l.py : 0.0;
-  l.x += -lpx * L * (dre + h * h) + lpy * L * dim - l.z * lh;
-  if (MO > 0) l.y += lpx * L * dim + lpy * L * dre;
-  l.d += lpx * lh;
+  if (PLAIN) {
+    p.px += L * (re + h * (h * x)) - lhd;
+    if (MO > 0) p.py -= L * im;
+    p.z += lh * x;
+    const double lpx = l.px, lpy = MO > 0 ? l.py : 0.0;
+    l.x += -lpx * L * (dre + h * h) + lpy * L * dim - l.z * lh;
+    if (MO > 0) l.y += lpx * L * dim + lpy * L * dre;
+    l.d += lpx * lh;
+  } else {
+    const double c3 = h / 6.0 * k1c - k2c / 3.0;
+    p.px += L * (re + h * (h * x + k1c * (x * x - 0.5 * y * y) + k2c * (x * x * x - (4.0 / 3.0) * x * y * y))) -
+            lhd;
+    p.py -= L * (im + h * (k1c * x * y + (4.0 / 3.0) * k2c * x * x * y + c3 * y * y * y));
+    p.z += lh * x;
+    const double fx = dre + h * (h + 2.0 * k1c * x + k2c * (3.0 * x * x - (4.0 / 3.0) * y * y));
+    const double fy = -dim - h * (k1c * y + (8.0 / 3.0) * k2c * x * y);
+    const double gx = dim + h * (k1c * y + (8.0 / 3.0) * k2c * x * y);
+    const double gy = dre + h * (k1c * x + (4.0 / 3.0) * k2c * x * x + 3.0 * c3 * y * y);
+    const double lpx = l.px, lpy = l.py;
+    l.x += L * (-lpx * fx + lpy * gx) - l.z * lh;
+    l.y += L * (-lpx * fy + lpy * gy);
+    l.d += lpx * lh;
+  }
 }
 
 // ---- quad6 from the power series, with the K derivative ----------------------

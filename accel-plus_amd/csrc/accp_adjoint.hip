@@ -14,26 +14,30 @@
 
 namespace accp {
 
-template <int MO, class PT>
+// BendMPole without radiation, PolyB[0..MO] in registers.  The record holds
+// PolyB[0..mo] (mo = MaxNPole - 1 <= MO); the orders above it are zeros,
+// which add exact zeros to the Horner sums.
+template <int MO, bool PLAIN, class PT>
 __device__ __forceinline__ void bendmp_rev(Part &p, Adj &l, const Hdr h, const PT *P) {
-  const int ns = h.a;
+  const int ns = h.a, mo = h.b & 0xff;
   const double psi1 = P[BM_PSI1], psi2 = P[BM_PSI2], h0 = P[BM_IRHO];
   const double l1 = P[BM_L1], l2 = P[BM_L2], k1 = P[BM_K1], k2 = P[BM_K2];
   const double hs1 = P[BM_HS1], hq1 = P[BM_HQ1], hs2 = P[BM_HS2], hq2 = P[BM_HQ2];
+  const double k1c = P[BM_K1C], k2c = P[BM_K2C];
   double b[MO + 1];
 #pragma unroll
-  for (int i = 0; i <= MO; ++i) b[i] = P[BM_B + i];
+  for (int i = 0; i <= MO; ++i) b[i] = i <= mo ? (double)P[BM_B + i] : 0.0;
   const double c4a = hs1 * 0.5, c4b = hs2 * 0.5, lh1 = k1 * h0, lh2 = k2 * h0;
   if (psi2 != 0) edge_rev(p, l, psi2);
   const HxC ca = hx_consts(p.ipd, l1, hs1, hq1), cb = hx_consts(p.ipd, l2, hs2, hq2);
   const double lhd1 = lh1 * p.d, lhd2 = lh2 * p.d;
   for (int m = 0; m < ns; ++m) {
     hxdrift_c_rev(p, l, ca, h0, c4a);
-    bend_kick_rev<MO>(p, l, b, k1, h0, lhd1, lh1);
+    bend_kick_rev<MO, PLAIN>(p, l, b, k1, h0, lhd1, lh1, k1c, k2c);
     hxdrift_c_rev(p, l, cb, h0, c4b);
-    bend_kick_rev<MO>(p, l, b, k2, h0, lhd2, lh2);
+    bend_kick_rev<MO, PLAIN>(p, l, b, k2, h0, lhd2, lh2, k1c, k2c);
     hxdrift_c_rev(p, l, cb, h0, c4b);
-    bend_kick_rev<MO>(p, l, b, k1, h0, lhd1, lh1);
+    bend_kick_rev<MO, PLAIN>(p, l, b, k1, h0, lhd1, lh1, k1c, k2c);
     hxdrift_c_rev(p, l, ca, h0, c4a);
   }
   if (psi1 != 0) edge_rev(p, l, psi1);
@@ -131,14 +135,18 @@ __device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const 
       }
       break;
     }
-    case OP_BENDMP:
+    case OP_BENDMP: {
+      // plain: both curvature coefficients zero (always so for MaxNPole = 1)
+      const bool plain = (h.b >> 8) & 1;
       switch (h.b & 0xff) {
-        case 0: bendmp_rev<0>(p, l, h, P); break;
-        case 1: bendmp_rev<1>(p, l, h, P); break;
-        case 2: bendmp_rev<2>(p, l, h, P); break;
-        default: bendmp_rev<3>(p, l, h, P); break;
+        case 0: bendmp_rev<0, true>(p, l, h, P); break;
+        case 1: plain ? bendmp_rev<1, true>(p, l, h, P) : bendmp_rev<1, false>(p, l, h, P); break;
+        case 2: plain ? bendmp_rev<2, true>(p, l, h, P) : bendmp_rev<2, false>(p, l, h, P); break;
+        case 3: plain ? bendmp_rev<3, true>(p, l, h, P) : bendmp_rev<3, false>(p, l, h, P); break;
+        default: plain ? bendmp_rev<9, true>(p, l, h, P) : bendmp_rev<9, false>(p, l, h, P); break;
       }
       break;
+    }
     case OP_STRMP:
       strmp_rev_dispatch<true>(p, l, h, P, kn, grad);
       break;

@@ -971,12 +971,10 @@ struct accp_adjoint {
 static bool adjoint_supports(const Hdr &h) {
   switch (h.op) {
     case OP_NOP: case OP_DRIFT: case OP_STRMP: case OP_STRMP_THIN:
-    case OP_CAVITY: case OP_APERTURE: case OP_CORRECTOR:
+    case OP_CAVITY: case OP_APERTURE: case OP_CORRECTOR: case OP_BENDMP:
       return true;
     case OP_QUAD:
       return h.a > 0;
-    case OP_BENDMP:
-      return ((h.b >> 8) & 1) && (h.b & 0xff) <= 3;
     default:
       return false;
   }
@@ -993,7 +991,8 @@ extern "C" int accp_adjoint_create(accp_lattice *L, int nparam, const accp_param
     if (!adjoint_supports(L->tab.hdr[e]))
       return fail(ACCP_E_LATTICE,
                   "element %d (type %d): no reverse map (radiation, wake, linear Bend, K=0 or "
-                  "out-of-range Quad, combined-function or high-order BendMPole)",
+                  "out-of-range Quad, or a type other than Drift, Quad, BendMPole, StrMPole, "
+                  "Cavity, Aperture, Corrector, Monitor)",
                   e, L->elems[e].type);
   std::vector<int32_t> slot(ne, -1), kind(ne, 0);
   for (int k = 0; k < nparam; ++k) {

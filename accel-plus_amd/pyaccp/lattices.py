@@ -48,6 +48,14 @@ def fodo_ring(n_cells=N_CELLS, kqf=1.2, kqd=-1.1, sf=20.0, sd=-25.0, bend_type="
     return out
 
 
+def combined_function_ring(gradient=-0.1, sext=2.0):
+    """The FODO ring with combined-function bends: BendMPole with PolyB =
+    [0, gradient, sext], so the curvature terms of PolyB[1] and PolyB[2]
+    enter every kick (at the defaults cos mu = -0.13 and 0.50)."""
+    return [el.bendmpole(1.5, e.theta, polyB=[0.0, gradient, sext], name="B") if e.type == el.BENDMPOLE else e
+            for e in fodo_ring()]
+
+
 def _error_kick(rng, strength, is_quad):
     b = rng.normal(0.0, ERR_SIGMA, 8)
     a = rng.normal(0.0, ERR_SIGMA, 8)

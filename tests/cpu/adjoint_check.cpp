@@ -111,10 +111,32 @@ int main() {
       const double h = 0.157 / 1.5, L = 0.15 * KICK1, lh = L * h;
       const double b0[1] = {0.0};
       check("bend kick MO=0", [=](Part &p, double) { bend_kick_c<0, true>(p, b0, L, h, lh * p.d, lh, 0, 0); },
-            [=](Part &p, Adj &l, double *) { bend_kick_rev<0>(p, l, b0, L, h, lh * p.d, lh); }, 0, false, s0);
+            [=](Part &p, Adj &l, double *) { bend_kick_rev<0, true>(p, l, b0, L, h, lh * p.d, lh); }, 0, false,
+            s0);
       const double b3[4] = {1e-3, 0.0, 5.0, 20.0};
       check("bend kick MO=3", [=](Part &p, double) { bend_kick_c<3, true>(p, b3, L, h, lh * p.d, lh, 0, 0); },
-            [=](Part &p, Adj &l, double *) { bend_kick_rev<3>(p, l, b3, L, h, lh * p.d, lh); }, 0, false, s0);
+            [=](Part &p, Adj &l, double *) { bend_kick_rev<3, true>(p, l, b3, L, h, lh * p.d, lh); }, 0, false,
+            s0);
+      // combined-function bends: the curvature terms of PolyB[1] and PolyB[2],
+      // at centimetre amplitudes where the cubic terms carry weight
+      const double s1[6] = {U(-2e-2, 2e-2), U(-1e-3, 1e-3), U(-1.5e-2, 1.5e-2), U(-1e-3, 1e-3), s0[4], d};
+      const double g1[2] = {0.0, -0.35};
+      check("bend kick gradient", [=](Part &p, double) { bend_kick_c<1, false>(p, g1, L, h, lh * p.d, lh, -0.35, 0); },
+            [=](Part &p, Adj &l, double *) { bend_kick_rev<1, false>(p, l, g1, L, h, lh * p.d, lh, -0.35, 0); }, 0,
+            false, s1);
+      const double g3[4] = {2e-4, 0.4, -6.0, 25.0};
+      check("bend kick grad+sext MO=3",
+            [=](Part &p, double) { bend_kick_c<3, false>(p, g3, L, h, lh * p.d, lh, 0.4, -6.0); },
+            [=](Part &p, Adj &l, double *) { bend_kick_rev<3, false>(p, l, g3, L, h, lh * p.d, lh, 0.4, -6.0); },
+            0, false, s1);
+      // orders above the record's padded with zeros (the MO = 9 reverse map
+      // serves every MaxNPole above 4)
+      const double g5[6] = {0.0, -0.2, 3.0, 0.0, 150.0, -900.0};
+      double g9[10] = {0.0, -0.2, 3.0, 0.0, 150.0, -900.0, 0.0, 0.0, 0.0, 0.0};
+      check("bend kick MO=5 as MO=9",
+            [=](Part &p, double) { bend_kick_c<5, false>(p, g5, L, h, lh * p.d, lh, -0.2, 3.0); },
+            [=](Part &p, Adj &l, double *) { bend_kick_rev<9, false>(p, l, g9, L, h, lh * p.d, lh, -0.2, 3.0); },
+            0, false, s1);
     }
     {  // multipole kicks with knobs on PolyB[2], PolyB[1], PolyA[3]
       const int kinds[3][2] = {{2, 2}, {2, 1}, {3, 3}};

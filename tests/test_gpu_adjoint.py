@@ -160,3 +160,57 @@ def test_adjoint_jacobian_matches_reference_getm66(gpu_lib):
         err = np.abs(J - ref).max() / np.abs(ref).max()
         print(f"dp={dp}: adjoint Jacobian vs reference getm66 {err:.2e}")
         assert err <= 1e-7
+
+
+def test_combined_function_bends(gpu_lib, oracle):
+    """Adjoint through BendMPole with gradient and sextupole curvature terms:
+    the knob gradients against central differences of the GPU map and of the
+    oracle, the two reverse forms against each other, and six backward
+    passes against the oracle's getm66 (the reference's 12-ray difference)."""
+    ring = lat.combined_function_ring()
+    knobs = lat.adjoint_knobs(ring)
+    assert [k[0] for k in knobs] == ["QF", "QD", "SF", "SD"]
+    # the ring's chromaticity is large: energy offsets of a few 1e-5 keep
+    # every particle through the 12 turns
+    beam = lat.gaussian_beam(64, seed=5) * 0.5
+    beam[:, 5] *= 0.05
+    L = gpu_lib.lattice(ring)
+    A = abi.Adjoint(gpu_lib, L, [(f, i, m) for _, f, i, m in knobs])
+    nturn = 12
+    out = A.forward(beam, nturn, ckpt_every=5)
+    assert np.array_equal(out, L.track(beam, nturn), equal_nan=True)
+    assert not np.isnan(out).any()
+    _, seed = loss_and_seed(out)
+    res = {}
+    for mode in (A.REVERSE_SEQUENTIAL, A.REVERSE_SEGMENT_JACOBIAN):
+        A.set_reverse(mode)
+        res[mode] = A.backward(seed)
+    (grad, dr0), (g2, d2) = res[A.REVERSE_SEQUENTIAL], res[A.REVERSE_SEGMENT_JACOBIAN]
+    assert np.all(np.abs(grad - g2) <= 1e-11 * np.abs(grad)), (grad, g2)
+    assert np.all(np.abs(dr0 - d2) <= 1e-11 * np.abs(dr0).max(axis=0))
+    for k, (name, f, i, m) in enumerate(knobs):
+        for track in (lambda r: gpu_lib.lattice(r).track(beam, nturn), lambda r: oracle.track(r, beam, nturn)):
+            rel, fd = best_fd(grad[k], lambda h: loss_and_seed(track(lat.shift_knob(ring, f, i, m, h)))[0],
+                              fd_steps(ring, f, i, m))
+            print(f"{name}: adjoint {grad[k]:+.12e}  best fd {fd:+.12e}  rel {rel:.1e}")
+            assert rel < 1e-6, name
+    for (pi, c) in ((3, 0), (40, 2), (63, 3)):
+        def loss_at(h):
+            b = beam[pi:pi + 1].copy()
+            b[0, c] += h
+            return loss_and_seed(L.track(b, nturn))[0]
+        steps = [r * max(abs(beam[pi, c]), 1e-6) for r in (1e-7, 1e-6, 1e-5, 1e-4, 1e-3)]
+        rel, fd = best_fd(dr0[pi, c], loss_at, steps)
+        assert rel < 1e-6, (pi, c, dr0[pi, c], fd)
+    A.set_reverse(A.REVERSE_SEQUENTIAL)
+    ref = oracle.getm66(ring)
+    r0 = np.zeros((1, 6))
+    A.forward(r0, 1, ckpt_every=1)
+    J = np.zeros((6, 6))
+    for i in range(6):
+        lam = np.zeros((1, 6))
+        lam[0, i] = 1.0
+        J[i] = A.backward(lam)[1][0]
+    err = np.abs(J - ref).max() / np.abs(ref).max()
+    print(f"adjoint Jacobian vs oracle getm66 {err:.2e}")
+    assert err <= 1e-7

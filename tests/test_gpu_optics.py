@@ -278,3 +278,27 @@ def test_momentum_aperture_matches_reference(gpu_lib):
         assert avg == pytest.approx(g["ma_avg"][0], rel=1e-15)
     with pytest.raises(abi.AccpError, match="location out of range"):
         L.trackma(0.01, 0.0005, 8, 19, [len(arr)])
+
+
+def test_adjoint_responses_combined_function_bends(gpu_lib):
+    """Tune and chromaticity responses by the adjoint on a ring whose bends
+    carry a gradient and a sextupole component (PolyB[1], PolyB[2] curvature
+    terms in every kick), against fourth-order central differences of the
+    GPU's own tunes and chromaticities, as in test_adjoint_responses."""
+    ring = lat.combined_function_ring()
+    qf, qd, sf, sd = fams(ring, "QF", "QD", "SF", "SD")
+    L = gpu_lib.lattice(ring)
+    for quantity, f1, f2, field, idx in ((TUNE, qf, qd, el.FIELD_K, 0), (CHROM, sf, sd, el.FIELD_POLYB, 2)):
+        adj = L.calcresp(quantity, f1, f2, ADJ)
+        measure = L.calctune if quantity == TUNE else L.calcchrom
+        for k, fam in enumerate((f1, f2)):
+            base = ring[fam[0]].k if field == el.FIELD_K else ring[fam[0]].polyB[idx]
+            h = (1e-4 if quantity == TUNE else 1e-2) * abs(base)
+            v = {}
+            for s in (-2, -1, 1, 2):
+                L.update(lat.shift_knob(ring, field, idx, fam, s * h))
+                v[s] = measure()
+            L.update(ring)
+            fd = (-v[2] + 8 * v[1] - 8 * v[-1] + v[-2]) / (12 * h)
+            print(quantity, k, "adjoint", adj[k], "fd", fd)
+            assert np.all(np.abs(adj[k] - fd) <= 1e-6 * np.abs(fd).max())
