@@ -61,20 +61,27 @@ __device__ __forceinline__ void bendmp(Part (&p)[PPL], const Hdr h, const PT *P,
   if (h.op == OP_BENDMP_QE) {
     const double sl = P[BM_SL], crad = P[BM_CRAD], cq = P[BM_CQ];
     const double half = 0.5 * sl, hs = P[BM_HSH], hq = P[BM_HQH];
-    // slices 2j and 2j+1 take the two normals of one Philox block
-    double godd[PPL];
-    for (int m = 0; m < ns; ++m) {
+    // slices 2j and 2j+1 take the two normals of one Philox block, drawn once
+    // per pair of slices (a loop over single slices gets the draw if-converted
+    // into every slice)
+    for (int m = 0; m < ns; m += 2) {
+      double g0[PPL], g1[PPL];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) gauss_pair(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, (uint32_t)(m >> 1), g0[k], g1[k]);
 #pragma unroll
       for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) {
-        double g;
-        if ((m & 1) == 0) gauss_pair(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, (uint32_t)(m >> 1), g, godd[k]);
-        else g = godd[k];
-        bend_kick_rad(p[k], B, mo, sl, h0, k1c, k2c, crad, cq, 2, g);
+      for (int k = 0; k < PPL; ++k) bend_kick_rad(p[k], B, mo, sl, h0, k1c, k2c, crad, cq, 2, g0[k]);
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
+      if (m + 1 < ns) {
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) bend_kick_rad(p[k], B, mo, sl, h0, k1c, k2c, crad, cq, 2, g1[k]);
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
       }
-#pragma unroll
-      for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
     }
   } else {
     const double l1 = P[BM_L1], l2 = P[BM_L2], k1 = P[BM_K1], k2 = P[BM_K2];

@@ -393,6 +393,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--tune", default="", help="block,ppl,staging(scalar|lds) instead of autotuning")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
+    ap.add_argument("--eq-turns", type=int, default=20000,
+                    help="turns of the collective leg's equilibrium run (0: skip it)")
     ap.add_argument("--legs", default="saturated,collective,adjoint,diffusion,plain,cpu",
                     help="comma list of the legs run after the batched one "
                          "(saturated, collective, adjoint, diffusion, plain, cpu); 'none' for the batched "
@@ -405,6 +407,8 @@ def main():
     for flag, leg in ((args.no_plain, "plain"), (args.no_cpu, "cpu"), (args.no_adjoint, "adjoint")):
         if flag:
             legs.discard(leg)
+    if args.eq_turns and args.eq_turns < 1000:
+        ap.error("--eq-turns is 0 or at least 1000 (the run goes in pieces of 1000 turns)")
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this backend has no "
               f"multi-GPU path: seeds and particles are independent and would shard with zero exchange.",
@@ -528,7 +532,7 @@ def main():
     if "saturated" in legs:
         result["saturated"] = saturated_leg(lib, glats[0], lats[0], tuning, stream)
     if "collective" in legs:
-        result["collective"] = collective_leg(lib, stream)
+        result["collective"] = collective_leg(lib, stream, eq_turns=args.eq_turns)
     if "adjoint" in legs:
         log("adjoint leg")
         result["adjoint"] = adjoint_leg(lib, stream)
