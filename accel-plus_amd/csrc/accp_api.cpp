@@ -445,8 +445,10 @@ int prepare_workset(WorkSet &ws, const int64_t *h_offs, int nseed, int per, int 
 // Enqueue a whole tracking run on stream s (no host synchronisation, no
 // memset: capturable).  Turns run in chunks of 1, 2, 4, ... up to
 // chunk_max; between chunks the lost particles leave the working set.
+// init = false reuses the working set and tile list that the previous run on
+// ws left: valid when that run had no compaction and the same particles.
 int enqueue_run(const TableRef &T, WorkSet &ws, int max_tiles, double *d_r, int64_t n,
-                const RunSpec &R, const accp_tuning *tuning, hipStream_t s) {
+                const RunSpec &R, const accp_tuning *tuning, hipStream_t s, bool init = true) {
   int block, ppl, chunk_max;
   bool lds;
   int rc = resolve_tuning(tuning, T.lds_need, block, ppl, lds, chunk_max);
@@ -454,10 +456,12 @@ int enqueue_run(const TableRef &T, WorkSet &ws, int max_tiles, double *d_r, int6
   const int per = block * ppl;
   const int64_t *d_offs = (const int64_t *)ws.offs.p;
   int cur = 0;
-  HIPCHK(launch_chunk_init((int32_t *)ws.idx[0].p, n, R.d_loss_turn, R.nturn, (int32_t *)ws.cnt[0].p, d_offs,
-                           T.nseed, s));
-  HIPCHK(launch_make_tiles((const int32_t *)ws.cnt[0].p, d_offs, T.nseed, per, (TileDesc *)ws.tiles.p,
-                           (int32_t *)ws.ntiles.p, s));
+  if (init) {
+    HIPCHK(launch_chunk_init((int32_t *)ws.idx[0].p, n, R.d_loss_turn, R.nturn, (int32_t *)ws.cnt[0].p, d_offs,
+                             T.nseed, s));
+    HIPCHK(launch_make_tiles((const int32_t *)ws.cnt[0].p, d_offs, T.nseed, per, (TileDesc *)ws.tiles.p,
+                             (int32_t *)ws.ntiles.p, s));
+  }
   KernelCtx c{};
   c.tiles = (const TileDesc *)ws.tiles.p;
   c.ntiles = (const int32_t *)ws.ntiles.p;
@@ -626,6 +630,7 @@ static int track_device_impl(accp_lattice *L, double *d_r, int64_t n, const RunS
   if (R.snap) return fail(ACCP_E_ARG, "monitor records with a wake element are not supported");
   accp_tuning one = tuning ? *tuning : accp_tuning{0, 0, 0, 0};
   one.compact = -1;  // every segment sees the whole beam
+  bool init = true;  // the first segment builds the tile list, the others reuse it
   for (int t = 0; t < R.nturn; ++t) {
     RunSpec seg = R;
     seg.nturn = 1;
@@ -637,7 +642,8 @@ static int track_device_impl(accp_lattice *L, double *d_r, int64_t n, const RunS
       if (b > a) {
         seg.st = a;
         seg.ed = b;
-        if ((rc = enqueue_run(T, L->ws, max_tiles, d_r, n, seg, &one, s))) return rc;
+        if ((rc = enqueue_run(T, L->ws, max_tiles, d_r, n, seg, &one, s, init))) return rc;
+        init = false;
       }
       if (k < cuts.size()) {
         if ((rc = wake_apply(L->wake, b, d_r, n, s))) return fail(rc, "wake element %d: %s", b, wake_last_error());
