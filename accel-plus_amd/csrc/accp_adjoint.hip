@@ -98,7 +98,9 @@ __device__ __forceinline__ void strmp_empty_rev(Part &p, Adj &l, const Hdr h, co
 template <bool THICK, class PT>
 __device__ __forceinline__ void strmp_rev_dispatch(Part &p, Adj &l, const Hdr h, const PT *P,
                                                    const Knob &kn, double *grad) {
-  switch (h.b) {
+  // the record keeps every coefficient up to hi; the STRMP_SINGLE flag only
+  // serves the forward kernel
+  switch (h.b == -1 ? -1 : (h.b & 0xff)) {
     case -1: strmp_empty_rev(p, l, h, P, THICK, kn, grad); break;
     case 0: strmp_rev<0, THICK>(p, l, h, P, kn, grad); break;
     case 1: strmp_rev<1, THICK>(p, l, h, P, kn, grad); break;

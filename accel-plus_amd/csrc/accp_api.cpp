@@ -215,7 +215,9 @@ int compile_lattice(const accp_elem *E, int n, Compiled &out) {
         if (e.max_npole < 1 || e.max_npole > ACCP_MAX_MPOLE || (e.len != 0 && e.nslice < 1))
           return fail(ACCP_E_LATTICE, "element %d: StrMPole needs 1<=MaxNPole<=10, nslice>=1", i);
         const int hi = highest_nonzero(e.polyB, e.polyA, e.max_npole);
-        h.b = hi;
+        bool single = hi >= 1 && hi <= 3;
+        for (int j = 0; j <= hi && single; ++j) single = e.polyA[j] == 0 && (j == hi || e.polyB[j] == 0);
+        h.b = hi | (single ? STRMP_SINGLE : 0);
         if (e.len == 0) {
           h.op = OP_STRMP_THIN;  // one integrated kick, L = 1 (accel.cpp:824-827)
         } else {

@@ -221,21 +221,29 @@ __device__ __forceinline__ void bendmp_fixed(Part (&p)[PPL], const Hdr h, const 
 
 // AStrMPole::pass with PolyB/PolyA[0..HI] in registers; thick (len > 0,
 // nslice Forest-Ruth slices, accel.cpp:829-843) or thin (one integrated kick,
-// accel.cpp:824-827).
-template <int PPL, int HI, bool THICK, class PT>
+// accel.cpp:824-827).  SINGLE: the field is PolyB[HI] alone (a sextupole,
+// say), and the kick sum skips the additions of the zero coefficients.
+template <int HI, bool SINGLE>
+__device__ __forceinline__ void strmp_sum(const double (&b)[HI + 1], const double (&a)[HI + 1], double x,
+                                          double y, double &re, double &im) {
+  if (SINGLE) horner_single<HI>(b[HI], x, y, re, im);
+  else horner_reg<HI>(b, a, x, y, re, im);
+}
+
+template <int PPL, int HI, bool THICK, bool SINGLE, class PT>
 __device__ __forceinline__ void strmp_fixed(Part (&p)[PPL], const Hdr h, const PT *P) {
   const PT *C = THICK ? P + 4 : P;
   double b[HI + 1], a[HI + 1];
 #pragma unroll
   for (int i = 0; i <= HI; ++i) {
-    b[i] = C[i];
-    a[i] = C[HI + 1 + i];
+    b[i] = (!SINGLE || i == HI) ? (double)C[i] : 0.0;
+    a[i] = SINGLE ? 0.0 : (double)C[HI + 1 + i];
   }
   if (!THICK) {
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
       double re, im;
-      horner_reg<HI>(b, a, p[k].x, p[k].y, re, im);
+      strmp_sum<HI, SINGLE>(b, a, p[k].x, p[k].y, re, im);
       p[k].px -= re;
       p[k].py += im;
     }
@@ -256,15 +264,15 @@ __device__ __forceinline__ void strmp_fixed(Part (&p)[PPL], const Hdr h, const P
     for (int k = 0; k < PPL; ++k) {
       double re, im;
       drift_c(p[k], n1[k], z1[k]);
-      horner_reg<HI>(b, a, p[k].x, p[k].y, re, im);
+      strmp_sum<HI, SINGLE>(b, a, p[k].x, p[k].y, re, im);
       p[k].px -= k1 * re;
       p[k].py += k1 * im;
       drift_c(p[k], n2[k], z2[k]);
-      horner_reg<HI>(b, a, p[k].x, p[k].y, re, im);
+      strmp_sum<HI, SINGLE>(b, a, p[k].x, p[k].y, re, im);
       p[k].px -= k2 * re;
       p[k].py += k2 * im;
       drift_c(p[k], n2[k], z2[k]);
-      horner_reg<HI>(b, a, p[k].x, p[k].y, re, im);
+      strmp_sum<HI, SINGLE>(b, a, p[k].x, p[k].y, re, im);
       p[k].px -= k1 * re;
       p[k].py += k1 * im;
       drift_c(p[k], n1[k], z1[k]);
@@ -292,20 +300,28 @@ __device__ __forceinline__ void strmp_empty(Part (&p)[PPL], const Hdr h, const P
 
 template <int PPL, bool THICK, class PT>
 __device__ __forceinline__ void strmp_dispatch(Part (&p)[PPL], const Hdr h, const PT *P) {
+  if (h.b != -1 && (h.b & STRMP_SINGLE)) {
+    switch (h.b & 0xff) {
+      case 1: strmp_fixed<PPL, 1, THICK, true>(p, h, P); break;
+      case 2: strmp_fixed<PPL, 2, THICK, true>(p, h, P); break;
+      default: strmp_fixed<PPL, 3, THICK, true>(p, h, P); break;
+    }
+    return;
+  }
   switch (h.b) {
     case -1:
       if (THICK) strmp_empty<PPL>(p, h, P);
       break;
-    case 0: strmp_fixed<PPL, 0, THICK>(p, h, P); break;
-    case 1: strmp_fixed<PPL, 1, THICK>(p, h, P); break;
-    case 2: strmp_fixed<PPL, 2, THICK>(p, h, P); break;
-    case 3: strmp_fixed<PPL, 3, THICK>(p, h, P); break;
-    case 4: strmp_fixed<PPL, 4, THICK>(p, h, P); break;
-    case 5: strmp_fixed<PPL, 5, THICK>(p, h, P); break;
-    case 6: strmp_fixed<PPL, 6, THICK>(p, h, P); break;
-    case 7: strmp_fixed<PPL, 7, THICK>(p, h, P); break;
-    case 8: strmp_fixed<PPL, 8, THICK>(p, h, P); break;
-    default: strmp_fixed<PPL, 9, THICK>(p, h, P); break;
+    case 0: strmp_fixed<PPL, 0, THICK, false>(p, h, P); break;
+    case 1: strmp_fixed<PPL, 1, THICK, false>(p, h, P); break;
+    case 2: strmp_fixed<PPL, 2, THICK, false>(p, h, P); break;
+    case 3: strmp_fixed<PPL, 3, THICK, false>(p, h, P); break;
+    case 4: strmp_fixed<PPL, 4, THICK, false>(p, h, P); break;
+    case 5: strmp_fixed<PPL, 5, THICK, false>(p, h, P); break;
+    case 6: strmp_fixed<PPL, 6, THICK, false>(p, h, P); break;
+    case 7: strmp_fixed<PPL, 7, THICK, false>(p, h, P); break;
+    case 8: strmp_fixed<PPL, 8, THICK, false>(p, h, P); break;
+    default: strmp_fixed<PPL, 9, THICK, false>(p, h, P); break;
   }
 }
 

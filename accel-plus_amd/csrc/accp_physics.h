@@ -289,6 +289,20 @@ ACCP_HD void horner_reg(const double (&b)[HI + 1], const double (&a)[HI + 1], do
     re = t;
   }
 }
+// b (x+iy)^HI: horner_reg on the coefficients {0, ..., 0, b} and A = 0 with
+// the additions of those zeros left out (each adds an exact zero) and the
+// first step's products by the zero imaginary part.
+template <int HI>
+ACCP_HD void horner_single(double b, double x, double y, double &re, double &im) {
+  re = b * x;
+  im = b * y;
+#pragma unroll
+  for (int i = HI - 2; i >= 0; --i) {
+    const double t = re * x - im * y;
+    im = im * x + re * y;
+    re = t;
+  }
+}
 template <int HI>
 ACCP_HD void horner_b_reg(const double (&b)[HI + 1], double x, double y, double &re, double &im) {
   re = b[HI];

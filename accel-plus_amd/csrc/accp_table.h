@@ -22,8 +22,8 @@ enum Op : int32_t {
   OP_BENDMP,      // BM_* constants then B0..Bmo             a=nslice b=mo | plain<<8
   OP_BENDMP_RAD,  // same layout, classical radiation in the kicks (flag 1)
   OP_BENDMP_QE,   // same layout, radiation + quantum excitation (flag 2, 2nd-order integrator)
-  OP_STRMP,       // [L1, L2, K1, K2, B0..Bhi, A0..Ahi]   a=nslice b=hi (-1: no field)
-  OP_STRMP_THIN,  // [B0..Bhi, A0..Ahi]                     b=hi
+  OP_STRMP,       // [L1, L2, K1, K2, B0..Bhi, A0..Ahi]   a=nslice b=hi | STRMP_SINGLE (-1: no field)
+  OP_STRMP_THIN,  // [B0..Bhi, A0..Ahi]                     b=hi | STRMP_SINGLE
   OP_CAVITY,      // [L, V/E, 2*pi*f, phis]                 a=on
   OP_APERTURE,    // [L, xmin, xmax, ymin, ymax]
   OP_CORRECTOR,   // [L, kickx, kicky]
@@ -43,6 +43,10 @@ struct Hdr {
   int32_t a;
   int32_t b;
 };
+
+// StrMPole header flag: the field is the single normal term PolyB[hi]
+// (PolyA zero, PolyB zero below hi), 1 <= hi <= 3.
+constexpr int STRMP_SINGLE = 0x100;
 
 // Offsets inside an OP_BENDMP* record.  HS* = irho*L and HQ* = (1/4)*HS*L
 // of the pseudo-drift lengths L1, L2 and SL/2; K1C/K2C are PolyB[1]/[2] as the
