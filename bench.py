@@ -32,8 +32,8 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 # HBM bytes per k_track launch of the default batched workload (64 seeds,
 # 1024 turns, 12 launches per step): rocprofv3 FETCH_SIZE and WRITE_SIZE
 # passes, scaled by what each counter reads for a plain device copy of known
-# size (1.9957 and 1.0); profiles/k_track_batched_da_64seeds_1024turns.md.
-TRAFFIC_BYTES_PER_LAUNCH = 7.16e6
+# size (1.9954 and 1.0); profiles/k_track_batched_da_64seeds_1024turns.md.
+TRAFFIC_BYTES_PER_LAUNCH = 7.13e6
 
 
 def chunk_launches(nturn, chunk_max=256):
