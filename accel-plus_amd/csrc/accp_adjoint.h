@@ -10,8 +10,9 @@
 // and their delta-derivatives enter l.d.
 //
 // Reference maps differentiated: fastdrift (atpass.cpp:416-428), quad6
-// (:15-72), ATbendhxdrift6 (:343-355), bndthinkick (:199-235, normal
-// multipoles, no curvature gradient terms), strthinkick (:358-414), edge
+// (:15-72, power-series and trigonometric forms), ATbendhxdrift6 (:343-355),
+// bndthinkick (:199-235, normal multipoles with the curvature terms of
+// PolyB[1] and PolyB[2]), strthinkick (:358-414), edge
 // (:190-196), ACavity::pass (accel.cpp:903-923), ACorrector::pass
 // (accel.cpp:270-292).
 #pragma once
@@ -207,15 +208,12 @@ ACCP_HD void ds_series(const PT *C, double u, double &sd, double &shd) {
   shd = de + u * dd;
 }
 
-template <int NT, class PT>
-ACCP_HD void quad_series_rev(Part &p, Adj &l, const PT *C, double L, double L2, bool foc, double aK,
-                             double sK, const Knob &kn, double *grad) {
+// quad6 reverse map from the functions of u = L^2 g, g = |K|/(1+delta):
+// c = cos sqrt u, ch = cosh sqrt u, s = sin sqrt u / sqrt u,
+// sh = sinh sqrt u / sqrt u and sd, shd = d s/du, d sh/du.
+ACCP_HD void quad_rev_core(Part &p, Adj &l, double c, double ch, double s, double sh, double sd, double shd,
+                           double L, double L2, bool foc, double g, double sK, const Knob &kn, double *grad) {
   const double ipd = p.ipd, opd = 1.0 + p.d;
-  const double g = aK * ipd;
-  const double u = L2 * g;
-  double c, ch, s, sh, sd, shd;
-  cs_series<NT>(C, u, c, ch, s, sh);
-  ds_series<NT>(C, u, sd, shd);
   // matrix and d/dg of its elements
   double mhd, m12, m21, mvd, m34, m43, mhd_g, m12_g, m21_g, mvd_g, m34_g, m43_g;
   const double cg = -0.5 * s * L2, chg = 0.5 * sh * L2, sg = sd * L2, shg = shd * L2;
@@ -274,6 +272,31 @@ ACCP_HD void quad_series_rev(Part &p, Adj &l, const PT *C, double L, double L2, 
   p.y = y;
   p.py = yp * opd;
   p.z = z;
+}
+
+template <int NT, class PT>
+ACCP_HD void quad_series_rev(Part &p, Adj &l, const PT *C, double L, double L2, bool foc, double aK,
+                             double sK, const Knob &kn, double *grad) {
+  const double g = aK * p.ipd;
+  const double u = L2 * g;
+  double c, ch, s, sh, sd, shd;
+  cs_series<NT>(C, u, c, ch, s, sh);
+  ds_series<NT>(C, u, sd, shd);
+  quad_rev_core(p, l, c, ch, s, sh, sd, shd, L, L2, foc, g, sK, kn, grad);
+}
+
+// The trigonometric path (quad, accp_physics.h), for u beyond the series
+// range: there u >= 6, so d/du (sin sqrt u / sqrt u) = (c - s)/(2u) and
+// d/du (sinh sqrt u / sqrt u) = (ch - sh)/(2u) lose nothing to cancellation.
+ACCP_HD void quad_trig_rev(Part &p, Adj &l, double L, double L2, bool foc, double aK, double sK, const Knob &kn,
+                           double *grad) {
+  const double g = aK * p.ipd;
+  const double u = L2 * g;
+  const double lt = sqrt(u);
+  double sn, cs;
+  sincos(lt, &sn, &cs);
+  const double c = cs, ch = cosh(lt), s = sn / lt, sh = sinh(lt) / lt;
+  quad_rev_core(p, l, c, ch, s, sh, (c - s) / (2.0 * u), (ch - sh) / (2.0 * u), L, L2, foc, g, sK, kn, grad);
 }
 
 // ---- RF cavity (ACavity::pass) ---------------------------------------------------

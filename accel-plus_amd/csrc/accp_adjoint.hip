@@ -127,6 +127,7 @@ __device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const 
       const bool foc = K > 0;
       const double sK = foc ? 1.0 : -1.0;
       switch (h.a) {
+        case 0: quad_trig_rev(p, l, L, L2, foc, aK, sK, kn, grad); break;
         case 4: quad_series_rev<4>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
         case 6: quad_series_rev<6>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
         case 8: quad_series_rev<8>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;

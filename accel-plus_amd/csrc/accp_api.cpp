@@ -979,10 +979,8 @@ struct accp_adjoint {
 static bool adjoint_supports(const Hdr &h) {
   switch (h.op) {
     case OP_NOP: case OP_DRIFT: case OP_STRMP: case OP_STRMP_THIN:
-    case OP_CAVITY: case OP_APERTURE: case OP_CORRECTOR: case OP_BENDMP:
+    case OP_CAVITY: case OP_APERTURE: case OP_CORRECTOR: case OP_BENDMP: case OP_QUAD:
       return true;
-    case OP_QUAD:
-      return h.a > 0;
     default:
       return false;
   }
@@ -998,9 +996,9 @@ extern "C" int accp_adjoint_create(accp_lattice *L, int nparam, const accp_param
   for (int e = 0; e < ne; ++e)
     if (!adjoint_supports(L->tab.hdr[e]))
       return fail(ACCP_E_LATTICE,
-                  "element %d (type %d): no reverse map (radiation, wake, linear Bend, K=0 or "
-                  "out-of-range Quad, or a type other than Drift, Quad, BendMPole, StrMPole, "
-                  "Cavity, Aperture, Corrector, Monitor)",
+                  "element %d (type %d): no reverse map (radiation, wake, linear Bend, K=0 Quad, "
+                  "or a type other than Drift, Quad, BendMPole, StrMPole, Cavity, Aperture, "
+                  "Corrector, Monitor)",
                   e, L->elems[e].type);
   std::vector<int32_t> slot(ne, -1), kind(ne, 0);
   for (int k = 0; k < nparam; ++k) {

@@ -46,10 +46,12 @@ static int failures = 0;
 
 // fwd(state, knob) -> state; rev(state_out, adj, grad) -> state_in, adj_in
 // knob_rel: relative finite-difference step on the knob (maps linear in
-// their knob take a large one, which is exact for them)
+// their knob take a large one, which is exact for them); coord_rel: the same
+// for the coordinates (strongly focusing maps amplify round-off and take a
+// larger one)
 static void check(const char *name, std::function<void(Part &, double)> fwd,
                   std::function<void(Part &, Adj &, double *)> rev, double knob, bool has_knob,
-                  const double in[6], double knob_rel = 1e-7, double tol = 2e-7) {
+                  const double in[6], double knob_rel = 1e-7, double tol = 2e-7, double coord_rel = 1e-6) {
   // forward and inverse round trip
   Part p = mk(in);
   fwd(p, knob);
@@ -68,7 +70,7 @@ static void check(const char *name, std::function<void(Part &, double)> fwd,
   double worst = 0;
   for (int i = 0; i <= 6; ++i) {
     if (i == 6 && !has_knob) break;
-    const double hstep = i < 6 ? 1e-7 * (std::fabs(in[i]) + 1e-4) : knob_rel * (std::fabs(knob) + 1e-3);
+    const double hstep = i < 6 ? coord_rel * (std::fabs(in[i]) + 1e-4) : knob_rel * (std::fabs(knob) + 1e-3);
     double vp[6], vm[6];
     for (int k = 0; k < 6; ++k) vp[k] = vm[k] = in[k];
     double kp = knob, km = knob;
@@ -174,12 +176,25 @@ int main() {
               K, true, s0);
       }
     }
+    {  // trigonometric quads (u = L^2 |K|/(1+delta) near 7.5, beyond the series range), K knob;
+       // cosh sqrt u ~ 15 amplifies the differences' round-off: 1e-5 coordinate steps
+      for (double K : {30.0, -30.0}) {
+        const double L = 0.5;
+        check(K > 0 ? "quad trig K>0" : "quad trig K<0",
+              [=](Part &p, double kv) { quad(p, L, kv, std::fabs(kv)); },
+              [=](Part &p, Adj &l, double *g) {
+                quad_trig_rev(p, l, L, L * L, K > 0, std::fabs(K), K > 0 ? 1.0 : -1.0, Knob{0, 1, 0}, g);
+              },
+              K, true, s0, 1e-7, 2e-7, 1e-5);
+      }
+    }
     {
       const double w = TWOPI * 562e6 / C0;
       check("cavity thin", [=](Part &p, double) { cavity(p, 0.0, 5e-4, w, 0.1, true); },
             [=](Part &p, Adj &l, double *) { cavity_rev(p, l, 0.0, 5e-4, w, 0.1, true); }, 0, false, s0);
       check("cavity thick", [=](Part &p, double) { cavity(p, 0.4, 5e-4, w, 0.1, true); },
-            [=](Part &p, Adj &l, double *) { cavity_rev(p, l, 0.4, 5e-4, w, 0.1, true); }, 0, false, s0);
+            [=](Part &p, Adj &l, double *) { cavity_rev(p, l, 0.4, 5e-4, w, 0.1, true); }, 0, false, s0, 1e-7,
+            2e-7, 1e-5);
       check("corrector", [=](Part &p, double) { corrector(p, 0.2, 1e-4, -2e-4); },
             [=](Part &p, Adj &l, double *) { corrector_rev(p, l, 0.2, 1e-4, -2e-4); }, 0, false, s0);
     }

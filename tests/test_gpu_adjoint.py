@@ -214,3 +214,38 @@ def test_combined_function_bends(gpu_lib, oracle):
     err = np.abs(J - ref).max() / np.abs(ref).max()
     print(f"adjoint Jacobian vs oracle getm66 {err:.2e}")
     assert err <= 1e-7
+
+
+def test_strong_quads_take_the_trigonometric_reverse_map(gpu_lib, oracle):
+    """Quads beyond the power-series range (2 L^2 |K| = 15) run the
+    trigonometric quad6 forward and reverse maps: a transfer line of two such
+    quads and a sextupole, tracked 2 passes; knob gradients against central
+    differences of the GPU map and of the oracle (1e-6), the two reverse
+    forms against each other.  Quads this strong have a phase advance of
+    2.7 rad each and no stable periodic cell: the line magnifies amplitudes
+    about 360-fold per pass, so the beam starts at 0.2% of the Gaussian
+    core's size."""
+    line = [el.drift(0.5, "D"), el.quad(0.5, 30.0, "QS"), el.drift(0.5, "D"), el.quad(0.5, -30.0, "QT"),
+            el.drift(0.5, "D"), el.strmpole(0.2, [0.0, 0.0, 40.0], name="SX"), el.drift(0.3, "D")]
+    knobs = [("QS", el.FIELD_K, 0, lat.family(line, "QS")), ("QT", el.FIELD_K, 0, lat.family(line, "QT")),
+             ("SX", el.FIELD_POLYB, 2, lat.family(line, "SX"))]
+    beam = lat.gaussian_beam(32, seed=9) * 0.002
+    L = gpu_lib.lattice(line)
+    A = abi.Adjoint(gpu_lib, L, [(f, i, m) for _, f, i, m in knobs])
+    out = A.forward(beam, 2, ckpt_every=1)
+    assert not np.isnan(out).any()
+    _, seed = loss_and_seed(out)
+    res = {}
+    for mode in (A.REVERSE_SEQUENTIAL, A.REVERSE_SEGMENT_JACOBIAN):
+        A.set_reverse(mode)
+        res[mode] = A.backward(seed)
+    (grad, dr0), (g2, d2) = res[A.REVERSE_SEQUENTIAL], res[A.REVERSE_SEGMENT_JACOBIAN]
+    assert np.all(np.abs(grad - g2) <= 1e-11 * np.abs(grad)), (grad, g2)
+    assert np.all(np.abs(dr0 - d2) <= 1e-11 * np.abs(dr0).max(axis=0))
+    for k, (name, f, i, m) in enumerate(knobs):
+        base = line[m[0]].k if f == el.FIELD_K else line[m[0]].polyB[i]
+        steps = [r * abs(base) for r in (1e-8, 1e-7, 1e-6, 1e-5)]
+        for track in (lambda r: gpu_lib.lattice(r).track(beam, 2), lambda r: oracle.track(r, beam, 2)):
+            rel, fd = best_fd(grad[k], lambda h: loss_and_seed(track(lat.shift_knob(line, f, i, m, h)))[0], steps)
+            print(f"{name}: adjoint {grad[k]:+.12e}  best fd {fd:+.12e}  rel {rel:.1e}")
+            assert rel < 1e-6, name
