@@ -286,6 +286,8 @@ class Reference:
                                          c_double, c_double]
         L.ref_trackdiffusion.restype = c_double
         L.ref_trackdnudexy.argtypes = [c_void_p, c_uint, c_double, c_double, DP]
+        L.ref_pass_crn.argtypes = [c_void_p, DP, c_int, c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
+        L.ref_pass_crn.restype = c_int
         self.lib = L
 
     def naff(self, data, xlow=0.0, xhi=0.5):
@@ -360,6 +362,14 @@ class RefLine:
     def track(self, r, nturn=1):
         r = np.ascontiguousarray(r, dtype=np.float64).copy()
         self.lib.ref_pass(self.h, _dp(r), r.size // 6, nturn)
+        return r.reshape(-1, 6)
+
+    def track_crn(self, r, nturn, seed=0, turn0=0, pid0=0):
+        """ref_pass_crn: the reference's tracking with the oracle's normals
+        in its quantum-excitation kicks (common random numbers)."""
+        r = np.ascontiguousarray(r, dtype=np.float64).copy()
+        if self.lib.ref_pass_crn(self.h, _dp(r), r.size // 6, nturn, seed, turn0, pid0) != 0:
+            raise ValueError("ref_pass_crn needs an even nslice in every BendMPole")
         return r.reshape(-1, 6)
 
     def track_range(self, r, st, ed):

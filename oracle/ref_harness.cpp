@@ -6,6 +6,8 @@
 // fixtures and to pin the oracle; bench.py times ref_pass as the CPU
 // baseline.  Nothing in the product links it.
 #include <cmath>
+#include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <string>
@@ -17,6 +19,34 @@
 #include "utility.h"
 
 namespace {
+// Common random numbers for quantum excitation.  The library is linked with
+// -Wl,--wrap=rand, so the reference's grandn (utility.cpp:194-217) draws its
+// uniforms from __wrap_rand below.  Outside ref_pass_crn that forwards to
+// the C library; inside it, while one particle passes one BendMPole, call c
+// returns the oracle's uniform u1 (c even) or u2 (c odd) of pair c/2 of that
+// (particle, turn, element): Philox4x32-10 (Salmon et al., SC'11) keyed by
+// the seed over the counter {pid, turn, element, pair}, the words taken
+// as the oracle's 53-bit uniform, rounded to rand()'s grid k/RAND_MAX.  The
+// reference's Box-Muller then yields the oracle's normals to ~1e-9.
+struct Crn {
+  bool on = false;
+  uint32_t key[2] = {0, 0};
+  uint32_t pid = 0, turn = 0, eidx = 0, calls = 0;
+} g_crn;
+
+void crn_philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int round = 0; round < 10; ++round) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[1] = (uint32_t)p1;
+    c[3] = (uint32_t)p0;
+    c[0] = n0;
+    c[2] = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
 bool g_map_ready = false;
 void ensure_map() {
   if (!g_map_ready) {
@@ -25,6 +55,22 @@ void ensure_map() {
   }
 }
 }  // namespace
+
+extern "C" int __real_rand(void);
+extern "C" int __wrap_rand(void) {
+  if (!g_crn.on) return __real_rand();
+  const uint32_t c = g_crn.calls++;
+  uint32_t w[4] = {g_crn.pid, g_crn.turn, g_crn.eidx, c >> 1};
+  crn_philox(w, g_crn.key[0], g_crn.key[1]);
+  const uint32_t a = (c & 1) ? w[2] : w[0], b = (c & 1) ? w[3] : w[1];
+  const uint64_t k = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
+  const double u = ((double)k + 0.5) * (1.0 / 9007199254740992.0);
+  long v = std::lround(u * RAND_MAX);
+  // u1 = 0 would make grandn draw again (utility.cpp:206-210)
+  if (v < 1) v = 1;
+  if (v > RAND_MAX) v = RAND_MAX;
+  return (int)v;
+}
 
 extern "C" {
 
@@ -228,5 +274,46 @@ double ref_trackdiffusion(void *h, int nturn, int np, double *x0, double *y0, do
 // ALine::trackdnudexy (aline.cpp:716-779)
 void ref_trackdnudexy(void *h, unsigned nturn, double xm, double ym, double *d) {
   static_cast<ALine *>(h)->trackdnudexy(d, nturn, xm, ym);
+}
+// ALine::pass(r, N, nturn) (aline.cpp:359-363) with the oracle's normals in
+// every quantum-excitation kick (common random numbers, see __wrap_rand):
+// elements in line order over the whole beam, except that each BendMPole
+// passes one particle at a time so that its kicks draw the counter stream of
+// (pid0 + i, turn0 + t, element).  Particles are independent (no collective
+// element may be in the line), so the order changes nothing else.  Every
+// BendMPole needs an even nslice, so that grandn's pairs (a cosine and a
+// sine draw) stay inside one particle.  Returns 0, or -1 for an odd nslice.
+int ref_pass_crn(void *h, double *r, int n, int nturn, uint64_t seed, uint64_t turn0, uint64_t pid0) {
+  ALine *line = static_cast<ALine *>(h);
+  const unsigned ne = line->getnumber();
+  for (unsigned e = 0; e < ne; ++e) {
+    AccelElement *el = line->getelement(e);
+    if (el->gettype() == "BendMPole") {
+      int ns = 0;
+      el->getfield(&ns, "NSlice");
+      if (ns % 2) return -1;
+    }
+  }
+  g_crn.key[0] = (uint32_t)seed;
+  g_crn.key[1] = (uint32_t)(seed >> 32);
+  for (int t = 0; t < nturn; ++t) {
+    for (unsigned e = 0; e < ne; ++e) {
+      AccelElement *el = line->getelement(e);
+      if (el->gettype() != "BendMPole") {
+        el->pass(r, n);
+        continue;
+      }
+      for (int i = 0; i < n; ++i) {
+        g_crn.pid = (uint32_t)(pid0 + (uint64_t)i);
+        g_crn.turn = (uint32_t)(turn0 + (uint64_t)t);
+        g_crn.eidx = e;
+        g_crn.calls = 0;
+        g_crn.on = true;
+        el->pass(r + 6 * i, 1);
+        g_crn.on = false;
+      }
+    }
+  }
+  return 0;
 }
 }

@@ -1,16 +1,24 @@
 """Radiation equilibrium of the compiled reference against the oracle.
 
-The reference draws its quantum-excitation noise from rand() (grandn,
-utility.cpp:194-217), the oracle from the counter-based Philox stream, so
-the two runs agree only statistically.  Both track the same 8192-particle
-bunch through the RADQE ring with the cavity at the synchronous phase for
-10000 turns, then record the beam every 100 turns for 2000 more.  Each code
-runs as 8 worker processes of 1024 particles: the reference workers are
-single-threaded (its rand() state is shared between threads) and seeded
-apart with srand, the oracle workers draw disjoint Philox streams (particle
-ids offset by the worker's first particle).  The statistics are those of
-the whole bunch at each record.  Writes tests/golden/equilibrium_ref.npz.
-Takes about 40 minutes on 8 cores: python tests/golden/make_equilibrium_ref.py"""
+Three runs of the same 8192-particle bunch through the RADQE ring with the
+cavity at the synchronous phase: 10000 turns, then a record every 100 turns
+for 2000 more.
+  oracle         the oracle, counter-based Philox normals;
+  reference      the reference with its own rand() noise (grandn,
+                 utility.cpp:194-217): a different random stream, so it
+                 agrees with the oracle only within sampling noise;
+  reference_crn  the reference fed the oracle's uniforms through rand()
+                 (oracle/ref_harness.cpp, ref_pass_crn): common random
+                 numbers, so any difference beyond round-off growth is a
+                 difference in the radiation physics.
+Each code runs as 8 worker processes of 1024 particles: the reference
+workers are single-threaded (its rand() state is shared between threads)
+and seeded apart with srand; the oracle and controlled workers draw the
+Philox streams of their particle ids (offset by the worker's first
+particle).  The statistics are those of the whole bunch at each record.
+Writes tests/golden/equilibrium_ref.npz; about 25 minutes per run on 8
+cores.  python tests/golden/make_equilibrium_ref.py [--only run,...]
+recomputes the named runs and keeps the others from the existing file."""
 import ctypes
 import multiprocessing as mp
 import os
@@ -45,6 +53,11 @@ def worker(args):
     if code == "oracle":
         def step(r, n, t0):
             return O.track_omp(ring, r, n, 1, seed=SEED, turn0=t0, pid0=k * per)
+    elif code == "reference_crn":
+        line = Reference().load(ring)
+
+        def step(r, n, t0):
+            return line.track_crn(r, n, seed=SEED, turn0=t0, pid0=k * per)
     else:
         ctypes.CDLL("libc.so.6").srand(1000 + k)
         line = Reference().load(ring)
@@ -67,18 +80,28 @@ def stats(O, r):
     return s[13], s[8], np.sqrt(max(s[8] ** 2 * s[9] ** 2 - s[14] ** 2, 0.0)), s[1]
 
 
+RUNS = ("oracle", "reference", "reference_crn")
+
+
 def main():
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", default=",".join(RUNS), help="comma list of runs to recompute")
+    todo = ap.parse_args().only.split(",")
+    assert set(todo) <= set(RUNS), todo
+    path = os.path.join(HERE, "equilibrium_ref.npz")
     O = Oracle()
     out = {}
+    if set(todo) != set(RUNS):
+        with np.load(path) as g:
+            out = {code: g[code] for code in RUNS if code in g and code not in todo}
     with mp.get_context("fork").Pool(WORKERS) as pool:
-        for code in ("oracle", "reference"):
+        for code in todo:
             parts = pool.map(worker, [(code, k) for k in range(WORKERS)])
             snaps = np.concatenate(parts, axis=1)  # (NBLOCK, N, 6)
             out[code] = np.array([stats(O, snaps[b]) for b in range(NBLOCK)])
             print(code, out[code][:, :3].mean(axis=0), "live", out[code][-1, 3], flush=True)
-    np.savez_compressed(os.path.join(HERE, "equilibrium_ref.npz"), oracle=out["oracle"],
-                        reference=out["reference"], n=np.array([N]), warm=np.array([WARM]),
-                        block=np.array([BLOCK]))
+    np.savez_compressed(path, **out, n=np.array([N]), warm=np.array([WARM]), block=np.array([BLOCK]))
 
 
 if __name__ == "__main__":
