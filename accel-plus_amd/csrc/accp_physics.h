@@ -290,8 +290,11 @@ ACCP_HD void horner_reg(const double (&b)[HI + 1], const double (&a)[HI + 1], do
   }
 }
 // b (x+iy)^HI: horner_reg on the coefficients {0, ..., 0, b} and A = 0 with
-// the additions of those zeros left out (each adds an exact zero) and the
-// first step's products by the zero imaginary part.
+// the additions of those zeros and the first step's products by the zero
+// imaginary part left out.  For finite coordinates each is an exact zero, so
+// the sum is the full one's; only an orbit that has already overflowed to
+// infinity can see a finite value here where the full sum has 0 * inf = NaN
+// (its x is infinite or NaN by then, so its loss is the same).
 template <int HI>
 ACCP_HD void horner_single(double b, double x, double y, double &re, double &im) {
   re = b * x;
