@@ -153,18 +153,33 @@ def test_element_fixtures2_bit_exact(oracle):
 
 
 def test_radiation_equilibrium_oracle_vs_reference():
-    """Quantum excitation (flag 2) is random in both codes: the reference's
-    rand() stream and the oracle's Philox stream give different particles
-    but the same equilibrium.  8192 particles, 10000 turns to equilibrium,
-    then 20 records 100 turns apart (tests/golden/make_equilibrium_ref.py):
-    energy spread, horizontal size and projected emittance within 2%, and
-    the same number of particles lost (within 1%)."""
+    """Quantum excitation (flag 2) on the FODO ring at chromaticity +1 with
+    the cavity at the synchronous phase: 8192 particles, 10000 turns to
+    equilibrium, then 20 records 100 turns apart
+    (tests/golden/make_equilibrium_ref.py); energy spread, horizontal size
+    and projected emittance averaged over the records.
+    * reference_crn: the compiled reference fed the oracle's normals through
+      rand() (common random numbers).  The tracks stay together, so the
+      equilibria agree to round-off growth: within 0.5% (0.06% measured), and
+      losses within 0.25% of the bunch.
+    * reference: the reference with its own rand() noise, an independent
+      sample of the same equilibrium.  With about 8150 particles a width
+      carries 0.8% sampling error per run (1/sqrt(2N)) and the emittance
+      ~1.1%, and the 2000-turn window is shorter than the damping times, so
+      the records are one sample: two runs differ by 1.1-1.6% at one standard
+      error.  Bound: 5% (three standard errors), losses within 1% of the
+      bunch."""
     g = golden("equilibrium_ref.npz")
-    o, r = g["oracle"], g["reference"]
-    for k, name in enumerate(("sigma_delta", "sigma_x", "emittance_x")):
-        mo, mr = o[:, k].mean(), r[:, k].mean()
-        assert abs(mo / mr - 1) < 0.02, (name, mo, mr)
-    assert abs(o[-1, 3] - r[-1, 3]) <= 0.01 * r[-1, 3]
+    o = g["oracle"]
+    n = int(g["n"][0])
+    for run, tol, loss_tol in (("reference_crn", 0.005, 0.0025), ("reference", 0.05, 0.01)):
+        r = g[run]
+        for k, name in enumerate(("sigma_delta", "sigma_x", "emittance_x")):
+            mo, mr = o[:, k].mean(), r[:, k].mean()
+            print(f"{run} {name}: oracle {mo:.5e} reference {mr:.5e} ({mo / mr - 1:+.2%})")
+            assert abs(mo / mr - 1) < tol, (run, name, mo, mr)
+        print(f"{run} live: oracle {o[-1, 3]:.0f} reference {r[-1, 3]:.0f}")
+        assert abs(o[-1, 3] - r[-1, 3]) <= loss_tol * n, (run, o[-1, 3], r[-1, 3])
 
 
 def test_transwake_bit_exact(oracle):
