@@ -10,7 +10,8 @@
 // and their delta-derivatives enter l.d.
 //
 // Reference maps differentiated: fastdrift (atpass.cpp:416-428), quad6
-// (:15-72, power-series and trigonometric forms), ATbendhxdrift6 (:343-355),
+// (:15-72, power-series and trigonometric forms), bend6 (:76-187),
+// ATbendhxdrift6 (:343-355),
 // bndthinkick (:199-235, normal multipoles with the curvature terms of
 // PolyB[1] and PolyB[2]), strthinkick (:358-414), edge
 // (:190-196), ACavity::pass (accel.cpp:903-923), ACorrector::pass
@@ -297,6 +298,157 @@ ACCP_HD void quad_trig_rev(Part &p, Adj &l, double L, double L2, bool foc, doubl
   sincos(lt, &sn, &cs);
   const double c = cs, ch = cosh(lt), s = sn / lt, sh = sinh(lt) / lt;
   quad_rev_core(p, l, c, ch, s, sh, (c - s) / (2.0 * u), (ch - sh) / (2.0 * u), L, L2, foc, g, sK, kn, grad);
+}
+
+
+// ---- bend6: linear sector bend with gradient (ABend, atpass.cpp:76-187) ----
+// Its map mixes delta into every output through g1, g2 and the dispersion
+// terms, so the vector-Jacobian product is formed from the Jacobian itself:
+// six forward evaluations in dual numbers (value and one directional
+// derivative), one per input coordinate.
+struct Dual {
+  double v, d;
+};
+ACCP_HD Dual dl(double v) { return Dual{v, 0.0}; }
+ACCP_HD Dual operator+(Dual a, Dual b) { return Dual{a.v + b.v, a.d + b.d}; }
+ACCP_HD Dual operator-(Dual a, Dual b) { return Dual{a.v - b.v, a.d - b.d}; }
+ACCP_HD Dual operator-(Dual a) { return Dual{-a.v, -a.d}; }
+ACCP_HD Dual operator*(Dual a, Dual b) { return Dual{a.v * b.v, a.d * b.v + a.v * b.d}; }
+ACCP_HD Dual operator/(Dual a, Dual b) { return Dual{a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+ACCP_HD Dual operator+(Dual a, double b) { return Dual{a.v + b, a.d}; }
+ACCP_HD Dual operator+(double a, Dual b) { return Dual{a + b.v, b.d}; }
+ACCP_HD Dual operator-(Dual a, double b) { return Dual{a.v - b, a.d}; }
+ACCP_HD Dual operator-(double a, Dual b) { return Dual{a - b.v, -b.d}; }
+ACCP_HD Dual operator*(Dual a, double b) { return Dual{a.v * b, a.d * b}; }
+ACCP_HD Dual operator*(double a, Dual b) { return Dual{a * b.v, a * b.d}; }
+ACCP_HD Dual operator/(Dual a, double b) { return Dual{a.v / b, a.d / b}; }
+ACCP_HD Dual operator/(double a, Dual b) { return Dual{a / b.v, -a * b.d / (b.v * b.v)}; }
+ACCP_HD Dual dsqrt(Dual a) {
+  const double r = sqrt(a.v);
+  return Dual{r, a.d * 0.5 / r};
+}
+ACCP_HD void dsincos(Dual a, Dual &s, Dual &c) {
+  double sv, cv;
+  sincos(a.v, &sv, &cv);
+  s = Dual{sv, cv * a.d};
+  c = Dual{cv, -sv * a.d};
+}
+ACCP_HD void dsinhcosh(Dual a, Dual &s, Dual &c) {
+  const double sv = sinh(a.v), cv = cosh(a.v);
+  s = Dual{sv, cv * a.d};
+  c = Dual{cv, sv * a.d};
+}
+
+// bend6's outputs from its inputs {x, px, y, py, z, delta} (the operations of
+// bend_linear in accp_physics.h); the branch on the focusing sign reads the
+// value part, which delta's positive 1/(1+delta) cannot flip.
+ACCP_HD void bend6_dual(const Dual (&in)[6], double L, double kx, double grd, Dual (&out)[6]) {
+  const Dual pn = 1.0 / (1.0 + in[5]);
+  const Dual g1 = (kx * kx + grd) * pn, g2 = -grd * pn;
+  Dual mhd, m12, m21, mvd, m34, m43, s1 = dl(0.0), c1 = dl(1.0), sg1 = dl(0.0);
+  if (g1.v == 0) {
+    mhd = dl(1.0); m12 = dl(L); m21 = dl(0.0);
+  } else if (g1.v > 0) {
+    sg1 = dsqrt(g1);
+    dsincos(L * sg1, s1, c1);
+    mhd = c1; m12 = s1 / sg1; m21 = -(s1 * sg1);
+  } else {
+    sg1 = dsqrt(-g1);
+    dsinhcosh(L * sg1, s1, c1);
+    mhd = c1; m12 = s1 / sg1; m21 = s1 * sg1;
+  }
+  if (g2.v == 0) {
+    mvd = dl(1.0); m34 = dl(L); m43 = dl(0.0);
+  } else if (g2.v > 0) {
+    const Dual sg2 = dsqrt(g2);
+    Dual s2, c2;
+    dsincos(L * sg2, s2, c2);
+    mvd = c2; m34 = s2 / sg2; m43 = -(s2 * sg2);
+  } else {
+    const Dual sg2 = dsqrt(-g2);
+    Dual s2, c2;
+    dsinhcosh(L * sg2, s2, c2);
+    mvd = c2; m34 = s2 / sg2; m43 = s2 * sg2;
+  }
+  const Dual x = in[0], xp = in[1] * pn, y = in[2], yp = in[3] * pn;
+  const Dual opd = 1.0 + in[5];
+  const Dual dp = in[5] * pn;
+  Dual nx = mhd * x + m12 * xp;
+  Dual npx = (m21 * x + mhd * xp) * opd;
+  if (g1.v == 0) {
+    nx = nx + dp * L * L * kx * 0.5;
+    npx = npx + dp * L * kx * opd;
+  } else {
+    nx = nx + dp * (1.0 - c1) * kx / g1;
+    npx = npx + dp * s1 * kx / (sg1 * pn);
+  }
+  Dual z = in[4] - xp * xp * (L + mhd * m12) * 0.25;
+  if (g1.v != 0) {
+    z = z - (L - mhd * m12) * (x * x * g1 + dp * dp * kx * kx / g1 - 2.0 * x * kx * dp) * 0.25;
+    z = z - m12 * m21 * (x * xp - xp * dp * kx / g1) * 0.5;
+    z = z - (kx * x * m12 + xp * (1.0 - mhd) * kx / g1 + dp * (L - m12) * kx * kx / g1);
+  }
+  z = z - ((L - mvd * m34) * y * y * g2 + yp * yp * (L + mvd * m34)) * 0.25;
+  z = z - m34 * m43 * y * yp * 0.5;
+  out[0] = nx;
+  out[1] = npx;
+  out[2] = mvd * y + m34 * yp;
+  out[3] = (m43 * y + mvd * yp) * opd;
+  out[4] = z;
+  out[5] = in[5];
+}
+
+// Inverse: delta passes unchanged and the transverse part is affine with a
+// unit-determinant matrix, so (x, x') = M^-1 ((x, x')_out - D); z takes back
+// the path-length change of the recovered input.  Then l <- J^T l.
+ACCP_HD void bend_linear_rev(Part &p, Adj &l, double L, double kx, double grd) {
+  const double d = p.d, pn = p.ipd, opd = 1.0 + d;
+  Dual in[6], out[6];
+  // the matrix and dispersion at this delta: the forward map of (0, 0, 0, 0, 0, d)
+  // gives D; unit vectors give M's columns
+  for (int k = 0; k < 6; ++k) in[k] = dl(0.0);
+  in[5] = dl(d);
+  bend6_dual(in, L, kx, grd, out);
+  const double Dx = out[0].v, Dxp = out[1].v * pn;
+  in[0] = dl(1.0);
+  bend6_dual(in, L, kx, grd, out);
+  const double mhd = out[0].v - Dx, m21 = out[1].v * pn - Dxp;
+  in[0] = dl(0.0);
+  in[1] = dl(opd);  // x' = 1
+  bend6_dual(in, L, kx, grd, out);
+  const double m12 = out[0].v - Dx;
+  in[1] = dl(0.0);
+  in[2] = dl(1.0);
+  bend6_dual(in, L, kx, grd, out);
+  const double mvd = out[2].v, m43 = out[3].v * pn;
+  in[2] = dl(0.0);
+  in[3] = dl(opd);
+  bend6_dual(in, L, kx, grd, out);
+  const double m34 = out[2].v;
+  const double a = p.x - Dx, b = p.px * pn - Dxp;
+  const double x = mhd * a - m12 * b, xp = -m21 * a + mhd * b;
+  const double y = mvd * p.y - m34 * p.py * pn, yp = -m43 * p.y + mvd * p.py * pn;
+  const double v[6] = {x, xp * opd, y, yp * opd, 0.0, d};
+  for (int k = 0; k < 6; ++k) in[k] = dl(v[k]);
+  bend6_dual(in, L, kx, grd, out);
+  const double z = p.z - out[4].v;  // out[4] is the change of z from 0
+  // J^T l, one column per input coordinate
+  const double lo[6] = {l.x, l.px, l.y, l.py, l.z, l.d};
+  double li[6];
+  const double vin[6] = {x, xp * opd, y, yp * opd, z, d};
+  for (int j = 0; j < 6; ++j) {
+    for (int k = 0; k < 6; ++k) in[k] = Dual{vin[k], k == j ? 1.0 : 0.0};
+    bend6_dual(in, L, kx, grd, out);
+    double acc = 0.0;
+    for (int k = 0; k < 6; ++k) acc += lo[k] * out[k].d;
+    li[j] = acc;
+  }
+  l = Adj{li[0], li[1], li[2], li[3], li[4], li[5]};
+  p.x = x;
+  p.px = xp * opd;
+  p.y = y;
+  p.py = yp * opd;
+  p.z = z;
 }
 
 // ---- RF cavity (ACavity::pass) ---------------------------------------------------

@@ -150,6 +150,13 @@ __device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const 
       }
       break;
     }
+    case OP_BEND: {  // edge, bend6, edge (ABend::pass, accel.cpp:523-550)
+      const double psi1 = P[4], psi2 = P[5];
+      if (psi2 != 0) edge_rev(p, l, psi2);
+      bend_linear_rev(p, l, P[0], P[1], P[2]);
+      if (psi1 != 0) edge_rev(p, l, psi1);
+      break;
+    }
     case OP_STRMP:
       strmp_rev_dispatch<true>(p, l, h, P, kn, grad);
       break;

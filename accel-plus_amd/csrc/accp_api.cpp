@@ -979,7 +979,7 @@ struct accp_adjoint {
 static bool adjoint_supports(const Hdr &h) {
   switch (h.op) {
     case OP_NOP: case OP_DRIFT: case OP_STRMP: case OP_STRMP_THIN:
-    case OP_CAVITY: case OP_APERTURE: case OP_CORRECTOR: case OP_BENDMP: case OP_QUAD:
+    case OP_CAVITY: case OP_APERTURE: case OP_CORRECTOR: case OP_BENDMP: case OP_QUAD: case OP_BEND:
       return true;
     default:
       return false;
@@ -996,8 +996,8 @@ extern "C" int accp_adjoint_create(accp_lattice *L, int nparam, const accp_param
   for (int e = 0; e < ne; ++e)
     if (!adjoint_supports(L->tab.hdr[e]))
       return fail(ACCP_E_LATTICE,
-                  "element %d (type %d): no reverse map (radiation, wake, linear Bend, K=0 Quad, "
-                  "or a type other than Drift, Quad, BendMPole, StrMPole, Cavity, Aperture, "
+                  "element %d (type %d): no reverse map (radiation, wake, K=0 Quad, or a type "
+                  "other than Drift, Quad, Bend, BendMPole, StrMPole, Cavity, Aperture, "
                   "Corrector, Monitor)",
                   e, L->elems[e].type);
   std::vector<int32_t> slot(ne, -1), kind(ne, 0);

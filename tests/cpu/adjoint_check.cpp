@@ -176,6 +176,15 @@ int main() {
               K, true, s0);
       }
     }
+    {  // linear sector bends (bend6): focusing, defocusing and zero-gradient vertical planes
+      const double L = 1.5, kx = 0.157 / 1.5;
+      for (double grd : {0.3, -0.5, 0.0}) {
+        char nm[64];
+        std::snprintf(nm, sizeof nm, "bend6 K=%+.1f", grd);
+        check(nm, [=](Part &p, double) { bend_linear(p, L, kx, grd); },
+              [=](Part &p, Adj &l, double *) { bend_linear_rev(p, l, L, kx, grd); }, 0, false, s0);
+      }
+    }
     {  // trigonometric quads (u = L^2 |K|/(1+delta) near 7.5, beyond the series range), K knob;
        // cosh sqrt u ~ 15 amplifies the differences' round-off: 1e-5 coordinate steps
       for (double K : {30.0, -30.0}) {

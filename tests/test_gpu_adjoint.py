@@ -249,3 +249,36 @@ def test_strong_quads_take_the_trigonometric_reverse_map(gpu_lib, oracle):
             rel, fd = best_fd(grad[k], lambda h: loss_and_seed(track(lat.shift_knob(line, f, i, m, h)))[0], steps)
             print(f"{name}: adjoint {grad[k]:+.12e}  best fd {fd:+.12e}  rel {rel:.1e}")
             assert rel < 1e-6, name
+
+
+def test_linear_bends(gpu_lib, oracle):
+    """The FODO ring built with the linear Bend (bend6) in place of
+    BendMPole: knob gradients against central differences of the GPU map and
+    of the oracle (1e-6), and the one-turn Jacobian from six backward passes
+    against the oracle's getm66."""
+    ring = lat.fodo_ring(bend_type="Bend")
+    knobs = lat.adjoint_knobs(ring)
+    # transverse amplitudes large enough that the sextupole knobs' gradients
+    # (1e-5 of the quadrupoles') stand above the differences' round-off;
+    # energy offsets small enough that no particle is lost
+    beam = lat.gaussian_beam(64, seed=6) * 2.0
+    beam[:, 5] *= 0.1
+    L = gpu_lib.lattice(ring)
+    A = abi.Adjoint(gpu_lib, L, [(f, i, m) for _, f, i, m in knobs])
+    nturn = 12
+    out = A.forward(beam, nturn, ckpt_every=5)
+    assert not np.isnan(out).any()
+    _, seed = loss_and_seed(out)
+    grad, dr0 = A.backward(seed)
+    for k, (name, f, i, m) in enumerate(knobs):
+        for track in (lambda r: gpu_lib.lattice(r).track(beam, nturn), lambda r: oracle.track(r, beam, nturn)):
+            rel, fd = best_fd(grad[k], lambda h: loss_and_seed(track(lat.shift_knob(ring, f, i, m, h)))[0],
+                              fd_steps(ring, f, i, m))
+            print(f"{name}: adjoint {grad[k]:+.12e}  best fd {fd:+.12e}  rel {rel:.1e}")
+            assert rel < 1e-6, name
+    ref = oracle.getm66(ring)
+    A.forward(np.zeros((1, 6)), 1, ckpt_every=1)
+    J = np.array([A.backward(np.eye(6)[i:i + 1])[1][0] for i in range(6)])
+    err = np.abs(J - ref).max() / np.abs(ref).max()
+    print(f"adjoint Jacobian vs oracle getm66 {err:.2e}")
+    assert err <= 1e-7
