@@ -104,7 +104,10 @@ def cpu_baseline(lats, beam, offs, nturn, budget_s=20.0):
 
 def saturated_leg(lib, glat, elems, tuning, stream):
     """The same kernel at full occupancy: a Gaussian core beam that survives
-    (2^18 particles on the seed-0 lattice, 16 turns)."""
+    (2^18 particles on the seed-0 lattice, 16 turns), in 256-lane workgroups
+    that share one LDS copy of the table per four waves (the batched leg's
+    autotuned shape suits its sparse tail: a 64-lane pick would hold a 63 KB
+    table per wave and cap a full GPU at two waves per CU)."""
     import numpy as np
 
     import pyaccp.flops as flops
@@ -530,7 +533,7 @@ def main():
     }
 
     if "saturated" in legs:
-        result["saturated"] = saturated_leg(lib, glats[0], lats[0], tuning, stream)
+        result["saturated"] = saturated_leg(lib, glats[0], lats[0], abi.Tuning(256, 1, 2, 0), stream)
     if "collective" in legs:
         result["collective"] = collective_leg(lib, stream, eq_turns=args.eq_turns)
     if "adjoint" in legs:
