@@ -115,7 +115,10 @@ __device__ __forceinline__ void strmp_rev_dispatch(Part &p, Adj &l, const Hdr h,
   }
 }
 
-template <class PT>
+// FULL = false leaves out the reverse maps of the trigonometric quad and the
+// linear Bend, whose registers would halve the occupancy of every lattice's
+// sweep; the host picks the variant per lattice.
+template <bool FULL, class PT>
 __device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const PT *P, const Knob &kn,
                                             double *grad) {
   switch (h.op) {
@@ -127,7 +130,9 @@ __device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const 
       const bool foc = K > 0;
       const double sK = foc ? 1.0 : -1.0;
       switch (h.a) {
-        case 0: quad_trig_rev(p, l, L, L2, foc, aK, sK, kn, grad); break;
+        case 0:
+          if (FULL) quad_trig_rev(p, l, L, L2, foc, aK, sK, kn, grad);
+          break;
         case 4: quad_series_rev<4>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
         case 6: quad_series_rev<6>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
         case 8: quad_series_rev<8>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
@@ -151,6 +156,7 @@ __device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const 
       break;
     }
     case OP_BEND: {  // edge, bend6, edge (ABend::pass, accel.cpp:523-550)
+      if (!FULL) break;
       const double psi1 = P[4], psi2 = P[5];
       if (psi2 != 0) edge_rev(p, l, psi2);
       bend_linear_rev(p, l, P[0], P[1], P[2]);
@@ -225,7 +231,7 @@ __device__ __forceinline__ RevTable stage_table(const AdjCtx &c, unsigned char *
 // particle inside the recomputed group of segments; at the start of every
 // turn the state is reset to the exact recomputed one.  The next element's
 // header and knob are fetched while the current element runs.
-template <bool LDS>
+template <bool LDS, bool FULL>
 __device__ __forceinline__ void reverse_turns(const AdjCtx &c, const RevTable &T, int64_t q, int t_lo,
                                               int t_hi, Part &p, Adj &l, double *grad) {
   for (int t = t_hi - 1; t >= t_lo; --t) {
@@ -241,7 +247,7 @@ __device__ __forceinline__ void reverse_turns(const AdjCtx &c, const RevTable &T
       Hdr hn;
       Knob kn_next;
       load_rev<LDS>(T.hdr, T.knob, e > c.st ? e - 1 : e, hn, kn_next);
-      element_rev(p, l, h, T.par + h.off, kn, grad);
+      element_rev<FULL>(p, l, h, T.par + h.off, kn, grad);
       h = hn;
       kn = kn_next;
     }
@@ -250,7 +256,7 @@ __device__ __forceinline__ void reverse_turns(const AdjCtx &c, const RevTable &T
   }
 }
 
-template <int BLOCK, bool LDS>
+template <int BLOCK, bool LDS, bool FULL>
 __global__ __launch_bounds__(BLOCK) void k_adj_reverse(AdjCtx c) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const RevTable T = stage_table<BLOCK, LDS>(c, lds_raw);
@@ -274,7 +280,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_reverse(AdjCtx c) {
   for (int t1 = c.t_hi; t1 > c.t_lo;) {
     const int seg = (t1 - 1) / c.ckpt_every;
     const int t0 = seg * c.ckpt_every > c.t_lo ? seg * c.ckpt_every : c.t_lo;
-    reverse_turns<LDS>(c, T, (int64_t)(seg - c.seg0) * c.n + i, t0, t1, p, l, grad);
+    reverse_turns<LDS, FULL>(c, T, (int64_t)(seg - c.seg0) * c.n + i, t0, t1, p, l, grad);
     t1 = t0;
   }
   pe[0] = p.x; pe[1] = p.px; pe[2] = p.y; pe[3] = p.py; pe[4] = p.z; pe[5] = p.d;
@@ -283,7 +289,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_reverse(AdjCtx c) {
   for (int k = 0; k < c.nparam; ++k) g[k] += grad[k];
 }
 
-template <int BLOCK, bool LDS>
+template <int BLOCK, bool LDS, bool FULL>
 __global__ __launch_bounds__(BLOCK) void k_adj_segjac(AdjCtx c) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const RevTable T = stage_table<BLOCK, LDS>(c, lds_raw);
@@ -306,7 +312,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_segjac(AdjCtx c) {
   for (int k = 0; k < ADJ_MAX_PARAM; ++k) grad[k] = 0.0;
   const int t0 = seg * c.ckpt_every;
   const int t1 = t0 + c.ckpt_every < c.nturn ? t0 + c.ckpt_every : c.nturn;
-  reverse_turns<LDS>(c, T, q, t0, t1, p, l, grad);
+  reverse_turns<LDS, FULL>(c, T, q, t0, t1, p, l, grad);
   double *o = c.jac + (q * 6 + col) * (6 + c.nparam);
   o[0] = l.x; o[1] = l.px; o[2] = l.y; o[3] = l.py; o[4] = l.z; o[5] = l.d;
   for (int k = 0; k < c.nparam; ++k) o[6 + k] = grad[k];
@@ -344,10 +350,14 @@ __global__ __launch_bounds__(64) void k_adj_combine(AdjCtx c) {
 hipError_t launch_adj_reverse(const AdjCtx &c, hipStream_t s) {
   constexpr int B = 64;
   const int64_t grid = (c.n + B - 1) / B;
-  if (c.lds_bytes > 0)
-    hipLaunchKernelGGL((k_adj_reverse<B, true>), dim3((unsigned)grid), dim3(B), c.lds_bytes, s, c);
+  if (c.lds_bytes > 0 && c.full)
+    hipLaunchKernelGGL((k_adj_reverse<B, true, true>), dim3((unsigned)grid), dim3(B), c.lds_bytes, s, c);
+  else if (c.lds_bytes > 0)
+    hipLaunchKernelGGL((k_adj_reverse<B, true, false>), dim3((unsigned)grid), dim3(B), c.lds_bytes, s, c);
+  else if (c.full)
+    hipLaunchKernelGGL((k_adj_reverse<B, false, true>), dim3((unsigned)grid), dim3(B), 0, s, c);
   else
-    hipLaunchKernelGGL((k_adj_reverse<B, false>), dim3((unsigned)grid), dim3(B), 0, s, c);
+    hipLaunchKernelGGL((k_adj_reverse<B, false, false>), dim3((unsigned)grid), dim3(B), 0, s, c);
   return hipGetLastError();
 }
 
@@ -358,11 +368,17 @@ hipError_t launch_adj_segjac(const AdjCtx &c, hipStream_t s) {
   if (c.lds_bytes > 0) {
     constexpr int B = 256;
     const int64_t grid = (c.group_n * 6 + B - 1) / B;
-    hipLaunchKernelGGL((k_adj_segjac<B, true>), dim3((unsigned)grid), dim3(B), c.lds_bytes, s, c);
+    if (c.full)
+      hipLaunchKernelGGL((k_adj_segjac<B, true, true>), dim3((unsigned)grid), dim3(B), c.lds_bytes, s, c);
+    else
+      hipLaunchKernelGGL((k_adj_segjac<B, true, false>), dim3((unsigned)grid), dim3(B), c.lds_bytes, s, c);
   } else {
     constexpr int B = 64;
     const int64_t grid = (c.group_n * 6 + B - 1) / B;
-    hipLaunchKernelGGL((k_adj_segjac<B, false>), dim3((unsigned)grid), dim3(B), 0, s, c);
+    if (c.full)
+      hipLaunchKernelGGL((k_adj_segjac<B, false, true>), dim3((unsigned)grid), dim3(B), 0, s, c);
+    else
+      hipLaunchKernelGGL((k_adj_segjac<B, false, false>), dim3((unsigned)grid), dim3(B), 0, s, c);
   }
   return hipGetLastError();
 }

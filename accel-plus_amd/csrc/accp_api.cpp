@@ -1155,6 +1155,9 @@ extern "C" int accp_adjoint_backward(accp_adjoint *A, const double *dl_dr_out, d
     c.npar = (int32_t)L->tab.par.size();
     const size_t lds = adj_lds_bytes(ne, L->tab.par.size());
     c.lds_bytes = lds <= LDS_LIMIT ? (int32_t)lds : 0;
+    c.full = 0;
+    for (const Hdr &hd : L->tab.hdr)
+      if (hd.op == OP_BEND || (hd.op == OP_QUAD && hd.a == 0)) c.full = 1;
     c.nturn = A->nturn;
     c.nseg_g = g;
     c.r_final = (const double *)A->d_r.p;
