@@ -30,10 +30,12 @@ for p in (os.path.join(ROOT, "accel-plus_amd"), os.path.join(ROOT, "oracle")):
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 
 # HBM bytes per k_track launch of the default batched workload (64 seeds,
-# 1024 turns, 12 launches per step): rocprofv3 FETCH_SIZE and WRITE_SIZE
-# passes, scaled by what each counter reads for a plain device copy of known
-# size (1.9954 and 1.0); profiles/k_track_batched_da_64seeds_1024turns.md.
-TRAFFIC_BYTES_PER_LAUNCH = 7.13e6
+# 1024 turns, 12 launches per step), per launch shape: rocprofv3 FETCH_SIZE
+# and WRITE_SIZE passes, scaled by what each counter reads for a plain device
+# copy of known size (about 1.9957 and 1.0);
+# profiles/k_track_batched_da_64seeds_1024turns.md.  Each workgroup stages its
+# seed's table in LDS, so 64-lane workgroups fetch more table bytes.
+TRAFFIC_BYTES_PER_LAUNCH = {"block=256,ppl=1,lds": 7.13e6, "block=64,ppl=1,lds": 9.48e6}
 
 
 def chunk_launches(nturn, chunk_max=256):
@@ -446,9 +448,11 @@ def main():
         tuning = abi.Tuning(int(bl), int(pp), {"scalar": 1, "lds": 2}[sg], 0)
     else:
         log("autotuning")
-        # the probe spans the dense first turns and the sparse tail
-        tuning, probe_ms = B.autotune(d_r.ptr, offs, 64)
-        log(f"autotune: {tuning} ({probe_ms:.2f} ms for 64 turns)")
+        # each shape runs the whole workload once: the sparse tail, not the
+        # dense first turns, sets the step time, and a short probe sees only
+        # the latter (12 shapes x 2 runs, a few seconds)
+        tuning, probe_ms = B.autotune(d_r.ptr, offs, nturn)
+        log(f"autotune: {tuning} ({probe_ms:.2f} ms for {nturn} turns)")
 
     B.capture(d_r.ptr, offs, nturn, tuning=tuning, d_loss_turn=d_lt.ptr)
 
@@ -521,7 +525,9 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-            "traffic": TRAFFIC_BYTES_PER_LAUNCH if default_workload else None,
+            "traffic": (TRAFFIC_BYTES_PER_LAUNCH.get(f"block={tuning.block},ppl={tuning.ppl},"
+                                                     f"{'lds' if tuning.staging == 2 else 'scalar'}")
+                        if default_workload else None),
             "traffic_unit": "HBM bytes per k_track launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "launches_per_step": launches,
             "ms_per_launch": round(kernel_ms / launches, 3),
