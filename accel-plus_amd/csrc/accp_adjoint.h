@@ -9,8 +9,9 @@
 // is fixed, so the delta-dependent constants are rebuilt from 1/(1+delta)
 // and their delta-derivatives enter l.d.
 //
-// Reference maps differentiated: fastdrift (atpass.cpp:416-428), quad6
-// (:15-72, power-series and trigonometric forms), bend6 (:76-187),
+// Reference maps differentiated: fastdrift (atpass.cpp:416-428), drift
+// (:6-13, quad6 at K = 0), quad6 (:15-72, power-series and trigonometric
+// forms), bend6 (:76-187),
 // ATbendhxdrift6 (:343-355),
 // bndthinkick (:199-235, normal multipoles with the curvature terms of
 // PolyB[1] and PolyB[2]), strthinkick (:358-414), edge
@@ -300,6 +301,23 @@ ACCP_HD void quad_trig_rev(Part &p, Adj &l, double L, double L2, bool foc, doubl
   quad_rev_core(p, l, c, ch, s, sh, (c - s) / (2.0 * u), (ch - sh) / (2.0 * u), L, L2, foc, g, sK, kn, grad);
 }
 
+
+// ---- quad6 with K == 0: the reference's drift() (atpass.cpp:6-13) --------------
+// x += px L, y += py L, then z -= pn^2 L (px + y^2)^2 / 2 with the updated y
+// (its SQR(r6[1]+SQR(r6[2])) quirk), delta fixed.
+ACCP_HD void drift_kzero_rev(Part &p, Adj &l, double L) {
+  const double pn = p.ipd, px = p.px, py = p.py, y1 = p.y;
+  const double w = px + y1 * y1, c = pn * pn * L * 0.5;
+  p.x -= px * L;
+  p.y -= py * L;
+  p.z += c * (w * w);
+  const Adj o = l;
+  const double dzw = -2.0 * c * w;  // dz'/dw
+  l.px = o.x * L + o.px + o.z * dzw;
+  l.y = o.y + o.z * dzw * 2.0 * y1;
+  l.py = o.y * L + o.py + o.z * dzw * 2.0 * y1 * L;
+  l.d = o.d + o.z * L * pn * pn * pn * (w * w);
+}
 
 // ---- bend6: linear sector bend with gradient (ABend, atpass.cpp:76-187) ----
 // Its map mixes delta into every output through g1, g2 and the dispersion

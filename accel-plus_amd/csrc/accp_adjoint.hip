@@ -125,6 +125,9 @@ __device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const 
     case OP_DRIFT:
       drift_nl_rev(p, l, P[0]);
       break;
+    case OP_QUAD_K0:
+      drift_kzero_rev(p, l, P[0]);
+      break;
     case OP_QUAD: {
       const double L = P[0], K = P[1], aK = P[2], L2 = P[3];
       const bool foc = K > 0;

@@ -978,7 +978,7 @@ struct accp_adjoint {
 // Element maps the reverse sweep carries (the others make create fail).
 static bool adjoint_supports(const Hdr &h) {
   switch (h.op) {
-    case OP_NOP: case OP_DRIFT: case OP_STRMP: case OP_STRMP_THIN:
+    case OP_NOP: case OP_DRIFT: case OP_QUAD_K0: case OP_STRMP: case OP_STRMP_THIN:
     case OP_CAVITY: case OP_APERTURE: case OP_CORRECTOR: case OP_BENDMP: case OP_QUAD: case OP_BEND:
       return true;
     default:
@@ -996,9 +996,8 @@ extern "C" int accp_adjoint_create(accp_lattice *L, int nparam, const accp_param
   for (int e = 0; e < ne; ++e)
     if (!adjoint_supports(L->tab.hdr[e]))
       return fail(ACCP_E_LATTICE,
-                  "element %d (type %d): no reverse map (radiation, wake, K=0 Quad, or a type "
-                  "other than Drift, Quad, Bend, BendMPole, StrMPole, Cavity, Aperture, "
-                  "Corrector, Monitor)",
+                  "element %d (type %d): no reverse map (radiation, wake, or a type other than "
+                  "Drift, Quad, Bend, BendMPole, StrMPole, Cavity, Aperture, Corrector, Monitor)",
                   e, L->elems[e].type);
   std::vector<int32_t> slot(ne, -1), kind(ne, 0);
   for (int k = 0; k < nparam; ++k) {

@@ -100,6 +100,9 @@ int main() {
     check("drift", [](Part &p, double) { drift_nl(p, 0.7 * p.ipd); },
           [](Part &p, Adj &l, double *) { drift_nl_rev(p, l, 0.7); }, 0, false, s0);
 
+    check("quad K=0 (drift quirk)", [](Part &p, double) { drift_kzero(p, 0.3); },
+          [](Part &p, Adj &l, double *) { drift_kzero_rev(p, l, 0.3); }, 0, false, s0);
+
     check("edge", [](Part &p, double) { edge(p, 0.013); }, [](Part &p, Adj &l, double *) { edge_rev(p, l, 0.013); },
           0, false, s0);
 

@@ -282,3 +282,27 @@ def test_linear_bends(gpu_lib, oracle):
     err = np.abs(J - ref).max() / np.abs(ref).max()
     print(f"adjoint Jacobian vs oracle getm66 {err:.2e}")
     assert err <= 1e-7
+
+
+def test_zero_strength_quads(gpu_lib, oracle):
+    """A Quad with K = 0 runs the reference's drift() quirk (z from
+    (px + y^2)^2); the adjoint carries it: the FODO ring with two K = 0 quads
+    inserted, knob gradients against central differences of the GPU map and
+    of the oracle (1e-6)."""
+    ring = lat.fodo_ring()
+    ring.insert(2, el.quad(0.2, 0.0, "QZ"))
+    ring.insert(40, el.quad(0.2, 0.0, "QZ"))
+    knobs = lat.adjoint_knobs(ring)
+    beam = lat.gaussian_beam(64, seed=12) * 2.0
+    beam[:, 5] *= 0.05
+    L = gpu_lib.lattice(ring)
+    A = abi.Adjoint(gpu_lib, L, [(f, i, m) for _, f, i, m in knobs])
+    out = A.forward(beam, 8, ckpt_every=3)
+    assert not np.isnan(out).any()
+    grad, _ = A.backward(loss_and_seed(out)[1])
+    for k, (name, f, i, m) in enumerate(knobs):
+        for track in (lambda r: gpu_lib.lattice(r).track(beam, 8), lambda r: oracle.track(r, beam, 8)):
+            rel, fd = best_fd(grad[k], lambda h: loss_and_seed(track(lat.shift_knob(ring, f, i, m, h)))[0],
+                              fd_steps(ring, f, i, m))
+            print(f"{name}: adjoint {grad[k]:+.12e}  best fd {fd:+.12e}  rel {rel:.1e}")
+            assert rel < 1e-6, name
