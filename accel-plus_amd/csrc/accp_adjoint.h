@@ -476,6 +476,71 @@ ACCP_HD void drift_nl_rev(Part &p, Adj &l, double L) {
   drift_c_rev(p, l, nl, nl * (0.5 * p.ipd));
 }
 
+// ---- AMatrix: 6x6 map between half drifts (accel.cpp:1697-1729) -------------
+// Inverse by Gaussian elimination with partial pivoting on M r_in = r_out;
+// the VJP of the matrix is l <- M^T l.  A singular matrix (no inverse)
+// leaves the state as it is, NaN included.
+template <class PT>
+ACCP_HD void matrix_rev(Part &p, Adj &l, double L, const PT *m) {
+  if (L != 0) drift_nl_rev(p, l, 0.5 * L);
+  double a[6][7];
+  const double rhs[6] = {p.x, p.px, p.y, p.py, p.z, p.d};
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) a[i][j] = m[i * 6 + j];
+    a[i][6] = rhs[i];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    int piv = c;
+#pragma unroll
+    for (int i = c + 1; i < 6; ++i)
+      if (fabs(a[i][c]) > fabs(a[piv][c])) piv = i;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {  // swap rows c and piv (selects keep indexing static)
+      double t = a[c][j];
+#pragma unroll
+      for (int i = c + 1; i < 6; ++i)
+        if (i == piv) {
+          a[c][j] = a[i][j];
+          a[i][j] = t;
+        }
+    }
+#pragma unroll
+    for (int i = c + 1; i < 6; ++i) {
+      const double f = a[i][c] / a[c][c];
+#pragma unroll
+      for (int j = c; j < 7; ++j) a[i][j] -= f * a[c][j];
+    }
+  }
+  double x[6];
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {
+    double acc = a[i][6];
+#pragma unroll
+    for (int j = i + 1; j < 6; ++j) acc -= a[i][j] * x[j];
+    x[i] = acc / a[i][i];
+  }
+  const double lo[6] = {l.x, l.px, l.y, l.py, l.z, l.d};
+  double li[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) acc += m[i * 6 + j] * lo[i];
+    li[j] = acc;
+  }
+  l = Adj{li[0], li[1], li[2], li[3], li[4], li[5]};
+  p.x = x[0];
+  p.px = x[1];
+  p.y = x[2];
+  p.py = x[3];
+  p.z = x[4];
+  set_delta(p, x[5]);
+  if (L != 0) drift_nl_rev(p, l, 0.5 * L);
+}
+
 ACCP_HD void cavity_rev(Part &p, Adj &l, double L, double nv, double w_c0, double phis, bool on) {
   if (L != 0) drift_nl_rev(p, l, 0.5 * L);
   if (on) {

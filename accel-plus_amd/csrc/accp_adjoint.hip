@@ -115,9 +115,9 @@ __device__ __forceinline__ void strmp_rev_dispatch(Part &p, Adj &l, const Hdr h,
   }
 }
 
-// FULL = false leaves out the reverse maps of the trigonometric quad and the
-// linear Bend, whose registers would halve the occupancy of every lattice's
-// sweep; the host picks the variant per lattice.
+// FULL = false leaves out the reverse maps of the trigonometric quad, the
+// linear Bend and the Matrix, whose registers would halve the occupancy of
+// every lattice's sweep; the host picks the variant per lattice.
 template <bool FULL, class PT>
 __device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const PT *P, const Knob &kn,
                                             double *grad) {
@@ -127,6 +127,9 @@ __device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const 
       break;
     case OP_QUAD_K0:
       drift_kzero_rev(p, l, P[0]);
+      break;
+    case OP_MATRIX:
+      if (FULL) matrix_rev(p, l, P[0], P + 1);
       break;
     case OP_QUAD: {
       const double L = P[0], K = P[1], aK = P[2], L2 = P[3];

@@ -978,7 +978,7 @@ struct accp_adjoint {
 // Element maps the reverse sweep carries (the others make create fail).
 static bool adjoint_supports(const Hdr &h) {
   switch (h.op) {
-    case OP_NOP: case OP_DRIFT: case OP_QUAD_K0: case OP_STRMP: case OP_STRMP_THIN:
+    case OP_NOP: case OP_DRIFT: case OP_QUAD_K0: case OP_MATRIX: case OP_STRMP: case OP_STRMP_THIN:
     case OP_CAVITY: case OP_APERTURE: case OP_CORRECTOR: case OP_BENDMP: case OP_QUAD: case OP_BEND:
       return true;
     default:
@@ -997,7 +997,8 @@ extern "C" int accp_adjoint_create(accp_lattice *L, int nparam, const accp_param
     if (!adjoint_supports(L->tab.hdr[e]))
       return fail(ACCP_E_LATTICE,
                   "element %d (type %d): no reverse map (radiation, wake, or a type other than "
-                  "Drift, Quad, Bend, BendMPole, StrMPole, Cavity, Aperture, Corrector, Monitor)",
+                  "Drift, Quad, Bend, BendMPole, StrMPole, Matrix, Cavity, Aperture, Corrector, "
+                  "Monitor)",
                   e, L->elems[e].type);
   std::vector<int32_t> slot(ne, -1), kind(ne, 0);
   for (int k = 0; k < nparam; ++k) {
@@ -1156,7 +1157,7 @@ extern "C" int accp_adjoint_backward(accp_adjoint *A, const double *dl_dr_out, d
     c.lds_bytes = lds <= LDS_LIMIT ? (int32_t)lds : 0;
     c.full = 0;
     for (const Hdr &hd : L->tab.hdr)
-      if (hd.op == OP_BEND || (hd.op == OP_QUAD && hd.a == 0)) c.full = 1;
+      if (hd.op == OP_BEND || hd.op == OP_MATRIX || (hd.op == OP_QUAD && hd.a == 0)) c.full = 1;
     c.nturn = A->nturn;
     c.nseg_g = g;
     c.r_final = (const double *)A->d_r.p;

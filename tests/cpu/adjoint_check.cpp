@@ -103,6 +103,13 @@ int main() {
     check("quad K=0 (drift quirk)", [](Part &p, double) { drift_kzero(p, 0.3); },
           [](Part &p, Adj &l, double *) { drift_kzero_rev(p, l, 0.3); }, 0, false, s0);
 
+    {  // a coupled 6x6 matrix between half drifts (near the identity, det != 1)
+      static double M[36];
+      for (int i = 0; i < 36; ++i) M[i] = (i % 7 == 0 ? 1.0 : 0.0) + 0.05 * std::sin(1.3 * i + 0.4);
+      check("matrix", [](Part &p, double) { matrix_map(p, 0.4, M); },
+            [](Part &p, Adj &l, double *) { matrix_rev(p, l, 0.4, M); }, 0, false, s0);
+    }
+
     check("edge", [](Part &p, double) { edge(p, 0.013); }, [](Part &p, Adj &l, double *) { edge_rev(p, l, 0.013); },
           0, false, s0);
 
