@@ -15,8 +15,8 @@
 // ATbendhxdrift6 (:343-355),
 // bndthinkick (:199-235, normal multipoles with the curvature terms of
 // PolyB[1] and PolyB[2]), strthinkick (:358-414), edge
-// (:190-196), ACavity::pass (accel.cpp:903-923), ACorrector::pass
-// (accel.cpp:270-292).
+// (:190-196), AMatrix::pass (accel.cpp:1697-1729), ACavity::pass
+// (accel.cpp:903-923), ACorrector::pass (accel.cpp:270-292).
 #pragma once
 #include "accp_physics.h"
 

@@ -242,9 +242,9 @@ typedef struct accp_param {
 } accp_param;
 
 /* The reverse sweep carries Drift, Quad, Bend, BendMPole without radiation,
- * StrMPole, Cavity, Aperture, Corrector and Monitor; a lattice holding any
- * other element (radiation, a wake, Matrix, CrabCav, RFMult, the wigglers)
- * fails with ACCP_E_LATTICE. */
+ * StrMPole, Matrix, Cavity, Aperture, Corrector and Monitor; a lattice
+ * holding any other element (radiation, a wake, CrabCav, RFMult, the
+ * wigglers) fails with ACCP_E_LATTICE. */
 int accp_adjoint_create(accp_lattice *lat, int nparam, const accp_param *params,
                         accp_adjoint **out);
 /* Forward: track r0 (host AoS, n particles) for nturn turns, keeping a
