@@ -478,8 +478,9 @@ ACCP_HD void drift_nl_rev(Part &p, Adj &l, double L) {
 
 // ---- AMatrix: 6x6 map between half drifts (accel.cpp:1697-1729) -------------
 // Inverse by Gaussian elimination with partial pivoting on M r_in = r_out;
-// the VJP of the matrix is l <- M^T l.  A singular matrix (no inverse)
-// leaves the state as it is, NaN included.
+// the VJP of the matrix is l <- M^T l.  The host rejects a singular or
+// nearly singular Matrix before any reverse sweep (smallest pivot below
+// 1e-12 of the largest entry).
 template <class PT>
 ACCP_HD void matrix_rev(Part &p, Adj &l, double L, const PT *m) {
   if (L != 0) drift_nl_rev(p, l, 0.5 * L);

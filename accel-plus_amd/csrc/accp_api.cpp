@@ -189,7 +189,9 @@ int compile_lattice(const accp_elem *E, int n, Compiled &out) {
         const double uce0 = 0.665 * E0 * E0 / 1.0e6 / 2.99792458E8 * std::fabs(irho) * 1000;
         const double cq = 55 * std::sqrt(3.0) / 72.0 * uce0;
         const double hs1 = irho * l1, hs2 = irho * l2, half = 0.5 * sl, hsh = irho * half;
-        const double k1c = mo >= 1 ? e.polyB[1] : 0.0;
+        // bndthinkick reads K1 = B[1] whatever MaxNPole is, K2 = B[2] only
+        // from max_order 2 (atpass.cpp:211-217)
+        const double k1c = e.polyB[1];
         const double k2c = mo >= 2 ? e.polyB[2] : 0.0;
         int hi = highest_nonzero(e.polyB, nullptr, mo + 1);
         if (hi < 0) hi = 0;
@@ -214,7 +216,9 @@ int compile_lattice(const accp_elem *E, int n, Compiled &out) {
       case ACCP_STRMPOLE: {
         if (e.max_npole < 1 || e.max_npole > ACCP_MAX_MPOLE || (e.len != 0 && e.nslice < 1))
           return fail(ACCP_E_LATTICE, "element %d: StrMPole needs 1<=MaxNPole<=10, nslice>=1", i);
-        const int hi = highest_nonzero(e.polyB, e.polyA, e.max_npole);
+        // strthinkick always sums to order MAX_MPOLE-1 (accel.cpp:826-839), and
+        // setfield accepts an index up to MaxNPole (accel.cpp:773-783)
+        const int hi = highest_nonzero(e.polyB, e.polyA, ACCP_MAX_MPOLE);
         bool single = hi >= 1 && hi <= 3;
         for (int j = 0; j <= hi && single; ++j) single = e.polyA[j] == 0 && (j == hi || e.polyB[j] == 0);
         h.b = hi | (single ? STRMP_SINGLE : 0);
@@ -986,6 +990,63 @@ static bool adjoint_supports(const Hdr &h) {
   }
 }
 
+// Smallest pivot of Gaussian elimination with partial pivoting on the 6x6
+// matrix, relative to its largest entry (0 for a singular matrix).
+static double matrix_pivot_ratio(const double *m) {
+  double a[6][6], big = 0.0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      a[i][j] = m[6 * i + j];
+      big = std::max(big, std::fabs(a[i][j]));
+    }
+  if (!(big > 0) || !std::isfinite(big)) return 0.0;
+  double small = big;
+  for (int c = 0; c < 6; ++c) {
+    int piv = c;
+    for (int i = c + 1; i < 6; ++i)
+      if (std::fabs(a[i][c]) > std::fabs(a[piv][c])) piv = i;
+    for (int j = 0; j < 6; ++j) std::swap(a[c][j], a[piv][j]);
+    small = std::min(small, std::fabs(a[c][c]));
+    if (a[c][c] == 0) return 0.0;
+    for (int i = c + 1; i < 6; ++i) {
+      const double f = a[i][c] / a[c][c];
+      for (int j = c; j < 6; ++j) a[i][j] -= f * a[c][j];
+    }
+  }
+  return small / big;
+}
+
+// The reverse sweep needs every element of the lattice to have a reverse
+// map, every Matrix to be invertible, and every K knob member to be a Quad
+// with K != 0.  accp_lattice_update can change the lattice after create,
+// so forward and backward check again.
+static int adjoint_check(const accp_lattice *L, const std::vector<int32_t> &slot,
+                         const std::vector<int32_t> &kind) {
+  const int ne = (int)L->elems.size();
+  if ((int)slot.size() != ne) return fail(ACCP_E_LATTICE, "the lattice changed its element count after accp_adjoint_create");
+  for (int e = 0; e < ne; ++e) {
+    const Hdr &h = L->tab.hdr[e];
+    if (!adjoint_supports(h))
+      return fail(ACCP_E_LATTICE,
+                  "element %d (type %d): no reverse map (radiation, wake, or a type other than "
+                  "Drift, Quad, Bend, BendMPole, StrMPole, Matrix, Cavity, Aperture, Corrector, "
+                  "Monitor)",
+                  e, L->elems[e].type);
+    if (h.op == OP_MATRIX && !(matrix_pivot_ratio(L->elems[e].mat) > 1e-12))
+      return fail(ACCP_E_LATTICE,
+                  "element %d: Matrix is singular or nearly so (smallest pivot below 1e-12 of its "
+                  "largest entry); the reverse sweep inverts it",
+                  e);
+    if (slot[e] >= 0 && kind[e] == 1 && h.op == OP_QUAD_K0)
+      return fail(ACCP_E_LATTICE,
+                  "knob %d: element %d is a Quad with K = 0, whose map (the reference's drift(), "
+                  "atpass.cpp:25-28) does not depend on K and jumps in z at K = 0; it has no K "
+                  "derivative there",
+                  slot[e], e);
+  }
+  return ACCP_OK;
+}
+
 extern "C" int accp_adjoint_create(accp_lattice *L, int nparam, const accp_param *params,
                                    accp_adjoint **out) {
   if (!L || !out || nparam < 0 || (nparam > 0 && !params))
@@ -993,13 +1054,6 @@ extern "C" int accp_adjoint_create(accp_lattice *L, int nparam, const accp_param
   *out = nullptr;
   if (nparam > ADJ_MAX_PARAM) return fail(ACCP_E_ARG, "at most %d knobs", ADJ_MAX_PARAM);
   const int ne = (int)L->elems.size();
-  for (int e = 0; e < ne; ++e)
-    if (!adjoint_supports(L->tab.hdr[e]))
-      return fail(ACCP_E_LATTICE,
-                  "element %d (type %d): no reverse map (radiation, wake, or a type other than "
-                  "Drift, Quad, Bend, BendMPole, StrMPole, Matrix, Cavity, Aperture, Corrector, "
-                  "Monitor)",
-                  e, L->elems[e].type);
   std::vector<int32_t> slot(ne, -1), kind(ne, 0);
   for (int k = 0; k < nparam; ++k) {
     const accp_param &pr = params[k];
@@ -1022,6 +1076,7 @@ extern "C" int accp_adjoint_create(accp_lattice *L, int nparam, const accp_param
       slot[e] = k;
     }
   }
+  if (int rc = adjoint_check(L, slot, kind)) return rc;
   accp_adjoint *A = new accp_adjoint;
   A->L = L;
   A->nparam = nparam;
@@ -1048,8 +1103,9 @@ extern "C" int accp_adjoint_forward(accp_adjoint *A, const double *r0, int n, in
   if (!A || !r0 || n <= 0 || nturn <= 0 || ckpt_every <= 0)
     return fail(ACCP_E_ARG, "accp_adjoint_forward: bad arguments");
   accp_lattice *L = A->L;
-  const size_t bytes = (size_t)n * 6 * sizeof(double);
   A->forward_done = false;
+  if (int rc = adjoint_check(L, A->knob_slot, A->knob_kind)) return rc;
+  const size_t bytes = (size_t)n * 6 * sizeof(double);
   A->n = n;
   A->nturn = nturn;
   A->ckpt = ckpt_every;
@@ -1093,6 +1149,7 @@ extern "C" int accp_adjoint_backward(accp_adjoint *A, const double *dl_dr_out, d
   if (!A || !dl_dr_out) return fail(ACCP_E_ARG, "accp_adjoint_backward: bad arguments");
   if (!A->forward_done) return fail(ACCP_E_STATE, "accp_adjoint_backward before accp_adjoint_forward");
   accp_lattice *L = A->L;
+  if (int rc = adjoint_check(L, A->knob_slot, A->knob_kind)) return rc;
   const int n = A->n, C = A->ckpt, nseg = A->nseg, np = A->nparam;
   const size_t bytes = (size_t)n * 6 * sizeof(double);
   // Sequential sweep: n lanes, each walking every turn.  Segment Jacobians:

@@ -320,18 +320,31 @@ extern "C" int accp_setrfphase(accp_lattice *L, double *phis) {
   Records R;
   int rc = R.load(L);
   if (rc) return rc;
-  // aline.cpp:468-474: cavities off, classical radiation, one turn from the origin
+  // aline.cpp:468-474: cavities off, classical radiation, one turn from the
+  // origin.  A LongWake is off for the probe: the lone probe particle would
+  // see its own field, the whole bunch charge's q k (0.31 MeV for 2 nC at
+  // 5 kOhm, 10 GHz, Q = 1), which is not the synchronous loss.  The
+  // reference's only wake, ATransWake, cannot kick a lone particle
+  // (accel.cpp:1638), so its probe sees no wake either.
+  std::vector<int32_t> wake_on;
   for (auto &e : R.e) {
     if (e.type == ACCP_CAVITY) e.on = 0;
     if (e.type == ACCP_BENDMPOLE) e.rad = ACCP_RAD_DAMP;
+    if (e.type == ACCP_LONGWAKE) {
+      wake_on.push_back(e.on);
+      e.on = 0;
+    }
   }
   if ((rc = R.store(L))) return rc;
   double r[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   accp_track_opts o{};
   if ((rc = accp_track(L, r, 1, 1, 0, -1, &o))) return rc;
   const double delta = -r[5];
-  for (auto &e : R.e)
+  size_t iw = 0;
+  for (auto &e : R.e) {
     if (e.type == ACCP_BENDMPOLE) e.rad = ACCP_RAD_OFF;
+    if (e.type == ACCP_LONGWAKE) e.on = wake_on[iw++];
+  }
   // aline.cpp:477-499
   double tot_volt = 0, energy = 0;
   int ncav = 0;

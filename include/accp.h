@@ -314,7 +314,9 @@ int accp_correct(accp_lattice *lat, int quantity, int method, const int32_t *fam
 /* ALine::setrfphase() (src/lib/accel/aline.cpp:465-501): one turn of a
  * particle at the origin with classical radiation and the cavities off gives
  * the energy loss U0/E; every cavity's phase becomes asin(U0/V_total).  Like
- * the reference it leaves the cavities off and radiation off.  *phis = 0 and
+ * the reference it leaves the cavities off and radiation off.  LongWake
+ * elements are off for the probe turn (a lone particle would see the whole
+ * bunch charge's own field) and keep their flag afterwards.  *phis = 0 and
  * nothing changes when the lattice has no cavity. */
 int accp_setrfphase(accp_lattice *lat, double *phis);
 /* ALine::trackDA (src/lib/accel/aline.cpp:607-715): nray (3..51) polar rays

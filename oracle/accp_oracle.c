@@ -808,14 +808,23 @@ void orc_correct(accp_elem *elems, int nelem, int kind, const int *fam1, int n1,
 /* aline.cpp:465-501 */
 double orc_setrfphase(accp_elem *elems, int nelem) {
   double r[6] = {0, 0, 0, 0, 0, 0};
+  /* LongWake (no reference counterpart) is off for the probe, which would
+   * otherwise see the whole bunch charge's own field; ATransWake, the
+   * reference's only wake, cannot kick a lone particle (accel.cpp:1638). */
+  int *wake_on = (int *)malloc(sizeof(int) * (size_t)(nelem > 0 ? nelem : 1));
   for (int k = 0; k < nelem; k++) {
     if (elems[k].type == ACCP_CAVITY) elems[k].on = 0;
     if (elems[k].type == ACCP_BENDMPOLE) elems[k].rad = 1;
+    wake_on[k] = elems[k].on;
+    if (elems[k].type == ACCP_LONGWAKE) elems[k].on = 0;
   }
   orc_line_pass(elems, nelem, r, 1, 0, nelem, 1, 0, 0, 0);
   double delta = -r[5];
-  for (int k = 0; k < nelem; k++)
+  for (int k = 0; k < nelem; k++) {
     if (elems[k].type == ACCP_BENDMPOLE) elems[k].rad = 0;
+    if (elems[k].type == ACCP_LONGWAKE) elems[k].on = wake_on[k];
+  }
+  free(wake_on);
   double volt = 0, energy = 0;
   int ncav = 0;
   for (int k = 0; k < nelem; k++)

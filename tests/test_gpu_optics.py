@@ -156,6 +156,12 @@ def test_setrfphase_matches_reference(gpu_lib):
     assert all(e.rad == 0 for e in arr if e.type == el.BENDMPOLE)
     assert all(e.on == 0 and e.phis == phis for e in arr if e.type == el.CAVITY)
     assert gpu_lib.lattice(lat.fodo_ring()).setrfphase() == 0.0  # no cavity: nothing to set
+    # a LongWake sits out the probe turn (a lone particle would see the whole
+    # bunch's own field) and keeps its flag
+    wk = el.longwake(5e3, 1.0, 10e9, 2e-9, 256, -0.15, 0.15, energy=3000.0, on=1)
+    Lw = gpu_lib.lattice(lat.da_ring(0) + [wk])
+    assert Lw.setrfphase() == phis
+    assert [e.on for e in Lw.get() if e.type == el.LONGWAKE] == [1]
 
 
 def da_lattice(oracle):

@@ -364,3 +364,62 @@ def test_fodo_1000_particles_1000_turns(gpu_lib, oracle):
     assert np.array_equal(fg, fo) and fg.sum() > 900
     err = assert_coords_close(g[fg], o[fo], 1e-9, what="1000 turns")
     print("max relative difference after 1000 turns", err)
+
+
+def da_maps_agree(surv_gpu, surv_ref, shape, what):
+    """DA survival maps identical except next to the survival boundary: a
+    point whose outcome differs must have both survivors and lost points in
+    the reference map within one grid cell of it."""
+    sg, so = surv_gpu.reshape(shape), surv_ref.reshape(shape)
+    for d, iy, ix in np.argwhere(sg != so):
+        nb = so[d, max(iy - 1, 0):iy + 2, max(ix - 1, 0):ix + 2]
+        assert nb.any() and not nb.all(), f"{what}: survival differs away from the boundary at {(d, iy, ix)}"
+    return int((sg != so).sum())
+
+
+def test_batched_da_64_seeds_full_size(gpu_lib, oracle):
+    """The batched DA workload at its full size: 64 error seeds (Quad K and
+    SF/SD strength errors on top of the multipole errors), uneven grids of
+    500-8000 points (274,385 in all), 1024 turns in one captured graph.
+    (1) Survival maps and coordinates equal 64 separate plain calls bit for
+    bit.  (2) On the seeds with the smallest and largest grids and two more,
+    the batched launch after one turn matches the oracle at 1e-12 per
+    coordinate (aline.cpp:359-363), and after 64 turns its survival maps
+    match the oracle's except within one grid cell of the boundary
+    (aline.cpp:607-715)."""
+    import pyaccp.abi as abi
+    from pyaccp.hipmem import DeviceArray, synchronize
+
+    elems, beam, offs = lat.batched_workload(64)
+    assert len(beam) == 274_385
+    lats = [gpu_lib.lattice(s) for s in elems]
+    B = abi.Batch(gpu_lib, lats)
+    d = DeviceArray(beam)
+    B.capture(d.ptr, offs, 1024)
+    B.replay()
+    synchronize()
+    batched = d.numpy()
+    plain = np.concatenate([L.track(beam[offs[s]:offs[s + 1]], 1024) for s, L in enumerate(lats)])
+    print("survivors after 1024 turns:", int((~np.isnan(batched[:, 0])).sum()))
+    assert np.array_equal(np.isnan(batched), np.isnan(plain))
+    assert np.array_equal(np.nan_to_num(batched), np.nan_to_num(plain))
+
+    sizes = np.diff(offs)
+    pick = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), 0, 37})
+    d1 = DeviceArray(beam)
+    B.track(d1.ptr, offs, 1)
+    d64 = DeviceArray(beam)
+    B.track(d64.ptr, offs, 64)
+    synchronize()
+    one, t64 = d1.numpy(), d64.numpy()
+    for s in pick:
+        sl = slice(offs[s], offs[s + 1])
+        nx, ny = lat.seed_grid_shape(s)
+        assert sizes[s] == 5 * nx * ny
+        err = assert_coords_close(one[sl], oracle.track(elems[s], beam[sl], 1), 1e-12,
+                                  what=f"seed {s}, one turn")
+        o64 = oracle.track_omp(elems[s], beam[sl], 64, 16)
+        nd = da_maps_agree(~np.isnan(t64[sl, 0]), ~np.isnan(o64[:, 0]), (5, ny, nx), f"seed {s}, 64 turns")
+        print(f"seed {s}: {sizes[s]} points ({nx}x{ny}x5), one-turn max scaled difference {err:.1e}, "
+              f"64-turn survivors gpu {int((~np.isnan(t64[sl, 0])).sum())} "
+              f"oracle {int((~np.isnan(o64[:, 0])).sum())}, boundary differences {nd}")

@@ -73,6 +73,10 @@ def test_radiation_and_rf_phase_bit_exact(oracle):
     assert same_bits([oracle.setrfphase(arr)], g["phis"])
     # the reference leaves radiation and cavities as setrfphase sets them
     assert all(arr[i].rad == 0 for i in range(len(arr)) if arr[i].type == el.BENDMPOLE)
+    # a LongWake sits out the probe turn and keeps its flag
+    wk = el.longwake(5e3, 1.0, 10e9, 2e-9, 256, -0.15, 0.15, energy=3000.0, on=1)
+    arrw = el.to_array(lat.da_ring(0) + [wk])
+    assert same_bits([oracle.setrfphase(arrw)], g["phis"]) and arrw[len(arrw) - 1].on == 1
 
 
 def test_trackda_bit_exact(oracle):
