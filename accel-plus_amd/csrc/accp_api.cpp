@@ -105,7 +105,17 @@ struct Compiled {
   std::vector<double> par;
   std::vector<int> wakes;  // element indices of LongWake records
   bool full = false;       // needs the kernel variant with every element map
+  // Grouped table for the tracker: a drift merged into the element after
+  // it and a thin StrMPole kick into the element before it, one dispatch
+  // per group (fhdr/fpar).  fmap[i] is the group starting at element i (-1
+  // inside a group), fmap[n] the number of groups; empty when the lattice
+  // holds elements keyed by their index (random numbers) or a wake.
+  std::vector<Hdr> fhdr;
+  std::vector<double> fpar;
+  std::vector<int32_t> fmap;
 };
+
+void build_groups(Compiled &c);
 
 // Elements the lean kernel variant does not carry.
 bool needs_full(const Hdr &h) {
@@ -334,10 +344,62 @@ int compile_lattice(const accp_elem *E, int n, Compiled &out) {
     out.full = out.full || needs_full(h);
   }
   if (out.par.empty()) out.par.push_back(0.0);
+  build_groups(out);
   return ACCP_OK;
 }
 
 size_t lds_bytes_for(size_t nelem, size_t npar) { return nelem * sizeof(Hdr) + npar * sizeof(double); }
+
+// Groups of the tracker's table.  Merging changes no arithmetic: the drift,
+// the element and the kick run the same maps in the same order in one
+// dispatch.  A particle lost on entry to a group stays frozen through all of
+// it, as it would through each element; an aperture never takes a trailing
+// kick, so its loss codes and frozen coordinates are untouched.  Elements
+// whose random numbers are keyed on their index, and wakes (the turn splits
+// there), leave the lattice ungrouped.
+void build_groups(Compiled &c) {
+  c.fhdr.clear();
+  c.fpar.clear();
+  c.fmap.clear();
+  const int n = (int)c.hdr.size();
+  for (const Hdr &h : c.hdr)
+    if (h.op == OP_WAKE || h.op == OP_BENDMP_QE || h.op == OP_SYNCRAD || h.op == OP_CRABCAV || h.op == OP_RFMULT)
+      return;
+  auto len = [&](int i) { return (i + 1 < n ? c.hdr[i + 1].off : (int32_t)c.par.size()) - c.hdr[i].off; };
+  auto main_ok = [&](int i) { return c.hdr[i].op != OP_NOP && c.hdr[i].op != OP_WAKE; };
+  c.fmap.assign(n + 1, -1);
+  int i = 0;
+  while (i < n) {
+    const int start = i;
+    Hdr g{0, (int32_t)c.fpar.size(), 0, 0};
+    int flags = 0;
+    if (c.hdr[i].op == OP_DRIFT && i + 1 < n && main_ok(i + 1)) {
+      flags |= FUSE_PRE;
+      c.fpar.push_back(c.par[c.hdr[i].off]);
+      ++i;
+    }
+    const Hdr m = c.hdr[i];
+    const size_t mstart = c.fpar.size();
+    c.fpar.insert(c.fpar.end(), c.par.begin() + m.off, c.par.begin() + m.off + len(i));
+    ++i;
+    const size_t delta = c.fpar.size() - mstart;
+    if (main_ok(i - 1) && m.op != OP_APERTURE && i < n && c.hdr[i].op == OP_STRMP_THIN && c.hdr[i].b != -1 &&
+        delta <= FUSE_POST_MAX_DELTA) {
+      const int pb = c.hdr[i].b;
+      flags |= FUSE_POST | ((int)delta << FUSE_POST_DELTA_SHIFT) | ((pb & 0xff) << FUSE_POST_HI_SHIFT) |
+               ((pb & STRMP_SINGLE) ? FUSE_POST_SINGLE : 0);
+      c.fpar.insert(c.fpar.end(), c.par.begin() + c.hdr[i].off, c.par.begin() + c.hdr[i].off + len(i));
+      ++i;
+    }
+    g.op = m.op | flags;
+    g.a = m.a;
+    g.b = m.b;
+    c.fmap[start] = (int32_t)c.fhdr.size();
+    c.fhdr.push_back(g);
+  }
+  c.fmap[n] = (int32_t)c.fhdr.size();
+  if (c.fpar.empty()) c.fpar.push_back(0.0);
+}
 
 constexpr size_t LDS_LIMIT = 64 * 1024;
 
@@ -518,12 +580,24 @@ struct accp_lattice {
   std::vector<std::vector<double>> tables;  // WigTable data the records point at
   Compiled tab;
   DevBuf d_hdr, d_par, d_meta;  // meta: lat_hdr0, lat_par0, lat_npar of one lattice
+  DevBuf d_fhdr, d_fpar, d_fmeta;  // the grouped table, when the lattice has one
   DevBuf d_beam, d_snap, d_stat, d_loss;
   WorkSet ws;
   WakeState wake;
   TableRef table() const {
     return TableRef{(const Hdr *)d_hdr.p, (const double *)d_par.p, (const int32_t *)d_meta.p, 1,
                     (int)tab.hdr.size(), lds_bytes_for(tab.hdr.size(), tab.par.size()), tab.full};
+  }
+  // The grouped table when [st, ed) starts and ends on group boundaries
+  // (st and ed are then rewritten as group indices), else the element table.
+  TableRef table_for(int &st, int &ed) const {
+    if (!tab.fmap.empty() && tab.fmap[st] >= 0 && tab.fmap[ed] >= 0) {
+      st = tab.fmap[st];
+      ed = tab.fmap[ed];
+      return TableRef{(const Hdr *)d_fhdr.p, (const double *)d_fpar.p, (const int32_t *)d_fmeta.p, 1,
+                      (int)tab.fhdr.size(), lds_bytes_for(tab.fhdr.size(), tab.fpar.size()), tab.full};
+    }
+    return table();
   }
 };
 
@@ -553,6 +627,16 @@ static int upload_lattice(accp_lattice *L) {
   HIPCHK(hipMemcpy(L->d_par.p, L->tab.par.data(), np * sizeof(double), hipMemcpyHostToDevice));
   int32_t meta[3] = {0, 0, (int32_t)np};
   HIPCHK(hipMemcpy(L->d_meta.p, meta, sizeof meta, hipMemcpyHostToDevice));
+  if (!L->tab.fmap.empty()) {
+    const size_t fh = L->tab.fhdr.size(), fp = L->tab.fpar.size();
+    if ((rc = L->d_fhdr.ensure(std::max<size_t>(fh, 1) * sizeof(Hdr)))) return rc;
+    if ((rc = L->d_fpar.ensure(fp * sizeof(double)))) return rc;
+    if ((rc = L->d_fmeta.ensure(3 * sizeof(int32_t)))) return rc;
+    if (fh) HIPCHK(hipMemcpy(L->d_fhdr.p, L->tab.fhdr.data(), fh * sizeof(Hdr), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L->d_fpar.p, L->tab.fpar.data(), fp * sizeof(double), hipMemcpyHostToDevice));
+    int32_t fmeta[3] = {0, 0, (int32_t)fp};
+    HIPCHK(hipMemcpy(L->d_fmeta.p, fmeta, sizeof fmeta, hipMemcpyHostToDevice));
+  }
   return wake_prepare(L->wake, L->elems.data(), (int)L->elems.size(), L->tab.wakes);
 }
 
@@ -606,6 +690,9 @@ extern "C" int accp_lattice_free(accp_lattice *L) {
   L->d_hdr.release();
   L->d_par.release();
   L->d_meta.release();
+  L->d_fhdr.release();
+  L->d_fpar.release();
+  L->d_fmeta.release();
   L->d_beam.release();
   L->d_snap.release();
   L->d_stat.release();
@@ -624,15 +711,16 @@ static int track_device_impl(accp_lattice *L, double *d_r, int64_t n, const RunS
   const int64_t offs[2] = {0, n};
   int block, ppl, chunk_max;
   bool lds;
-  const TableRef T = L->table();
+  std::vector<int> cuts;
+  for (int w : L->tab.wakes)
+    if (w >= R.st && w < R.ed) cuts.push_back(w);
+  RunSpec Rg = R;
+  const TableRef T = cuts.empty() ? L->table_for(Rg.st, Rg.ed) : L->table();
   int rc = resolve_tuning(tuning, T.lds_need, block, ppl, lds, chunk_max);
   if (rc) return rc;
   int max_tiles = 0;
   if ((rc = prepare_workset(L->ws, offs, 1, block * ppl, max_tiles))) return rc;
-  std::vector<int> cuts;
-  for (int w : L->tab.wakes)
-    if (w >= R.st && w < R.ed) cuts.push_back(w);
-  if (cuts.empty()) return enqueue_run(T, L->ws, max_tiles, d_r, n, R, tuning, s);
+  if (cuts.empty()) return enqueue_run(T, L->ws, max_tiles, d_r, n, Rg, tuning, s);
   if (R.snap) return fail(ACCP_E_ARG, "monitor records with a wake element are not supported");
   accp_tuning one = tuning ? *tuning : accp_tuning{0, 0, 0, 0};
   one.compact = -1;  // every segment sees the whole beam
@@ -808,20 +896,26 @@ extern "C" int accp_batch_create(accp_lattice *const *seeds, int nseed, accp_bat
     if ((int)seeds[s]->elems.size() != ne) return fail(ACCP_E_LATTICE, "seed %d: element count differs", s);
     if (!seeds[s]->tab.wakes.empty()) return fail(ACCP_E_LATTICE, "seed %d: wake elements are not batched", s);
   }
+  // whole turns only: the grouped tables when every seed has one with the
+  // same group boundaries
+  bool grouped = !seeds[0]->tab.fmap.empty();
+  for (int s = 1; s < nseed && grouped; ++s) grouped = seeds[s]->tab.fmap == seeds[0]->tab.fmap;
   accp_batch *b = new accp_batch;
   b->nseed = nseed;
-  b->nelem = ne;
+  b->nelem = grouped ? (int)seeds[0]->tab.fhdr.size() : ne;
   std::vector<Hdr> hdr;
   std::vector<double> par;
   std::vector<int32_t> h0, p0, np;
   for (int s = 0; s < nseed; ++s) {
     const Compiled &c = seeds[s]->tab;
+    const std::vector<Hdr> &ch = grouped ? c.fhdr : c.hdr;
+    const std::vector<double> &cp = grouped ? c.fpar : c.par;
     h0.push_back((int32_t)hdr.size());
     p0.push_back((int32_t)par.size());
-    np.push_back((int32_t)c.par.size());
-    hdr.insert(hdr.end(), c.hdr.begin(), c.hdr.end());
-    par.insert(par.end(), c.par.begin(), c.par.end());
-    b->max_lds = std::max(b->max_lds, lds_bytes_for(c.hdr.size(), c.par.size()));
+    np.push_back((int32_t)cp.size());
+    hdr.insert(hdr.end(), ch.begin(), ch.end());
+    par.insert(par.end(), cp.begin(), cp.end());
+    b->max_lds = std::max(b->max_lds, lds_bytes_for(ch.size(), cp.size()));
     b->full = b->full || c.full;
   }
   std::vector<int32_t> meta(h0);

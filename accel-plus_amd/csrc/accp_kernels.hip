@@ -503,6 +503,10 @@ __device__ __forceinline__ void element_body(Part (&p)[PPL], const Hdr h, const 
 // slower on the latency-bound tail of a dynamic-aperture run.)  Padding
 // lanes of a partly filled tile are never written back, so they compute
 // freely: the restore runs only in waves that hold a lost particle.
+//
+// A record of the grouped table may also carry a drift run before the
+// element (FUSE_PRE, its length first) and a thin StrMPole kick after it
+// (FUSE_POST): one dispatch, the same maps in the same order.
 template <int PPL, bool FULL, class PT>
 __device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const PT *P,
                                               const KernelCtx &ctx, const uint64_t (&pid)[PPL],
@@ -514,25 +518,42 @@ __device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const
     keep[k] = p[k];
     live[k] = is_live(p[k]) || !inb[k];
   }
-  element_body<PPL, FULL>(p, h, P, ctx, pid, turn, eidx);
+  const PT *Pm = P;
+  if (h.op & FUSE_PRE) {
+    const double L = P[0];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) drift_nl(p[k], L * p[k].ipd);
+    Pm = P + 1;
+  }
+  element_body<PPL, FULL>(p, Hdr{h.op & FUSE_OP_MASK, h.off, h.a, h.b}, Pm, ctx, pid, turn, eidx);
+  if (h.op & FUSE_POST) {
+    const PT *Pk = Pm + ((h.op >> FUSE_POST_DELTA_SHIFT) & FUSE_POST_MAX_DELTA);
+    const int pb = ((h.op >> FUSE_POST_HI_SHIFT) & 0xf) | ((h.op & FUSE_POST_SINGLE) ? STRMP_SINGLE : 0);
+    strmp_dispatch<PPL, false>(p, Hdr{OP_STRMP_THIN, 0, 0, pb}, Pk);
+  }
 #pragma unroll
   for (int k = 0; k < PPL; ++k)
     if (!live[k]) p[k] = keep[k];
 }
 
+// Headers: the next one is read while the current element runs.  From LDS
+// it arrives in vector registers and becomes wave-uniform (readfirstlane)
+// only after that element, so the read's latency hides behind it.
+struct HdrRaw {
+  int32_t op, off, a, b;
+};
+
 template <bool LDS>
-__device__ __forceinline__ Hdr load_hdr(const Hdr *hdr, int e) {
-  Hdr h;
-  if (LDS) {
-    const Hdr v = hdr[e];
-    h.op = __builtin_amdgcn_readfirstlane(v.op);
-    h.off = __builtin_amdgcn_readfirstlane(v.off);
-    h.a = __builtin_amdgcn_readfirstlane(v.a);
-    h.b = __builtin_amdgcn_readfirstlane(v.b);
-  } else {
-    h = hdr[e];
-  }
-  return h;
+__device__ __forceinline__ HdrRaw load_hdr_raw(const Hdr *hdr, int e) {
+  const Hdr v = hdr[e];
+  return HdrRaw{v.op, v.off, v.a, v.b};
+}
+
+template <bool LDS>
+__device__ __forceinline__ Hdr uniform_hdr(const HdrRaw &v) {
+  if (!LDS) return Hdr{v.op, v.off, v.a, v.b};
+  return Hdr{__builtin_amdgcn_readfirstlane(v.op), __builtin_amdgcn_readfirstlane(v.off),
+             __builtin_amdgcn_readfirstlane(v.a), __builtin_amdgcn_readfirstlane(v.b)};
 }
 
 // ---- the persistent tracker ---------------------------------------------------
@@ -613,11 +634,11 @@ __global__ __launch_bounds__(BLOCK) void k_track(KernelCtx ctx) {
     if (!__any(any)) break;  // every particle of this wave is lost: frozen from here on
     const uint64_t turn = ctx.turn0 + (uint64_t)t;
     // header of the next element is fetched while the current one runs
-    Hdr h = load_hdr<LDS>(hdr, ctx.st);
+    Hdr h = uniform_hdr<LDS>(load_hdr_raw<LDS>(hdr, ctx.st));
     for (int e = ctx.st; e < ctx.ed; ++e) {
-      const Hdr hn = load_hdr<LDS>(hdr, e + 1 < ctx.ed ? e + 1 : e);
+      const HdrRaw hn = load_hdr_raw<LDS>(hdr, e + 1 < ctx.ed ? e + 1 : e);
       apply_element<PPL, FULL>(p, h, par + h.off, ctx, pid, inb, turn, e);
-      h = hn;
+      h = uniform_hdr<LDS>(hn);
     }
   }
   // monitor slots after an early exit hold the frozen state

@@ -48,9 +48,22 @@ struct Hdr {
 // (PolyA zero, PolyB zero below hi), 1 <= hi <= 3.
 constexpr int STRMP_SINGLE = 0x100;
 
+// Grouped table (the tracker's): Hdr::op carries the op code in its low
+// byte and, above it, a drift run before the element (its length first
+// in the record) and a thin StrMPole kick run after it (PolyB[0..hi],
+// PolyA[0..hi] at FUSE_POST_DELTA doubles past the element's own constants).
+constexpr int32_t FUSE_OP_MASK = 0xff;
+constexpr int32_t FUSE_PRE = 1 << 8;
+constexpr int32_t FUSE_POST = 1 << 9;
+constexpr int FUSE_POST_DELTA_SHIFT = 10;
+constexpr int FUSE_POST_MAX_DELTA = 1023;
+constexpr int FUSE_POST_HI_SHIFT = 20;
+constexpr int32_t FUSE_POST_SINGLE = 1 << 24;
+
 // Offsets inside an OP_BENDMP* record.  HS* = irho*L and HQ* = (1/4)*HS*L
 // of the pseudo-drift lengths L1, L2 and SL/2; K1C/K2C are PolyB[1]/[2] as the
-// curvature terms of bndthinkick read them (zero beyond MaxNPole); CQ is
+// curvature terms of bndthinkick read them (PolyB[1] whatever MaxNPole,
+// PolyB[2] from MaxNPole 3 on); CQ is
 // 55*sqrt(3)/72 * uc/E0 of the quantum-excitation sigma.
 enum {
   BM_PSI1 = 0, BM_PSI2, BM_IRHO, BM_L1, BM_L2, BM_K1, BM_K2, BM_SL, BM_CRAD, BM_CQ,

@@ -5,8 +5,9 @@ resonator wake (5 kOhm, Q = 1, 10 GHz, 2 nC in 256 bins over +-15 cm) at the
 end of the ring; 4096 Gaussian particles, 8000 turns (two longitudinal
 damping times), with the counter-based normal stream the GPU draws.  Every
 250 turns it records the survivors, the energy spread, the bunch length, the
-horizontal emittance and the bunch centroid.  Writes
-tests/golden/collective.npz.
+projected horizontal emittance, the bunch centroid, and the betatron
+emittance (dispersion taken out) twice: from the second moments and from
+the median of the betatron action.  Writes tests/golden/collective.npz.
 
 The wake couples the particles, so each turn runs the elements before the
 wake in parallel over particles (they are independent there) and then the
@@ -51,16 +52,46 @@ def initial_beam():
     return lat.gaussian_beam(N, seed=13)
 
 
-def record(stat, turn):
-    """turn, survivors, sigma_delta, sigma_z, horizontal emittance, <z>, <delta>
-    from a calc_beam_stat record."""
+def optics_at_start(oracle, ring):
+    """beta_x, alpha_x and the dispersion (eta_x, eta_x') at the ring start
+    from the oracle's one-turn matrix of the ring without radiation, cavity
+    and wake."""
+    lin = [e.copy() for e in ring]
+    for e in lin:
+        if e.type == el.BENDMPOLE:
+            e.rad = 0
+        if e.type in (el.CAVITY, el.LONGWAKE):
+            e.on = 0
+    m = oracle.getm66(lin)
+    eta = np.linalg.solve(np.eye(4) - m[:4, :4], m[:4, 5])
+    c = 0.5 * (m[0, 0] + m[1, 1])
+    s = np.copysign(np.sqrt(1.0 - c * c), m[0, 1])
+    return m[0, 1] / s, (m[0, 0] - m[1, 1]) / (2.0 * s), eta[0], eta[1]
+
+
+def record(stat, r, turn, optics):
+    """turn, survivors, sigma_delta, sigma_z, projected horizontal emittance,
+    <z>, <delta> from a calc_beam_stat record; then the betatron emittance
+    of the live particles (x - eta delta, x' - eta' delta) from the second
+    moments, and from the median betatron action: for a Gaussian beam the
+    action J is exponential with mean eps, so eps = median(J) / ln 2 (a
+    handful of halo particles near the aperture barely moves it)."""
     emit = np.sqrt(max(stat[8] ** 2 * stat[9] ** 2 - stat[14] ** 2, 0.0))
-    return [turn, stat[1], stat[13], stat[12], emit, stat[6], stat[7]]
+    beta, alpha, eta, etap = optics
+    live = r[~np.isnan(r[:, 0])]
+    d = live[:, 5] - live[:, 5].mean()
+    xb = live[:, 0] - live[:, 0].mean() - eta * d
+    pb = live[:, 1] - live[:, 1].mean() - etap * d
+    e_rms = np.sqrt(max(np.mean(xb * xb) * np.mean(pb * pb) - np.mean(xb * pb) ** 2, 0.0))
+    gamma = (1.0 + alpha * alpha) / beta
+    action = 0.5 * (gamma * xb * xb + 2.0 * alpha * xb * pb + beta * pb * pb)
+    return [turn, stat[1], stat[13], stat[12], emit, stat[6], stat[7], e_rms, np.median(action) / np.log(2.0)]
 
 
 def main():
     O = Oracle()
     ring = collective_ring(O)
+    optics = optics_at_start(O, ring)
     body, wk = ring[:-1], [ring[-1]]
     r = initial_beam()
     recs = []
@@ -70,9 +101,10 @@ def main():
         r = O.track_omp(body, r, 1, threads, seed=SEED, turn0=t)
         r = O.track(wk, r, 1)
         if (t + 1) % EVERY == 0:
-            recs.append(record(O.beam_stat(r), t + 1))
+            recs.append(record(O.beam_stat(r), r, t + 1, optics))
             print(f"turn {t + 1}: {recs[-1][1]:.0f} left, sigma_delta {recs[-1][2]:.4e}, "
-                  f"sigma_z {recs[-1][3]:.4e}, emittance {recs[-1][4]:.4e} ({time.time() - t0:.0f} s)", flush=True)
+                  f"sigma_z {recs[-1][3]:.4e}, emittance {recs[-1][4]:.4e}, betatron {recs[-1][7]:.4e} "
+                  f"(median action {recs[-1][8]:.4e}) ({time.time() - t0:.0f} s)", flush=True)
     np.savez_compressed(os.path.join(HERE, "collective.npz"), records=np.array(recs), n=np.array([N]),
                         seed=np.array([SEED]), turns=np.array([TURNS]), every=np.array([EVERY]))
 

@@ -383,10 +383,15 @@ def test_batched_da_64_seeds_full_size(gpu_lib, oracle):
     500-8000 points (274,385 in all), 1024 turns in one captured graph.
     (1) Survival maps and coordinates equal 64 separate plain calls bit for
     bit.  (2) On the seeds with the smallest and largest grids and two more,
-    the batched launch after one turn matches the oracle at 1e-12 per
-    coordinate (aline.cpp:359-363), and after 64 turns its survival maps
-    match the oracle's except within one grid cell of the boundary
-    (aline.cpp:607-715)."""
+    the batched launch after one turn has the oracle's survivors, its
+    particles that stay finite match the oracle's at 1e-12 per coordinate
+    (aline.cpp:359-363) and its aperture loss codes are the oracle's; after
+    64 turns its survival maps match the oracle's except within one grid cell
+    of the boundary (aline.cpp:607-715).  Most grid points at large
+    amplitude overflow within the first turn (the sextupoles of this
+    uncorrected ring throw them to infinity before the aperture at the end
+    of the ring); which of their coordinates end as inf or NaN depends on
+    the order of the operations, so only their loss is compared."""
     import pyaccp.abi as abi
     from pyaccp.hipmem import DeviceArray, synchronize
 
@@ -416,8 +421,14 @@ def test_batched_da_64_seeds_full_size(gpu_lib, oracle):
         sl = slice(offs[s], offs[s + 1])
         nx, ny = lat.seed_grid_shape(s)
         assert sizes[s] == 5 * nx * ny
-        err = assert_coords_close(one[sl], oracle.track(elems[s], beam[sl], 1), 1e-12,
-                                  what=f"seed {s}, one turn")
+        o1 = oracle.track(elems[s], beam[sl], 1)
+        g1 = one[sl]
+        assert np.array_equal(np.isnan(g1[:, 0]), np.isnan(o1[:, 0])), f"seed {s}: survivors after one turn"
+        fin = np.isfinite(o1).all(axis=1)
+        assert np.array_equal(np.isfinite(g1).all(axis=1), fin), f"seed {s}: finite particles after one turn"
+        err = assert_coords_close(g1[fin], o1[fin], 1e-12, what=f"seed {s}, one turn")
+        ap = np.isnan(o1[:, 0]) & np.isfinite(o1[:, 4]) & np.isfinite(g1[:, 4])
+        assert np.array_equal(g1[ap, 4], o1[ap, 4]), f"seed {s}: aperture codes"
         o64 = oracle.track_omp(elems[s], beam[sl], 64, 16)
         nd = da_maps_agree(~np.isnan(t64[sl, 0]), ~np.isnan(o64[:, 0]), (5, ny, nx), f"seed {s}, 64 turns")
         print(f"seed {s}: {sizes[s]} points ({nx}x{ny}x5), one-turn max scaled difference {err:.1e}, "
