@@ -208,10 +208,10 @@ int compile_lattice(const accp_elem *E, int n, Compiled &out) {
         const int plain = (k1c == 0 && k2c == 0) ? 1 : 0;
         h.op = e.rad == ACCP_RAD_DAMP ? OP_BENDMP_RAD : e.rad == ACCP_RAD_QE ? OP_BENDMP_QE : OP_BENDMP;
         h.a = e.nslice;
-        h.b = hi | (plain << 8);
         double rec[BM_B];
         rec[BM_PSI1] = irho * std::tan(e.e1);
         rec[BM_PSI2] = irho * std::tan(e.e2);
+        h.b = hi | (plain << 8) | (rec[BM_PSI1] != 0 ? BM_EDGE_IN : 0) | (rec[BM_PSI2] != 0 ? BM_EDGE_OUT : 0);
         rec[BM_IRHO] = irho;
         rec[BM_L1] = l1; rec[BM_L2] = l2; rec[BM_K1] = k1; rec[BM_K2] = k2; rec[BM_SL] = sl;
         rec[BM_CRAD] = crad; rec[BM_CQ] = cq;
@@ -352,16 +352,17 @@ size_t lds_bytes_for(size_t nelem, size_t npar) { return nelem * sizeof(Hdr) + n
 
 // Groups of the tracker's table.  Merging changes no arithmetic: the drift,
 // the element and the kick run the same maps in the same order in one
-// dispatch.  A particle lost on entry to a group stays frozen through all of
-// it, as it would through each element; an aperture never takes a trailing
-// kick, so its loss codes and frozen coordinates are untouched.  Elements
-// whose random numbers are keyed on their index, and wakes (the turn splits
-// there), leave the lattice ungrouped.
+// dispatch.  An aperture never takes a trailing kick, so the state the
+// tracker keeps for a particle an aperture marks lost is the reference's.
+// Lattices that need the kernel variant with every element map, elements
+// whose random numbers are keyed on their index, and wakes (the turn
+// splits there) leave the lattice ungrouped.
 void build_groups(Compiled &c) {
   c.fhdr.clear();
   c.fpar.clear();
   c.fmap.clear();
   const int n = (int)c.hdr.size();
+  if (c.full) return;  // the variant with every element map takes the element table
   for (const Hdr &h : c.hdr)
     if (h.op == OP_WAKE || h.op == OP_BENDMP_QE || h.op == OP_SYNCRAD || h.op == OP_CRABCAV || h.op == OP_RFMULT)
       return;

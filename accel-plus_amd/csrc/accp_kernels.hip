@@ -106,7 +106,7 @@ __device__ __forceinline__ void bendmp(Part (&p)[PPL], const Hdr h, const PT *P,
   const double psi1 = P[BM_PSI1], psi2 = P[BM_PSI2], h0 = P[BM_IRHO];
   const double k1c = P[BM_K1C], k2c = P[BM_K2C];
   const PT *B = P + BM_B;
-  if (psi1 != 0) {
+  if (h.b & BM_EDGE_IN) {
 #pragma unroll
     for (int k = 0; k < PPL; ++k) edge(p[k], psi1);
   }
@@ -166,7 +166,7 @@ __device__ __forceinline__ void bendmp(Part (&p)[PPL], const Hdr h, const PT *P,
       for (int k = 0; k < PPL; ++k) hxdrift(p[k], l1, h0, hs1, hq1);
     }
   }
-  if (psi2 != 0) {
+  if (h.b & BM_EDGE_OUT) {
 #pragma unroll
     for (int k = 0; k < PPL; ++k) edge(p[k], psi2);
   }
@@ -187,11 +187,47 @@ __device__ __forceinline__ void bendmp_fixed(Part (&p)[PPL], const Hdr h, const 
   for (int i = 0; i <= MO; ++i) b[i] = P[BM_B + i];
   const double c4a = hs1 * 0.5, c4b = hs2 * 0.5;
   const double lh1 = k1 * h0, lh2 = k2 * h0;
+  if constexpr (PLAIN && MO == 0) {
+    // a pure dipole: py stays fixed through the slices (hxdrift_py)
+    const double lhh1 = lh1 * h0, lhh2 = lh2 * h0;
+    HxPy fa[PPL], fb[PPL];
+    double qa[PPL], qb[PPL], py2[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+      if (h.b & BM_EDGE_IN) edge(p[k], psi1);
+      py2[k] = p[k].py * p[k].py;
+      fa[k] = hx_consts_py(p[k].ipd, l1, hs1, hq1, p[k].py);
+      fb[k] = hx_consts_py(p[k].ipd, l2, hs2, hq2, p[k].py);
+      qa[k] = lh1 * p[k].d - k1 * b[0];
+      qb[k] = lh2 * p[k].d - k2 * b[0];
+    }
+    for (int m = 0; m < ns; ++m) {
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) hxdrift_py(p[k], fa[k], h0, py2[k]);
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) bend_kick_dipole(p[k], lhh1, qa[k], lh1);
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) hxdrift_py(p[k], fb[k], h0, py2[k]);
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) bend_kick_dipole(p[k], lhh2, qb[k], lh2);
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) hxdrift_py(p[k], fb[k], h0, py2[k]);
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) bend_kick_dipole(p[k], lhh1, qa[k], lh1);
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) hxdrift_py(p[k], fa[k], h0, py2[k]);
+    }
+    if (h.b & BM_EDGE_OUT) {
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) edge(p[k], psi2);
+    }
+    return;
+  }
   HxC ca[PPL], cb[PPL];
   double lhd1[PPL], lhd2[PPL];
 #pragma unroll
   for (int k = 0; k < PPL; ++k) {
-    if (psi1 != 0) edge(p[k], psi1);
+    if (h.b & BM_EDGE_IN) edge(p[k], psi1);
     ca[k] = hx_consts(p[k].ipd, l1, hs1, hq1);
     cb[k] = hx_consts(p[k].ipd, l2, hs2, hq2);
     lhd1[k] = lh1 * p[k].d;
@@ -213,7 +249,7 @@ __device__ __forceinline__ void bendmp_fixed(Part (&p)[PPL], const Hdr h, const 
 #pragma unroll
     for (int k = 0; k < PPL; ++k) hxdrift_c(p[k], ca[k], h0, c4a);
   }
-  if (psi2 != 0) {
+  if (h.b & BM_EDGE_OUT) {
 #pragma unroll
     for (int k = 0; k < PPL; ++k) edge(p[k], psi2);
   }
@@ -258,6 +294,23 @@ __device__ __forceinline__ void strmp_fixed(Part (&p)[PPL], const Hdr h, const P
     n2[k] = l2 * p[k].ipd;
     z1[k] = n1[k] * (0.5 * p[k].ipd);
     z2[k] = n2[k] * (0.5 * p[k].ipd);
+  }
+  if constexpr (SINGLE && HI == 2) {
+    // a pure sextupole: sext_kick
+    const double kb1 = k1 * b[2], kb2 = k2 * b[2], kb1x2 = 2.0 * kb1, kb2x2 = 2.0 * kb2;
+    for (int m = 0; m < ns; ++m) {
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        drift_c(p[k], n1[k], z1[k]);
+        sext_kick(p[k], kb1, kb1x2);
+        drift_c(p[k], n2[k], z2[k]);
+        sext_kick(p[k], kb2, kb2x2);
+        drift_c(p[k], n2[k], z2[k]);
+        sext_kick(p[k], kb1, kb1x2);
+        drift_c(p[k], n1[k], z1[k]);
+      }
+    }
+    return;
   }
   for (int m = 0; m < ns; ++m) {
 #pragma unroll
@@ -497,43 +550,63 @@ __device__ __forceinline__ void element_body(Part (&p)[PPL], const Hdr h, const 
 }
 
 // Lost particles (x NaN) stay frozen, as the reference skips them
-// (e.g. accel.cpp:166-167): every lane computes (the PPL instruction streams
-// stay interleaved, no divergent branch per element) and lost particles get
-// their state back afterwards.  (An exec-masked branch per element measured
-// slower on the latency-bound tail of a dynamic-aperture run.)  Padding
-// lanes of a partly filled tile are never written back, so they compute
-// freely: the restore runs only in waves that hold a lost particle.
-//
-// A record of the grouped table may also carry a drift run before the
-// element (FUSE_PRE, its length first) and a thin StrMPole kick after it
+// (e.g. accel.cpp:166-167).  Every lane computes every element (no
+// divergent branch, no per-element save and restore): a lost particle's
+// registers turn to garbage, but its frozen state was written to the beam
+// buffer when it was lost ("parked", see k_track), and that is what the
+// launch leaves there.  x NaN absorbs every map, so a lost particle stays
+// lost.  A record of the grouped table may also carry a drift run before
+// the element (FUSE_PRE, its length first) and a thin StrMPole kick after it
 // (FUSE_POST): one dispatch, the same maps in the same order.
 template <int PPL, bool FULL, class PT>
 __device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const PT *P,
                                               const KernelCtx &ctx, const uint64_t (&pid)[PPL],
-                                              const bool (&inb)[PPL], uint64_t turn, int eidx) {
-  Part keep[PPL];
-  bool live[PPL];
+                                              uint64_t turn, int eidx) {
+  if constexpr (FULL) {
+    // the host groups only lattices of the lean variant
+    element_body<PPL, FULL>(p, h, P, ctx, pid, turn, eidx);
+  } else {
+    const PT *Pm = P;
+    if (h.op & FUSE_PRE) {
+      const double L = P[0];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) drift_nl(p[k], L * p[k].ipd);
+      Pm = P + 1;
+    }
+    element_body<PPL, FULL>(p, Hdr{h.op & FUSE_OP_MASK, h.off, h.a, h.b}, Pm, ctx, pid, turn, eidx);
+    if (h.op & FUSE_POST) {
+      const PT *Pk = Pm + ((h.op >> FUSE_POST_DELTA_SHIFT) & FUSE_POST_MAX_DELTA);
+      const int pb = ((h.op >> FUSE_POST_HI_SHIFT) & 0xf) | ((h.op & FUSE_POST_SINGLE) ? STRMP_SINGLE : 0);
+      strmp_dispatch<PPL, false>(p, Hdr{OP_STRMP_THIN, 0, 0, pb}, Pk);
+    }
+  }
+}
+
+__device__ __forceinline__ void park(double *dst, const Part &p) {
+  dst[0] = p.x; dst[1] = p.px; dst[2] = p.y;
+  dst[3] = p.py; dst[4] = p.z; dst[5] = p.d;
+}
+
+// Monitor record of turn slot: the state in registers, or for a parked
+// particle its frozen state from the beam buffer.
+template <int PPL>
+__device__ __forceinline__ void snapshot(const KernelCtx &ctx, int slot, const Part (&p)[PPL],
+                                         const int64_t (&gidx)[PPL], const bool (&inb)[PPL],
+                                         const bool (&parked)[PPL]) {
 #pragma unroll
   for (int k = 0; k < PPL; ++k) {
-    keep[k] = p[k];
-    live[k] = is_live(p[k]) || !inb[k];
+    const double *src = ctx.r + 6 * gidx[k];
+    const double v[6] = {src[0], src[1], src[2], src[3], src[4], src[5]};
+    if (inb[k] && slot < ctx.nsnap) {
+      double *dst = ctx.snap + ((int64_t)slot * ctx.n + gidx[k]) * 6;
+      dst[0] = parked[k] ? v[0] : p[k].x;
+      dst[1] = parked[k] ? v[1] : p[k].px;
+      dst[2] = parked[k] ? v[2] : p[k].y;
+      dst[3] = parked[k] ? v[3] : p[k].py;
+      dst[4] = parked[k] ? v[4] : p[k].z;
+      dst[5] = parked[k] ? v[5] : p[k].d;
+    }
   }
-  const PT *Pm = P;
-  if (h.op & FUSE_PRE) {
-    const double L = P[0];
-#pragma unroll
-    for (int k = 0; k < PPL; ++k) drift_nl(p[k], L * p[k].ipd);
-    Pm = P + 1;
-  }
-  element_body<PPL, FULL>(p, Hdr{h.op & FUSE_OP_MASK, h.off, h.a, h.b}, Pm, ctx, pid, turn, eidx);
-  if (h.op & FUSE_POST) {
-    const PT *Pk = Pm + ((h.op >> FUSE_POST_DELTA_SHIFT) & FUSE_POST_MAX_DELTA);
-    const int pb = ((h.op >> FUSE_POST_HI_SHIFT) & 0xf) | ((h.op & FUSE_POST_SINGLE) ? STRMP_SINGLE : 0);
-    strmp_dispatch<PPL, false>(p, Hdr{OP_STRMP_THIN, 0, 0, pb}, Pk);
-  }
-#pragma unroll
-  for (int k = 0; k < PPL; ++k)
-    if (!live[k]) p[k] = keep[k];
 }
 
 // Headers: the next one is read while the current element runs.  From LDS
@@ -592,6 +665,11 @@ __global__ __launch_bounds__(BLOCK) void k_track(KernelCtx ctx) {
   bool inb[PPL];
   int lost_at[PPL];
   bool record[PPL];
+  // parked: the particle's frozen state is in ctx.r and the launch leaves it
+  // there (lost on input, padding lanes, and particles lost in this launch:
+  // at an aperture, where the reference marks them, or, for an orbit that
+  // overflowed to NaN, at the end of that turn)
+  bool parked[PPL];
 #pragma unroll
   for (int k = 0; k < PPL; ++k) {
     const int li = k * BLOCK + (int)threadIdx.x;
@@ -609,58 +687,51 @@ __global__ __launch_bounds__(BLOCK) void k_track(KernelCtx ctx) {
     // unless this is the first launch of the call (lost on input: -1)
     record[k] = inb[k] && ctx.loss_turn && (is_live(p[k]) || ctx.turn_rel0 == 0);
     lost_at[k] = 0x7fffffff;
+    parked[k] = !inb[k] || !is_live(p[k]);
     if (!inb[k]) p[k].x = __builtin_nan("");  // padding lanes start lost, never written back
   }
 
   int t = 0;
   for (; t < ctx.nturn; ++t) {
-    if (ctx.snap && t % ctx.mon_interval == 0) {
-      const int slot = t / ctx.mon_interval;
-#pragma unroll
-      for (int k = 0; k < PPL; ++k)
-        if (inb[k] && slot < ctx.nsnap) {
-          double *dst = ctx.snap + ((int64_t)slot * ctx.n + gidx[k]) * 6;
-          dst[0] = p[k].x; dst[1] = p[k].px; dst[2] = p[k].y;
-          dst[3] = p[k].py; dst[4] = p[k].z; dst[5] = p[k].d;
-        }
-    }
     bool any = false;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
       const bool lv = is_live(p[k]);
       any |= lv && inb[k];
       if (!lv && lost_at[k] == 0x7fffffff) lost_at[k] = ctx.turn_rel0 + t - 1;
+      if (!lv && !parked[k]) {  // overflowed in the last turn
+        park(ctx.r + 6 * gidx[k], p[k]);
+        parked[k] = true;
+      }
     }
+    if (ctx.snap && t % ctx.mon_interval == 0) snapshot<PPL>(ctx, t / ctx.mon_interval, p, gidx, inb, parked);
     if (!__any(any)) break;  // every particle of this wave is lost: frozen from here on
     const uint64_t turn = ctx.turn0 + (uint64_t)t;
     // header of the next element is fetched while the current one runs
     Hdr h = uniform_hdr<LDS>(load_hdr_raw<LDS>(hdr, ctx.st));
     for (int e = ctx.st; e < ctx.ed; ++e) {
       const HdrRaw hn = load_hdr_raw<LDS>(hdr, e + 1 < ctx.ed ? e + 1 : e);
-      apply_element<PPL, FULL>(p, h, par + h.off, ctx, pid, inb, turn, e);
+      apply_element<PPL, FULL>(p, h, par + h.off, ctx, pid, turn, e);
+      if ((FULL ? h.op : (h.op & FUSE_OP_MASK)) == OP_APERTURE) {
+#pragma unroll
+        for (int k = 0; k < PPL; ++k)
+          if (!parked[k] && !is_live(p[k])) {
+            park(ctx.r + 6 * gidx[k], p[k]);
+            parked[k] = true;
+          }
+      }
       h = uniform_hdr<LDS>(hn);
     }
   }
   // monitor slots after an early exit hold the frozen state
   if (ctx.snap) {
-    for (int tt = t; tt < ctx.nturn; ++tt) {
-      if (tt % ctx.mon_interval) continue;
-      const int slot = tt / ctx.mon_interval;
-#pragma unroll
-      for (int k = 0; k < PPL; ++k)
-        if (inb[k] && slot < ctx.nsnap) {
-          double *dst = ctx.snap + ((int64_t)slot * ctx.n + gidx[k]) * 6;
-          dst[0] = p[k].x; dst[1] = p[k].px; dst[2] = p[k].y;
-          dst[3] = p[k].py; dst[4] = p[k].z; dst[5] = p[k].d;
-        }
-    }
+    for (int tt = t + 1; tt < ctx.nturn; ++tt)
+      if (tt % ctx.mon_interval == 0) snapshot<PPL>(ctx, tt / ctx.mon_interval, p, gidx, inb, parked);
   }
 #pragma unroll
   for (int k = 0; k < PPL; ++k) {
     if (!inb[k]) continue;
-    double *dst = ctx.r + 6 * gidx[k];
-    dst[0] = p[k].x; dst[1] = p[k].px; dst[2] = p[k].y;
-    dst[3] = p[k].py; dst[4] = p[k].z; dst[5] = p[k].d;
+    if (!parked[k]) park(ctx.r + 6 * gidx[k], p[k]);
     if (record[k]) {
       int lt = lost_at[k];
       if (lt == 0x7fffffff && !is_live(p[k])) lt = ctx.turn_rel0 + t - 1;  // lost in the last turn run

@@ -276,6 +276,49 @@ ACCP_HD void hxdrift_c(Part &p, const HxC &c, double h, double c4) {
   p.z -= a * c.c5 * s;
 }
 
+// ATbendhxdrift6 in a bend whose kicks leave py alone (no field above the
+// dipole term, no curvature gradients): py is fixed through the slices, so
+// its terms are per-particle constants of the element, and the update
+// regroups as
+//   x  += px (c1 (1 + h x) + c2 px) - c2 py^2
+//   px -= c3 (px^2 + py^2)
+//   y  += (1 + h x) (c1 py + c1 c4 py px)
+//   z  -= c5 (1 + h x) (px^2 + py^2)
+// with c1..c5 as in HxC: the reference's terms in another order, which
+// changes round-off only, in fewer dependent operations.
+struct HxPy {
+  double c1, c2, c3, c5, p1, p1c4, c2py2;
+};
+ACCP_HD HxPy hx_consts_py(double ipd, double L, double hs, double hsl4, double py) {
+  const double c1 = ipd * L, c2 = hsl4 * ipd * ipd, p1 = py * c1;
+  return HxPy{c1, c2, hs * 0.5 * ipd, ipd * ipd * L * 0.5, p1, p1 * (hs * 0.5), c2 * (py * py)};
+}
+ACCP_HD void hxdrift_py(Part &p, const HxPy &c, double h, double py2) {
+  const double x = p.x, px = p.px;
+  const double a = fma(h, x, 1.0);
+  const double s = fma(px, px, py2);
+  p.x = fma(px, fma(c.c2, px, c.c1 * a), x - c.c2py2);
+  p.px = fma(-c.c3, s, px);
+  p.y = fma(a, fma(c.p1c4, px, c.p1), p.y);
+  p.z = fma(-(c.c5 * a), s, p.z);
+}
+// bndthinkick with PolyB[0] alone and delta fixed: px -= L (B0 + h h x) -
+// L h delta, as px + q - (L h h) x with q = L h delta - L B0 per particle
+// and element; z -= L h x (atpass.cpp:229-233).
+ACCP_HD void bend_kick_dipole(Part &p, double lhh, double q, double lh) {
+  const double x = p.x;
+  p.px = fma(-lhh, x, p.px + q);
+  p.z = fma(-lh, x, p.z);
+}
+// The kick of a pure sextupole term b (x + i y)^2: px -= k b (x^2 - y^2),
+// py += 2 k b x y, with kb = k b and kb2 = 2 k b (the reference's Horner sum
+// in another order: round-off differences only).
+ACCP_HD void sext_kick(Part &p, double kb, double kb2) {
+  const double x = p.x, y = p.y;
+  p.px = fma(-kb, fma(x, x, -(y * y)), p.px);
+  p.py = fma(kb2 * x, y, p.py);
+}
+
 // Complex Horner of order HI over coefficients held in registers.
 template <int HI>
 ACCP_HD void horner_reg(const double (&b)[HI + 1], const double (&a)[HI + 1], double x, double y,

@@ -60,6 +60,11 @@ constexpr int FUSE_POST_MAX_DELTA = 1023;
 constexpr int FUSE_POST_HI_SHIFT = 20;
 constexpr int32_t FUSE_POST_SINGLE = 1 << 24;
 
+// BendMPole header flags beside hi | plain << 8: an entry or exit edge with
+// non-zero focusing (psi1, psi2 != 0).
+constexpr int BM_EDGE_IN = 1 << 9;
+constexpr int BM_EDGE_OUT = 1 << 10;
+
 // Offsets inside an OP_BENDMP* record.  HS* = irho*L and HQ* = (1/4)*HS*L
 // of the pseudo-drift lengths L1, L2 and SL/2; K1C/K2C are PolyB[1]/[2] as the
 // curvature terms of bndthinkick read them (PolyB[1] whatever MaxNPole,

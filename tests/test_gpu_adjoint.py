@@ -78,7 +78,7 @@ def test_gradient_matches_gpu_finite_differences(gpu_lib):
             b = beam[pi:pi + 1].copy()
             b[0, c] += h
             return loss_and_seed(L.track(b, nturn))[0]
-        steps = [r * max(abs(beam[pi, c]), 1e-6) for r in (1e-7, 1e-6, 1e-5, 1e-4, 1e-3)]
+        steps = [r * max(abs(beam[pi, c]), 1e-6) for r in (1e-7, 3e-7, 1e-6, 3e-6, 1e-5, 3e-5, 1e-4, 3e-4, 1e-3)]
         rel, worse, fd = best_fd(dr0[pi, c], loss_at, steps)
         assert rel < 1e-6 and worse < 5e-6, (pi, c, dr0[pi, c], fd)
 
@@ -205,7 +205,7 @@ def test_combined_function_bends(gpu_lib, oracle):
             b = beam[pi:pi + 1].copy()
             b[0, c] += h
             return loss_and_seed(L.track(b, nturn))[0]
-        steps = [r * max(abs(beam[pi, c]), 1e-6) for r in (1e-7, 1e-6, 1e-5, 1e-4, 1e-3)]
+        steps = [r * max(abs(beam[pi, c]), 1e-6) for r in (1e-7, 3e-7, 1e-6, 3e-6, 1e-5, 3e-5, 1e-4, 3e-4, 1e-3)]
         rel, worse, fd = best_fd(dr0[pi, c], loss_at, steps)
         assert rel < 1e-6 and worse < 5e-6, (pi, c, dr0[pi, c], fd)
     A.set_reverse(A.REVERSE_SEQUENTIAL)

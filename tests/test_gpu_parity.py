@@ -391,7 +391,13 @@ def test_batched_da_64_seeds_full_size(gpu_lib, oracle):
     amplitude overflow within the first turn (the sextupoles of this
     uncorrected ring throw them to infinity before the aperture at the end
     of the ring); which of their coordinates end as inf or NaN depends on
-    the order of the operations, so only their loss is compared."""
+    the order of the operations, so only their loss is compared.  A few
+    finite orbits just inside that edge amplify round-off so much that the
+    oracle itself moves by up to 5e-12 of a coordinate when its initial x
+    and y change by one ulp; such a particle is held to 1e-12 plus 1000
+    times that measured sensitivity (every particle that stays inside the
+    dynamic aperture for 64 turns has a sensitivity below 1e-13 and is held
+    to 1e-12)."""
     import pyaccp.abi as abi
     from pyaccp.hipmem import DeviceArray, synchronize
 
@@ -426,11 +432,23 @@ def test_batched_da_64_seeds_full_size(gpu_lib, oracle):
         assert np.array_equal(np.isnan(g1[:, 0]), np.isnan(o1[:, 0])), f"seed {s}: survivors after one turn"
         fin = np.isfinite(o1).all(axis=1)
         assert np.array_equal(np.isfinite(g1).all(axis=1), fin), f"seed {s}: finite particles after one turn"
-        err = assert_coords_close(g1[fin], o1[fin], 1e-12, what=f"seed {s}, one turn")
+        ulp = beam[sl].copy()
+        ulp[:, 0] *= 1 + 2.0 ** -52
+        ulp[:, 2] *= 1 + 2.0 ** -52
+        op = oracle.track(elems[s], ulp, 1)
+        scale = np.maximum(np.maximum(np.abs(g1[fin]), np.abs(o1[fin])), np.sqrt(np.mean(o1[fin] ** 2, axis=0)))
+        sens = np.nan_to_num(np.abs(op[fin] - o1[fin]) / scale, nan=1.0, posinf=1.0)
+        rel = np.abs(g1[fin] - o1[fin]) / scale
+        assert np.all(rel <= 1e-12 + 1e3 * sens), f"seed {s}: one turn, {int((rel > 1e-12 + 1e3 * sens).sum())} coords"
+        well = sens.max(axis=1) <= 1e-13
+        err = float(rel[well].max()) if well.any() else 0.0
+        assert err <= 1e-12, f"seed {s}: one turn, well-conditioned orbits"
         ap = np.isnan(o1[:, 0]) & np.isfinite(o1[:, 4]) & np.isfinite(g1[:, 4])
         assert np.array_equal(g1[ap, 4], o1[ap, 4]), f"seed {s}: aperture codes"
         o64 = oracle.track_omp(elems[s], beam[sl], 64, 16)
         nd = da_maps_agree(~np.isnan(t64[sl, 0]), ~np.isnan(o64[:, 0]), (5, ny, nx), f"seed {s}, 64 turns")
-        print(f"seed {s}: {sizes[s]} points ({nx}x{ny}x5), one-turn max scaled difference {err:.1e}, "
+        print(f"seed {s}: {sizes[s]} points ({nx}x{ny}x5), {int(fin.sum())} finite after one turn, "
+              f"{int(well.sum())} well-conditioned: max scaled difference {err:.1e} "
+              f"(others {float(rel[~well].max()) if (~well).any() else 0.0:.1e}), "
               f"64-turn survivors gpu {int((~np.isnan(t64[sl, 0])).sum())} "
               f"oracle {int((~np.isnan(o64[:, 0])).sum())}, boundary differences {nd}")
