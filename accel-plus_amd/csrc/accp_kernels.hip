@@ -196,8 +196,8 @@ __device__ __forceinline__ void bendmp_fixed(Part (&p)[PPL], const Hdr h, const 
     for (int k = 0; k < PPL; ++k) {
       if (h.b & BM_EDGE_IN) edge(p[k], psi1);
       py2[k] = p[k].py * p[k].py;
-      fa[k] = hx_consts_py(p[k].ipd, l1, hs1, hq1, p[k].py);
-      fb[k] = hx_consts_py(p[k].ipd, l2, hs2, hq2, p[k].py);
+      fa[k] = hx_consts_py(p[k].ipd, l1, hs1, hq1, p[k].py, h0);
+      fb[k] = hx_consts_py(p[k].ipd, l2, hs2, hq2, p[k].py, h0);
       qa[k] = lh1 * p[k].d - k1 * b[0];
       qb[k] = lh2 * p[k].d - k2 * b[0];
     }

@@ -287,17 +287,17 @@ ACCP_HD void hxdrift_c(Part &p, const HxC &c, double h, double c4) {
 // with c1..c5 as in HxC: the reference's terms in another order, which
 // changes round-off only, in fewer dependent operations.
 struct HxPy {
-  double c1, c2, c3, c5, p1, p1c4, c2py2;
+  double c1, c1h, c2, c3, c5, p1, p1c4, c2py2;
 };
-ACCP_HD HxPy hx_consts_py(double ipd, double L, double hs, double hsl4, double py) {
+ACCP_HD HxPy hx_consts_py(double ipd, double L, double hs, double hsl4, double py, double h) {
   const double c1 = ipd * L, c2 = hsl4 * ipd * ipd, p1 = py * c1;
-  return HxPy{c1, c2, hs * 0.5 * ipd, ipd * ipd * L * 0.5, p1, p1 * (hs * 0.5), c2 * (py * py)};
+  return HxPy{c1, c1 * h, c2, hs * 0.5 * ipd, ipd * ipd * L * 0.5, p1, p1 * (hs * 0.5), c2 * (py * py)};
 }
 ACCP_HD void hxdrift_py(Part &p, const HxPy &c, double h, double py2) {
   const double x = p.x, px = p.px;
   const double a = fma(h, x, 1.0);
   const double s = fma(px, px, py2);
-  p.x = fma(px, fma(c.c2, px, c.c1 * a), x - c.c2py2);
+  p.x = fma(px, fma(c.c2, px, fma(c.c1h, x, c.c1)), x - c.c2py2);
   p.px = fma(-c.c3, s, px);
   p.y = fma(a, fma(c.p1c4, px, c.p1), p.y);
   p.z = fma(-(c.c5 * a), s, p.z);

@@ -1,0 +1,96 @@
+// CPU check of the regrouped forward maps of the tracking kernel
+// (accel-plus_amd/csrc/accp_physics.h, compiled here for the host): the
+// dipole pseudo-drift and kick (hxdrift_py, bend_kick_dipole) and the pure
+// sextupole kick (sext_kick) against the same maps written in the
+// reference's order (hxdrift_c and bend_kick_c<0, true>, which follow
+// atpass.cpp:199-235, 343-355; horner_single<2>).  They must agree to
+// round-off over random states; a dropped term or a wrong sign does not.
+// Prints one line per check and exits non-zero on a failure.
+#include <cmath>
+#include <cstdio>
+#include <random>
+
+#include "accp_physics.h"
+
+using namespace accp;
+
+static std::mt19937_64 rng(2024);
+static double U(double a, double b) { return std::uniform_real_distribution<double>(a, b)(rng); }
+
+static Part random_part(double scale) {
+  Part p{U(-1, 1) * scale, U(-1, 1) * scale * 0.2, U(-1, 1) * scale * 0.5, U(-1, 1) * scale * 0.1,
+         U(-1, 1) * 1e-2, 0, 0};
+  set_delta(p, U(-0.03, 0.03));
+  return p;
+}
+
+static double rel(const Part &a, const Part &b, const double sc[6]) {
+  const double d[6] = {a.x - b.x, a.px - b.px, a.y - b.y, a.py - b.py, a.z - b.z, a.d - b.d};
+  double m = 0;
+  for (int i = 0; i < 6; ++i) m = std::fmax(m, std::fabs(d[i]) / sc[i]);
+  return m;
+}
+
+int main() {
+  int fail = 0;
+  // a 1.5 m bend of angle 2 pi / 40 in 10 slices, as in the benchmark lattices
+  const double L = 1.5, theta = 6.283185307179586 / 40, h = theta / L, sl = L / 10;
+  const double l1 = sl * DRIFT1, l2 = sl * DRIFT2, k1 = sl * KICK1, k2 = sl * KICK2;
+  const double hs1 = h * l1, hq1 = 0.25 * hs1 * l1, hs2 = h * l2, hq2 = 0.25 * hs2 * l2;
+  const double b0s[2] = {0.0, 3e-4};
+  double worst = 0;
+  for (int it = 0; it < 20000; ++it) {
+    for (double b0 : b0s) {
+      const double scale = it % 2 ? 1e-3 : 2e-2;
+      Part p = random_part(scale), q = p;
+      const double sc[6] = {scale, scale * 0.2, scale * 0.5, scale * 0.1, 1e-2, 0.03};
+      const double b[1] = {b0};
+      // reference order
+      const HxC ca = hx_consts(p.ipd, l1, hs1, hq1), cb = hx_consts(p.ipd, l2, hs2, hq2);
+      const double lh1 = k1 * h, lh2 = k2 * h;
+      for (int m = 0; m < 10; ++m) {
+        hxdrift_c(p, ca, h, hs1 * 0.5);
+        bend_kick_c<0, true>(p, b, k1, h, lh1 * p.d, lh1, 0, 0);
+        hxdrift_c(p, cb, h, hs2 * 0.5);
+        bend_kick_c<0, true>(p, b, k2, h, lh2 * p.d, lh2, 0, 0);
+        hxdrift_c(p, cb, h, hs2 * 0.5);
+        bend_kick_c<0, true>(p, b, k1, h, lh1 * p.d, lh1, 0, 0);
+        hxdrift_c(p, ca, h, hs1 * 0.5);
+      }
+      // regrouped
+      const HxPy fa = hx_consts_py(q.ipd, l1, hs1, hq1, q.py, h), fb = hx_consts_py(q.ipd, l2, hs2, hq2, q.py, h);
+      const double py2 = q.py * q.py, qa = lh1 * q.d - k1 * b0, qb = lh2 * q.d - k2 * b0;
+      for (int m = 0; m < 10; ++m) {
+        hxdrift_py(q, fa, h, py2);
+        bend_kick_dipole(q, lh1 * h, qa, lh1);
+        hxdrift_py(q, fb, h, py2);
+        bend_kick_dipole(q, lh2 * h, qb, lh2);
+        hxdrift_py(q, fb, h, py2);
+        bend_kick_dipole(q, lh1 * h, qa, lh1);
+        hxdrift_py(q, fa, h, py2);
+      }
+      worst = std::fmax(worst, rel(p, q, sc));
+    }
+  }
+  std::printf("dipole bend, 10 slices, 40000 states: max scaled difference %.2e\n", worst);
+  if (!(worst < 1e-13)) fail = 1;
+
+  // sextupole kick
+  worst = 0;
+  for (int it = 0; it < 20000; ++it) {
+    const double scale = it % 2 ? 1e-3 : 2e-2;
+    Part p = random_part(scale), q = p;
+    const double sc[6] = {scale, scale * 0.2, scale * 0.5, scale * 0.1, 1e-2, 0.03};
+    const double bs = U(-40, 40), k = U(-0.2, 0.2);
+    double re, im;
+    horner_single<2>(bs, p.x, p.y, re, im);
+    p.px -= k * re;
+    p.py += k * im;
+    sext_kick(q, k * bs, 2.0 * k * bs);
+    worst = std::fmax(worst, rel(p, q, sc));
+  }
+  std::printf("sextupole kick, 20000 states: max scaled difference %.2e\n", worst);
+  if (!(worst < 1e-14)) fail = 1;
+  std::printf(fail ? "FAILED\n" : "all regrouped maps ok\n");
+  return fail;
+}

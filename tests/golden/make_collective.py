@@ -2,7 +2,7 @@
 aperture, multipole errors) with quantum excitation in every bend, the
 cavity at the synchronous phase from setrfphase, and the longitudinal
 resonator wake (5 kOhm, Q = 1, 10 GHz, 2 nC in 256 bins over +-15 cm) at the
-end of the ring; 4096 Gaussian particles, 8000 turns (two longitudinal
+end of the ring; 32768 Gaussian particles, 8000 turns (two longitudinal
 damping times), with the counter-based normal stream the GPU draws.  Every
 250 turns it records the survivors, the energy spread, the bunch length, the
 projected horizontal emittance, the bunch centroid, and the betatron
@@ -12,7 +12,7 @@ the median of the betatron action.  Writes tests/golden/collective.npz.
 The wake couples the particles, so each turn runs the elements before the
 wake in parallel over particles (they are independent there) and then the
 wake over the whole beam; the wake is the last element, so every other
-element keeps the index its random numbers are keyed on.  Takes about ten
+element keeps the index its random numbers are keyed on.  Takes about 70
 minutes on 8 cores: python tests/golden/make_collective.py"""
 import os
 import sys
@@ -29,7 +29,7 @@ import pyaccp.elements as el  # noqa: E402
 import pyaccp.lattices as lat  # noqa: E402
 from pyoracle import Oracle  # noqa: E402
 
-N, SEED, TURNS, EVERY = 4096, 3, 8000, 250
+N, SEED, TURNS, EVERY = 32768, 3, 8000, 250
 
 
 def wake():
