@@ -448,6 +448,22 @@ struct WorkSet {
   }
 };
 
+// The working-set buffers one tracking run uses: owned by a WorkSet, or a
+// caller's fixed workspace (accp_track_device_noalloc).
+struct WsView {
+  int32_t *idx[2];
+  int32_t *cnt[2];
+  TileDesc *tiles;
+  int32_t *ntiles;
+  int64_t *offs;
+};
+
+WsView view_of(const WorkSet &ws) {
+  return WsView{{(int32_t *)ws.idx[0].p, (int32_t *)ws.idx[1].p},
+                {(int32_t *)ws.cnt[0].p, (int32_t *)ws.cnt[1].p},
+                (TileDesc *)ws.tiles.p, (int32_t *)ws.ntiles.p, (int64_t *)ws.offs.p};
+}
+
 // Device-resident element tables of nseed lattices.
 struct TableRef {
   const Hdr *hdr;
@@ -516,24 +532,22 @@ int prepare_workset(WorkSet &ws, const int64_t *h_offs, int nseed, int per, int 
 // chunk_max; between chunks the lost particles leave the working set.
 // init = false reuses the working set and tile list that the previous run on
 // ws left: valid when that run had no compaction and the same particles.
-int enqueue_run(const TableRef &T, WorkSet &ws, int max_tiles, double *d_r, int64_t n,
+int enqueue_run(const TableRef &T, const WsView &ws, int max_tiles, double *d_r, int64_t n,
                 const RunSpec &R, const accp_tuning *tuning, hipStream_t s, bool init = true) {
   int block, ppl, chunk_max;
   bool lds;
   int rc = resolve_tuning(tuning, T.lds_need, block, ppl, lds, chunk_max);
   if (rc) return rc;
   const int per = block * ppl;
-  const int64_t *d_offs = (const int64_t *)ws.offs.p;
+  const int64_t *d_offs = ws.offs;
   int cur = 0;
   if (init) {
-    HIPCHK(launch_chunk_init((int32_t *)ws.idx[0].p, n, R.d_loss_turn, R.nturn, (int32_t *)ws.cnt[0].p, d_offs,
-                             T.nseed, s));
-    HIPCHK(launch_make_tiles((const int32_t *)ws.cnt[0].p, d_offs, T.nseed, per, (TileDesc *)ws.tiles.p,
-                             (int32_t *)ws.ntiles.p, s));
+    HIPCHK(launch_chunk_init(ws.idx[0], n, R.d_loss_turn, R.nturn, ws.cnt[0], d_offs, T.nseed, s));
+    HIPCHK(launch_make_tiles(ws.cnt[0], d_offs, T.nseed, per, ws.tiles, ws.ntiles, s));
   }
   KernelCtx c{};
-  c.tiles = (const TileDesc *)ws.tiles.p;
-  c.ntiles = (const int32_t *)ws.ntiles.p;
+  c.tiles = ws.tiles;
+  c.ntiles = ws.ntiles;
   c.hdr = T.hdr;
   c.par = T.par;
   c.lat_hdr0 = T.meta;
@@ -554,18 +568,16 @@ int enqueue_run(const TableRef &T, WorkSet &ws, int max_tiles, double *d_r, int6
   int t = 0, chunk = 1;
   while (t < R.nturn) {
     const int cn = compact ? std::min(chunk, R.nturn - t) : R.nturn;
-    c.idx = (const int32_t *)ws.idx[cur].p;
+    c.idx = ws.idx[cur];
     c.turn0 = R.turn0 + (uint64_t)t;
     c.turn_rel0 = t;
     c.nturn = cn;
     HIPCHK(launch_track(c, max_tiles, block, ppl, lds, lds ? T.lds_need : 0, T.full, s));
     t += cn;
     if (t < R.nturn) {
-      HIPCHK(launch_compact((const TileDesc *)ws.tiles.p, (const int32_t *)ws.ntiles.p, max_tiles,
-                            (const int32_t *)ws.idx[cur].p, d_r, d_offs, (int32_t *)ws.cnt[1 - cur].p,
-                            (int32_t *)ws.idx[1 - cur].p, T.nseed, s));
-      HIPCHK(launch_make_tiles((const int32_t *)ws.cnt[1 - cur].p, d_offs, T.nseed, per,
-                               (TileDesc *)ws.tiles.p, (int32_t *)ws.ntiles.p, s));
+      HIPCHK(launch_compact(ws.tiles, ws.ntiles, max_tiles, ws.idx[cur], d_r, d_offs, ws.cnt[1 - cur],
+                            ws.idx[1 - cur], T.nseed, s));
+      HIPCHK(launch_make_tiles(ws.cnt[1 - cur], d_offs, T.nseed, per, ws.tiles, ws.ntiles, s));
       cur = 1 - cur;
       chunk = std::min(chunk * 2, chunk_max);
     }
@@ -707,8 +719,10 @@ extern "C" int accp_lattice_free(accp_lattice *L) {
 // ---- tracking -------------------------------------------------------------------
 // Run on one lattice; turns are split at active LongWake elements, which need
 // the whole beam (histogram over all particles) before anyone passes them.
+// ext: a caller's fixed workspace (no allocation, offsets set by a kernel);
+// null: the lattice's own working set, grown as needed.
 static int track_device_impl(accp_lattice *L, double *d_r, int64_t n, const RunSpec &R,
-                             const accp_tuning *tuning, hipStream_t s) {
+                             const accp_tuning *tuning, hipStream_t s, const WsView *ext = nullptr) {
   const int64_t offs[2] = {0, n};
   int block, ppl, chunk_max;
   bool lds;
@@ -720,8 +734,16 @@ static int track_device_impl(accp_lattice *L, double *d_r, int64_t n, const RunS
   int rc = resolve_tuning(tuning, T.lds_need, block, ppl, lds, chunk_max);
   if (rc) return rc;
   int max_tiles = 0;
-  if ((rc = prepare_workset(L->ws, offs, 1, block * ppl, max_tiles))) return rc;
-  if (cuts.empty()) return enqueue_run(T, L->ws, max_tiles, d_r, n, Rg, tuning, s);
+  WsView ws;
+  if (ext) {
+    ws = *ext;
+    max_tiles = (int)((n + block * ppl - 1) / (block * ppl));
+    HIPCHK(launch_set_offsets(ws.offs, n, s));
+  } else {
+    if ((rc = prepare_workset(L->ws, offs, 1, block * ppl, max_tiles))) return rc;
+    ws = view_of(L->ws);
+  }
+  if (cuts.empty()) return enqueue_run(T, ws, max_tiles, d_r, n, Rg, tuning, s);
   if (R.snap) return fail(ACCP_E_ARG, "monitor records with a wake element are not supported");
   accp_tuning one = tuning ? *tuning : accp_tuning{0, 0, 0, 0};
   one.compact = -1;  // every segment sees the whole beam
@@ -737,7 +759,7 @@ static int track_device_impl(accp_lattice *L, double *d_r, int64_t n, const RunS
       if (b > a) {
         seg.st = a;
         seg.ed = b;
-        if ((rc = enqueue_run(T, L->ws, max_tiles, d_r, n, seg, &one, s, init))) return rc;
+        if ((rc = enqueue_run(T, ws, max_tiles, d_r, n, seg, &one, s, init))) return rc;
         init = false;
       }
       if (k < cuts.size()) {
@@ -781,6 +803,26 @@ extern "C" int accp_track_device(accp_lattice *L, double *d_r, int n, int nturn,
   RunSpec R = spec_from(opts, nturn, st, ed);
   R.d_loss_turn = opts ? opts->loss_turn : nullptr;
   return track_device_impl(L, d_r, n, R, opts ? &opts->tuning : nullptr, (hipStream_t)stream);
+}
+
+int accp::track_device_fixed(accp_lattice *L, double *d_r, int n, int nturn, int st, int ed,
+                             const accp_track_opts *opts, void *stream, const FixedWorkspace &fw) {
+  if (!L || (!d_r && n > 0)) return fail(ACCP_E_ARG, "accp_track_device_noalloc: null argument");
+  int rc = check_range(L, n, nturn, st, ed);
+  if (rc) return rc;
+  if (opts && opts->mon_interval > 0)
+    return fail(ACCP_E_ARG, "accp_track_device_noalloc: no monitor records (they need a host buffer)");
+  if (n > fw.max_n)
+    return fail(ACCP_E_ARG, "accp_track_device_noalloc: %d particles, at most %lld", n, (long long)fw.max_n);
+  for (int w : L->tab.wakes)
+    if (w >= st && w < ed && L->elems[w].type == ACCP_TRANSWAKE)
+      return fail(ACCP_E_LATTICE, "accp_track_device_noalloc: element %d is a TransWake, whose sort needs "
+                                  "buffers sized to the beam", w);
+  if (n == 0 || nturn == 0 || st == ed) return ACCP_OK;
+  RunSpec R = spec_from(opts, nturn, st, ed);
+  R.d_loss_turn = opts ? opts->loss_turn : nullptr;
+  const WsView v{{fw.idx0, fw.idx1}, {fw.cnt0, fw.cnt1}, (TileDesc *)fw.tiles, fw.ntiles, fw.offs};
+  return track_device_impl(L, d_r, n, R, opts ? &opts->tuning : nullptr, (hipStream_t)stream, &v);
 }
 
 int accp::track_record(accp_lattice *L, double *r, int n, int nturn, const double **d_rec, uint64_t pid0) {
@@ -958,7 +1000,7 @@ static int batched_run(accp_batch *b, double *d_r, const int64_t *offsets, int n
   R.st = 0;
   R.ed = b->nelem;
   R.d_loss_turn = d_loss_turn;
-  return enqueue_run(T, b->ws, max_tiles, d_r, n, R, tuning, s);
+  return enqueue_run(T, view_of(b->ws), max_tiles, d_r, n, R, tuning, s);
 }
 
 extern "C" int accp_track_batched(accp_batch *b, double *d_r, const int64_t *offsets, int nturn,

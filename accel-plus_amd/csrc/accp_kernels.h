@@ -37,6 +37,10 @@ struct KernelCtx {
   int32_t turn_rel0;         // turn of this launch relative to the call start
 };
 
+// offs[0] = 0, offs[1] = n for a one-lattice run, written by a kernel (no
+// host copy).
+hipError_t launch_set_offsets(int64_t *offs, int64_t n, hipStream_t s);
+
 // Chunked driver helpers (device-resident bookkeeping, no host round trips,
 // so a whole run can be captured in a HIP graph).
 hipError_t launch_chunk_init(int32_t *idx, int64_t n, int32_t *loss_turn, int32_t nturn,

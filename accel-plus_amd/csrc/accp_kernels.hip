@@ -910,6 +910,18 @@ hipError_t launch_track(const KernelCtx &ctx, int grid, int block, int ppl, bool
   return hipErrorInvalidValue;
 }
 
+__global__ void k_set_offsets(int64_t *offs, int64_t n) {
+  if (threadIdx.x == 0) {
+    offs[0] = 0;
+    offs[1] = n;
+  }
+}
+
+hipError_t launch_set_offsets(int64_t *offs, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_offsets, dim3(1), dim3(64), 0, s, offs, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_chunk_init(int32_t *idx, int64_t n, int32_t *loss_turn, int32_t nturn,
                              int32_t *cnt, const int64_t *offs, int nseed, hipStream_t s) {
   const int grid = (int)std::min<int64_t>((n + 255) / 256 + 1, 4096);

@@ -53,7 +53,7 @@ class Param(ctypes.Structure):
 EXPORTS = (
     "accp_last_error", "accp_abi_version", "accp_device_count",
     "accp_lattice_build", "accp_lattice_update", "accp_lattice_nelem", "accp_lattice_free",
-    "accp_track", "accp_track_device", "accp_beam_stat",
+    "accp_track", "accp_track_device", "accp_track_device_noalloc", "accp_beam_stat",
     "accp_batch_create", "accp_track_batched", "accp_batch_capture", "accp_batch_replay",
     "accp_autotune", "accp_batch_free",
     "accp_adjoint_create", "accp_adjoint_forward", "accp_adjoint_backward", "accp_adjoint_set_reverse",
@@ -87,6 +87,8 @@ class Library:
         L.accp_lattice_free.argtypes = [VP]
         L.accp_track.argtypes = [VP, DP, c_int, c_int, c_int, c_int, POINTER(TrackOpts)]
         L.accp_track_device.argtypes = [VP, c_void_p, c_int, c_int, c_int, c_int, POINTER(TrackOpts), c_void_p]
+        L.accp_track_device_noalloc.argtypes = [VP, c_void_p, c_int, c_int, c_int, c_int, POINTER(TrackOpts),
+                                                c_void_p]
         L.accp_beam_stat.argtypes = [DP, c_int, DP]
         L.accp_batch_create.argtypes = [POINTER(VP), c_int, POINTER(VP)]
         L.accp_track_batched.argtypes = [VP, c_void_p, POINTER(c_int64), c_int, POINTER(Tuning), c_void_p, c_void_p]
@@ -307,16 +309,17 @@ class Lattice:
         return out if len(out) > 1 else out[0]
 
     def track_device(self, d_ptr, n, nturn, st=0, ed=-1, seed=0, turn0=0, pid0=0, tuning=None, stream=None,
-                     d_loss_turn=None):
+                     d_loss_turn=None, noalloc=False):
+        """accp_track_device (noalloc: accp_track_device_noalloc)."""
         o = TrackOpts()
         o.rng_seed, o.turn0, o.pid0 = seed, turn0, pid0
         if d_loss_turn:
             o.loss_turn = ctypes.cast(c_void_p(d_loss_turn), POINTER(c_int32))
         if tuning is not None:
             o.tuning = tuning
-        self.L.check(self.L.lib.accp_track_device(self.h, c_void_p(d_ptr), n, nturn, st, ed, byref(o),
-                                                  c_void_p(stream) if stream else None),
-                     "accp_track_device")
+        fn = self.L.lib.accp_track_device_noalloc if noalloc else self.L.lib.accp_track_device
+        self.L.check(fn(self.h, c_void_p(d_ptr), n, nturn, st, ed, byref(o), c_void_p(stream) if stream else None),
+                     "accp_track_device_noalloc" if noalloc else "accp_track_device")
 
 
 class Batch:

@@ -203,6 +203,19 @@ int accp_track(accp_lattice *lat, double *r, int n, int nturn, int st, int ed,
  * variant. */
 int accp_track_device(accp_lattice *lat, double *d_r, int n, int nturn, int st,
                       int ed, const accp_track_opts *opts, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): accp_track_device with the run's working set (live
+ * particle index lists, counts, tile list) in a fixed array of the library
+ * (about 8.3 MB on each GPU it runs on, held until the process ends)
+ * instead of buffers the library allocates; it allocates nothing.  At most
+ * ACCP_NOALLOC_MAX_PARTICLES particles (more: ACCP_E_ARG); no monitor
+ * records; no TransWake in the range (its sort needs buffers sized to the
+ * beam: ACCP_E_LATTICE).  Results equal accp_track_device's bit for bit.
+ * All calls on one GPU share the array: calls that overlap on one GPU race. */
+#define ACCP_NOALLOC_MAX_PARTICLES (1 << 20)
+int accp_track_device_noalloc(accp_lattice *lat, double *d_r, int n, int nturn, int st,
+                              int ed, const accp_track_opts *opts, void *stream);
 
 /* calc_beam_stat on a host AoS buffer, computed on the device. */
 int accp_beam_stat(const double *r, int n, double *stat /* ACCP_STAT_LEN */);
