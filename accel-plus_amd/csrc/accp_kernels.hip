@@ -573,11 +573,25 @@ __device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const
       for (int k = 0; k < PPL; ++k) drift_nl(p[k], L * p[k].ipd);
       Pm = P + 1;
     }
-    element_body<PPL, FULL>(p, Hdr{h.op & FUSE_OP_MASK, h.off, h.a, h.b}, Pm, ctx, pid, turn, eidx);
+    // the commonest elements first, each one test: a pure dipole BendMPole,
+    // a pure sextupole, an 8-term series quad, then the general dispatch
+    const Hdr hm{h.op & FUSE_OP_MASK, h.off, h.a, h.b};
+    if (hm.op == OP_BENDMP && (hm.b & 0x1ff) == (1 << 8)) {
+      bendmp_fixed<PPL, 0, true>(p, hm, Pm);
+    } else if (hm.op == OP_STRMP && hm.b != -1 && (hm.b & (0xff | STRMP_SINGLE)) == (2 | STRMP_SINGLE)) {
+      strmp_fixed<PPL, 2, true, true>(p, hm, Pm);
+    } else if (hm.op == OP_QUAD && hm.a == 8) {
+      const double L = Pm[0], K = Pm[1], aK = Pm[2], L2 = Pm[3];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) quad_series<8>(p[k], Pm + 4, L, L2, K > 0, aK);
+    } else {
+      element_body<PPL, FULL>(p, hm, Pm, ctx, pid, turn, eidx);
+    }
     if (h.op & FUSE_POST) {
       const PT *Pk = Pm + ((h.op >> FUSE_POST_DELTA_SHIFT) & FUSE_POST_MAX_DELTA);
       const int pb = ((h.op >> FUSE_POST_HI_SHIFT) & 0xf) | ((h.op & FUSE_POST_SINGLE) ? STRMP_SINGLE : 0);
-      strmp_dispatch<PPL, false>(p, Hdr{OP_STRMP_THIN, 0, 0, pb}, Pk);
+      if (pb == 9) strmp_fixed<PPL, 9, false, false>(p, Hdr{OP_STRMP_THIN, 0, 0, pb}, Pk);
+      else strmp_dispatch<PPL, false>(p, Hdr{OP_STRMP_THIN, 0, 0, pb}, Pk);
     }
   }
 }
