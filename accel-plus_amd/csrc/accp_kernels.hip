@@ -201,7 +201,7 @@ __device__ __forceinline__ void bendmp_fixed(Part (&p)[PPL], const Hdr h, const 
       qa[k] = lh1 * p[k].d - k1 * b[0];
       qb[k] = lh2 * p[k].d - k2 * b[0];
     }
-    for (int m = 0; m < ns; ++m) {
+    auto slice = [&]() {
 #pragma unroll
       for (int k = 0; k < PPL; ++k) hxdrift_py(p[k], fa[k], h0, py2[k]);
 #pragma unroll
@@ -216,6 +216,12 @@ __device__ __forceinline__ void bendmp_fixed(Part (&p)[PPL], const Hdr h, const 
       for (int k = 0; k < PPL; ++k) bend_kick_dipole(p[k], lhh1, qa[k], lh1);
 #pragma unroll
       for (int k = 0; k < PPL; ++k) hxdrift_py(p[k], fa[k], h0, py2[k]);
+    };
+    if (ns == 10) {  // the reference's default nslice (accel.h:195): straight-line code
+#pragma unroll
+      for (int m = 0; m < 10; ++m) slice();
+    } else {
+      for (int m = 0; m < ns; ++m) slice();
     }
     if (h.b & BM_EDGE_OUT) {
 #pragma unroll
@@ -262,7 +268,8 @@ __device__ __forceinline__ void bendmp_fixed(Part (&p)[PPL], const Hdr h, const 
 template <int HI, bool SINGLE>
 __device__ __forceinline__ void strmp_sum(const double (&b)[HI + 1], const double (&a)[HI + 1], double x,
                                           double y, double &re, double &im) {
-  if (SINGLE) horner_single<HI>(b[HI], x, y, re, im);
+  if constexpr (SINGLE) horner_single<HI>(b[HI], x, y, re, im);
+  else if constexpr (HI == 9) estrin9(b, a, x, y, re, im);
   else horner_reg<HI>(b, a, x, y, re, im);
 }
 
@@ -298,7 +305,7 @@ __device__ __forceinline__ void strmp_fixed(Part (&p)[PPL], const Hdr h, const P
   if constexpr (SINGLE && HI == 2) {
     // a pure sextupole: sext_kick
     const double kb1 = k1 * b[2], kb2 = k2 * b[2], kb1x2 = 2.0 * kb1, kb2x2 = 2.0 * kb2;
-    for (int m = 0; m < ns; ++m) {
+    auto slice = [&]() {
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
         drift_c(p[k], n1[k], z1[k]);
@@ -309,6 +316,12 @@ __device__ __forceinline__ void strmp_fixed(Part (&p)[PPL], const Hdr h, const P
         sext_kick(p[k], kb1, kb1x2);
         drift_c(p[k], n1[k], z1[k]);
       }
+    };
+    if (ns == 10) {  // the reference's default nslice (accel.h:242): straight-line code
+#pragma unroll
+      for (int m = 0; m < 10; ++m) slice();
+    } else {
+      for (int m = 0; m < ns; ++m) slice();
     }
     return;
   }

@@ -3,7 +3,9 @@
 // dipole pseudo-drift and kick (hxdrift_py, bend_kick_dipole) and the pure
 // sextupole kick (sext_kick) against the same maps written in the
 // reference's order (hxdrift_c and bend_kick_c<0, true>, which follow
-// atpass.cpp:199-235, 343-355; horner_single<2>).  They must agree to
+// atpass.cpp:199-235, 343-355; horner_single<2>), and the order-9 multipole
+// sum by Estrin's scheme (estrin9) against Horner's (horner_reg<9>, the
+// order of strthinkick, atpass.cpp:358-414).  They must agree to
 // round-off over random states; a dropped term or a wrong sign does not.
 // Prints one line per check and exits non-zero on a failure.
 #include <cmath>
@@ -90,6 +92,26 @@ int main() {
     worst = std::fmax(worst, rel(p, q, sc));
   }
   std::printf("sextupole kick, 20000 states: max scaled difference %.2e\n", worst);
+  if (!(worst < 1e-14)) fail = 1;
+  // order-9 multipole sum: Estrin against Horner (relative to the sum's
+  // scale, the largest term)
+  worst = 0;
+  for (int it = 0; it < 20000; ++it) {
+    double b[10], a[10];
+    for (int i = 0; i < 10; ++i) {
+      b[i] = U(-1, 1) * std::pow(10.0, 2 * i - 3);
+      a[i] = U(-1, 1) * std::pow(10.0, 2 * i - 3);
+    }
+    const double x = U(-0.02, 0.02), y = U(-0.01, 0.01);
+    double hr, hi, er, ei;
+    horner_reg<9>(b, a, x, y, hr, hi);
+    estrin9(b, a, x, y, er, ei);
+    double scale = 0, zn = 1;
+    for (int i = 0; i < 10; ++i, zn *= std::hypot(x, y)) scale = std::fmax(scale, std::hypot(b[i], a[i]) * zn);
+    worst = std::fmax(worst, std::fmax(std::fabs(hr - er), std::fabs(hi - ei)) / scale);
+  }
+  std::printf("order-9 multipole sum, Estrin vs Horner, 20000 cases: max difference %.2e of the largest term\n",
+              worst);
   if (!(worst < 1e-14)) fail = 1;
   std::printf(fail ? "FAILED\n" : "all regrouped maps ok\n");
   return fail;
