@@ -35,13 +35,15 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 # copy of known size; profiles/k_track_batched_da_64seeds_1024turns.md.  These
 # are constants of that profile, not measured by this run (the counters need
 # rocprofv3 around the process); other shapes and workloads report null.
-TRAFFIC_BYTES_PER_LAUNCH = {"block=256,ppl=1,lds": 7.13e6, "block=64,ppl=1,lds": 9.48e6}
+TRAFFIC_BYTES_PER_LAUNCH = {"block=64,ppl=1,lds": 8.99e6}
 TRAFFIC_SOURCE = "profile constant: profiles/k_track_batched_da_64seeds_1024turns.md (not measured in this run)"
 
 # Executed FP64 work of the saturated leg's kernel from rocprofv3 counters
 # (SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 per wave x 64 lanes, FMA = 2 flops),
 # per particle-turn of the seed-0 DA lattice; profiles/k_track_saturated_262144x16.md.
-EXECUTED_FP64_FLOPS_PER_PARTICLE_TURN = None
+EXECUTED_FP64_FLOPS_PER_PARTICLE_TURN = 74968.0
+# the same counters over one replay of the default batched workload, per launch shape
+EXECUTED_FP64_FLOPS_PER_STEP = {"block=64,ppl=1,lds": 4.935e11}
 
 
 def chunk_launches(nturn, chunk_max=256):
@@ -334,6 +336,52 @@ def collective_leg(lib, stream, nturn=200, eq_turns=20000):
     return res
 
 
+def tail_leg(lib, stream, n=48, nturn=256):
+    """The sparse tail of the batched DA run, isolated: one partly filled
+    wave (48 particles inside the dynamic aperture, 64-lane workgroup, table
+    in LDS, no compaction) tracking nturn turns of the seed-0 DA lattice, and
+    of sub-lattices holding one element class of it (its 40 dipole
+    BendMPoles, 40 sextupoles, 80 thin error multipoles, 40 quads, 120
+    drifts).  Each costs one wave's instruction stream per turn: the
+    per-turn time of the batched run's last launches."""
+    import numpy as np
+
+    import pyaccp.abi as abi
+    import pyaccp.elements as el
+    import pyaccp.lattices as lat
+    from pyaccp.hipmem import DeviceArray, Event
+
+    log("tail leg")
+    rng = np.random.default_rng(5)
+    beam = np.zeros((n, 6))
+    beam[:, 0] = rng.uniform(-1e-3, 1e-3, n)
+    beam[:, 2] = rng.uniform(0.0, 5e-4, n)
+    da = lat.da_ring(0)
+    groups = (("full lattice", da),
+              ("dipole bends", [e for e in da if e.type == el.BENDMPOLE]),
+              ("sextupoles", [e for e in da if e.type == el.STRMPOLE and e.len > 0]),
+              ("thin error multipoles", [e for e in da if e.type == el.STRMPOLE and e.len == 0]),
+              ("quads", [e for e in da if e.type == el.QUAD]),
+              ("drifts", [e for e in da if e.type == el.DRIFT]))
+    tuning = abi.Tuning(64, 1, 2, -1)
+    res = {}
+    for name, elems in groups:
+        L = lib.lattice(elems)
+        d = DeviceArray(beam)
+        L.track_device(d.ptr, n, 4, stream=stream.handle, tuning=tuning)
+        d.upload(beam)
+        e0, e1 = Event(), Event()
+        e0.record(stream.handle)
+        L.track_device(d.ptr, n, nturn, stream=stream.handle, tuning=tuning)
+        e1.record(stream.handle)
+        us = e0.elapsed_ms(e1) / nturn * 1e3
+        res[name] = {"elements": len(elems), "us_per_turn": round(us, 2),
+                     "ns_per_element": round(us / len(elems) * 1e3, 1),
+                     "lost": int(np.isnan(d.numpy()[:, 0]).sum())}
+    return {"workload": f"one wave of {n} particles (64-lane workgroup, LDS table, no compaction), {nturn} turns",
+            "per_class": res}
+
+
 def start_optics(lib, ring):
     """beta_x, alpha_x, eta_x, eta_x' at the ring start from the GPU's
     one-turn matrix of the ring with radiation, cavity and wake off."""
@@ -542,6 +590,25 @@ def diffusion_leg(lib, nx=256, ny=128, nturn=1024, cpu_sample=32):
     return res
 
 
+def side_legs(lib, stream, legs, lat0, glat0, args, result):
+    """The legs beside the batched one that need only the seed-0 lattice."""
+    import pyaccp.abi as abi
+
+    if "saturated" in legs:
+        result["saturated"] = saturated_leg(lib, glat0, lat0, abi.Tuning(256, 1, 2, 0), stream)
+    if "tail" in legs:
+        result["tail"] = tail_leg(lib, stream)
+    if "configs" in legs:
+        result["configs"] = config_legs(lib, stream)
+    if "collective" in legs:
+        result["collective"] = collective_leg(lib, stream, eq_turns=args.eq_turns)
+    if "adjoint" in legs:
+        log("adjoint leg")
+        result["adjoint"] = adjoint_leg(lib, stream)
+    if "diffusion" in legs:
+        result["diffusion"] = diffusion_leg(lib)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -555,13 +622,15 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
     ap.add_argument("--eq-turns", type=int, default=20000,
                     help="turns of the collective leg's equilibrium run (0: skip it)")
-    ap.add_argument("--legs", default="saturated,configs,collective,adjoint,diffusion,plain,cpu",
+    ap.add_argument("--legs", default="saturated,tail,configs,collective,adjoint,diffusion,plain,cpu",
                     help="comma list of the legs run after the batched one "
-                         "(saturated, configs, collective, adjoint, diffusion, plain, cpu); 'none' for the "
+                         "(saturated, tail, configs, collective, adjoint, diffusion, plain, cpu); 'none' for the "
                          "batched leg alone")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched leg (and its headline fields) and run only the listed legs")
     args = ap.parse_args()
     legs = set() if args.legs == "none" else set(args.legs.split(","))
-    unknown = legs - {"saturated", "configs", "collective", "adjoint", "diffusion", "plain", "cpu"}
+    unknown = legs - {"saturated", "tail", "configs", "collective", "adjoint", "diffusion", "plain", "cpu"}
     if unknown:
         ap.error(f"unknown legs {sorted(unknown)}")
     for flag, leg in ((args.no_plain, "plain"), (args.no_cpu, "cpu"), (args.no_adjoint, "adjoint")):
@@ -587,6 +656,15 @@ def main():
     if lib.device_count() < 1:
         raise SystemExit("no HIP device")
     stream = Stream()
+
+    if args.no_batched:
+        if legs & {"plain", "cpu"}:
+            ap.error("the plain and cpu legs compare with the batched run (drop --no-batched)")
+        result = {"legs": sorted(legs)}
+        lat0 = lat.da_ring(0, strength_errors=True)
+        side_legs(lib, stream, legs, lat0, lib.lattice(lat0), args, result)
+        print(json.dumps(result), flush=True)
+        return
 
     log("building workload")
     lats, beam, offs = lat.batched_workload(args.nseed)
@@ -694,19 +772,16 @@ def main():
             "kernel_ms": round(kernel_ms, 3),
         },
     }
+    shape = f"block={tuning.block},ppl={tuning.ppl},{'lds' if tuning.staging == 2 else 'scalar'}"
+    if default_workload and shape in EXECUTED_FP64_FLOPS_PER_STEP:
+        ex = EXECUTED_FP64_FLOPS_PER_STEP[shape] / (kernel_ms * 1e-3) / 1e12
+        result["roofline"]["executed_tflops"] = round(ex, 4)
+        result["roofline"]["executed_frac"] = round(ex / FP64_PEAK_TFLOPS, 5)
+        result["roofline"]["executed_source"] = ("FP64 VALU instructions of one replay from rocprofv3 counters "
+                                                 "(profiles/k_track_batched_da_64seeds_1024turns.md) / this "
+                                                 "run's kernel time")
 
-    if "saturated" in legs:
-        result["saturated"] = saturated_leg(lib, glats[0], lats[0], abi.Tuning(256, 1, 2, 0), stream)
-    if "configs" in legs:
-        result["configs"] = config_legs(lib, stream)
-    if "collective" in legs:
-        result["collective"] = collective_leg(lib, stream, eq_turns=args.eq_turns)
-    if "adjoint" in legs:
-        log("adjoint leg")
-        result["adjoint"] = adjoint_leg(lib, stream)
-
-    if "diffusion" in legs:
-        result["diffusion"] = diffusion_leg(lib)
+    side_legs(lib, stream, legs, lats[0], glats[0], args, result)
 
     if "plain" in legs:
         log("plain per-seed GPU path")
