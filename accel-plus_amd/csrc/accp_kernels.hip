@@ -547,11 +547,15 @@ __device__ __forceinline__ void element_body(Part (&p)[PPL], const Hdr h, const 
       if (!FULL) break;
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
+        // normals in the reference's draw order (accel.cpp:1015-1022): the
+        // phase's first when it draws, then x's and y's amplitudes
         double g0 = 0, g1 = 0, g2 = 0;
-        if ((h.a & 1) && (h.a & 2)) g0 = gauss_kick(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, 0);
+        const bool ph = (h.a & 1) && (h.a & 2);
+        const uint32_t ja = ph ? 1u : 0u;
+        if (ph) g0 = gauss_kick(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, 0);
         if ((h.a & 1) && (h.a & 4)) {
-          g1 = gauss_kick(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, 1);
-          g2 = gauss_kick(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, 2);
+          g1 = gauss_kick(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, ja);
+          g2 = gauss_kick(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, ja + 1);
         }
         crabcav(p[k], P, h.a, g0, g1, g2);
       }

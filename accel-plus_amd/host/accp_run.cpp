@@ -241,6 +241,11 @@ std::vector<accp_elem> read_lattice(const string &path, std::deque<std::vector<d
     } else if (type == "RFMult") {  // accel.cpp:1058-1068: L f phis E sigphi sigdVV MaxNPole / PolyB / PolyA
       e = new_elem(ACCP_RFMULT, fam);
       ps >> e.len >> e.freq >> e.phis >> e.energy >> e.sigphi >> e.sigdvv >> e.max_npole;
+      // The reference parses the noise sigmas, then builds the element with
+      // a constructor that clears them (accel.cpp:1065-1071, accel.h:376):
+      // an RFMult read from a lattice file has no phase noise.
+      e.sigphi = 0;
+      e.sigdvv = 0;
       if (e.max_npole < 0 || e.max_npole > ACCP_MAX_MPOLE)
         throw std::runtime_error("RFMult " + fam + ": MaxNPole out of range");
       for (int i = 0; i < e.max_npole; ++i) in >> e.polyB[i];

@@ -627,12 +627,15 @@ void orc_elem_pass(const accp_elem *e, double *r, int n, const orc_ctx *ctx) {
         if (e->on) {
           const uint64_t pid = ctx->pid0 + (uint64_t)i;
           const uint32_t ei = (uint32_t)ctx->eidx;
+          /* normals in the reference's draw order (accel.cpp:1015-1022):
+           * the phase's first when it draws, then x's and y's amplitudes */
           double om, nvxa = nvx, nvya = nvy;
-          if (e->sigphi > 0) om = -k * p[4] + e->phis + e->sigphi * orc_gauss(ctx->seed, pid, ctx->turn, ei, 0);
+          uint32_t j = 0;
+          if (e->sigphi > 0) om = -k * p[4] + e->phis + e->sigphi * orc_gauss(ctx->seed, pid, ctx->turn, ei, j++);
           else om = -k * p[4] + e->phis;
           if (e->sigdvv > 0) {
-            nvxa = nvx * (1.0 + e->sigdvv * orc_gauss(ctx->seed, pid, ctx->turn, ei, 1));
-            nvya = nvy * (1.0 + e->sigdvv * orc_gauss(ctx->seed, pid, ctx->turn, ei, 2));
+            nvxa = nvx * (1.0 + e->sigdvv * orc_gauss(ctx->seed, pid, ctx->turn, ei, j));
+            nvya = nvy * (1.0 + e->sigdvv * orc_gauss(ctx->seed, pid, ctx->turn, ei, j + 1));
           }
           p[1] += nvxa * sin(om);
           p[3] += nvya * sin(om);

@@ -316,6 +316,10 @@ class Reference:
                 self.lib.ref_setfield_int(h, i, b"NSlice", e.nslice)
             if e.type == el.BENDMPOLE and e.energy != el.DEFAULT_E0:
                 self.lib.ref_setfield(h, i, b"energy", e.energy)
+            if e.type == el.RFMULT and e.sigphi != 0:
+                # an RFMult read from a file drops its noise sigmas
+                # (accel.cpp:1065-1071); SigPhi is settable (accel.cpp:1085)
+                self.lib.ref_setfield(h, i, b"SigPhi", e.sigphi)
             if e.type == el.IDEALWIG and e.energy != el.DEFAULT_E0:
                 raise ValueError("the reference's IdealWig energy is not settable from the lattice file")
         rads = {e.rad for e in elems if e.type in (el.BENDMPOLE, el.SYNCRAD, el.IDEALWIG)}
@@ -369,7 +373,7 @@ class RefLine:
         in its quantum-excitation kicks (common random numbers)."""
         r = np.ascontiguousarray(r, dtype=np.float64).copy()
         if self.lib.ref_pass_crn(self.h, _dp(r), r.size // 6, nturn, seed, turn0, pid0) != 0:
-            raise ValueError("ref_pass_crn needs an even nslice in every BendMPole")
+            raise ValueError("ref_pass_crn failed")
         return r.reshape(-1, 6)
 
     def track_range(self, r, st, ed):

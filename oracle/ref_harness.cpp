@@ -22,12 +22,18 @@ namespace {
 // Common random numbers for quantum excitation.  The library is linked with
 // -Wl,--wrap=rand, so the reference's grandn (utility.cpp:194-217) draws its
 // uniforms from __wrap_rand below.  Outside ref_pass_crn that forwards to
-// the C library; inside it, while one particle passes one BendMPole, call c
-// returns the oracle's uniform u1 (c even) or u2 (c odd) of pair c/2 of that
-// (particle, turn, element): Philox4x32-10 (Salmon et al., SC'11) keyed by
-// the seed over the counter {pid, turn, element, pair}, the words taken
-// as the oracle's 53-bit uniform, rounded to rand()'s grid k/RAND_MAX.  The
-// reference's Box-Muller then yields the oracle's normals to ~1e-9.
+// the C library; inside it, while one particle passes one element that draws
+// normals (BendMPole with quantum excitation, SyncRad, CrabCav, RFMult),
+// call c returns the oracle's uniform u1 (c even) or u2 (c odd) of pair c/2
+// of that (particle, turn, element): Philox4x32-10 (Salmon et al., SC'11)
+// keyed by the seed over the counter {pid, turn, element, pair}, the words
+// taken as the oracle's 53-bit uniform, rounded to rand()'s grid
+// k/RAND_MAX.  The reference's Box-Muller then yields the oracle's normals to
+// ~1e-9: the j-th normal a particle draws in an element is the cosine (j
+// even) or sine (j odd) branch of pair j/2.  grandn keeps the second normal
+// of a pair in static state and flips a static toggle per call
+// (utility.cpp:194-217), so after each particle the harness brings the
+// toggle back to "draw a new pair" (sync_grandn).
 struct Crn {
   bool on = false;
   uint32_t key[2] = {0, 0};
@@ -45,6 +51,20 @@ void crn_philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
   }
+}
+
+// Leave grandn's toggle at "draw a new pair": one call draws (two rand()
+// calls) when it was there, and then a second call flips it back; a call
+// that draws nothing consumed the pending second normal and already did.
+void sync_grandn() {
+  const uint32_t before = g_crn.calls;
+  grandn();
+  if (g_crn.calls != before) grandn();
+}
+
+bool draws_normals(AccelElement *el) {
+  const std::string &t = el->gettype();
+  return t == "BendMPole" || t == "SyncRad" || t == "CrabCav" || t == "RFMult";
 }
 
 bool g_map_ready = false;
@@ -276,30 +296,23 @@ void ref_trackdnudexy(void *h, unsigned nturn, double xm, double ym, double *d) 
   static_cast<ALine *>(h)->trackdnudexy(d, nturn, xm, ym);
 }
 // ALine::pass(r, N, nturn) (aline.cpp:359-363) with the oracle's normals in
-// every quantum-excitation kick (common random numbers, see __wrap_rand):
-// elements in line order over the whole beam, except that each BendMPole
-// passes one particle at a time so that its kicks draw the counter stream of
+// every noise draw (common random numbers, see __wrap_rand): elements in line
+// order over the whole beam, except that each element that draws normals
+// passes one particle at a time, so that its draws take the counter stream of
 // (pid0 + i, turn0 + t, element).  Particles are independent (no collective
-// element may be in the line), so the order changes nothing else.  Every
-// BendMPole needs an even nslice, so that grandn's pairs (a cosine and a
-// sine draw) stay inside one particle.  Returns 0, or -1 for an odd nslice.
+// element may be in the line), so the order changes nothing else.  Returns 0.
 int ref_pass_crn(void *h, double *r, int n, int nturn, uint64_t seed, uint64_t turn0, uint64_t pid0) {
   ALine *line = static_cast<ALine *>(h);
   const unsigned ne = line->getnumber();
-  for (unsigned e = 0; e < ne; ++e) {
-    AccelElement *el = line->getelement(e);
-    if (el->gettype() == "BendMPole") {
-      int ns = 0;
-      el->getfield(&ns, "NSlice");
-      if (ns % 2) return -1;
-    }
-  }
   g_crn.key[0] = (uint32_t)seed;
   g_crn.key[1] = (uint32_t)(seed >> 32);
+  g_crn.on = true;
+  sync_grandn();
+  g_crn.on = false;
   for (int t = 0; t < nturn; ++t) {
     for (unsigned e = 0; e < ne; ++e) {
       AccelElement *el = line->getelement(e);
-      if (el->gettype() != "BendMPole") {
+      if (!draws_normals(el)) {
         el->pass(r, n);
         continue;
       }
@@ -310,6 +323,7 @@ int ref_pass_crn(void *h, double *r, int n, int nturn, uint64_t seed, uint64_t t
         g_crn.calls = 0;
         g_crn.on = true;
         el->pass(r + 6 * i, 1);
+        sync_grandn();
         g_crn.on = false;
       }
     }

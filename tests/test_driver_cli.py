@@ -120,6 +120,32 @@ def test_export_is_lossless_and_reads_back(tmp_path):
     assert set(ref) <= set(vals)
 
 
+def test_rfmult_noise_is_not_read_from_a_lattice_file(tmp_path, reference):
+    """The reference parses RFMult's noise sigmas and then builds the element
+    with a constructor that clears them (accel.cpp:1065-1071, accel.h:376),
+    while CrabCav passes its own to the constructor (accel.cpp:943-945).  The
+    driver reads both files the same way: the exported RFMult carries zero
+    sigmas, the CrabCav its own; the compiled reference, loading the same
+    file, reports SigPhi 0 and 0.01."""
+    from pyoracle import RefLine
+    ring = [el.rfmult(0.1, 5e8, 0.2, [1e-4, 2e-3], [0.0, 1e-3], sigphi=0.01, sigdvv=0.02, name="RFM"),
+            el.drift(0.5), el.crabcav(0.2, 1.5, 0.5, 1.5e9, 0.3, 3000.0, sigphi=0.01, sigdvv=0.02, name="CRAB")]
+    path = tmp_path / "noise.lat"
+    el.write_lattice(ring, str(path))
+    p = run(tmp_path, "Lattice\n file_name = noise.lat\n export_element: type=RING\nEnd\n", "--parse-only")
+    assert p.returncode == 0, p.stderr
+    body = export_block(p.stdout)
+    rfm = [float(t) for t in body[[i for i, s in enumerate(body) if "RFMult" in s][0]].split()[3:9]]
+    crab = [float(t) for t in body[[i for i, s in enumerate(body) if "CrabCav" in s][0]].split()[3:11]]
+    assert rfm[4:6] == [0.0, 0.0], rfm
+    assert crab[6:8] == [0.01, 0.02], crab
+    h = reference.lib.ref_load(str(path).encode())
+    assert h
+    line = RefLine(reference.lib, h)
+    assert line.getfield(0, "SigPhi") == 0.0
+    assert line.getfield(2, "SigPhi") == 0.01
+
+
 def test_apply_setpt(tmp_path):
     """Set points (process.cpp:2273-2392): the first element must match, then
     one PolyB[2] value per SF member."""

@@ -275,6 +275,31 @@ def test_matrix_syncrad_crabcav_rfmult_wiggler(gpu_lib, oracle):
                         1e-10, what="lumped ring, 50 turns")
 
 
+def test_noise_against_reference_crn(gpu_lib, oracle):
+    """The noise cases of noise.npz on the GPU (CrabCav phase and amplitude
+    jitter, RFMult phase jitter, SyncRad excitation, BendMPole quantum
+    excitation, a line of three noisy elements): against the compiled
+    reference fed the same normals in its draw order, within 1e-6 of the
+    noise (the reference's normals come through rand()'s grid), and against
+    the oracle on the same Philox stream at 1e-12."""
+    from conftest import golden
+    from tests.golden.make_noise import noise_cases
+    from tests.test_oracle_golden import noise_bound
+    g = golden("noise.npz")
+    P = g["particles"]
+    kw = dict(seed=int(g["seed"][0]), turn0=int(g["turn0"][0]), pid0=int(g["pid0"][0]))
+    nt = int(g["turns"][0])
+    for name, noisy, quiet in noise_cases():
+        r = g[name]
+        out = gpu_lib.lattice(noisy).track(P, nt, **kw)
+        tol, fin = noise_bound(r, oracle.track(quiet, P, nt, **kw))
+        assert np.array_equal(np.isfinite(out[:, 0]), fin), name
+        err = np.abs(out - r)[fin].max(axis=0)
+        print(f"{name}: GPU - reference {err.max():.2e} (bound {tol.max():.2e})", flush=True)
+        assert (err <= tol).all(), (name, err, tol)
+        assert_coords_close(out, oracle.track(noisy, P, nt, **kw), 1e-12, what=name)
+
+
 def test_wigtable(gpu_lib, oracle):
     """WigTable on the GPU against the reference fixtures (wigtable.npz) at
     1e-12: thin and sliced tables, descending and non-uniform grids up to

@@ -186,6 +186,42 @@ def test_radiation_equilibrium_oracle_vs_reference():
         assert abs(o[-1, 3] - r[-1, 3]) <= loss_tol * n, (run, o[-1, 3], r[-1, 3])
 
 
+def noise_bound(r, quiet):
+    """Per coordinate: 1e-6 of the largest noise displacement |r - quiet|
+    over the finite particles.  The harness rounds each Philox uniform u to
+    rand()'s grid (relative 5e-10), and Box-Muller's radius sqrt(-2 ln u)
+    magnifies that by 1/(2 u ln(1/u)); over the few thousand draws of a case
+    the smallest u is ~1e-4, so single normals differ by up to ~1e-7 of
+    their size (5e-8 of the noise measured)."""
+    fin = np.isfinite(r[:, 0])
+    return 1e-6 * np.abs(r - quiet)[fin].max(axis=0), fin
+
+
+def test_noise_oracle_vs_reference_crn(oracle):
+    """CrabCav phase and amplitude jitter (both, each alone), RFMult phase
+    jitter, SyncRad excitation, BendMPole quantum excitation and a line of
+    three noisy elements: 3 turns from turn 7 with particle ids from 1000
+    against the compiled reference fed the same normals in its own draw
+    order (noise.npz, tests/golden/make_noise.py).  A normal taken in
+    another order or from another (particle, turn, element) moves a
+    coordinate by the whole noise, 1e6 times the bound."""
+    from tests.golden.make_noise import noise_cases
+    g = golden("noise.npz")
+    P = g["particles"]
+    kw = dict(seed=int(g["seed"][0]), turn0=int(g["turn0"][0]), pid0=int(g["pid0"][0]))
+    nt = int(g["turns"][0])
+    for name, noisy, quiet in noise_cases():
+        r = g[name]
+        q = oracle.track(quiet, P, nt, **kw)
+        tol, fin = noise_bound(r, q)
+        assert tol.max() > 1e-12, name  # the noise is there
+        o = oracle.track(noisy, P, nt, **kw)
+        assert np.array_equal(np.isfinite(o[:, 0]), fin), name
+        err = np.abs(o - r)[fin].max(axis=0)
+        print(f"{name}: noise up to {tol.max() * 1e6:.2e}, oracle - reference {err.max():.2e}")
+        assert (err <= tol).all(), (name, err, tol)
+
+
 def test_transwake_bit_exact(oracle):
     """The transverse wake over the whole beam (one reference thread),
     pairwise as the reference sums it: bit for bit."""
