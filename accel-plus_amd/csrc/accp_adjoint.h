@@ -16,7 +16,16 @@
 // bndthinkick (:199-235, normal multipoles with the curvature terms of
 // PolyB[1] and PolyB[2]), strthinkick (:358-414), edge
 // (:190-196), AMatrix::pass (accel.cpp:1697-1729), ACavity::pass
-// (accel.cpp:903-923), ACorrector::pass (accel.cpp:270-292).
+// (accel.cpp:903-923), ACorrector::pass (accel.cpp:270-292); with delta
+// changing inside the element: bndthinkickrad with the classical loss
+// (atpass.cpp:255-339, flag 1), ACrabCav::pass and ARFMult::pass without
+// noise (accel.cpp:968-1034, 1146-1201), ASyncRad::pass with damping only
+// (accel.cpp:1813-1854), AIdealWig::pass with and without its classical
+// loss (accel.cpp:1463-1537) and AWigTable::pass (accel.cpp:1338-1382).
+// The loss maps keep x and y and move delta by a function of the angles
+// and of delta itself; given the angles after the loss, the delta before it
+// is the root near 1 + delta of a quadratic, so these inverses stay in
+// closed form too.
 #pragma once
 #include "accp_physics.h"
 
@@ -559,6 +568,273 @@ ACCP_HD void corrector_rev(Part &p, Adj &l, double L, double kx, double ky) {
   p.py -= ky * p.ipd;
   l.d += -(l.px * kx + l.py * ky) * p.ipd * p.ipd;
   if (L != 0) drift_nl_rev(p, l, 0.5 * L);
+}
+
+// ---- bndthinkickrad, classical loss (bend_kick_rad with flag 1) ----------------
+// Forward: x' = px/(1+d), y' = py/(1+d); d1 = d - C (1+d)^2 with
+// C = crad L B2perp(x, x', y, y') (1 + h x + (x'^2 + y'^2)/2); then
+// px = x' (1+d1) - L (-h d1 + F(x, y)), py = y' (1+d1) + L G(x, y),
+// z -= L h x.  Inverse: x' and y' from the output momenta and d1, C from
+// them, and 1+d = 2 (1+d1) / (1 + sqrt(1 - 4 C (1+d1))).  The VJP runs the
+// forward operations backwards.
+template <class PT>
+ACCP_HD void bend_kick_rad_rev(Part &p, Adj &l, const PT *B, int mo, double L, double h, double k1c, double k2c,
+                               double crad) {
+  const double x = p.x, y = p.y;
+  double re = B[mo], im = 0.0, dre = 0.0, dim = 0.0;
+  for (int i = mo - 1; i >= 0; --i) {
+    const double tdr = dre * x - dim * y + re;
+    dim = dim * x + dre * y + im;
+    dre = tdr;
+    const double t = re * x - im * y + B[i];
+    im = im * x + re * y;
+    re = t;
+  }
+  const double hk = h * (k1c * h - k2c);
+  im += hk * y * y * y * (1.0 / 6.0);
+  re += -k1c * h * y * y * 0.5 + hk * x * y * y * 0.5;
+  // partial derivatives of the corrected sums (Cauchy-Riemann for P)
+  const double re_x = dre + hk * y * y * 0.5, re_y = -dim - k1c * h * y + hk * x * y;
+  const double im_x = dim, im_y = dre + hk * y * y * 0.5;
+  const double c3 = h / 6.0 * k1c - k2c / 3.0;
+  const double fx = re + h * (h * x + k1c * (x * x - 0.5 * y * y) + k2c * (x * x * x - (4.0 / 3.0) * x * y * y));
+  const double gy = im + h * (k1c * x * y + (4.0 / 3.0) * k2c * x * x * y + c3 * y * y * y);
+  const double fx_x = re_x + h * (h + 2.0 * k1c * x + k2c * (3.0 * x * x - (4.0 / 3.0) * y * y));
+  const double fx_y = re_y - h * (k1c * y + (8.0 / 3.0) * k2c * x * y);
+  const double gy_x = im_x + h * (k1c * y + (8.0 / 3.0) * k2c * x * y);
+  const double gy_y = im_y + h * (k1c * x + (4.0 / 3.0) * k2c * x * x + 3.0 * c3 * y * y);
+  // the state before the kick
+  const double d1 = p.d, u1 = 1.0 + d1;
+  const double xp = (p.px + L * (-h * d1 + fx)) / u1;
+  const double yp = (p.py - L * gy) / u1;
+  const double a = 1.0 + x * h;
+  const double vq = a * a + xp * xp + yp * yp;
+  const double by = re + h, bx = im;
+  const double t1 = by * a, t2 = bx * a, t3 = bx * yp - by * xp;
+  const double nn = t1 * t1 + t2 * t2 + t3 * t3;
+  const double b2p = nn / vq;
+  const double S = a + (xp * xp + yp * yp) * 0.5;
+  const double C = crad * b2p * S * L;
+  const double u0 = 2.0 * u1 / (1.0 + sqrt(1.0 - 4.0 * C * u1));
+  // VJP
+  const Adj o = l;
+  const double gd1 = o.d + o.px * (xp + L * h) + o.py * yp;
+  double gxp = o.px * u1, gyp = o.py * u1;
+  double gx = -o.px * L * fx_x + o.py * L * gy_x - o.z * L * h;
+  double gyy = -o.px * L * fx_y + o.py * L * gy_y;
+  // d1 = d - mean, mean = crad L u0^2 b2p S
+  const double gmean = -gd1;
+  double gu = gmean * crad * L * 2.0 * u0 * b2p * S;
+  const double gb2p = gmean * crad * L * u0 * u0 * S;
+  const double gS = gmean * crad * L * u0 * u0 * b2p;
+  double ga = gS;
+  gxp += gS * xp;
+  gyp += gS * yp;
+  const double gnn = gb2p / vq, gvq = -gb2p * b2p / vq;
+  ga += gvq * 2.0 * a;
+  gxp += gvq * 2.0 * xp;
+  gyp += gvq * 2.0 * yp;
+  const double gt1 = 2.0 * t1 * gnn, gt2 = 2.0 * t2 * gnn, gt3 = 2.0 * t3 * gnn;
+  const double gby = gt1 * a - gt3 * xp, gbx = gt2 * a + gt3 * yp;
+  ga += gt1 * by + gt2 * bx;
+  gxp -= gt3 * by;
+  gyp += gt3 * bx;
+  gx += ga * h + gby * re_x + gbx * im_x;
+  gyy += gby * re_y + gbx * im_y;
+  const double iu0 = 1.0 / u0;
+  gu -= (gxp * xp + gyp * yp) * iu0;
+  l.x = o.x + gx;
+  l.px = gxp * iu0;
+  l.y = o.y + gyy;
+  l.py = gyp * iu0;
+  l.z = o.z;
+  l.d = gd1 + gu;
+  p.px = xp * u0;
+  p.py = yp * u0;
+  p.z += L * h * x;
+  set_delta(p, u0 - 1.0);
+}
+
+// ABendMPole::pass with the classical loss (flag 1): the 4th-order slices
+// walked back, the pseudo-drift constants rebuilt from the delta of each
+// sub-step.
+template <class PT>
+ACCP_HD void bendmp_rad_rev(Part &p, Adj &l, const Hdr h, const PT *P) {
+  const int ns = h.a, mo = h.b & 0xff;
+  const double psi1 = P[BM_PSI1], psi2 = P[BM_PSI2], h0 = P[BM_IRHO];
+  const double l1 = P[BM_L1], l2 = P[BM_L2], k1 = P[BM_K1], k2 = P[BM_K2];
+  const double hs1 = P[BM_HS1], hq1 = P[BM_HQ1], hs2 = P[BM_HS2], hq2 = P[BM_HQ2];
+  const double k1c = P[BM_K1C], k2c = P[BM_K2C], crad = P[BM_CRAD];
+  const PT *B = P + BM_B;
+  if (psi2 != 0) edge_rev(p, l, psi2);
+  for (int m = 0; m < ns; ++m) {
+    hxdrift_c_rev(p, l, hx_consts(p.ipd, l1, hs1, hq1), h0, hs1 * 0.5);
+    bend_kick_rad_rev(p, l, B, mo, k1, h0, k1c, k2c, crad);
+    hxdrift_c_rev(p, l, hx_consts(p.ipd, l2, hs2, hq2), h0, hs2 * 0.5);
+    bend_kick_rad_rev(p, l, B, mo, k2, h0, k1c, k2c, crad);
+    hxdrift_c_rev(p, l, hx_consts(p.ipd, l2, hs2, hq2), h0, hs2 * 0.5);
+    bend_kick_rad_rev(p, l, B, mo, k1, h0, k1c, k2c, crad);
+    hxdrift_c_rev(p, l, hx_consts(p.ipd, l1, hs1, hq1), h0, hs1 * 0.5);
+  }
+  if (psi1 != 0) edge_rev(p, l, psi1);
+}
+
+// ---- ASyncRad, damping only (syncrad with damp, no excitation) -------------
+template <class PT>
+ACCP_HD void syncrad_rev(Part &p, Adj &l, const PT *C, bool damp) {
+  const double L = C[0];
+  if (L != 0) drift_nl_rev(p, l, 0.5 * L);
+  if (damp) {
+    p.x /= C[1];
+    p.y /= C[2];
+    set_delta(p, p.d / C[3]);
+    l.x *= C[1];
+    l.y *= C[2];
+    l.d *= C[3];
+  }
+  if (L != 0) drift_nl_rev(p, l, 0.5 * L);
+}
+
+// ---- ACrabCav without noise ---------------------------------------------------
+// Kick at om = -k z + phis: px += nvx sin om, py += nvy sin om,
+// d1 = d - A/(1+d) with A = (nvy y + nvx x) k cos om; 1+d is the positive
+// root of u^2 - (1+d1) u - A = 0.
+template <class PT>
+ACCP_HD void crabcav_rev(Part &p, Adj &l, const PT *C, bool on) {
+  const double L = C[0];
+  if (L != 0) drift_nl_rev(p, l, 0.5 * L);
+  if (on) {
+    const double nvx = C[1], nvy = C[2], k = C[3];
+    double so, co;
+    sincos(-k * p.z + C[4], &so, &co);
+    const double sxy = nvy * p.y + nvx * p.x;
+    const double A = sxy * k * co;
+    const double hh = 0.5 * (1.0 + p.d);
+    const double u0 = hh + sqrt(hh * hh + A);
+    const double iu = 1.0 / u0;
+    const Adj o = l;
+    l.x = o.x - o.d * nvx * k * co * iu;
+    l.y = o.y - o.d * nvy * k * co * iu;
+    l.z = o.z - k * co * (o.px * nvx + o.py * nvy) - o.d * sxy * k * k * so * iu;
+    l.d = o.d * (1.0 + A * iu * iu);
+    p.px -= nvx * so;
+    p.py -= nvy * so;
+    set_delta(p, u0 - 1.0);
+  }
+  if (L != 0) drift_nl_rev(p, l, 0.5 * L);
+}
+
+// ---- ARFMult without noise -----------------------------------------------------
+// In Cartesian form the kick is px += cos om Re G(w), py -= cos om Im G(w),
+// d1 = d + k sin om Re F(w)/(1+d), with w = x + iy,
+// G(w) = sum_m (B_m - i A_m) w^m and F' = G, F(0) = 0 (the polar sums of
+// the reference are these series term by term); 1+d is the root near
+// 1+d1 of u^2 - (1+d1) u + k sin om Re F = 0.
+template <class PT>
+ACCP_HD void rfmult_rev(Part &p, Adj &l, const PT *C, int n) {
+  const double L = C[0], k = C[1];
+  if (L != 0) drift_nl_rev(p, l, 0.5 * L);
+  const PT *B = C + 4, *A = C + 4 + n;
+  const double x = p.x, y = p.y;
+  double gr = 0, gi = 0, dr = 0, di = 0, hr = 0, hi = 0;
+  for (int m = n - 1; m >= 0; --m) {
+    const double tdr = dr * x - di * y + gr;
+    di = dr * y + di * x + gi;
+    dr = tdr;
+    const double tgr = gr * x - gi * y + B[m];
+    gi = gr * y + gi * x - A[m];
+    gr = tgr;
+    const double inv = 1.0 / (m + 1);
+    const double thr = hr * x - hi * y + B[m] * inv;
+    hi = hr * y + hi * x - A[m] * inv;
+    hr = thr;
+  }
+  const double fr = hr * x - hi * y;  // Re F, F = w H
+  double so, co;
+  sincos(-k * p.z + C[2], &so, &co);
+  const double bq = k * fr * so;
+  const double hh = 0.5 * (1.0 + p.d);
+  const double u0 = hh + sqrt(hh * hh - bq);
+  const double iu = 1.0 / u0;
+  const Adj o = l;
+  l.x = o.x + co * (o.px * dr - o.py * di) + o.d * k * so * gr * iu;
+  l.y = o.y - co * (o.px * di + o.py * dr) - o.d * k * so * gi * iu;
+  l.z = o.z + k * so * (o.px * gr - o.py * gi) - o.d * k * k * co * fr * iu;
+  l.d = o.d * (1.0 - bq * iu * iu);
+  p.px -= co * gr;
+  p.py += co * gi;
+  set_delta(p, u0 - 1.0);
+  if (L != 0) drift_nl_rev(p, l, 0.5 * L);
+}
+
+// ---- AIdealWig -------------------------------------------------------------------
+// The loss step keeps the angles: d1 = d - rq (1+d)^2, p1 = p (1+d1)/(1+d).
+ACCP_HD void wig_loss_rev(Part &p, Adj &l, double rq) {
+  const double u1 = 1.0 + p.d;
+  const double u0 = 2.0 * u1 / (1.0 + sqrt(1.0 - 4.0 * rq * u1));
+  const double r = u1 / u0, dd = 1.0 - 2.0 * rq * u0;
+  const double xp = p.px / u1, yp = p.py / u1;
+  const Adj o = l;
+  l.px = o.px * r;
+  l.py = o.py * r;
+  l.d = o.d * dd + (o.px * xp + o.py * yp) * (dd - r);
+  p.px = xp * u0;
+  p.py = yp * u0;
+  set_delta(p, u0 - 1.0);
+}
+
+template <class PT>
+ACCP_HD void idealwig_rev(Part &p, Adj &l, const PT *C, int nslice, int type, bool rad) {
+  const double k2 = C[1], hh = C[2], lns = C[3], half = C[4], rq = C[5];
+  for (int j = 0; j < nslice; ++j) {
+    drift_nl_rev(p, l, half);
+    if (rad) wig_loss_rev(p, l, rq);
+    if (type == 0) {
+      const double y = p.y, y2 = y * y;
+      p.py += hh * (0.5 * y + 1.0 / 3.0 * y2 * y * k2 + 1.0 / 15.0 * y2 * y2 * y * k2 * k2) * lns;
+      l.y -= l.py * hh * (0.5 + y2 * k2 + 1.0 / 3.0 * y2 * y2 * k2 * k2) * lns;
+    } else if (type == 1) {
+      const double x = p.x, x2 = x * x;
+      p.px += hh * (0.5 * x + 1.0 / 3.0 * x2 * x * k2 + 1.0 / 15.0 * x2 * x2 * x * k2 * k2) * lns;
+      l.x -= l.px * hh * (0.5 + x2 * k2 + 1.0 / 3.0 * x2 * x2 * k2 * k2) * lns;
+    }
+    if (rad) wig_loss_rev(p, l, rq);
+    drift_nl_rev(p, l, half);
+  }
+}
+
+// ---- AWigTable: the bilinear kick and its piecewise-constant gradient ---------
+template <class PT>
+ACCP_HD void wt_kick_rev(Part &p, Adj &l, const PT *xl, int mx, const PT *yl, int my, const PT *xk,
+                         const PT *yk, double w) {
+  const double x = p.x, y = p.y;
+  const int i = wt_cell(xl, mx, x), j = wt_cell(yl, my, y);
+  const double dx = xl[i + 1] - xl[i], dy = yl[j + 1] - yl[j];
+  const double t = (x - xl[i]) / dx, u = (y - yl[j]) / dy;
+  const int o0 = j * mx + i, o1 = o0 + mx;
+  const double ax = ((1. - u) * (xk[o0 + 1] - xk[o0]) + u * (xk[o1 + 1] - xk[o1])) / dx;
+  const double ay = ((1. - t) * (xk[o1] - xk[o0]) + t * (xk[o1 + 1] - xk[o0 + 1])) / dy;
+  const double bx = ((1. - u) * (yk[o0 + 1] - yk[o0]) + u * (yk[o1 + 1] - yk[o1])) / dx;
+  const double by = ((1. - t) * (yk[o1] - yk[o0]) + t * (yk[o1 + 1] - yk[o0 + 1])) / dy;
+  p.px -= wt_interp(xl, mx, yl, my, xk, x, y) * w;
+  p.py -= wt_interp(xl, mx, yl, my, yk, x, y) * w;
+  l.x += w * (l.px * ax + l.py * bx);
+  l.y += w * (l.px * ay + l.py * by);
+}
+
+template <class PT>
+ACCP_HD void wigtable_rev(Part &p, Adj &l, const PT *C, int nslice, int mx, int my) {
+  const PT *xl = C + 2, *yl = xl + mx, *xk = yl + my, *yk = xk + mx * my;
+  if (nslice == 0) {
+    wt_kick_rev(p, l, xl, mx, yl, my, xk, yk, 1.0);
+    return;
+  }
+  const double ns = C[1], hl = C[0] / ns;
+  for (int s = 0; s < nslice; ++s) {
+    drift_nl_rev(p, l, hl);
+    wt_kick_rev(p, l, xl, mx, yl, my, xk, yk, 1.0 / ns);
+    drift_nl_rev(p, l, hl);
+  }
 }
 
 }  // namespace accp

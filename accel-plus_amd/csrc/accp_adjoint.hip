@@ -116,8 +116,11 @@ __device__ __forceinline__ void strmp_rev_dispatch(Part &p, Adj &l, const Hdr h,
 }
 
 // FULL = false leaves out the reverse maps of the trigonometric quad, the
-// linear Bend and the Matrix, whose registers would halve the occupancy of
-// every lattice's sweep; the host picks the variant per lattice.
+// linear Bend, the Matrix and the elements that move delta inside them
+// (radiating BendMPole, SyncRad, CrabCav, RFMult, IdealWig) or read a kick
+// table (WigTable), whose registers would halve the occupancy of every
+// lattice's sweep; the host picks the variant per lattice.  The host also
+// refuses what has no reverse map (noise, quantum excitation, wakes).
 template <bool FULL, class PT>
 __device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const PT *P, const Knob &kn,
                                             double *grad) {
@@ -185,6 +188,24 @@ __device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const 
       break;
     case OP_CORRECTOR:
       corrector_rev(p, l, P[0], P[1], P[2]);
+      break;
+    case OP_BENDMP_RAD:
+      if (FULL) bendmp_rad_rev(p, l, h, P);
+      break;
+    case OP_SYNCRAD:
+      if (FULL) syncrad_rev(p, l, P, (h.a & 1) != 0);
+      break;
+    case OP_CRABCAV:
+      if (FULL) crabcav_rev(p, l, P, (h.a & 1) != 0);
+      break;
+    case OP_RFMULT:
+      if (FULL) rfmult_rev(p, l, P, h.b);
+      break;
+    case OP_IDEALWIG:
+      if (FULL) idealwig_rev(p, l, P, h.a, h.b & 0xff, ((h.b >> 8) & 1) != 0);
+      break;
+    case OP_WIGTABLE:
+      if (FULL) wigtable_rev(p, l, P, h.a, h.b & 0xffff, h.b >> 16);
       break;
     default:
       break;

@@ -27,7 +27,7 @@ struct AdjCtx {
   int32_t nparam;
   int32_t npar;              // doubles in `par`
   int32_t lds_bytes;         // > 0: stage headers, knobs and parameters in LDS
-  int32_t full;              // the lattice holds a trigonometric Quad or a linear Bend
+  int32_t full;              // the lattice holds a map only the full sweep carries
   // segment-Jacobian mode
   int32_t nturn;             // turns of the whole forward run
   int32_t nseg_g;            // segments held in `states` (group_n = nseg_g * n)

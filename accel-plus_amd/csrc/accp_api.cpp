@@ -1121,7 +1121,27 @@ static bool adjoint_supports(const Hdr &h) {
   switch (h.op) {
     case OP_NOP: case OP_DRIFT: case OP_QUAD_K0: case OP_MATRIX: case OP_STRMP: case OP_STRMP_THIN:
     case OP_CAVITY: case OP_APERTURE: case OP_CORRECTOR: case OP_BENDMP: case OP_QUAD: case OP_BEND:
+    case OP_BENDMP_RAD: case OP_IDEALWIG: case OP_WIGTABLE:
       return true;
+    case OP_SYNCRAD:  // damping only: the excitation draws noise
+      return (h.a & 2) == 0;
+    case OP_CRABCAV:  // off, or on without phase or amplitude noise
+      return (h.a & 1) == 0 || (h.a & 6) == 0;
+    case OP_RFMULT:  // no phase noise
+      return h.a == 0;
+    default:  // wakes, quantum excitation in BendMPole
+      return false;
+  }
+}
+
+// The reverse maps only the full variant of the sweep carries.
+static bool adjoint_needs_full(const Hdr &h) {
+  switch (h.op) {
+    case OP_BEND: case OP_MATRIX: case OP_BENDMP_RAD: case OP_SYNCRAD: case OP_CRABCAV: case OP_RFMULT:
+    case OP_IDEALWIG: case OP_WIGTABLE:
+      return true;
+    case OP_QUAD:
+      return h.a == 0;
     default:
       return false;
   }
@@ -1165,9 +1185,8 @@ static int adjoint_check(const accp_lattice *L, const std::vector<int32_t> &slot
     const Hdr &h = L->tab.hdr[e];
     if (!adjoint_supports(h))
       return fail(ACCP_E_LATTICE,
-                  "element %d (type %d): no reverse map (radiation, wake, or a type other than "
-                  "Drift, Quad, Bend, BendMPole, StrMPole, Matrix, Cavity, Aperture, Corrector, "
-                  "Monitor)",
+                  "element %d (type %d): no reverse map (quantum excitation, RF noise or a wake: "
+                  "maps that draw random numbers or couple particles)",
                   e, L->elems[e].type);
     if (h.op == OP_MATRIX && !(matrix_pivot_ratio(L->elems[e].mat) > 1e-12))
       return fail(ACCP_E_LATTICE,
@@ -1351,7 +1370,7 @@ extern "C" int accp_adjoint_backward(accp_adjoint *A, const double *dl_dr_out, d
     c.lds_bytes = lds <= LDS_LIMIT ? (int32_t)lds : 0;
     c.full = 0;
     for (const Hdr &hd : L->tab.hdr)
-      if (hd.op == OP_BEND || hd.op == OP_MATRIX || (hd.op == OP_QUAD && hd.a == 0)) c.full = 1;
+      if (adjoint_needs_full(hd)) c.full = 1;
     c.nturn = A->nturn;
     c.nseg_g = g;
     c.r_final = (const double *)A->d_r.p;

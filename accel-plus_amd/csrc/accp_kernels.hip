@@ -28,46 +28,12 @@ __device__ inline double rcp_near1(double x) {
   return fma(r, e, r);
 }
 
-// bndthinkickrad (atpass.cpp:255-339).  crad = CGAMMA*E0^3/(2*pi*1e9);
-// cq = 55*sqrt(3)/72*uc/E0; flag 1 = classical loss, flag 2 = loss plus
-// gauss*sigma (quantum excitation).  The two reciprocals (the velocity norm
-// of B2perp, atpass.cpp:242, and 1/(1+delta) after the loss) take
-// rcp_near1: both arguments are within a few percent of 1.
-template <class PT>
-__device__ inline void bend_kick_rad(Part &p, const PT *B, int mo, double L, double h, double k1c, double k2c,
-                                     double crad, double cq, int flag, double gauss) {
-  const double pn = p.ipd;
-  const double x = p.x, xp = p.px * pn, y = p.y, yp = p.py * pn;
-  double re, im;
-  horner_b(B, mo, x, y, re, im);
-  const double hk = h * (k1c * h - k2c);
-  im += hk * y * y * y * (1.0 / 6.0);
-  re += -k1c * h * y * y * 0.5 + hk * x * y * y * 0.5;
-  const double a = 1.0 + x * h;
-  const double vn2 = rcp_near1(a * a + xp * xp + yp * yp);
-  const double by = re + h, bx = im;
-  const double t1 = by * a, t2 = bx * a, t3 = bx * yp - by * xp;
-  const double b2p = (t1 * t1 + t2 * t2 + t3 * t3) * vn2;
-  const double opd = 1.0 + p.d;
-  const double mean = crad * (opd * opd) * b2p * (a + (xp * xp + yp * yp) * 0.5) * L;
-  double dnew;
-  if (flag == 1) {
-    dnew = p.d - mean;
-  } else {
-    const double sig = sqrt(cq * fabs(mean));
-    dnew = p.d - (mean + gauss * sig);
-  }
-  const double opn = 1.0 + dnew;
-  p.d = dnew;
-  p.ipd = rcp_near1(opn);
-  p.px = xp * opn;
-  p.py = yp * opn;
-  p.px -= L * (-h * dnew + re + h * (h * x + k1c * (x * x - 0.5 * y * y) +
-                                     k2c * (x * x * x - (4.0 / 3.0) * x * y * y)));
-  p.py += L * (im + h * (k1c * x * y + (4.0 / 3.0) * k2c * x * x * y +
-                         (h / 6.0 * k1c - k2c / 3.0) * y * y * y));
-  p.z -= L * h * x;
-}
+// bndthinkickrad (bend_kick_rad, accp_physics.h) with its two reciprocals
+// (the velocity norm of B2perp, atpass.cpp:242, and 1/(1+delta) after the
+// loss) taken by rcp_near1: both arguments are within a few percent of 1.
+struct RcpNear1 {
+  __device__ double operator()(double x) const { return rcp_near1(x); }
+};
 
 // Two standard normals for (seed, particle, turn, element, pair): two 53-bit
 // uniforms in (0,1) from the four Philox words, Box-Muller with both
@@ -123,14 +89,14 @@ __device__ __forceinline__ void bendmp(Part (&p)[PPL], const Hdr h, const PT *P,
 #pragma unroll
       for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) bend_kick_rad(p[k], B, mo, sl, h0, k1c, k2c, crad, cq, 2, g0[k]);
+      for (int k = 0; k < PPL; ++k) bend_kick_rad(p[k], B, mo, sl, h0, k1c, k2c, crad, cq, 2, g0[k], RcpNear1{});
 #pragma unroll
       for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
       if (m + 1 < ns) {
 #pragma unroll
         for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
 #pragma unroll
-        for (int k = 0; k < PPL; ++k) bend_kick_rad(p[k], B, mo, sl, h0, k1c, k2c, crad, cq, 2, g1[k]);
+        for (int k = 0; k < PPL; ++k) bend_kick_rad(p[k], B, mo, sl, h0, k1c, k2c, crad, cq, 2, g1[k], RcpNear1{});
 #pragma unroll
         for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
       }
@@ -145,21 +111,21 @@ __device__ __forceinline__ void bendmp(Part (&p)[PPL], const Hdr h, const PT *P,
       for (int k = 0; k < PPL; ++k) hxdrift(p[k], l1, h0, hs1, hq1);
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
-        if (rad) bend_kick_rad(p[k], B, mo, k1, h0, k1c, k2c, crad, 0.0, 1, 0.0);
+        if (rad) bend_kick_rad(p[k], B, mo, k1, h0, k1c, k2c, crad, 0.0, 1, 0.0, RcpNear1{});
         else bend_kick(p[k], B, mo, plain, k1, h0, k1c, k2c);
       }
 #pragma unroll
       for (int k = 0; k < PPL; ++k) hxdrift(p[k], l2, h0, hs2, hq2);
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
-        if (rad) bend_kick_rad(p[k], B, mo, k2, h0, k1c, k2c, crad, 0.0, 1, 0.0);
+        if (rad) bend_kick_rad(p[k], B, mo, k2, h0, k1c, k2c, crad, 0.0, 1, 0.0, RcpNear1{});
         else bend_kick(p[k], B, mo, plain, k2, h0, k1c, k2c);
       }
 #pragma unroll
       for (int k = 0; k < PPL; ++k) hxdrift(p[k], l2, h0, hs2, hq2);
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
-        if (rad) bend_kick_rad(p[k], B, mo, k1, h0, k1c, k2c, crad, 0.0, 1, 0.0);
+        if (rad) bend_kick_rad(p[k], B, mo, k1, h0, k1c, k2c, crad, 0.0, 1, 0.0, RcpNear1{});
         else bend_kick(p[k], B, mo, plain, k1, h0, k1c, k2c);
       }
 #pragma unroll

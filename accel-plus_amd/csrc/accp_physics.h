@@ -471,6 +471,48 @@ ACCP_HD void matrix_map(Part &p, double L, const PT *m) {
   if (L != 0) drift_nl(p, 0.5 * L * p.ipd);
 }
 
+// ---- bndthinkickrad (atpass.cpp:255-339) -------------------------------------
+// crad = CGAMMA*E0^3/(2*pi*1e9); cq = 55*sqrt(3)/72*uc/E0; flag 1 = classical
+// loss, flag 2 = loss plus gauss*sigma (quantum excitation).  The curvature
+// (Iselin) terms are added to the multipole sum before it enters both the
+// radiated power and the kick.  rcp(v) returns 1/v for the velocity norm
+// (atpass.cpp:242) and for 1/(1+delta) after the loss.
+template <class PT, class Rcp>
+ACCP_HD void bend_kick_rad(Part &p, const PT *B, int mo, double L, double h, double k1c, double k2c,
+                           double crad, double cq, int flag, double gauss, Rcp rcp) {
+  const double pn = p.ipd;
+  const double x = p.x, xp = p.px * pn, y = p.y, yp = p.py * pn;
+  double re, im;
+  horner_b(B, mo, x, y, re, im);
+  const double hk = h * (k1c * h - k2c);
+  im += hk * y * y * y * (1.0 / 6.0);
+  re += -k1c * h * y * y * 0.5 + hk * x * y * y * 0.5;
+  const double a = 1.0 + x * h;
+  const double vn2 = rcp(a * a + xp * xp + yp * yp);
+  const double by = re + h, bx = im;
+  const double t1 = by * a, t2 = bx * a, t3 = bx * yp - by * xp;
+  const double b2p = (t1 * t1 + t2 * t2 + t3 * t3) * vn2;
+  const double opd = 1.0 + p.d;
+  const double mean = crad * (opd * opd) * b2p * (a + (xp * xp + yp * yp) * 0.5) * L;
+  double dnew;
+  if (flag == 1) {
+    dnew = p.d - mean;
+  } else {
+    const double sig = sqrt(cq * fabs(mean));
+    dnew = p.d - (mean + gauss * sig);
+  }
+  const double opn = 1.0 + dnew;
+  p.d = dnew;
+  p.ipd = rcp(opn);
+  p.px = xp * opn;
+  p.py = yp * opn;
+  p.px -= L * (-h * dnew + re + h * (h * x + k1c * (x * x - 0.5 * y * y) +
+                                     k2c * (x * x * x - (4.0 / 3.0) * x * y * y)));
+  p.py += L * (im + h * (k1c * x * y + (4.0 / 3.0) * k2c * x * x * y +
+                         (h / 6.0 * k1c - k2c / 3.0) * y * y * y));
+  p.z -= L * h * x;
+}
+
 // ---- ASyncRad: lumped damping and excitation (accel.cpp:1813-1854) ----------
 // C = [L, 1-ax, 1-ay, 1-az, sigx sx, sigy sy, sigdpp sz]; g = three normals.
 template <class PT>

@@ -665,7 +665,14 @@ void cmd_correct(Line &L, const string &line, int quantity) {
     if (a == string::npos || b == string::npos) throw std::runtime_error("correct: [target1 target2] expected");
     std::istringstream(s.substr(a + 1, b - a - 1)) >> target[0] >> target[1];
   }
-  const int method = up.find("ADJOINT") != string::npos ? ACCP_RESP_ADJOINT : ACCP_RESP_FD;
+  // Responses: by default the adjoint's exact derivatives of the getm66
+  // tune; METHOD=FD asks for the reference's +1e-5 forward difference
+  // (aline.cpp:1187-1242, 1310-1355), METHOD=ADJOINT insists on the adjoint.
+  // Without METHOD, a lattice the reverse sweep cannot carry (noise,
+  // quantum excitation, a wake) falls back to the forward difference.
+  const bool want_fd = up.find("METHOD=FD") != string::npos;
+  const bool want_adjoint = up.find("METHOD=ADJOINT") != string::npos;
+  int method = want_fd ? ACCP_RESP_FD : ACCP_RESP_ADJOINT;
   std::cout << (tune ? "Target tunes: " : "Target chromaticities: ") << fmt(target[0]) << "\t" << fmt(target[1])
             << "\t" << std::endl;
   L.set_rad(ACCP_RAD_OFF);
@@ -684,8 +691,15 @@ void cmd_correct(Line &L, const string &line, int quantity) {
   const int steps = up.find("REPEAT") != string::npos ? 2 : 1;
   for (int k = 0; k < steps; ++k) {
     double t[2] = {target[0], target[1]};
-    check(accp_correct(L.device(), quantity, method, f1.data(), (int)f1.size(), f2.data(), (int)f2.size(), t),
-          "accp_correct");
+    int rc = accp_correct(L.device(), quantity, method, f1.data(), (int)f1.size(), f2.data(), (int)f2.size(), t);
+    if (rc == ACCP_E_LATTICE && method == ACCP_RESP_ADJOINT && !want_adjoint) {
+      std::cout << "note: finite-difference responses (" << accp_last_error() << ")" << std::endl;
+      method = ACCP_RESP_FD;
+      t[0] = target[0];
+      t[1] = target[1];
+      rc = accp_correct(L.device(), quantity, method, f1.data(), (int)f1.size(), f2.data(), (int)f2.size(), t);
+    }
+    check(rc, "accp_correct");
     L.pull();
     std::cout << (tune ? "corrected tunes: " : "corrected chromaticities: ") << fmt(t[0]) << "\t" << fmt(t[1])
               << "\t" << std::endl;

@@ -254,10 +254,13 @@ typedef struct accp_param {
   const int32_t *elems; /* member element indices                             */
 } accp_param;
 
-/* The reverse sweep carries Drift, Quad, Bend, BendMPole without radiation,
- * StrMPole, Matrix, Cavity, Aperture, Corrector and Monitor; a lattice
- * holding any other element (radiation, a wake, CrabCav, RFMult, the
- * wigglers) fails with ACCP_E_LATTICE. */
+/* The reverse sweep carries every map that is deterministic and acts on one
+ * particle: Drift, Quad, Bend, BendMPole (radiation off or classical),
+ * StrMPole, Matrix, Cavity, Aperture, Corrector, Monitor, SyncRad with
+ * damping only, CrabCav and RFMult without noise, IdealWig (with or without
+ * its classical loss) and WigTable.  A lattice holding quantum excitation,
+ * RF noise or a wake fails with ACCP_E_LATTICE, at create and again at
+ * forward and backward (the lattice can change in between). */
 int accp_adjoint_create(accp_lattice *lat, int nparam, const accp_param *params,
                         accp_adjoint **out);
 /* Forward: track r0 (host AoS, n particles) for nturn turns, keeping a

@@ -106,7 +106,7 @@ def test_gradient_matches_oracle_finite_differences(gpu_lib, oracle):
 
 def test_adjoint_rejects_unsupported_lattices(gpu_lib):
     ring = lat.fodo_ring()
-    lat.set_rad(ring, 1)
+    lat.set_rad(ring, 2)  # quantum excitation draws noise
     L = gpu_lib.lattice(ring)
     with pytest.raises(abi.AccpError, match="no reverse map"):
         abi.Adjoint(gpu_lib, L, [])
@@ -348,7 +348,7 @@ def test_adjoint_lattice_checks(gpu_lib):
     forward and backward (the lattice can change in between): a K knob on a
     K = 0 Quad (the reference's drift() does not depend on K and quad6 jumps
     in z at K = 0), a singular Matrix, and radiation switched on after
-    create."""
+    create (quantum excitation, which draws noise)."""
     ring = lat.fodo_ring()
     ring.insert(2, el.quad(0.2, 0.0, "QZ"))
     L = gpu_lib.lattice(ring)
@@ -370,7 +370,7 @@ def test_adjoint_lattice_checks(gpu_lib):
     out = A.forward(beam, 2, ckpt_every=1)
     A.backward(2 * out)
     rad = [e.copy() for e in good]
-    lat.set_rad(rad, 1)
+    lat.set_rad(rad, 2)
     Lg.update(rad)
     with pytest.raises(abi.AccpError, match="no reverse map"):
         A.backward(2 * out)
@@ -449,3 +449,89 @@ def test_multipole_index_up_to_max_npole(gpu_lib, oracle):
             rel, worse, fd = best_fd(grad[k], lambda h: loss_and_seed(track(lat.shift_knob(ring, f, i, mm, h)))[0], steps)
             print(f"{name}: adjoint {grad[k]:+.12e}  fd {fd:+.12e}  rel {rel:.1e} / {worse:.1e}")
             assert rel < 1e-6 and worse < 5e-6, name
+
+
+def radiating_rf_ring():
+    """The FODO ring with classical radiation in its bends and, between its
+    cells, a thin crab cavity, a thick RF multipole, lumped damping, a
+    horizontal ideal wiggler with its classical loss, a vertical one without,
+    a sliced kick table (a third of the wigtable.npz table), and an RF
+    cavity: every deterministic single-particle map whose delta changes
+    inside the element, or whose kick comes from a table."""
+    from tests.golden.make_golden import wigtable_cases
+    ring = lat.fodo_ring()
+    lat.set_rad(ring, 1)
+    xl, yl, xk, yk = el.wigtable_arrays(wigtable_cases()[1][1])
+    ins = [(1, el.crabcav(0.0, 0.5, 0.2, 1.5e9, 0.0, 3000.0, name="CRAB")),
+           (30, el.rfmult(0.1, 5e8, 0.2, [1e-5, 2e-3, 0.5], [0.0, 1e-3, 0.2], energy=3000.0, name="RFM")),
+           (60, el.syncrad(0.2, 3000.0, 0.75, 1.3, 1.0, 5e-9, 5e-11, 8e-4, 10.0, 5.0, rad=el.RAD_LUMP_DAMP)),
+           (90, el.idealwig(1.0, 10, 0.1, 0.1, 0, rad=el.RAD_DAMP, name="WH")),
+           (120, el.idealwig(0.6, 6, 0.08, 0.08, 1, name="WV")),
+           (150, el.wigtable(0.05, 4, xl, yl, 0.3 * xk, 0.3 * yk))]
+    for i, e in reversed(ins):
+        ring.insert(i, e)
+    ring.append(el.cavity(0.0, 1.5, lat.RF_FREQ, 0.0, el.DEFAULT_E0, on=1, name="RF"))
+    return ring
+
+
+def test_radiation_rf_wigglers_and_tables(gpu_lib, oracle):
+    """The reverse maps of the elements that move delta inside them
+    (radiating BendMPole, SyncRad damping, CrabCav and RFMult without noise,
+    IdealWig with its loss) and of WigTable, in one ring: forward equals
+    tracking, the two reverse forms agree, knob gradients and dL/dr0 against
+    central differences of the GPU map and of the oracle (1e-6 at two
+    adjacent steps), and the one-turn Jacobian from six backward passes
+    against the oracle's getm66 (the reference's 12-ray difference).  Noise,
+    quantum excitation and wakes stay errors."""
+    ring = radiating_rf_ring()
+    knobs = lat.adjoint_knobs(ring)
+    assert [k[0] for k in knobs] == ["QF", "QD", "SF", "SD"]
+    beam = lat.gaussian_beam(64, seed=7) * 2.0
+    beam[:, 5] *= 0.05
+    L = gpu_lib.lattice(ring)
+    A = abi.Adjoint(gpu_lib, L, [(f, i, m) for _, f, i, m in knobs])
+    nturn = 8
+    out = A.forward(beam, nturn, ckpt_every=3)
+    assert np.array_equal(out, L.track(beam, nturn), equal_nan=True)
+    assert not np.isnan(out).any()
+    _, seed = loss_and_seed(out)
+    res = {}
+    for mode in (A.REVERSE_SEQUENTIAL, A.REVERSE_SEGMENT_JACOBIAN):
+        A.set_reverse(mode)
+        res[mode] = A.backward(seed)
+    (grad, dr0), (g2, d2) = res[A.REVERSE_SEQUENTIAL], res[A.REVERSE_SEGMENT_JACOBIAN]
+    assert np.all(np.abs(grad - g2) <= 1e-11 * np.abs(grad)), (grad, g2)
+    assert np.all(np.abs(dr0 - d2) <= 1e-11 * np.abs(dr0).max(axis=0))
+    for k, (name, f, i, m) in enumerate(knobs):
+        for track in (lambda r: gpu_lib.lattice(r).track(beam, nturn), lambda r: oracle.track(r, beam, nturn)):
+            rel, worse, fd = best_fd(grad[k], lambda h: loss_and_seed(track(lat.shift_knob(ring, f, i, m, h)))[0],
+                                     fd_steps(ring, f, i, m))
+            print(f"{name}: adjoint {grad[k]:+.12e}  fd {fd:+.12e}  rel {rel:.1e} / {worse:.1e}", flush=True)
+            assert rel < 1e-6 and worse < 5e-6, name
+    for (pi, c) in ((2, 0), (30, 1), (41, 2), (63, 4), (50, 5)):
+        def loss_at(h):
+            b = beam[pi:pi + 1].copy()
+            b[0, c] += h
+            return loss_and_seed(oracle.track(ring, b, nturn))[0]
+        steps = [r * max(abs(beam[pi, c]), 1e-6) for r in (1e-5, 3e-5, 1e-4, 3e-4, 1e-3, 3e-3, 1e-2, 3e-2)]
+        rel, worse, fd = best_fd(dr0[pi, c], loss_at, steps)
+        print(f"dL/dr0[{pi},{c}]: adjoint {dr0[pi, c]:+.12e}  fd {fd:+.12e}  rel {rel:.1e} / {worse:.1e}")
+        assert rel < 1e-6 and worse < 5e-6, (pi, c, dr0[pi, c], fd)
+    A.set_reverse(A.REVERSE_SEQUENTIAL)
+    ref = oracle.getm66(ring)
+    A.forward(np.zeros((1, 6)), 1, ckpt_every=1)
+    J = np.array([A.backward(np.eye(6)[i:i + 1])[1][0] for i in range(6)])
+    err = np.abs(J - ref).max() / np.abs(ref).max()
+    print(f"adjoint Jacobian vs oracle getm66 {err:.2e}")
+    assert err <= 1e-7
+    # what has no reverse map
+    for e in (el.crabcav(0.0, 0.5, 0.2, 1.5e9, 0.0, 3000.0, sigphi=1e-3),
+              el.rfmult(0.0, 5e8, 0.2, [1e-5], [0.0], sigphi=1e-3),
+              el.syncrad(0.0, 3000.0, 0.75, 1.3, 1.0, 5e-9, 5e-11, 8e-4, 10.0, 5.0, rad=el.RAD_LUMP_QE)):
+        with pytest.raises(abi.AccpError, match="no reverse map"):
+            abi.Adjoint(gpu_lib, gpu_lib.lattice(lat.fodo_ring() + [e]), [])
+    qe = lat.fodo_ring()
+    lat.set_rad(qe, 2)
+    with pytest.raises(abi.AccpError, match="no reverse map"):
+        abi.Adjoint(gpu_lib, gpu_lib.lattice(qe), [])
+

@@ -20,11 +20,14 @@ pytestmark = pytest.mark.gpu
 RUN = os.path.join(ROOT, "accel-plus_amd", "accp-run")
 
 
-def run(tmp_path, text):
+def run(tmp_path, text, check=True):
+    """The driver's output lines; with check=False the finished process."""
     (tmp_path / "cmd.txt").write_text(text)
     print("accp-run", tmp_path / "cmd.txt", flush=True)
     p = subprocess.run([RUN, "--precision", "17", "cmd.txt"], cwd=tmp_path, capture_output=True, text=True,
                        timeout=300)
+    if not check:
+        return p
     assert p.returncode == 0, p.stdout[-2000:] + p.stderr
     return p.stdout.splitlines()
 
@@ -114,18 +117,45 @@ def test_correct_tune_and_chrom(tmp_path):
     el.write_lattice(lat.fodo_ring(), str(tmp_path / "fodo.lat"))
     out = run(tmp_path, """Lattice
   file_name = fodo.lat
-  correct_tune: type=Quad, famname=(QF QD) [0.28, 0.24]
-  correct_chrom: type=StrMPole, famname=(SF SD) [2.0, 1.0] REPEAT method=adjoint
+  correct_tune: type=Quad, famname=(QF QD) [0.28, 0.24] method=fd
+  correct_chrom: type=StrMPole, famname=(SF SD) [2.0, 1.0] REPEAT
   correct_tune: type=Quad, famname=(QF QD) [0.27, 0.25] REPEAT method=adjoint
 End
 """)
     tunes = [numbers(s) for s in out if s.startswith("corrected tunes:")]
     chroms = [numbers(s) for s in out if s.startswith("corrected chromaticities:")]
-    # the first step is the reference's own (finite-difference responses)
+    # method=fd is the reference's own step (finite-difference responses)
     assert np.allclose(tunes[0], g["corrected_tune"], rtol=0, atol=1e-9)
-    # adjoint responses: two Newton steps reach the targets
+    # adjoint responses (the default): two Newton steps reach the targets
     assert np.allclose(chroms[-1], [2.0, 1.0], rtol=0, atol=1e-5)
     assert np.allclose(tunes[-1], [0.27, 0.25], rtol=0, atol=1e-8)
+    assert not any(s.startswith("note:") for s in out)
+
+
+def test_correction_falls_back_to_finite_differences(tmp_path):
+    """A crab cavity switched on with phase noise has no reverse map:
+    without METHOD the correction notes it and takes the reference's
+    forward difference, with METHOD=ADJOINT it is an error.  The cavity
+    has no voltage, so the FODO ring's reference step still applies."""
+    g = golden("optics.npz")
+    ring = lat.fodo_ring() + [el.crabcav(0.0, 0.0, 0.0, 1.5e9, 0.0, 3000.0, sigphi=1e-3)]
+    el.write_lattice(ring, str(tmp_path / "crab.lat"))
+    out = run(tmp_path, """Lattice
+  file_name = crab.lat
+  set_flags: crabcav=on
+  correct_tune: type=Quad, famname=(QF QD) [0.28, 0.24]
+End
+""")
+    assert any(s.startswith("note: finite-difference responses") for s in out)
+    tunes = [numbers(s) for s in out if s.startswith("corrected tunes:")]
+    assert np.allclose(tunes[0], g["corrected_tune"], rtol=0, atol=1e-9)
+    p = run(tmp_path, """Lattice
+  file_name = crab.lat
+  set_flags: crabcav=on
+  correct_tune: type=Quad, famname=(QF QD) [0.28, 0.24] method=adjoint
+End
+""", check=False)
+    assert p.returncode != 0 and "no reverse map" in p.stderr
 
 
 def test_optics_and_momentum_aperture_commands(tmp_path, oracle):
