@@ -160,33 +160,38 @@ __global__ __launch_bounds__(NB) void k_naff(const double *__restrict__ sig, con
   const double lo = (imx - 2) * 1.0 / m / 2.0, hi = (imx + 2) * 1.0 / m / 2.0;
   xlow = xlow > lo ? xlow : lo;
   xhi = xhi < hi ? xhi : hi;
-  const double R = 0.61803399, C = 1.0 - R, tol = 3.0e-8;
-  const double ax = xlow, cx = xhi, bx = (xlow + xhi) / 2.0;
-  double x0 = ax, x3 = cx, x1, x2;
-  if (fabs(cx - bx) > fabs(bx - ax)) {
-    x1 = bx;
-    x2 = bx + C * (cx - bx);
-  } else {
-    x2 = bx;
-    x1 = bx - C * (bx - ax);
-  }
-  double f1 = projection(lds, m, x1, red), f2 = projection(lds, m, x2, red);
-  for (int it = 0; it < GOLDEN_MAX_ITER && fabs(x3 - x0) > tol * (fabs(x1) + fabs(x2)); ++it) {
-    if (f2 < f1) {
-      x0 = x1;
-      x1 = x2;
-      x2 = R * x2 + C * x3;
-      f1 = f2;
-      f2 = projection(lds, m, x2, red);
-    } else {
-      x3 = x2;
-      x2 = x1;
-      x1 = R * x1 + C * x0;
-      f2 = f1;
-      f1 = projection(lds, m, x1, red);
+  // Golden-section search for the minimum of -|projection| on [xlow, xhi]:
+  // the bracket [a, d] holds two probes b < c; each step keeps the side of
+  // the better probe, moves the bracket's far end onto the worse probe and
+  // places one new probe a golden fraction into the longer part, until the
+  // bracket is narrower than 3e-8 of |b| + |c|.  The ratio (0.61803399),
+  // the first probes (the midpoint and a point into the longer half) and
+  // the stopping rule are the reference's (Golden::minimize, mins.h:75-108,
+  // called from naff, utility.cpp:74), so the tunes agree with its to the
+  // stopping tolerance.
+  const double g = 0.61803399, gc = 1.0 - g, tol = 3.0e-8;
+  const double mid = (xlow + xhi) / 2.0;
+  double a = xlow, d = xhi;
+  const bool right_longer = fabs(d - mid) > fabs(mid - a);
+  double b = right_longer ? mid : mid - gc * (mid - a);
+  double c = right_longer ? mid + gc * (d - mid) : mid;
+  double fb = projection(lds, m, b, red), fc = projection(lds, m, c, red);
+  for (int it = 0; it < GOLDEN_MAX_ITER && fabs(d - a) > tol * (fabs(b) + fabs(c)); ++it) {
+    if (fc < fb) {  // the minimum lies in [b, d]
+      a = b;
+      b = c;
+      fb = fc;
+      c = g * c + gc * d;
+      fc = projection(lds, m, c, red);
+    } else {  // the minimum lies in [a, c]
+      d = c;
+      c = b;
+      fc = fb;
+      b = g * b + gc * a;
+      fb = projection(lds, m, b, red);
     }
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = f1 < f2 ? x1 : x2;
+  if (threadIdx.x == 0) out[blockIdx.x] = fb < fc ? b : c;
 }
 
 }  // namespace
