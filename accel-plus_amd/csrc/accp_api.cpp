@@ -19,6 +19,7 @@
 #include "accp_error.h"
 #include "accp_internal.h"
 #include "accp_kernels.h"
+#include "accp_spec.h"
 #include "accp_table.h"
 #include "accp_wake.h"
 
@@ -472,6 +473,9 @@ struct TableRef {
   int nseed, nelem;
   size_t lds_need;
   bool full;  // kernel variant with every element map
+  // host copy of the headers every seed shares (offsets included), when the
+  // table can be specialized (ACCP_STAGE_SPECIALIZED), else null
+  const std::vector<Hdr> *shared_hdr = nullptr;
 };
 
 struct RunSpec {
@@ -486,9 +490,11 @@ int resolve_tuning(const accp_tuning *t, size_t lds_need, int &block, int &ppl, 
   ppl = (t && t->ppl) ? t->ppl : 1;
   const int st = t ? t->staging : ACCP_STAGE_AUTO;
   chunk_max = (t && t->compact) ? t->compact : 256;
+  if (st < ACCP_STAGE_AUTO || st > ACCP_STAGE_SPECIALIZED) return fail(ACCP_E_ARG, "unknown staging %d", st);
   if (st == ACCP_STAGE_LDS && lds_need > LDS_LIMIT)
     return fail(ACCP_E_ARG, "element table (%zu B) exceeds the LDS staging limit", lds_need);
-  lds = (st == ACCP_STAGE_LDS) || (st == ACCP_STAGE_AUTO && lds_need <= LDS_LIMIT);
+  // the specialized kernel keeps its constants in LDS when they fit
+  lds = (st == ACCP_STAGE_LDS) || ((st == ACCP_STAGE_AUTO || st == ACCP_STAGE_SPECIALIZED) && lds_need <= LDS_LIMIT);
   // default shape: with the table in LDS, 256-lane workgroups share one copy
   // among four waves (a 64-lane workgroup holding tens of KB of LDS would
   // cap a CU below the waves its registers allow)
@@ -538,6 +544,13 @@ int enqueue_run(const TableRef &T, const WsView &ws, int max_tiles, double *d_r,
   bool lds;
   int rc = resolve_tuning(tuning, T.lds_need, block, ppl, lds, chunk_max);
   if (rc) return rc;
+  const SpecKernel *spec = nullptr;
+  if (tuning && tuning->staging == ACCP_STAGE_SPECIALIZED) {
+    if (!T.shared_hdr || T.full || R.st != 0 || R.ed != T.nelem)
+      return fail(ACCP_E_ARG, "ACCP_STAGE_SPECIALIZED runs whole turns of lean lattices whose seeds share one "
+                              "grouped table");
+    if ((rc = spec_kernel(*T.shared_hdr, block, ppl, lds, &spec))) return rc;
+  }
   const int per = block * ppl;
   const int64_t *d_offs = ws.offs;
   int cur = 0;
@@ -572,7 +585,8 @@ int enqueue_run(const TableRef &T, const WsView &ws, int max_tiles, double *d_r,
     c.turn0 = R.turn0 + (uint64_t)t;
     c.turn_rel0 = t;
     c.nturn = cn;
-    HIPCHK(launch_track(c, max_tiles, block, ppl, lds, lds ? T.lds_need : 0, T.full, s));
+    if (spec) HIPCHK(launch_spec(spec, c, max_tiles, lds ? T.lds_need : 0, s));
+    else HIPCHK(launch_track(c, max_tiles, block, ppl, lds, lds ? T.lds_need : 0, T.full, s));
     t += cn;
     if (t < R.nturn) {
       HIPCHK(launch_compact(ws.tiles, ws.ntiles, max_tiles, ws.idx[cur], d_r, d_offs, ws.cnt[1 - cur],
@@ -608,7 +622,7 @@ struct accp_lattice {
       st = tab.fmap[st];
       ed = tab.fmap[ed];
       return TableRef{(const Hdr *)d_fhdr.p, (const double *)d_fpar.p, (const int32_t *)d_fmeta.p, 1,
-                      (int)tab.fhdr.size(), lds_bytes_for(tab.fhdr.size(), tab.fpar.size()), tab.full};
+                      (int)tab.fhdr.size(), lds_bytes_for(tab.fhdr.size(), tab.fpar.size()), tab.full, &tab.fhdr};
     }
     return table();
   }
@@ -814,6 +828,9 @@ int accp::track_device_fixed(accp_lattice *L, double *d_r, int n, int nturn, int
     return fail(ACCP_E_ARG, "accp_track_device_noalloc: no monitor records (they need a host buffer)");
   if (n > fw.max_n)
     return fail(ACCP_E_ARG, "accp_track_device_noalloc: %d particles, at most %lld", n, (long long)fw.max_n);
+  if (opts && opts->tuning.staging == ACCP_STAGE_SPECIALIZED)
+    return fail(ACCP_E_ARG, "accp_track_device_noalloc: ACCP_STAGE_SPECIALIZED loads a code object, which allocates "
+                            "device memory");
   for (int w : L->tab.wakes)
     if (w >= st && w < ed && L->elems[w].type == ACCP_TRANSWAKE)
       return fail(ACCP_E_LATTICE, "accp_track_device_noalloc: element %d is a TransWake, whose sort needs "
@@ -915,6 +932,7 @@ struct accp_batch {
   int nseed = 0, nelem = 0;
   size_t max_lds = 0;
   bool full = false;
+  std::vector<Hdr> shared_hdr;  // the grouped headers, when every seed has the same ones
   DevBuf d_hdr, d_par, d_meta, d_scratch;
   WorkSet ws;
   hipGraph_t graph = nullptr;
@@ -922,7 +940,7 @@ struct accp_batch {
   hipStream_t cap_stream = nullptr;
   TableRef table() const {
     return TableRef{(const Hdr *)d_hdr.p, (const double *)d_par.p, (const int32_t *)d_meta.p, nseed, nelem,
-                    max_lds, full};
+                    max_lds, full, shared_hdr.empty() ? nullptr : &shared_hdr};
   }
 };
 
@@ -960,6 +978,17 @@ extern "C" int accp_batch_create(accp_lattice *const *seeds, int nseed, accp_bat
     par.insert(par.end(), cp.begin(), cp.end());
     b->max_lds = std::max(b->max_lds, lds_bytes_for(ch.size(), cp.size()));
     b->full = b->full || c.full;
+  }
+  // one specialized kernel serves every seed when their grouped headers
+  // (constant offsets included) are the same
+  if (grouped) {
+    bool same = true;
+    for (int s = 1; s < nseed && same; ++s) {
+      const std::vector<Hdr> &a = seeds[0]->tab.fhdr, &c = seeds[s]->tab.fhdr;
+      for (size_t k = 0; k < a.size() && same; ++k)
+        same = a[k].op == c[k].op && a[k].off == c[k].off && a[k].a == c[k].a && a[k].b == c[k].b;
+    }
+    if (same) b->shared_hdr = seeds[0]->tab.fhdr;
   }
   std::vector<int32_t> meta(h0);
   meta.insert(meta.end(), p0.begin(), p0.end());
@@ -1017,8 +1046,16 @@ extern "C" int accp_batch_capture(accp_batch *b, double *d_r, const int64_t *off
   int rc = resolve_tuning(tuning, b->max_lds, block, ppl, lds, chunk_max);
   if (rc) return rc;
   int max_tiles = 0;
-  // offsets and buffers go to the device before the capture starts
+  // offsets and buffers go to the device before the capture starts, and so
+  // does the specialized kernel's code object
   if ((rc = prepare_workset(b->ws, offsets, b->nseed, block * ppl, max_tiles))) return rc;
+  if (tuning && tuning->staging == ACCP_STAGE_SPECIALIZED) {
+    const SpecKernel *k = nullptr;
+    if (b->shared_hdr.empty() || b->full)
+      return fail(ACCP_E_ARG, "ACCP_STAGE_SPECIALIZED runs whole turns of lean lattices whose seeds share one "
+                              "grouped table");
+    if ((rc = spec_kernel(b->shared_hdr, block, ppl, lds, &k))) return rc;
+  }
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
@@ -1057,11 +1094,15 @@ extern "C" int accp_autotune(accp_batch *b, double *d_r, const int64_t *offsets,
   double bestv = 1e300;
   accp_tuning bt{64, 1, ACCP_STAGE_SCALAR, 0};
   const int blocks[3] = {64, 128, 256}, ppls[2] = {1, 2};
-  const int stages[2] = {ACCP_STAGE_SCALAR, ACCP_STAGE_LDS};
+  const int stages[3] = {ACCP_STAGE_SCALAR, ACCP_STAGE_LDS, ACCP_STAGE_SPECIALIZED};
   for (int bl : blocks)
     for (int pp : ppls)
       for (int sg : stages) {
         if (sg == ACCP_STAGE_LDS && b->max_lds > LDS_LIMIT) continue;
+        // specialized kernels (one compilation each) at one particle per lane
+        if (sg == ACCP_STAGE_SPECIALIZED &&
+            (pp != 1 || b->shared_hdr.empty() || b->full || !spec_unsupported(b->shared_hdr).empty()))
+          continue;
         accp_tuning t{bl, pp, sg, 0};
         // untimed launch (code object load, buffers), then the timed probe
         HIPCHK(hipMemcpy(b->d_scratch.p, d_r, bytes, hipMemcpyDeviceToDevice));

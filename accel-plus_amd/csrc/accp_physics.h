@@ -9,13 +9,17 @@
 // per-coordinate tolerance of the parity tests (1e-12 relative after one
 // turn) covers these ulp-level effects.
 #pragma once
+#if !defined(__HIPCC_RTC__)  // runtime compilation provides the device math itself
 #include <math.h>
+#endif
 #include <stdint.h>
 
 #include "accp_table.h"
 
 #if defined(__HIPCC__)
+#if !defined(__HIPCC_RTC__)
 #include <hip/hip_runtime.h>
+#endif
 #define ACCP_HD __host__ __device__ __forceinline__
 #else
 #define ACCP_HD inline

@@ -1,0 +1,51 @@
+// accp_spec.h -- the lattice-specialized tracker: the persistent tracker of
+// accp_track.h compiled at run time (hiprtc) for one lattice structure, with
+// the turn's group sequence written out as straight-line code instead of a
+// walk over the element table.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "accp_ctx.h"
+#include "accp_table.h"
+
+namespace accp {
+
+// Groups [s, s + p*r) of a table are r copies of the p groups at s: equal
+// op, a and b, and constants `stride` doubles after the previous copy's.
+struct Period {
+  int s = 0, p = 0, r = 0, stride = 0;
+};
+
+// The repeated block covering the most groups (r >= 2, p <= max_p; the
+// shortest period among equals), or r = 0 when there is none.
+Period find_period(const std::vector<Hdr> &h, int max_p);
+
+// Why the grouped table h cannot be specialized, or "" when it can (a lean
+// table whose code stays within a bounded number of straight-line groups).
+std::string spec_unsupported(const std::vector<Hdr> &h);
+
+// HIP source of the specialized kernel `accp_spec_track` for table h, launch
+// shape (block, ppl) and staging (lds): the turn walks all groups of h.
+std::string spec_source(const std::vector<Hdr> &h, int block, int ppl, bool lds);
+
+// Compiles source (from spec_source) for offload target arch ("gfx950",
+// or a device's gcnArchName) into a code object: ACCP_OK, or an error code
+// with the compiler's log.  No device needed.
+int spec_compile(const std::string &src, const std::string &arch, std::vector<char> &code, std::string &log);
+
+struct SpecKernel;
+
+// The compiled kernel for (h, block, ppl, lds) on the current device,
+// compiled on first use and kept for the life of the process.  Call it
+// outside any stream capture (module loading is not a stream operation).
+int spec_kernel(const std::vector<Hdr> &h, int block, int ppl, bool lds, const SpecKernel **out);
+
+// Milliseconds the last compilation of this process took (0: none yet).
+double spec_last_compile_ms();
+
+hipError_t launch_spec(const SpecKernel *k, const KernelCtx &ctx, int grid, size_t lds_bytes, hipStream_t s);
+
+}  // namespace accp

@@ -14,7 +14,8 @@ from . import elements as el
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(HERE), "libaccp.so")
 
-STAGE_AUTO, STAGE_SCALAR, STAGE_LDS = 0, 1, 2
+STAGE_AUTO, STAGE_SCALAR, STAGE_LDS, STAGE_SPECIALIZED = 0, 1, 2, 3
+STAGE_NAMES = {0: "auto", 1: "scalar", 2: "lds", 3: "specialized"}
 STAT_LEN = 20
 
 
@@ -26,12 +27,12 @@ class Tuning(ctypes.Structure):
     _fields_ = [("block", c_int32), ("ppl", c_int32), ("staging", c_int32), ("compact", c_int32)]
 
     def __repr__(self):
-        st = {0: "auto", 1: "scalar", 2: "lds"}.get(self.staging, self.staging)
+        st = STAGE_NAMES.get(self.staging, self.staging)
         return f"Tuning(block={self.block}, ppl={self.ppl}, staging={st}, compact={self.compact})"
 
     def as_dict(self):
         return {"block": self.block, "ppl": self.ppl,
-                "staging": {0: "auto", 1: "scalar", 2: "lds"}.get(self.staging, self.staging),
+                "staging": STAGE_NAMES.get(self.staging, self.staging),
                 "compact": self.compact}
 
 

@@ -46,6 +46,12 @@ EXECUTED_FP64_FLOPS_PER_PARTICLE_TURN = 74968.0
 EXECUTED_FP64_FLOPS_PER_STEP = {"block=64,ppl=1,lds": 4.935e11}
 
 
+def shape_name(t):
+    """Key of a launch shape in the profile tables."""
+    import pyaccp.abi as abi
+    return f"block={t.block},ppl={t.ppl},{abi.STAGE_NAMES.get(t.staging, t.staging)}"
+
+
 def chunk_launches(nturn, chunk_max=256):
     """Tracking launches of one call: chunks of 1, 2, 4, ... turns up to
     chunk_max, compaction in between."""
@@ -189,7 +195,7 @@ def config_legs(lib, stream):
     return res
 
 
-def saturated_leg(lib, glat, elems, tuning, stream):
+def saturated_leg(lib, glat, elems, tuning, stream, counters=True):
     """The same kernel at full occupancy: a Gaussian core beam that survives
     (2^18 particles on the seed-0 lattice, 16 turns), in 256-lane workgroups
     that share one LDS copy of the table per four waves (the batched leg's
@@ -201,7 +207,7 @@ def saturated_leg(lib, glat, elems, tuning, stream):
     import pyaccp.lattices as lat
     from pyaccp.hipmem import DeviceArray, Event
 
-    log("saturated leg")
+    log(f"saturated leg ({shape_name(tuning)})")
     nsat, tsat = 1 << 18, 16
     sat_beam = lat.gaussian_beam(nsat, seed=11)
     d_sat0 = DeviceArray(sat_beam)
@@ -225,8 +231,9 @@ def saturated_leg(lib, glat, elems, tuning, stream):
         "achieved_tflops": round(sat_fl / (sat_best * 1e-3) / 1e12, 3),
         "frac_of_fp64_peak": round(sat_fl / (sat_best * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
         "achieved_counts": "algorithmic flops (the reference's formulas, useful terms) / kernel time",
+        "tuning": tuning.as_dict(),
     }
-    if EXECUTED_FP64_FLOPS_PER_PARTICLE_TURN:
+    if counters and EXECUTED_FP64_FLOPS_PER_PARTICLE_TURN:
         ex = nsat * tsat * EXECUTED_FP64_FLOPS_PER_PARTICLE_TURN / (sat_best * 1e-3) / 1e12
         res["executed_tflops"] = round(ex, 3)
         res["executed_frac_of_fp64_peak"] = round(ex / FP64_PEAK_TFLOPS, 4)
@@ -363,22 +370,26 @@ def tail_leg(lib, stream, n=48, nturn=256):
               ("thin error multipoles", [e for e in da if e.type == el.STRMPOLE and e.len == 0]),
               ("quads", [e for e in da if e.type == el.QUAD]),
               ("drifts", [e for e in da if e.type == el.DRIFT]))
-    tuning = abi.Tuning(64, 1, 2, -1)
     res = {}
-    for name, elems in groups:
-        L = lib.lattice(elems)
-        d = DeviceArray(beam)
-        L.track_device(d.ptr, n, 4, stream=stream.handle, tuning=tuning)
-        d.upload(beam)
-        e0, e1 = Event(), Event()
-        e0.record(stream.handle)
-        L.track_device(d.ptr, n, nturn, stream=stream.handle, tuning=tuning)
-        e1.record(stream.handle)
-        us = e0.elapsed_ms(e1) / nturn * 1e3
-        res[name] = {"elements": len(elems), "us_per_turn": round(us, 2),
-                     "ns_per_element": round(us / len(elems) * 1e3, 1),
-                     "lost": int(np.isnan(d.numpy()[:, 0]).sum())}
-    return {"workload": f"one wave of {n} particles (64-lane workgroup, LDS table, no compaction), {nturn} turns",
+    for kernel, tuning in (("table walker", abi.Tuning(64, 1, abi.STAGE_LDS, -1)),
+                           ("specialized", abi.Tuning(64, 1, abi.STAGE_SPECIALIZED, -1))):
+        res[kernel] = {}
+        for name, elems in groups:
+            L = lib.lattice(elems)
+            d = DeviceArray(beam)
+            # untimed: compiles the specialized kernel of this structure
+            L.track_device(d.ptr, n, 4, stream=stream.handle, tuning=tuning)
+            d.upload(beam)
+            e0, e1 = Event(), Event()
+            e0.record(stream.handle)
+            L.track_device(d.ptr, n, nturn, stream=stream.handle, tuning=tuning)
+            e1.record(stream.handle)
+            us = e0.elapsed_ms(e1) / nturn * 1e3
+            res[kernel][name] = {"elements": len(elems), "us_per_turn": round(us, 2),
+                                 "ns_per_element": round(us / len(elems) * 1e3, 1),
+                                 "lost": int(np.isnan(d.numpy()[:, 0]).sum())}
+    return {"workload": f"one wave of {n} particles (64-lane workgroup, constants in LDS, no compaction), "
+                        f"{nturn} turns; the table-walking kernel and the kernel specialized for each lattice",
             "per_class": res}
 
 
@@ -595,7 +606,9 @@ def side_legs(lib, stream, legs, lat0, glat0, args, result):
     import pyaccp.abi as abi
 
     if "saturated" in legs:
-        result["saturated"] = saturated_leg(lib, glat0, lat0, abi.Tuning(256, 1, 2, 0), stream)
+        result["saturated"] = saturated_leg(lib, glat0, lat0, abi.Tuning(256, 1, abi.STAGE_LDS, 0), stream)
+        result["saturated"]["specialized"] = saturated_leg(
+            lib, glat0, lat0, abi.Tuning(256, 1, abi.STAGE_SPECIALIZED, 0), stream, counters=False)
     if "tail" in legs:
         result["tail"] = tail_leg(lib, stream)
     if "configs" in legs:
@@ -618,7 +631,8 @@ def main():
     ap.add_argument("--nturn", type=int, default=1024)
     ap.add_argument("--no-plain", action="store_true", help="skip the per-seed plain comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--tune", default="", help="block,ppl,staging(scalar|lds) instead of autotuning")
+    ap.add_argument("--tune", default="",
+                    help="block,ppl,staging (scalar|lds|specialized) instead of autotuning")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
     ap.add_argument("--eq-turns", type=int, default=20000,
                     help="turns of the collective leg's equilibrium run (0: skip it)")
@@ -678,12 +692,14 @@ def main():
 
     if args.tune:
         bl, pp, sg = args.tune.split(",")
-        tuning = abi.Tuning(int(bl), int(pp), {"scalar": 1, "lds": 2}[sg], 0)
+        tuning = abi.Tuning(int(bl), int(pp), {"scalar": abi.STAGE_SCALAR, "lds": abi.STAGE_LDS,
+                                               "specialized": abi.STAGE_SPECIALIZED}[sg], 0)
     else:
         log("autotuning")
         # each shape runs the whole workload once: the sparse tail, not the
         # dense first turns, sets the step time, and a short probe sees only
-        # the latter (12 shapes x 2 runs, a few seconds)
+        # the latter (15 shapes x 2 runs, a few seconds, plus the
+        # specialized kernels' compilations)
         tuning, probe_ms = B.autotune(d_r.ptr, offs, nturn)
         log(f"autotune: {tuning} ({probe_ms:.2f} ms for {nturn} turns)")
 
@@ -759,9 +775,7 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-            "traffic": (TRAFFIC_BYTES_PER_LAUNCH.get(f"block={tuning.block},ppl={tuning.ppl},"
-                                                     f"{'lds' if tuning.staging == 2 else 'scalar'}")
-                        if default_workload else None),
+            "traffic": TRAFFIC_BYTES_PER_LAUNCH.get(shape_name(tuning)) if default_workload else None,
             "traffic_unit": "HBM bytes per k_track launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "traffic_source": TRAFFIC_SOURCE,
             "launches_per_step": launches,
@@ -772,7 +786,7 @@ def main():
             "kernel_ms": round(kernel_ms, 3),
         },
     }
-    shape = f"block={tuning.block},ppl={tuning.ppl},{'lds' if tuning.staging == 2 else 'scalar'}"
+    shape = shape_name(tuning)
     if default_workload and shape in EXECUTED_FP64_FLOPS_PER_STEP:
         ex = EXECUTED_FP64_FLOPS_PER_STEP[shape] / (kernel_ms * 1e-3) / 1e12
         result["roofline"]["executed_tflops"] = round(ex, 4)
@@ -784,13 +798,16 @@ def main():
     side_legs(lib, stream, legs, lats[0], glats[0], args, result)
 
     if "plain" in legs:
+        # the same kernel and launch shape, called once per seed (no batching,
+        # no graph); the first call of a specialized shape compiles nothing
+        # here, the batched run already did
         log("plain per-seed GPU path")
         d_p = DeviceArray(beam)
         synchronize()
         t0 = time.perf_counter()
         for s in range(len(lats)):
             cnt = int(offs[s + 1] - offs[s])
-            glats[s].track_device(d_p.ptr + int(offs[s]) * 48, cnt, nturn, stream=stream.handle)
+            glats[s].track_device(d_p.ptr + int(offs[s]) * 48, cnt, nturn, tuning=tuning, stream=stream.handle)
         synchronize()
         plain_s = time.perf_counter() - t0
         pout = d_p.numpy()
@@ -800,6 +817,7 @@ def main():
             "ms": round(plain_s * 1e3, 3),
             "batched_speedup": round(plain_s * 1e3 / ms_per_step, 3),
             "identical_survival_maps": same,
+            "tuning": tuning.as_dict(),
         }
 
     if "cpu" in legs:

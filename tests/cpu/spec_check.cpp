@@ -1,0 +1,130 @@
+// CPU check of the lattice-specialized tracker's generator
+// (accel-plus_amd/csrc/accp_spec.cpp, linked from libaccp.so): the repeated
+// cell it finds in a grouped element table, the tables it refuses, and that
+// the kernel source it writes for a table holding every group form of the
+// lean tracker compiles for gfx950 (hiprtc; no device needed).  Prints one
+// line per check and exits non-zero on a failure.
+#include <cstdio>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "accp_spec.h"
+#include "accp_table.h"
+
+using namespace accp;
+
+static int fails = 0;
+static void check(bool ok, const char *what) {
+  std::printf("%s: %s\n", ok ? "ok" : "FAILED", what);
+  if (!ok) ++fails;
+}
+
+// A table of `ncell` cells of the group shapes in `cell`, between a prefix
+// and a suffix; constants laid out back to back with `len` doubles a group.
+static std::vector<Hdr> ring(const std::vector<Hdr> &pre, const std::vector<Hdr> &cell, int ncell,
+                             const std::vector<Hdr> &post, int len) {
+  std::vector<Hdr> h;
+  int off = 0;
+  auto put = [&](Hdr g) {
+    g.off = off;
+    off += len;
+    h.push_back(g);
+  };
+  for (const Hdr &g : pre) put(g);
+  for (int c = 0; c < ncell; ++c)
+    for (const Hdr &g : cell) put(g);
+  for (const Hdr &g : post) put(g);
+  return h;
+}
+
+int main() {
+  // a FODO-like cell: focusing and defocusing quads (8 and 10 series terms),
+  // two sextupole families, two bends with their error kicks
+  const Hdr qf{OP_QUAD | FUSE_PRE, 0, 8, 0}, qd{OP_QUAD | FUSE_PRE, 0, 10, 0};
+  const Hdr sf{OP_STRMP | FUSE_PRE, 0, 10, 2 | STRMP_SINGLE}, sd{OP_STRMP | FUSE_PRE, 0, 10, 3 | STRMP_SINGLE};
+  const Hdr bend{OP_BENDMP | FUSE_PRE | FUSE_POST | (30 << FUSE_POST_DELTA_SHIFT) | (9 << FUSE_POST_HI_SHIFT), 0, 10,
+                 1 << 8};
+  const Hdr cav{OP_CAVITY, 0, 1, 0}, ap{OP_APERTURE, 0, 0, 0};
+  const std::vector<Hdr> cell = {qf, sf, bend, qd, sd, bend};
+
+  // 20 cells between a cavity and an aperture: the cells are found
+  {
+    const std::vector<Hdr> h = ring({cav}, cell, 20, {ap}, 7);
+    const Period p = find_period(h, 64);
+    std::printf("  period: s=%d p=%d r=%d stride=%d\n", p.s, p.p, p.r, p.stride);
+    check(p.s == 1 && p.p == 6 && p.r == 20 && p.stride == 42, "20 identical cells after a prefix group");
+    check(spec_unsupported(h).empty(), "a lean table of cells is specialized");
+  }
+  // a cell that differs (one quad with more series terms) splits the ring:
+  // the longest run of identical cells is taken (here it starts one group
+  // into cell 7, a rotated cell)
+  {
+    std::vector<Hdr> h = ring({}, cell, 20, {}, 5);
+    h[6 * 7].a = 12;  // cell 7's first group
+    const Period p = find_period(h, 64);
+    std::printf("  period: s=%d p=%d r=%d\n", p.s, p.p, p.r);
+    check(p.p == 6 && p.r == 12 && p.s > 6 * 7 && p.s + p.p * p.r <= 120, "12 cells after the differing group");
+  }
+  // constants not evenly spaced: no repetition
+  {
+    std::vector<Hdr> h = ring({}, cell, 4, {}, 5);
+    h[13].off += 1;
+    for (size_t i = 14; i < h.size(); ++i) h[i].off += 1;
+    const Period p = find_period(h, 64);
+    check(p.r == 0 || p.p * p.r <= 13, "no repetition across a break in the constant stride");
+  }
+  // refusals
+  {
+    std::vector<Hdr> h = ring({}, cell, 3, {}, 5);
+    h[2].op = OP_BENDMP_RAD;
+    check(!spec_unsupported(h).empty(), "a radiating bend needs the full variant: refused");
+    std::vector<Hdr> big;  // series terms drawn at random: no repeating pattern
+    std::mt19937 gen(7);
+    for (int i = 0; i < 300; ++i) big.push_back(Hdr{OP_QUAD, i * 40, 4 + 2 * (int)(gen() % 7), 0});
+    check(!spec_unsupported(big).empty(), "300 groups without a repeated cell: refused");
+  }
+  // every group form of the lean tracker, in a table of two cells, compiles
+  {
+    const std::vector<Hdr> forms = {
+        {OP_NOP, 0, 0, 0},
+        {OP_DRIFT, 0, 0, 0},
+        {OP_QUAD, 0, 4, 0},
+        {OP_QUAD | FUSE_PRE, 0, 8, 0},
+        {OP_QUAD, 0, 16, 0},
+        {OP_QUAD_K0 | FUSE_PRE, 0, 0, 0},
+        {OP_BENDMP | FUSE_PRE, 0, 10, 1 << 8},
+        {OP_BENDMP, 0, 10, (1 << 8) | BM_EDGE_IN | BM_EDGE_OUT},
+        {OP_BENDMP, 0, 7, 1 << 8},
+        {OP_BENDMP, 0, 10, 1},
+        {OP_BENDMP, 0, 10, 3 | (1 << 8)},
+        {OP_STRMP | FUSE_PRE, 0, 10, 2 | STRMP_SINGLE},
+        {OP_STRMP, 0, 4, 1 | STRMP_SINGLE},
+        {OP_STRMP, 0, 10, 5},
+        {OP_STRMP, 0, 10, -1},
+        {OP_STRMP_THIN, 0, 0, 9},
+        {OP_STRMP_THIN, 0, 0, 3 | STRMP_SINGLE},
+        {OP_DRIFT | FUSE_POST | (1 << FUSE_POST_DELTA_SHIFT) | (9 << FUSE_POST_HI_SHIFT), 0, 0, 0},
+        {OP_QUAD | FUSE_POST | (28 << FUSE_POST_DELTA_SHIFT) | (2 << FUSE_POST_HI_SHIFT) | FUSE_POST_SINGLE, 0, 8, 0},
+        {OP_CAVITY, 0, 1, 0},
+        {OP_CAVITY | FUSE_PRE, 0, 0, 0},
+        {OP_CORRECTOR, 0, 0, 0},
+        {OP_APERTURE, 0, 0, 0},
+    };
+    const std::vector<Hdr> h = ring({}, forms, 2, {}, 40);
+    check(spec_unsupported(h).empty(), "every lean group form is accepted");
+    for (int lds = 0; lds < 2; ++lds) {
+      const std::string src = spec_source(h, 64, lds ? 1 : 2, lds != 0);
+      std::vector<char> code;
+      std::string log;
+      const int rc = spec_compile(src, "gfx950", code, log);
+      std::printf("  %s staging, %zu source bytes -> %zu code bytes\n", lds ? "LDS" : "scalar", src.size(),
+                  code.size());
+      if (rc) std::printf("%s\n", log.c_str());
+      check(rc == 0 && !code.empty(), lds ? "every group form compiles (LDS, 1 per lane)"
+                                          : "every group form compiles (scalar loads, 2 per lane)");
+    }
+  }
+  std::printf(fails ? "FAILED\n" : "specialized tracker generator ok\n");
+  return fails != 0;
+}

@@ -216,7 +216,8 @@ def test_autotune_picks_a_valid_shape(gpu_lib):
     offs = np.array([0, len(grid), 2 * len(grid)], dtype=np.int64)
     d = DeviceArray(beam)
     best, ms = B.autotune(d.ptr, offs, 2)
-    assert best.block in (64, 128, 256) and best.ppl in (1, 2) and best.staging in (1, 2) and ms > 0
+    assert best.block in (64, 128, 256) and best.ppl in (1, 2) and best.staging in (1, 2, 3) and ms > 0
+    assert best.staging != 3 or best.ppl == 1  # specialized kernels are tried at one particle per lane
     assert np.array_equal(d.numpy(), beam)  # probes ran on a scratch copy
 
 
