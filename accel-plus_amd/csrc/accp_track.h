@@ -270,26 +270,10 @@ __device__ __forceinline__ void strmp_fixed(Part (&p)[PPL], const Hdr h, const P
     z2[k] = n2[k] * (0.5 * p[k].ipd);
   }
   if constexpr (SINGLE && HI == 2) {
-    // a pure sextupole: sext_kick
+    // a pure sextupole: sext_slices
     const double kb1 = k1 * b[2], kb2 = k2 * b[2], kb1x2 = 2.0 * kb1, kb2x2 = 2.0 * kb2;
-    auto slice = [&]() {
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) {
-        drift_c(p[k], n1[k], z1[k]);
-        sext_kick(p[k], kb1, kb1x2);
-        drift_c(p[k], n2[k], z2[k]);
-        sext_kick(p[k], kb2, kb2x2);
-        drift_c(p[k], n2[k], z2[k]);
-        sext_kick(p[k], kb1, kb1x2);
-        drift_c(p[k], n1[k], z1[k]);
-      }
-    };
-    if (ns == 10) {  // the reference's default nslice (accel.h:242): straight-line code
-#pragma unroll
-      for (int m = 0; m < 10; ++m) slice();
-    } else {
-      for (int m = 0; m < ns; ++m) slice();
-    }
+    for (int k = 0; k < PPL; ++k) sext_slices(p[k], ns, n1[k], z1[k], n2[k], z2[k], kb1, kb1x2, kb2, kb2x2);
     return;
   }
   for (int m = 0; m < ns; ++m) {

@@ -3,7 +3,9 @@
 // dipole pseudo-drift and kick (hxdrift_py, bend_kick_dipole) and the pure
 // sextupole kick (sext_kick) against the same maps written in the
 // reference's order (hxdrift_c and bend_kick_c<0, true>, which follow
-// atpass.cpp:199-235, 343-355; horner_single<2>), and the order-9 multipole
+// atpass.cpp:199-235, 343-355; horner_single<2>), a sextupole's 10 slices
+// with the drifts between slices merged (sext_slices) against the slice
+// sequence of accel.cpp:829-843, and the order-9 multipole
 // sum by Estrin's scheme (estrin9) against Horner's (horner_reg<9>, the
 // order of strthinkick, atpass.cpp:358-414).  They must agree to
 // round-off over random states; a dropped term or a wrong sign does not.
@@ -93,6 +95,37 @@ int main() {
   }
   std::printf("sextupole kick, 20000 states: max scaled difference %.2e\n", worst);
   if (!(worst < 1e-14)) fail = 1;
+  // the 10 slices of a sextupole with the drifts between slices merged,
+  // against the reference's slice sequence (horner_single kicks, every drift)
+  worst = 0;
+  for (int it = 0; it < 20000; ++it) {
+    const double scale = it % 2 ? 1e-3 : 2e-2;
+    Part p = random_part(scale), q = p;
+    const double sc[6] = {scale, scale * 0.2, scale * 0.5, scale * 0.1, 1e-2, 0.03};
+    const double len = 0.2, bs = U(-40, 40), sls = len / 10;
+    const double l1 = sls * DRIFT1, l2 = sls * DRIFT2, k1 = sls * KICK1, k2 = sls * KICK2;
+    const double n1 = l1 * p.ipd, n2 = l2 * p.ipd, z1 = n1 * (0.5 * p.ipd), z2 = n2 * (0.5 * p.ipd);
+    for (int m = 0; m < 10; ++m) {
+      double re, im;
+      drift_c(p, n1, z1);
+      horner_single<2>(bs, p.x, p.y, re, im);
+      p.px -= k1 * re;
+      p.py += k1 * im;
+      drift_c(p, n2, z2);
+      horner_single<2>(bs, p.x, p.y, re, im);
+      p.px -= k2 * re;
+      p.py += k2 * im;
+      drift_c(p, n2, z2);
+      horner_single<2>(bs, p.x, p.y, re, im);
+      p.px -= k1 * re;
+      p.py += k1 * im;
+      drift_c(p, n1, z1);
+    }
+    sext_slices(q, 10, n1, z1, n2, z2, k1 * bs, 2.0 * k1 * bs, k2 * bs, 2.0 * k2 * bs);
+    worst = std::fmax(worst, rel(p, q, sc));
+  }
+  std::printf("sextupole, 10 slices with merged drifts, 20000 states: max scaled difference %.2e\n", worst);
+  if (!(worst < 1e-13)) fail = 1;
   // order-9 multipole sum: Estrin against Horner (relative to the sum's
   // scale, the largest term)
   worst = 0;
