@@ -473,8 +473,8 @@ struct TableRef {
   int nseed, nelem;
   size_t lds_need;
   bool full;  // kernel variant with every element map
-  // host copy of the headers every seed shares (offsets included), when the
-  // table can be specialized (ACCP_STAGE_SPECIALIZED), else null
+  // host copy of the headers every seed shares (offsets included), for the
+  // specialized kernel (ACCP_STAGE_SPECIALIZED); null when the seeds differ
   const std::vector<Hdr> *shared_hdr = nullptr;
 };
 
@@ -546,10 +546,8 @@ int enqueue_run(const TableRef &T, const WsView &ws, int max_tiles, double *d_r,
   if (rc) return rc;
   const SpecKernel *spec = nullptr;
   if (tuning && tuning->staging == ACCP_STAGE_SPECIALIZED) {
-    if (!T.shared_hdr || T.full || R.st != 0 || R.ed != T.nelem)
-      return fail(ACCP_E_ARG, "ACCP_STAGE_SPECIALIZED runs whole turns of lean lattices whose seeds share one "
-                              "grouped table");
-    if ((rc = spec_kernel(*T.shared_hdr, block, ppl, lds, &spec))) return rc;
+    if (!T.shared_hdr) return fail(ACCP_E_ARG, "ACCP_STAGE_SPECIALIZED: the seeds differ in structure");
+    if ((rc = spec_kernel(*T.shared_hdr, SpecRange{R.st, R.ed, T.full}, block, ppl, lds, &spec))) return rc;
   }
   const int per = block * ppl;
   const int64_t *d_offs = ws.offs;
@@ -613,7 +611,7 @@ struct accp_lattice {
   WakeState wake;
   TableRef table() const {
     return TableRef{(const Hdr *)d_hdr.p, (const double *)d_par.p, (const int32_t *)d_meta.p, 1,
-                    (int)tab.hdr.size(), lds_bytes_for(tab.hdr.size(), tab.par.size()), tab.full};
+                    (int)tab.hdr.size(), lds_bytes_for(tab.hdr.size(), tab.par.size()), tab.full, &tab.hdr};
   }
   // The grouped table when [st, ed) starts and ends on group boundaries
   // (st and ed are then rewritten as group indices), else the element table.
@@ -932,7 +930,7 @@ struct accp_batch {
   int nseed = 0, nelem = 0;
   size_t max_lds = 0;
   bool full = false;
-  std::vector<Hdr> shared_hdr;  // the grouped headers, when every seed has the same ones
+  std::vector<Hdr> shared_hdr;  // the table's headers, when every seed has the same ones
   DevBuf d_hdr, d_par, d_meta, d_scratch;
   WorkSet ws;
   hipGraph_t graph = nullptr;
@@ -979,16 +977,18 @@ extern "C" int accp_batch_create(accp_lattice *const *seeds, int nseed, accp_bat
     b->max_lds = std::max(b->max_lds, lds_bytes_for(ch.size(), cp.size()));
     b->full = b->full || c.full;
   }
-  // one specialized kernel serves every seed when their grouped headers
-  // (constant offsets included) are the same
-  if (grouped) {
+  // one specialized kernel serves every seed when their headers (constant
+  // offsets included) are the same
+  {
+    const std::vector<Hdr> &h0 = grouped ? seeds[0]->tab.fhdr : seeds[0]->tab.hdr;
     bool same = true;
     for (int s = 1; s < nseed && same; ++s) {
-      const std::vector<Hdr> &a = seeds[0]->tab.fhdr, &c = seeds[s]->tab.fhdr;
-      for (size_t k = 0; k < a.size() && same; ++k)
-        same = a[k].op == c[k].op && a[k].off == c[k].off && a[k].a == c[k].a && a[k].b == c[k].b;
+      const std::vector<Hdr> &c = grouped ? seeds[s]->tab.fhdr : seeds[s]->tab.hdr;
+      same = c.size() == h0.size();
+      for (size_t k = 0; k < h0.size() && same; ++k)
+        same = h0[k].op == c[k].op && h0[k].off == c[k].off && h0[k].a == c[k].a && h0[k].b == c[k].b;
     }
-    if (same) b->shared_hdr = seeds[0]->tab.fhdr;
+    if (same) b->shared_hdr = h0;
   }
   std::vector<int32_t> meta(h0);
   meta.insert(meta.end(), p0.begin(), p0.end());
@@ -1051,10 +1051,8 @@ extern "C" int accp_batch_capture(accp_batch *b, double *d_r, const int64_t *off
   if ((rc = prepare_workset(b->ws, offsets, b->nseed, block * ppl, max_tiles))) return rc;
   if (tuning && tuning->staging == ACCP_STAGE_SPECIALIZED) {
     const SpecKernel *k = nullptr;
-    if (b->shared_hdr.empty() || b->full)
-      return fail(ACCP_E_ARG, "ACCP_STAGE_SPECIALIZED runs whole turns of lean lattices whose seeds share one "
-                              "grouped table");
-    if ((rc = spec_kernel(b->shared_hdr, block, ppl, lds, &k))) return rc;
+    if (b->shared_hdr.empty()) return fail(ACCP_E_ARG, "ACCP_STAGE_SPECIALIZED: the seeds differ in structure");
+    if ((rc = spec_kernel(b->shared_hdr, SpecRange{0, b->nelem, b->full}, block, ppl, lds, &k))) return rc;
   }
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -1101,7 +1099,8 @@ extern "C" int accp_autotune(accp_batch *b, double *d_r, const int64_t *offsets,
         if (sg == ACCP_STAGE_LDS && b->max_lds > LDS_LIMIT) continue;
         // specialized kernels (one compilation each) at one particle per lane
         if (sg == ACCP_STAGE_SPECIALIZED &&
-            (pp != 1 || b->shared_hdr.empty() || b->full || !spec_unsupported(b->shared_hdr).empty()))
+            (pp != 1 || b->shared_hdr.empty() ||
+             !spec_unsupported(b->shared_hdr, SpecRange{0, b->nelem, b->full}).empty()))
           continue;
         accp_tuning t{bl, pp, sg, 0};
         // untimed launch (code object load, buffers), then the timed probe

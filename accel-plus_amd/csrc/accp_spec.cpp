@@ -34,8 +34,8 @@ namespace accp {
 
 namespace {
 
-constexpr int MAX_PERIOD = 64;     // groups in one loop body
-constexpr int MAX_STRAIGHT = 256;  // groups written out once (outside the loop, plus its body)
+constexpr int MAX_PERIOD = 64;     // entries in one loop body
+constexpr int MAX_STRAIGHT = 256;  // entries written out once (outside the loop, plus its body)
 
 bool same_shape(const Hdr &a, const Hdr &b) { return a.op == b.op && a.a == b.a && a.b == b.b; }
 
@@ -52,6 +52,7 @@ bool lean_op(int32_t op) {
     case OP_CAVITY:
     case OP_APERTURE:
     case OP_CORRECTOR:
+    case OP_WAKE:  // the host splits the turn at an active wake; inside a pass it does nothing
       return true;
     default:
       return false;
@@ -95,22 +96,29 @@ Period find_period(const std::vector<Hdr> &h, int max_p) {
   return best;
 }
 
-std::string spec_unsupported(const std::vector<Hdr> &h) {
-  if (h.empty()) return "empty table";
-  for (const Hdr &x : h)
-    if (!lean_op(x.op)) return "an element needs the full tracker variant";
-  const Period pr = find_period(h, MAX_PERIOD);
-  const int straight = (int)h.size() - (pr.r >= 2 ? pr.p * (pr.r - 1) : 0);
-  if (straight > MAX_STRAIGHT) return "more than 256 groups outside a repeated cell";
+std::string spec_unsupported(const std::vector<Hdr> &h, const SpecRange &r) {
+  if (r.st < 0 || r.ed > (int)h.size() || r.st >= r.ed) return "empty range";
+  const std::vector<Hdr> sub(h.begin() + r.st, h.begin() + r.ed);
+  if (!r.full)
+    for (const Hdr &x : sub)
+      if (!lean_op(x.op)) return "an element needs the full tracker variant";
+  const Period pr = find_period(sub, MAX_PERIOD);
+  const int straight = (int)sub.size() - (pr.r >= 2 ? pr.p * (pr.r - 1) : 0);
+  if (straight > MAX_STRAIGHT) return "more than 256 elements outside a repeated cell";
   return "";
 }
 
-std::string spec_source(const std::vector<Hdr> &h, int block, int ppl, bool lds) {
+std::string spec_source(const std::vector<Hdr> &hall, const SpecRange &r, int block, int ppl, bool lds) {
+  const std::vector<Hdr> h(hall.begin() + r.st, hall.begin() + r.ed);
   const int g = (int)h.size();
   const Period pr = find_period(h, MAX_PERIOD);
+  const char *full = r.full ? "true" : "false";
   std::ostringstream o;
-  o << "// the persistent tracker specialized for one lattice structure: " << g << " groups";
-  if (pr.r >= 2) o << ", groups " << pr.s << ".." << pr.s + pr.p * pr.r - 1 << " as " << pr.r << " cells of " << pr.p;
+  o << "// the persistent tracker specialized for one lattice structure: entries " << r.st << ".." << r.ed - 1
+    << " of the " << (r.full ? "element" : "grouped") << " table";
+  if (pr.r >= 2)
+    o << ", entries " << r.st + pr.s << ".." << r.st + pr.s + pr.p * pr.r - 1 << " as " << pr.r << " cells of "
+      << pr.p;
   o << "\n#include \"accp_track.h\"\n\nnamespace accp {\n\n"
        "struct SpecWalker {\n"
        "  template <int PPL, bool FULL, bool LDS, class PT>\n"
@@ -118,26 +126,26 @@ std::string spec_source(const std::vector<Hdr> &h, int block, int ppl, bool lds)
        "                                              const uint64_t (&pid)[PPL], uint64_t turn,\n"
        "                                              const int64_t (&gidx)[PPL], bool (&parked)[PPL]) {\n";
   auto emit = [&](const Hdr &x, const char *base, int off, const std::string &e, const char *indent) {
-    o << indent << "track_step<PPL, false>(p, Hdr{" << x.op << ", " << off << ", " << x.a << ", " << x.b << "}, "
+    o << indent << "track_step<PPL, FULL>(p, Hdr{" << x.op << ", " << off << ", " << x.a << ", " << x.b << "}, "
       << base << " + " << off << ", ctx, pid, turn, " << e << ", gidx, parked);\n";
   };
   int i = 0;
   if (pr.r >= 2) {
-    for (; i < pr.s; ++i) emit(h[i], "par", h[i].off, std::to_string(i), "    ");
+    for (; i < pr.s; ++i) emit(h[i], "par", h[i].off, std::to_string(r.st + i), "    ");
     o << "#pragma unroll 1\n"
       << "    for (int c = 0; c < " << pr.r << "; ++c) {\n"
       << "      const PT *q = par + " << h[pr.s].off << " + c * " << pr.stride << ";\n"
-      << "      const int e0 = " << pr.s << " + c * " << pr.p << ";\n";
+      << "      const int e0 = " << r.st + pr.s << " + c * " << pr.p << ";\n";
     for (int j = 0; j < pr.p; ++j)
       emit(h[pr.s + j], "q", h[pr.s + j].off - h[pr.s].off, "e0 + " + std::to_string(j), "      ");
     o << "    }\n";
     i = pr.s + pr.p * pr.r;
   }
-  for (; i < g; ++i) emit(h[i], "par", h[i].off, std::to_string(i), "    ");
+  for (; i < g; ++i) emit(h[i], "par", h[i].off, std::to_string(r.st + i), "    ");
   o << "  }\n};\n\n}  // namespace accp\n\n"
     << "extern \"C\" __global__ __launch_bounds__(" << block << ") void accp_spec_track(accp::KernelCtx ctx) {\n"
-    << "  accp::track_tile<" << block << ", " << ppl << ", " << (lds ? "true" : "false")
-    << ", false, accp::SpecWalker>(ctx);\n}\n";
+    << "  accp::track_tile<" << block << ", " << ppl << ", " << (lds ? "true" : "false") << ", " << full
+    << ", accp::SpecWalker>(ctx);\n}\n";
   return o.str();
 }
 
@@ -179,13 +187,14 @@ int spec_compile(const std::string &src, const std::string &arch, std::vector<ch
   return ACCP_OK;
 }
 
-int spec_kernel(const std::vector<Hdr> &h, int block, int ppl, bool lds, const SpecKernel **out) {
+int spec_kernel(const std::vector<Hdr> &h, const SpecRange &r, int block, int ppl, bool lds,
+                const SpecKernel **out) {
   *out = nullptr;
-  const std::string why = spec_unsupported(h);
+  const std::string why = spec_unsupported(h, r);
   if (!why.empty()) return err(ACCP_E_ARG, "the lattice cannot be specialized: " + why);
   int dev = -1;
   if (hipGetDevice(&dev) != hipSuccess) return err(ACCP_E_HIP, "specialized tracker: no current HIP device");
-  const std::string src = spec_source(h, block, ppl, lds);
+  const std::string src = spec_source(h, r, block, ppl, lds);
   std::lock_guard<std::mutex> lock(g_mu);
   auto it = g_cache.find({dev, src});
   if (it != g_cache.end()) {

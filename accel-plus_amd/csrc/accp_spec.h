@@ -1,7 +1,7 @@
 // accp_spec.h -- the lattice-specialized tracker: the persistent tracker of
 // accp_track.h compiled at run time (hiprtc) for one lattice structure, with
-// the turn's group sequence written out as straight-line code instead of a
-// walk over the element table.
+// the turn's element sequence (or a range of it) written out as straight-line
+// code instead of a walk over the element table.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -23,13 +23,22 @@ struct Period {
 // shortest period among equals), or r = 0 when there is none.
 Period find_period(const std::vector<Hdr> &h, int max_p);
 
-// Why the grouped table h cannot be specialized, or "" when it can (a lean
-// table whose code stays within a bounded number of straight-line groups).
-std::string spec_unsupported(const std::vector<Hdr> &h);
+// Which entries [st, ed) of a table a specialized kernel runs, and in which
+// tracker variant: full = the element table with every element map (radiation,
+// noise, linear Bend, ...), otherwise the grouped table of the lean variant.
+struct SpecRange {
+  int st = 0, ed = 0;
+  bool full = false;
+};
 
-// HIP source of the specialized kernel `accp_spec_track` for table h, launch
-// shape (block, ppl) and staging (lds): the turn walks all groups of h.
-std::string spec_source(const std::vector<Hdr> &h, int block, int ppl, bool lds);
+// Why entries [r.st, r.ed) of table h cannot be specialized, or "" when they
+// can (ops of the variant, code within a bounded number of straight-line
+// entries).
+std::string spec_unsupported(const std::vector<Hdr> &h, const SpecRange &r);
+
+// HIP source of the specialized kernel `accp_spec_track` for entries
+// [r.st, r.ed) of table h, launch shape (block, ppl) and staging (lds).
+std::string spec_source(const std::vector<Hdr> &h, const SpecRange &r, int block, int ppl, bool lds);
 
 // Compiles source (from spec_source) for offload target arch ("gfx950",
 // or a device's gcnArchName) into a code object: ACCP_OK, or an error code
@@ -38,10 +47,10 @@ int spec_compile(const std::string &src, const std::string &arch, std::vector<ch
 
 struct SpecKernel;
 
-// The compiled kernel for (h, block, ppl, lds) on the current device,
+// The compiled kernel for (h, r, block, ppl, lds) on the current device,
 // compiled on first use and kept for the life of the process.  Call it
 // outside any stream capture (module loading is not a stream operation).
-int spec_kernel(const std::vector<Hdr> &h, int block, int ppl, bool lds, const SpecKernel **out);
+int spec_kernel(const std::vector<Hdr> &h, const SpecRange &r, int block, int ppl, bool lds, const SpecKernel **out);
 
 // Milliseconds the last compilation of this process took (0: none yet).
 double spec_last_compile_ms();

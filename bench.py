@@ -144,10 +144,13 @@ def config_legs(lib, stream):
     """The plain workloads on the GPU, one tracking call each, inputs in HBM:
     the 20-cell FODO ring with 1000 Gaussian particles for 1000 turns, and
     the full DA scan of the seed-0 lattice (101 x 51 x 5 = 25,755 points,
-    2048 turns); the reference CPU tracking of the same FODO beam, and of
-    every tenth DA grid point, beside them (median of 3)."""
+    2048 turns), each on the kernel specialized for its lattice (64-lane
+    workgroups: both end as a few waves' instruction streams) with the
+    table-walking kernel beside it; the reference CPU tracking of the same
+    FODO beam, and of every tenth DA grid point (median of 3)."""
     import numpy as np
 
+    import pyaccp.abi as abi
     import pyaccp.lattices as lat
     from pyaccp.hipmem import DeviceArray, Event
 
@@ -166,18 +169,23 @@ def config_legs(lib, stream):
         d0 = DeviceArray(beam)
         d = DeviceArray(beam)
         e0, e1 = Event(), Event()
-        ms = []
-        for _ in range(3):
-            d.copy_from(d0, stream.handle)
-            e0.record(stream.handle)
-            L.track_device(d.ptr, len(beam), nturn, stream=stream.handle)
-            e1.record(stream.handle)
-            ms.append(e0.elapsed_ms(e1))
-        out = d.numpy()
-        gms = float(np.median(ms))
+        times = {}
+        for kernel, tuning in (("specialized", abi.Tuning(64, 1, abi.STAGE_SPECIALIZED, 0)), ("table walker", None)):
+            L.track_device(d.ptr, len(beam), 1, stream=stream.handle, tuning=tuning)  # compiles the specialized one
+            ms = []
+            for _ in range(3):
+                d.copy_from(d0, stream.handle)
+                e0.record(stream.handle)
+                L.track_device(d.ptr, len(beam), nturn, stream=stream.handle, tuning=tuning)
+                e1.record(stream.handle)
+                ms.append(e0.elapsed_ms(e1))
+            times[kernel] = (float(np.median(ms)), d.numpy())
+        gms, out = times["specialized"]
         entry = {"particles": len(beam), "turns": nturn, "elements": len(ring), "ms": round(gms, 3),
                  "particle_turns_per_s": len(beam) * nturn / (gms * 1e-3),
-                 "survivors": int((~np.isnan(out[:, 0])).sum())}
+                 "survivors": int((~np.isnan(out[:, 0])).sum()), "kernel": "specialized, 64 lanes",
+                 "table_walker": {"ms": round(times["table walker"][0], 3),
+                                  "survivors": int((~np.isnan(times["table walker"][1][:, 0])).sum())}}
         if ref is not None:
             sample = beam[::cpu_stride]
             line = ref.load(ring)
@@ -247,11 +255,13 @@ def collective_leg(lib, stream, nturn=200, eq_turns=20000):
     synchronous phase from the GPU's setrfphase, longitudinal resonator wake
     of 5 kOhm, Q = 1, 10 GHz, 2 nC in 256 bins over +-15 cm) of 1e5 Gaussian particles on
     the seed-0 DA lattice; every turn splits at the wake for the histogram,
-    convolution and kick.  Timed over nturn turns; then the whole workload
-    (eq_turns turns) for the equilibrium energy spread, bunch length and
-    emittance."""
+    convolution and kick.  Timed over nturn turns on the table-walking kernel
+    and on kernels specialized for each segment of the turn; then the whole
+    workload (eq_turns turns) on the faster, for the equilibrium energy
+    spread, bunch length and emittance."""
     import numpy as np
 
+    import pyaccp.abi as abi
     import pyaccp.elements as el
     import pyaccp.flops as flops
     import pyaccp.lattices as lat
@@ -274,13 +284,17 @@ def collective_leg(lib, stream, nturn=200, eq_turns=20000):
     d0 = DeviceArray(beam)
     d = DeviceArray(beam)
     e0, e1 = Event(), Event()
-    L.track_device(d.ptr, n, 10, seed=3, stream=stream.handle)  # warmup
-    d.copy_from(d0, stream.handle)
-    e0.record(stream.handle)
-    L.track_device(d.ptr, n, nturn, seed=3, stream=stream.handle)
-    e1.record(stream.handle)
-    ms = e0.elapsed_ms(e1)
-    out = d.numpy()
+    times = {}
+    for kernel, tuning in (("specialized", abi.Tuning(256, 1, abi.STAGE_SPECIALIZED, 0)), ("table walker", None)):
+        d.copy_from(d0, stream.handle)
+        L.track_device(d.ptr, n, 10, seed=3, stream=stream.handle, tuning=tuning)  # warmup, compilation
+        d.copy_from(d0, stream.handle)
+        e0.record(stream.handle)
+        L.track_device(d.ptr, n, nturn, seed=3, stream=stream.handle, tuning=tuning)
+        e1.record(stream.handle)
+        times[kernel] = (e0.elapsed_ms(e1), d.numpy(), tuning)
+    kernel = min(times, key=lambda k: times[k][0])
+    ms, out, tuning = times[kernel]
     alive = int((~np.isnan(out[:, 0])).sum())
     fl = n * nturn * flops.flops_per_turn([arr[i] for i in range(len(arr))], "useful")
     res = {
@@ -290,6 +304,8 @@ def collective_leg(lib, stream, nturn=200, eq_turns=20000):
         "ms_per_turn": round(ms / nturn, 4),
         "particle_turns_per_s": n * nturn / (ms * 1e-3),
         "achieved_tflops": round(fl / (ms * 1e-3) / 1e12, 3),
+        "kernel": kernel,
+        "ms_per_turn_by_kernel": {k: round(v[0] / nturn, 4) for k, v in times.items()},
     }
     if eq_turns:
         # the whole collective workload: 20000 turns (five longitudinal
@@ -302,7 +318,7 @@ def collective_leg(lib, stream, nturn=200, eq_turns=20000):
         recs = []
         t0 = time.perf_counter()
         for k in range(eq_turns // piece):
-            L.track_device(d.ptr, n, piece, seed=3, turn0=k * piece, stream=stream.handle)
+            L.track_device(d.ptr, n, piece, seed=3, turn0=k * piece, stream=stream.handle, tuning=tuning)
             r = d.numpy()
             recs.append([(k + 1) * piece] + beam_record(lib.beam_stat(r), r, beta, alpha, eta, etap))
         sec = time.perf_counter() - t0

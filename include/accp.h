@@ -146,14 +146,17 @@ typedef struct accp_adjoint accp_adjoint; /* reverse-mode turn map context    */
 #define ACCP_STAGE_AUTO 0   /* library default                                */
 #define ACCP_STAGE_SCALAR 1 /* wave-uniform scalar loads through the K$       */
 #define ACCP_STAGE_LDS 2    /* whole table copied to LDS per workgroup        */
-/* The lattice's element sequence compiled at run time (hiprtc) into a
- * kernel of its own, with the constants in LDS when they fit: no per-element
- * dispatch in the turn.  Lean lattices (no radiation, noise, wake, linear
- * Bend, trigonometric quads or BendMPole above octupole order), whole turns,
- * and in a batch seeds with the same structure; otherwise ACCP_E_ARG.  The
- * first call for a structure and launch shape compiles for a few seconds;
- * the kernel is kept for the life of the process.  Never chosen by
- * ACCP_STAGE_AUTO; accp_autotune tries it. */
+/* The lattice's element sequence (the call's element range) compiled at
+ * run time (hiprtc) into a kernel of its own, with the constants in LDS when
+ * they fit: no per-element dispatch in the turn, a ring of identical cells a
+ * loop over one cell.  Any lattice (up to 256 elements outside a repeated
+ * cell); in a batch, seeds of one structure; otherwise ACCP_E_ARG.  Results
+ * differ from the table-walking kernel's in the last bits (the compiler may
+ * fuse operations across elements).  The first call for a structure, range
+ * and launch shape compiles for about a second; the kernel is kept for the
+ * life of the process.  Not for accp_track_device_noalloc (loading a kernel
+ * allocates device memory).  Never chosen by ACCP_STAGE_AUTO;
+ * accp_autotune tries it. */
 #define ACCP_STAGE_SPECIALIZED 3
 
 /* Launch shape of the tracking kernel; zero fields mean "library default".

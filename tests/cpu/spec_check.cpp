@@ -1,8 +1,9 @@
 // CPU check of the lattice-specialized tracker's generator
 // (accel-plus_amd/csrc/accp_spec.cpp, linked from libaccp.so): the repeated
 // cell it finds in a grouped element table, the tables it refuses, and that
-// the kernel source it writes for a table holding every group form of the
-// lean tracker compiles for gfx950 (hiprtc; no device needed).  Prints one
+// the kernel source it writes compiles for gfx950 (hiprtc; no device needed)
+// for a table holding every group form of the lean tracker and for a range
+// of a table holding every element form of the full one.  Prints one
 // line per check and exits non-zero on a failure.
 #include <cstdio>
 #include <random>
@@ -54,7 +55,7 @@ int main() {
     const Period p = find_period(h, 64);
     std::printf("  period: s=%d p=%d r=%d stride=%d\n", p.s, p.p, p.r, p.stride);
     check(p.s == 1 && p.p == 6 && p.r == 20 && p.stride == 42, "20 identical cells after a prefix group");
-    check(spec_unsupported(h).empty(), "a lean table of cells is specialized");
+    check(spec_unsupported(h, SpecRange{0, (int)h.size(), false}).empty(), "a lean table of cells is specialized");
   }
   // a cell that differs (one quad with more series terms) splits the ring:
   // the longest run of identical cells is taken (here it starts one group
@@ -78,11 +79,15 @@ int main() {
   {
     std::vector<Hdr> h = ring({}, cell, 3, {}, 5);
     h[2].op = OP_BENDMP_RAD;
-    check(!spec_unsupported(h).empty(), "a radiating bend needs the full variant: refused");
+    check(!spec_unsupported(h, SpecRange{0, (int)h.size(), false}).empty(),
+          "a radiating bend in the lean variant: refused");
+    check(spec_unsupported(h, SpecRange{0, (int)h.size(), true}).empty(), "a radiating bend in the full variant");
+    check(!spec_unsupported(h, SpecRange{4, 4, true}).empty(), "an empty range: refused");
     std::vector<Hdr> big;  // series terms drawn at random: no repeating pattern
     std::mt19937 gen(7);
     for (int i = 0; i < 300; ++i) big.push_back(Hdr{OP_QUAD, i * 40, 4 + 2 * (int)(gen() % 7), 0});
-    check(!spec_unsupported(big).empty(), "300 groups without a repeated cell: refused");
+    check(!spec_unsupported(big, SpecRange{0, 300, false}).empty(), "300 groups without a repeated cell: refused");
+    check(spec_unsupported(big, SpecRange{20, 220, false}).empty(), "200 of them: accepted");
   }
   // every group form of the lean tracker, in a table of two cells, compiles
   {
@@ -112,9 +117,9 @@ int main() {
         {OP_APERTURE, 0, 0, 0},
     };
     const std::vector<Hdr> h = ring({}, forms, 2, {}, 40);
-    check(spec_unsupported(h).empty(), "every lean group form is accepted");
+    check(spec_unsupported(h, SpecRange{0, (int)h.size(), false}).empty(), "every lean group form is accepted");
     for (int lds = 0; lds < 2; ++lds) {
-      const std::string src = spec_source(h, 64, lds ? 1 : 2, lds != 0);
+      const std::string src = spec_source(h, SpecRange{0, (int)h.size(), false}, 64, lds ? 1 : 2, lds != 0);
       std::vector<char> code;
       std::string log;
       const int rc = spec_compile(src, "gfx950", code, log);
@@ -124,6 +129,46 @@ int main() {
       check(rc == 0 && !code.empty(), lds ? "every group form compiles (LDS, 1 per lane)"
                                           : "every group form compiles (scalar loads, 2 per lane)");
     }
+  }
+  // every element form of the full variant (radiation, noise, the linear
+  // Bend, trigonometric quads, wigglers, tables, the Matrix, a disabled wake),
+  // over a range of the table, compiles
+  {
+    const std::vector<Hdr> forms = {
+        {OP_NOP, 0, 0, 0},
+        {OP_DRIFT, 0, 0, 0},
+        {OP_QUAD, 0, 0, 0},
+        {OP_QUAD, 0, 8, 0},
+        {OP_QUAD_K0, 0, 0, 0},
+        {OP_BEND, 0, 0, 0},
+        {OP_BENDMP, 0, 10, 1 << 8},
+        {OP_BENDMP, 0, 10, 7},
+        {OP_BENDMP_RAD, 0, 10, 1 << 8},
+        {OP_BENDMP_QE, 0, 10, (1 << 8) | BM_EDGE_IN},
+        {OP_STRMP, 0, 10, 2 | STRMP_SINGLE},
+        {OP_STRMP, 0, 10, 9},
+        {OP_STRMP_THIN, 0, 0, 9},
+        {OP_CAVITY, 0, 1, 0},
+        {OP_APERTURE, 0, 0, 0},
+        {OP_CORRECTOR, 0, 0, 0},
+        {OP_WAKE, 0, 0, 0},
+        {OP_MATRIX, 0, 0, 0},
+        {OP_SYNCRAD, 0, 3, 0},
+        {OP_CRABCAV, 0, 7, 0},
+        {OP_RFMULT, 0, 1, 3},
+        {OP_IDEALWIG, 0, 20, 1 | (1 << 8)},
+        {OP_WIGTABLE, 0, 4, 5 | (3 << 16)},
+    };
+    const std::vector<Hdr> h = ring({}, forms, 3, {}, 40);
+    const SpecRange r{5, (int)h.size() - 3, true};
+    check(spec_unsupported(h, r).empty(), "every element form of the full variant is accepted");
+    const std::string src = spec_source(h, r, 256, 1, true);
+    std::vector<char> code;
+    std::string log;
+    const int rc = spec_compile(src, "gfx950", code, log);
+    std::printf("  full variant, %zu source bytes -> %zu code bytes\n", src.size(), code.size());
+    if (rc) std::printf("%s\n", log.c_str());
+    check(rc == 0 && !code.empty(), "every element form of the full variant compiles (a range, LDS)");
   }
   std::printf(fails ? "FAILED\n" : "specialized tracker generator ok\n");
   return fails != 0;

@@ -129,21 +129,56 @@ def test_specialized_monitor_records(gpu_lib):
     np.testing.assert_allclose(a[:, 2:], b[:, 2:], rtol=1e-11, atol=1e-20)
 
 
+def test_specialized_full_variant_and_ranges(gpu_lib, oracle):
+    """Lattices of the full tracker variant and element ranges: radiation
+    (flag 1) and quantum excitation on the oracle's random stream, one turn
+    at 1e-12; an element range and a turn offset at 1e-12; a ring whose turn
+    splits at a LongWake (one kernel per segment), 3 turns at 1e-10; the
+    element types beyond the tracking set (Matrix, SyncRad, CrabCav, RFMult,
+    both wigglers) against the reference's fixtures at 1e-12, and a line of
+    the noisy ones against the oracle on the same stream, 3 turns at 1e-12."""
+    beam = lat.gaussian_beam(500, seed=2)
+    ring = lat.fodo_ring()
+    lat.set_rad(ring, 1)
+    assert_coords_close(gpu_lib.lattice(ring).track(beam, 1, tuning=spec(256)), oracle.track(ring, beam, 1), 1e-12,
+                        what="rad flag 1")
+    qe = lat.da_ring(0)
+    lat.set_rad(qe, 2)
+    kw = dict(seed=1234, turn0=7, pid0=100)
+    assert_coords_close(gpu_lib.lattice(qe).track(beam, 1, tuning=spec(), **kw), oracle.track(qe, beam, 1, **kw),
+                        1e-12, what="rad flag 2")
+    da = lat.da_ring(0, extra_quads=True)
+    L = gpu_lib.lattice(da)
+    for st, ed in ((0, 1), (17, 100), (100, len(da))):
+        assert_coords_close(L.track(beam, 1, st, ed, turn0=3, tuning=spec()),
+                            oracle.track(da, beam, 1, st, ed, turn0=3), 1e-12, what=f"range {st}..{ed}")
+    wring = lat.da_ring(0)
+    wring.insert(120, el.longwake(5e3, 1.0, 10e9, 2e-9, 256, -0.015, 0.015, energy=3000.0, on=1))
+    wbeam = lat.gaussian_beam(3000, seed=10)
+    assert_coords_close(gpu_lib.lattice(wring).track(wbeam, 3, tuning=spec()), oracle.track(wring, wbeam, 3), 1e-10,
+                        what="ring with a wake, 3 turns")
+    from conftest import golden
+    from tests.golden.make_golden import elements2_cases
+    g = golden("elements2.npz")
+    P = g["particles"]
+    for name, e in elements2_cases():
+        assert_coords_close(gpu_lib.lattice([e]).track(P, 1, tuning=spec()), g[name], 1e-12, what=name)
+    noisy = [el.rfmult(0.1, 5e8, 0.2, [1e-4, 2e-3, 0.5], [0.0, 1e-3, 0.2], sigphi=2e-3),
+             el.syncrad(0.0, 3000.0, 0.75, 1.3, 1.0, 5e-9, 5e-11, 8e-4, 10.0, 5.0, rad=el.RAD_LUMP_QE),
+             el.crabcav(0.3, 1.5, 0.5, 1.5e9, 0.3, 3000.0, sigphi=1e-3, sigdvv=1e-2)]
+    kw = dict(seed=17, turn0=5, pid0=100)
+    assert_coords_close(gpu_lib.lattice(noisy).track(beam, 3, tuning=spec(), **kw),
+                        oracle.track(noisy, beam, 3, **kw), 1e-12, what="noisy line, 3 turns")
+
+
 def test_specialized_refusals(gpu_lib):
-    """Lattices and calls the specialized kernel does not take fail with an
-    error and change nothing: radiation in the bends, an element range that
-    is not a whole turn, the no-allocation entry point (it would load a code
-    object), and a batch whose seeds differ in structure."""
+    """Calls the specialized kernel does not take fail with an error and
+    change nothing: the no-allocation entry point (it would load a code
+    object) and a batch whose seeds differ in structure."""
     from pyaccp.hipmem import DeviceArray
 
     beam = lat.gaussian_beam(64, seed=1)
-    rad = lat.da_ring(0)
-    lat.set_rad(rad, 1)
-    with pytest.raises(abi.AccpError, match="specialized|grouped table"):
-        gpu_lib.lattice(rad).track(beam, 1, tuning=spec())
     L = gpu_lib.lattice(lat.da_ring(0))
-    with pytest.raises(abi.AccpError, match="whole turns"):
-        L.track(beam, 1, st=5, ed=100, tuning=spec())
     d = DeviceArray(beam)
     with pytest.raises(abi.AccpError, match="noalloc"):
         L.track_device(d.ptr, len(beam), 1, tuning=spec(), noalloc=True)
@@ -152,8 +187,9 @@ def test_specialized_refusals(gpu_lib):
     B = abi.Batch(gpu_lib, mixed)
     both = np.concatenate([beam, beam])
     d2 = DeviceArray(both)
-    with pytest.raises(abi.AccpError):
+    with pytest.raises(abi.AccpError, match="differ in structure"):
         B.track(d2.ptr, np.array([0, 64, 128], dtype=np.int64), 1, tuning=spec())
+    assert np.array_equal(d2.numpy(), both)
 
 
 def test_specialized_batched_64_seeds_full_size(gpu_lib, oracle):
