@@ -35,15 +35,16 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 # copy of known size; profiles/k_track_batched_da_64seeds_1024turns.md.  These
 # are constants of that profile, not measured by this run (the counters need
 # rocprofv3 around the process); other shapes and workloads report null.
-TRAFFIC_BYTES_PER_LAUNCH = {"block=64,ppl=1,lds": 8.99e6}
+TRAFFIC_BYTES_PER_LAUNCH = {"block=64,ppl=1,lds": 8.99e6, "block=64,ppl=1,specialized": 9.26e6}
 TRAFFIC_SOURCE = "profile constant: profiles/k_track_batched_da_64seeds_1024turns.md (not measured in this run)"
 
-# Executed FP64 work of the saturated leg's kernel from rocprofv3 counters
+# Executed FP64 work of the saturated leg's kernels from rocprofv3 counters
 # (SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 per wave x 64 lanes, FMA = 2 flops),
-# per particle-turn of the seed-0 DA lattice; profiles/k_track_saturated_262144x16.md.
-EXECUTED_FP64_FLOPS_PER_PARTICLE_TURN = 74968.0
+# per particle-turn of the seed-0 DA lattice, per launch shape;
+# profiles/k_track_saturated_262144x16.md.
+EXECUTED_FP64_FLOPS_PER_PARTICLE_TURN = {"block=256,ppl=1,lds": 72888.0, "block=256,ppl=1,specialized": 73703.0}
 # the same counters over one replay of the default batched workload, per launch shape
-EXECUTED_FP64_FLOPS_PER_STEP = {"block=64,ppl=1,lds": 4.935e11}
+EXECUTED_FP64_FLOPS_PER_STEP = {"block=64,ppl=1,lds": 4.935e11, "block=64,ppl=1,specialized": 4.851e11}
 
 
 def shape_name(t):
@@ -203,7 +204,7 @@ def config_legs(lib, stream):
     return res
 
 
-def saturated_leg(lib, glat, elems, tuning, stream, counters=True):
+def saturated_leg(lib, glat, elems, tuning, stream):
     """The same kernel at full occupancy: a Gaussian core beam that survives
     (2^18 particles on the seed-0 lattice, 16 turns), in 256-lane workgroups
     that share one LDS copy of the table per four waves (the batched leg's
@@ -241,8 +242,9 @@ def saturated_leg(lib, glat, elems, tuning, stream, counters=True):
         "achieved_counts": "algorithmic flops (the reference's formulas, useful terms) / kernel time",
         "tuning": tuning.as_dict(),
     }
-    if counters and EXECUTED_FP64_FLOPS_PER_PARTICLE_TURN:
-        ex = nsat * tsat * EXECUTED_FP64_FLOPS_PER_PARTICLE_TURN / (sat_best * 1e-3) / 1e12
+    per_pt = EXECUTED_FP64_FLOPS_PER_PARTICLE_TURN.get(shape_name(tuning))
+    if per_pt:
+        ex = nsat * tsat * per_pt / (sat_best * 1e-3) / 1e12
         res["executed_tflops"] = round(ex, 3)
         res["executed_frac_of_fp64_peak"] = round(ex / FP64_PEAK_TFLOPS, 4)
         res["executed_source"] = ("FP64 VALU instructions per particle-turn from rocprofv3 counters "
@@ -624,7 +626,7 @@ def side_legs(lib, stream, legs, lat0, glat0, args, result):
     if "saturated" in legs:
         result["saturated"] = saturated_leg(lib, glat0, lat0, abi.Tuning(256, 1, abi.STAGE_LDS, 0), stream)
         result["saturated"]["specialized"] = saturated_leg(
-            lib, glat0, lat0, abi.Tuning(256, 1, abi.STAGE_SPECIALIZED, 0), stream, counters=False)
+            lib, glat0, lat0, abi.Tuning(256, 1, abi.STAGE_SPECIALIZED, 0), stream)
     if "tail" in legs:
         result["tail"] = tail_leg(lib, stream)
     if "configs" in legs:
