@@ -17,7 +17,6 @@
 
 #include <hip/hiprtc.h>
 
-#include <chrono>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -60,8 +59,18 @@ bool lean_op(int32_t op) {
 }
 
 std::mutex g_mu;
-std::map<std::pair<int, std::string>, SpecKernel *> g_cache;  // (device, source) -> kernel
-double g_last_ms = 0;
+// Compiled kernels by (device, source), and by (device, the entries' raw
+// headers, range and launch shape): a call that finds its key in the second
+// map builds no source (a turn split at a wake asks once per segment and turn).
+std::map<std::pair<int, std::string>, SpecKernel *> g_cache;
+std::map<std::pair<int, std::string>, SpecKernel *> g_by_table;
+
+std::string table_key(const std::vector<Hdr> &h, const SpecRange &r, int block, int ppl, bool lds) {
+  const int32_t shape[6] = {r.st, r.ed, r.full ? 1 : 0, block, ppl, lds ? 1 : 0};
+  std::string k(reinterpret_cast<const char *>(shape), sizeof shape);
+  k.append(reinterpret_cast<const char *>(h.data() + r.st), sizeof(Hdr) * (size_t)(r.ed - r.st));
+  return k;
+}
 
 int err(int code, const std::string &msg) { return set_error(code, msg.c_str()); }
 
@@ -190,18 +199,25 @@ int spec_compile(const std::string &src, const std::string &arch, std::vector<ch
 int spec_kernel(const std::vector<Hdr> &h, const SpecRange &r, int block, int ppl, bool lds,
                 const SpecKernel **out) {
   *out = nullptr;
-  const std::string why = spec_unsupported(h, r);
-  if (!why.empty()) return err(ACCP_E_ARG, "the lattice cannot be specialized: " + why);
   int dev = -1;
   if (hipGetDevice(&dev) != hipSuccess) return err(ACCP_E_HIP, "specialized tracker: no current HIP device");
-  const std::string src = spec_source(h, r, block, ppl, lds);
+  if (r.st < 0 || r.ed > (int)h.size() || r.st >= r.ed) return err(ACCP_E_ARG, "specialized tracker: empty range");
+  const std::string tkey = table_key(h, r, block, ppl, lds);
   std::lock_guard<std::mutex> lock(g_mu);
+  auto hit = g_by_table.find({dev, tkey});
+  if (hit != g_by_table.end()) {
+    *out = hit->second;
+    return ACCP_OK;
+  }
+  const std::string why = spec_unsupported(h, r);
+  if (!why.empty()) return err(ACCP_E_ARG, "the lattice cannot be specialized: " + why);
+  const std::string src = spec_source(h, r, block, ppl, lds);
   auto it = g_cache.find({dev, src});
   if (it != g_cache.end()) {
+    g_by_table[{dev, tkey}] = it->second;
     *out = it->second;
     return ACCP_OK;
   }
-  const auto t0 = std::chrono::steady_clock::now();
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess)
     return err(ACCP_E_HIP, "specialized tracker: hipGetDeviceProperties failed");
@@ -219,14 +235,9 @@ int spec_kernel(const std::vector<Hdr> &h, const SpecRange &r, int block, int pp
     return err(ACCP_E_HIP, std::string("specialized tracker: module load failed: ") + hipGetErrorString(e));
   }
   g_cache[{dev, src}] = k;
-  g_last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  g_by_table[{dev, tkey}] = k;
   *out = k;
   return ACCP_OK;
-}
-
-double spec_last_compile_ms() {
-  std::lock_guard<std::mutex> lock(g_mu);
-  return g_last_ms;
 }
 
 hipError_t launch_spec(const SpecKernel *k, const KernelCtx &ctx, int grid, size_t lds_bytes, hipStream_t s) {

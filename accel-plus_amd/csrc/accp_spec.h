@@ -52,9 +52,6 @@ struct SpecKernel;
 // outside any stream capture (module loading is not a stream operation).
 int spec_kernel(const std::vector<Hdr> &h, const SpecRange &r, int block, int ppl, bool lds, const SpecKernel **out);
 
-// Milliseconds the last compilation of this process took (0: none yet).
-double spec_last_compile_ms();
-
 hipError_t launch_spec(const SpecKernel *k, const KernelCtx &ctx, int grid, size_t lds_bytes, hipStream_t s);
 
 }  // namespace accp

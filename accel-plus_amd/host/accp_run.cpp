@@ -1,15 +1,18 @@
 // accp_run.cpp -- the Accel+ command-file driver on the GPU backend.
 //
-//   accp-run [--precision N] [--parse-only] command_file
+//   accp-run [--precision N] [--specialized] [--parse-only] command_file
 //
 // Reads the reference's command files (src/process.cpp) and runs their
 // Lattice, Optics, Track and DAMA blocks through libaccp: lattice files in
 // the reference grammar (src/lib/accel/accel.cpp:67-133 and each create()),
 // the same keys, defaults and printed blocks ($$$...), tracking, optics and
 // DA on the GPU.  Numbers print with 6 significant digits like the
-// reference's cout unless --precision asks for more.  --parse-only reads the
-// command file and the lattice and prints the element count without
-// touching the GPU.
+// reference's cout unless --precision asks for more.  --specialized runs
+// Do_Track on the kernel compiled at run time for the lattice's structure
+// (ACCP_STAGE_SPECIALIZED: about half a second of compilation, then a
+// faster turn; long runs gain).
+// --parse-only reads the command file and the lattice and prints the element
+// count without touching the GPU.
 //
 // Commands handled (the path's callers in the reference):
 //   Lattice: file_name=F[, type=ERING]; set_flags: radiation=, cavity=, wake=;
@@ -53,6 +56,19 @@ namespace {
 using std::string;
 
 int g_prec = 6;
+bool g_specialized = false;  // --specialized: Do_Track on the specialized kernel
+
+// Tracking options of Do_Track: the launch shape that suits the beam (64-lane
+// workgroups for beams that leave the GPU partly idle, 256 otherwise).
+accp_track_opts track_opts(int np) {
+  accp_track_opts o{};
+  if (g_specialized) {
+    o.tuning.staging = ACCP_STAGE_SPECIALIZED;
+    o.tuning.block = np >= 65536 ? 256 : 64;
+    o.tuning.ppl = 1;
+  }
+  return o;
+}
 
 // ---- text helpers (process.cpp:109-160) ----------------------------------------
 string trim(const string &s) {
@@ -1242,7 +1258,7 @@ void do_track(Line &L, std::vector<double> &r, const Monitor &mon, const string 
     }
   }
   accp_lattice *lat = L.device();
-  accp_track_opts o{};
+  accp_track_opts o = track_opts(np);
   o.rng_seed = seed;
   if (mon.flag == 2 || mon.flag == 3) {
     // statistics at the start of every interval-th turn, on the device
@@ -1518,11 +1534,12 @@ int main(int argc, char **argv) {
   for (int i = 1; i < argc; ++i) {
     const string a = argv[i];
     if (a == "--precision" && i + 1 < argc) g_prec = std::max(1, std::min(17, std::atoi(argv[++i])));
+    else if (a == "--specialized") g_specialized = true;
     else if (a == "--parse-only") parse_only = true;
     else cmdfile = a;
   }
   if (cmdfile.empty()) {
-    std::cerr << "usage: accp-run [--precision N] [--parse-only] command_file" << std::endl;
+    std::cerr << "usage: accp-run [--precision N] [--specialized] [--parse-only] command_file" << std::endl;
     return 2;
   }
   std::ifstream in(cmdfile);

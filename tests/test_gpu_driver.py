@@ -20,11 +20,11 @@ pytestmark = pytest.mark.gpu
 RUN = os.path.join(ROOT, "accel-plus_amd", "accp-run")
 
 
-def run(tmp_path, text, check=True):
+def run(tmp_path, text, check=True, flags=()):
     """The driver's output lines; with check=False the finished process."""
     (tmp_path / "cmd.txt").write_text(text)
-    print("accp-run", tmp_path / "cmd.txt", flush=True)
-    p = subprocess.run([RUN, "--precision", "17", "cmd.txt"], cwd=tmp_path, capture_output=True, text=True,
+    print("accp-run", *flags, tmp_path / "cmd.txt", flush=True)
+    p = subprocess.run([RUN, "--precision", "17", *flags, "cmd.txt"], cwd=tmp_path, capture_output=True, text=True,
                        timeout=300)
     if not check:
         return p
@@ -86,6 +86,39 @@ STOP
     final = np.array([numbers(s)[1:7] for s in after(out, "$$$Track-Data: Full export", 201)[1:]])
     assert final.shape == (200, 6)
     assert_coords_close(final, r, 1e-10, what="20 turns through the driver")
+
+
+def test_specialized_tracking(tmp_path, oracle):
+    """Do_Track with --specialized (the kernel compiled for the lattice's
+    structure): the final coordinates against the oracle, as without it, and
+    the monitor statistics against the default kernel's."""
+    ring = lat.da_ring(1)
+    el.write_lattice(ring, str(tmp_path / "da.lat"))
+    beam = lat.gaussian_beam(300, seed=6)
+    el.write_plain_beam(beam, str(tmp_path / "beam.txt"))
+    cmd = """Lattice
+  file_name = da.lat, type=ERING
+End
+Track
+  Distribution: Load beam.txt, format=plain
+  Monitor: interval=4, printflag=Stat1
+  Do_Track: NTurn=12, radiation=off, RFCavity=on, PrintFinal=Full
+End
+STOP
+"""
+    spec = run(tmp_path, cmd, flags=("--specialized",))
+    walk = run(tmp_path, cmd)
+    arr = el.to_array(ring)
+    oracle.setrfphase(arr)  # Do_Track's RFCavity=on sets the synchronous phase, as the reference does
+    for i in range(len(arr)):
+        if arr[i].type == el.CAVITY:
+            arr[i].on = 1
+    final = np.array([numbers(s)[1:7] for s in after(spec, "$$$Track-Data: Full export", 301)[1:]])
+    assert_coords_close(final, oracle.track(arr, beam, 12), 1e-10, what="12 turns, --specialized")
+    st_s = np.array([numbers(s) for s in after(spec, "$$$Track-Data: Stat1", 5)[2:5]])
+    st_w = np.array([numbers(s) for s in after(walk, "$$$Track-Data: Stat1", 5)[2:5]])
+    assert np.array_equal(st_s[:, :2], st_w[:, :2])
+    np.testing.assert_allclose(st_s[:, 2:], st_w[:, 2:], rtol=1e-10, atol=1e-18)
 
 
 def test_track_da_with_loss_turns(tmp_path):
