@@ -609,6 +609,7 @@ struct accp_lattice {
   DevBuf d_beam, d_snap, d_stat, d_loss;
   WorkSet ws;
   WakeState wake;
+  accp_tuning def_tuning{};  // for calls that leave their tuning at zero (accp_lattice_set_tuning)
   TableRef table() const {
     return TableRef{(const Hdr *)d_hdr.p, (const double *)d_par.p, (const int32_t *)d_meta.p, 1,
                     (int)tab.hdr.size(), lds_bytes_for(tab.hdr.size(), tab.par.size()), tab.full, &tab.hdr};
@@ -710,6 +711,18 @@ extern "C" int accp_lattice_nelem(const accp_lattice *L, int *n) {
   return ACCP_OK;
 }
 
+extern "C" int accp_lattice_set_tuning(accp_lattice *L, const accp_tuning *t) {
+  if (!L) return fail(ACCP_E_ARG, "accp_lattice_set_tuning: null lattice");
+  if (t) {
+    int block, ppl, chunk_max;
+    bool lds;
+    int rc = resolve_tuning(t, lds_bytes_for(L->tab.hdr.size(), L->tab.par.size()), block, ppl, lds, chunk_max);
+    if (rc) return rc;
+  }
+  L->def_tuning = t ? *t : accp_tuning{};
+  return ACCP_OK;
+}
+
 extern "C" int accp_lattice_free(accp_lattice *L) {
   if (!L) return ACCP_OK;
   L->d_hdr.release();
@@ -733,8 +746,15 @@ extern "C" int accp_lattice_free(accp_lattice *L) {
 // the whole beam (histogram over all particles) before anyone passes them.
 // ext: a caller's fixed workspace (no allocation, offsets set by a kernel);
 // null: the lattice's own working set, grown as needed.
+static bool tuning_unset(const accp_tuning *t) {
+  return !t || (t->block == 0 && t->ppl == 0 && t->staging == ACCP_STAGE_AUTO && t->compact == 0);
+}
+
 static int track_device_impl(accp_lattice *L, double *d_r, int64_t n, const RunSpec &R,
                              const accp_tuning *tuning, hipStream_t s, const WsView *ext = nullptr) {
+  // the lattice's own launch shape for calls that name none (not for the
+  // no-allocation entry point: a specialized default would load a kernel)
+  if (!ext && tuning_unset(tuning)) tuning = &L->def_tuning;
   const int64_t offs[2] = {0, n};
   int block, ppl, chunk_max;
   bool lds;

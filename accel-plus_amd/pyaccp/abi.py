@@ -54,6 +54,7 @@ class Param(ctypes.Structure):
 EXPORTS = (
     "accp_last_error", "accp_abi_version", "accp_device_count",
     "accp_lattice_build", "accp_lattice_update", "accp_lattice_nelem", "accp_lattice_free",
+    "accp_lattice_set_tuning",
     "accp_track", "accp_track_device", "accp_track_device_noalloc", "accp_beam_stat",
     "accp_batch_create", "accp_track_batched", "accp_batch_capture", "accp_batch_replay",
     "accp_autotune", "accp_batch_free",
@@ -86,6 +87,7 @@ class Library:
         L.accp_lattice_update.argtypes = [VP, E, c_int]
         L.accp_lattice_nelem.argtypes = [VP, POINTER(c_int)]
         L.accp_lattice_free.argtypes = [VP]
+        L.accp_lattice_set_tuning.argtypes = [VP, POINTER(Tuning)]
         L.accp_track.argtypes = [VP, DP, c_int, c_int, c_int, c_int, POINTER(TrackOpts)]
         L.accp_track_device.argtypes = [VP, c_void_p, c_int, c_int, c_int, c_int, POINTER(TrackOpts), c_void_p]
         L.accp_track_device_noalloc.argtypes = [VP, c_void_p, c_int, c_int, c_int, c_int, POINTER(TrackOpts),
@@ -140,6 +142,12 @@ class Lattice:
         h = c_void_p()
         lib.check(lib.lib.accp_lattice_build(self.arr, len(self.arr), byref(h)), "accp_lattice_build")
         self.h = h
+
+    def set_tuning(self, tuning=None):
+        """accp_lattice_set_tuning: the launch shape of calls on this lattice
+        that name none (None: the library default)."""
+        self.L.check(self.L.lib.accp_lattice_set_tuning(self.h, byref(tuning) if tuning is not None else None),
+                     "accp_lattice_set_tuning")
 
     def update(self, elems):
         self.arr = elems if isinstance(elems, ctypes.Array) else el.to_array(elems)

@@ -204,6 +204,15 @@ int accp_lattice_build(const accp_elem *elems, int n, accp_lattice **out);
 int accp_lattice_update(accp_lattice *lat, const accp_elem *elems, int n);
 int accp_lattice_nelem(const accp_lattice *lat, int *n);
 int accp_lattice_free(accp_lattice *lat);
+/* No counterpart in the reference (whose ALine has one OpenMP loop).  The
+ * launch shape of every tracking call on this lattice that leaves its tuning
+ * at zero (ACCP_STAGE_AUTO): accp_track / accp_track_device without a
+ * tuning, and the lattice services that track internally (getm66, tunes,
+ * responses and corrections, trackDA, trackMA, Twiss, the tune diagnostics,
+ * the adjoint's forward pass).  NULL restores the library default.  A
+ * caller that wants those services on the specialized kernel sets
+ * ACCP_STAGE_SPECIALIZED here.  accp_track_device_noalloc ignores it. */
+int accp_lattice_set_tuning(accp_lattice *lat, const accp_tuning *tuning);
 
 /* ---- tracking ---------------------------------------------------------- */
 /* ALine::pass over elements [st, ed) repeated nturn times on a host AoS

@@ -171,6 +171,47 @@ def test_specialized_full_variant_and_ranges(gpu_lib, oracle):
                         oracle.track(noisy, beam, 3, **kw), 1e-12, what="noisy line, 3 turns")
 
 
+def test_lattice_default_tuning(gpu_lib, oracle):
+    """accp_lattice_set_tuning: calls that name no launch shape take the
+    lattice's.  With the specialized kernel as the lattice's default, plain
+    tracking equals an explicit specialized call bit for bit; trackDA (which
+    tracks internally) keeps the oracle's survival of every ray point except
+    next to the boundary; the adjoint, whose forward pass then runs on the
+    specialized kernel, gives the walker's gradient to round-off; resetting
+    the default restores the walker's results bit for bit."""
+    da = lat.da_ring(0)
+    L = gpu_lib.lattice(da)
+    beam = lat.gaussian_beam(500, seed=3)
+    walker = L.track(beam, 5)
+    L.set_tuning(spec())
+    assert same_bits(L.track(beam, 5), L.track(beam, 5, tuning=spec()))
+    area_s, rx_s, ry_s, lt_s = L.trackda(0.02, 0.001, 0.01, 64, 11, 10, loss_turn=True)
+    L.set_tuning(None)
+    assert same_bits(L.track(beam, 5), walker)
+    area_w, rx_w, ry_w, lt_w = L.trackda(0.02, 0.001, 0.01, 64, 11, 10, loss_turn=True)
+    print("trackDA area: specialized", area_s, "walker", area_w, "loss turns differ at", int((lt_s != lt_w).sum()))
+    assert abs(area_s - area_w) <= 0.05 * area_w and (lt_s != lt_w).sum() <= 3
+
+    ring = lat.da_ring(0, extra_quads=True)
+    lat.set_rad(ring, 0)
+    lat.set_cavity(ring, 1)
+    knobs = [(f, i, m) for _, f, i, m in lat.adjoint_knobs(ring)]
+    b4 = lat.gaussian_beam(512, seed=1) * np.array([0.5, 0.5, 0.5, 0.5, 0.25, 0.25])
+    La = gpu_lib.lattice(ring)
+    grads = []
+    for t in (None, spec()):
+        La.set_tuning(t)
+        A = abi.Adjoint(gpu_lib, La, knobs)
+        out = A.forward(b4, 64, ckpt_every=16)
+        g, dr0 = A.backward(np.where(np.isnan(out[:, :1]), 0.0, 2.0 * np.nan_to_num(out)))
+        grads.append((g, dr0))
+        A.free()
+    (gw, dw), (gs, ds) = grads
+    print("adjoint gradient, specialized forward vs walker:", np.max(np.abs(gs - gw) / np.abs(gw)))
+    np.testing.assert_allclose(gs, gw, rtol=1e-9)
+    np.testing.assert_allclose(ds, dw, rtol=1e-9, atol=1e-12 * np.abs(dw).max())
+
+
 def test_specialized_refusals(gpu_lib):
     """Calls the specialized kernel does not take fail with an error and
     change nothing: the no-allocation entry point (it would load a code
