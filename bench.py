@@ -571,9 +571,12 @@ def diffusion_leg(lib, nx=256, ny=128, nturn=1024, cpu_sample=32):
     points (x0 to 8 mm, y0 to 4 mm), 1024 turns, NAFF of both halves of
     each plane; the whole call (one tracking launch sequence recording every
     turn, one NAFF launch).  The reference's own trackDiffusion, one particle
-    after another on one core, on the first cpu_sample points of the grid."""
+    after another on one core, on the first cpu_sample points of the grid.
+    The tracking runs on the kernel specialized for the lattice (its default
+    launch shape); the table walker's time is reported beside it."""
     import numpy as np
 
+    import pyaccp.abi as abi
     import pyaccp.elements as el
     import pyaccp.lattices as lat
 
@@ -589,10 +592,15 @@ def diffusion_leg(lib, nx=256, ny=128, nturn=1024, cpu_sample=32):
     L = lib.lattice(ring)
     X, Y = np.meshgrid(np.linspace(1e-4, 8e-3, nx), np.linspace(1e-4, 4e-3, ny))
     x0, y0 = X.ravel(), Y.ravel()
-    L.trackdiffusion(64, x0[:8], y0[:8], 0.05, 0.45, 0.05, 0.45)  # warmup
-    t0 = time.perf_counter()
-    tot, nux, nuy, dif = L.trackdiffusion(nturn, x0, y0, 0.05, 0.45, 0.05, 0.45)
-    gpu_s = time.perf_counter() - t0
+    walker_s = None
+    for tuning in (None, abi.Tuning(64, 1, abi.STAGE_SPECIALIZED, 0)):
+        L.set_tuning(tuning)
+        L.trackdiffusion(64, x0[:8], y0[:8], 0.05, 0.45, 0.05, 0.45)  # warmup, compilation
+        t0 = time.perf_counter()
+        tot, nux, nuy, dif = L.trackdiffusion(nturn, x0, y0, 0.05, 0.45, 0.05, 0.45)
+        gpu_s = time.perf_counter() - t0
+        if tuning is None:
+            walker_s = gpu_s
     live = int((~np.isnan(dif)).sum())
     sig = np.cos(2 * np.pi * np.outer(np.linspace(0.1, 0.4, 4 * live), np.arange(nturn // 2)))
     t0 = time.perf_counter()
@@ -603,6 +611,7 @@ def diffusion_leg(lib, nx=256, ny=128, nturn=1024, cpu_sample=32):
                     f"({len(ring)} elements), {live} of {nx * ny} points survive",
         "ms": round(gpu_s * 1e3, 3),
         "points_per_s": nx * ny / gpu_s,
+        "table_walker_ms": round(walker_s * 1e3, 3),
         "naff_ms": round(naff_s * 1e3, 3),
         "naff_signals": 4 * live,
         "naff_samples": nturn // 2,
