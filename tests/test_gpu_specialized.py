@@ -176,8 +176,9 @@ def test_lattice_default_tuning(gpu_lib, oracle):
     lattice's.  With the specialized kernel as the lattice's default, plain
     tracking equals an explicit specialized call bit for bit; trackDA (which
     tracks internally) keeps the oracle's survival of every ray point except
-    next to the boundary; the adjoint, whose forward pass then runs on the
-    specialized kernel, gives the walker's gradient to round-off; resetting
+    next to the boundary; the adjoint, whose forward and recomputation
+    passes then run on the specialized kernel, gives the walker's gradient
+    to round-off in both reverse forms; resetting
     the default restores the walker's results bit for bit."""
     da = lat.da_ring(0)
     L = gpu_lib.lattice(da)
@@ -198,18 +199,21 @@ def test_lattice_default_tuning(gpu_lib, oracle):
     knobs = [(f, i, m) for _, f, i, m in lat.adjoint_knobs(ring)]
     b4 = lat.gaussian_beam(512, seed=1) * np.array([0.5, 0.5, 0.5, 0.5, 0.25, 0.25])
     La = gpu_lib.lattice(ring)
-    grads = []
-    for t in (None, spec()):
-        La.set_tuning(t)
-        A = abi.Adjoint(gpu_lib, La, knobs)
-        out = A.forward(b4, 64, ckpt_every=16)
-        g, dr0 = A.backward(np.where(np.isnan(out[:, :1]), 0.0, 2.0 * np.nan_to_num(out)))
-        grads.append((g, dr0))
-        A.free()
-    (gw, dw), (gs, ds) = grads
-    print("adjoint gradient, specialized forward vs walker:", np.max(np.abs(gs - gw) / np.abs(gw)))
-    np.testing.assert_allclose(gs, gw, rtol=1e-9)
-    np.testing.assert_allclose(ds, dw, rtol=1e-9, atol=1e-12 * np.abs(dw).max())
+    for mode in (abi.Adjoint.REVERSE_SEQUENTIAL, abi.Adjoint.REVERSE_SEGMENT_JACOBIAN):
+        grads = []
+        for t in (None, spec()):
+            La.set_tuning(t)
+            A = abi.Adjoint(gpu_lib, La, knobs)
+            A.set_reverse(mode)
+            out = A.forward(b4, 64, ckpt_every=16)
+            g, dr0 = A.backward(np.where(np.isnan(out[:, :1]), 0.0, 2.0 * np.nan_to_num(out)))
+            grads.append((g, dr0))
+            A.free()
+        (gw, dw), (gs, ds) = grads
+        print(f"adjoint (reverse mode {mode}), specialized forward vs the walker's:",
+              np.max(np.abs(gs - gw) / np.abs(gw)))
+        np.testing.assert_allclose(gs, gw, rtol=1e-9)
+        np.testing.assert_allclose(ds, dw, rtol=1e-9, atol=1e-12 * np.abs(dw).max())
 
 
 def test_specialized_refusals(gpu_lib):
