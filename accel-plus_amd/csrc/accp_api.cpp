@@ -390,6 +390,8 @@ void build_groups(Compiled &c) {
       const int pb = c.hdr[i].b;
       flags |= FUSE_POST | ((int)delta << FUSE_POST_DELTA_SHIFT) | ((pb & 0xff) << FUSE_POST_HI_SHIFT) |
                ((pb & STRMP_SINGLE) ? FUSE_POST_SINGLE : 0);
+      const double *K = c.par.data() + c.hdr[i].off;  // [B0..B9, A0..A9] when pb == 9
+      if (pb == 9 && K[0] == 0 && K[1] == 0 && K[10] == 0 && K[11] == 0) flags |= FUSE_POST_LO2;
       c.fpar.insert(c.fpar.end(), c.par.begin() + c.hdr[i].off, c.par.begin() + c.hdr[i].off + len(i));
       ++i;
     }

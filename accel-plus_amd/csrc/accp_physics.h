@@ -282,38 +282,45 @@ ACCP_HD void hxdrift_c(Part &p, const HxC &c, double h, double c4) {
 
 // ATbendhxdrift6 in a bend whose kicks leave py alone (no field above the
 // dipole term, no curvature gradients): py is fixed through the slices, so
-// its terms are per-particle constants of the element, and the update
-// regroups as
-//   x  += px (c1 (1 + h x) + c2 px) - c2 py^2
+// its terms are per-particle constants of the element.  With
+// g = c1 (1 + h x) (the reference's a*i1*L) the update regroups as
+//   x  += px (g + c2 px) - c2 py^2
 //   px -= c3 (px^2 + py^2)
-//   y  += (1 + h x) (c1 py + c1 c4 py px)
-//   z  -= c5 (1 + h x) (px^2 + py^2)
-// with c1..c5 as in HxC: the reference's terms in another order, which
-// changes round-off only, in fewer dependent operations.
-struct HxPy {
-  double c1, c1h, c2, c3, c5, p1, p1c4, c2py2;
+//   y  += g (py + c4 py px)
+//   z  -= (1/(2(1+delta))) g (px^2 + py^2)
+// with c1..c4 as in HxC.  Neither the pseudo-drifts nor the kicks read z,
+// so its increments are summed in w (in units of 2(1+delta), where each
+// pseudo-drift adds g s in one fused operation) and added to z once at the
+// end of the element (dipole_close).  The reference's terms in another
+// order: round-off differences only, 9 operations per pseudo-drift where
+// the reference's order takes 11.
+struct HxW {
+  double c1, c1h, c2, c3, c2py2, pyc4;
 };
-ACCP_HD HxPy hx_consts_py(double ipd, double L, double hs, double hsl4, double py, double h) {
-  const double c1 = ipd * L, c2 = hsl4 * ipd * ipd, p1 = py * c1;
-  return HxPy{c1, c1 * h, c2, hs * 0.5 * ipd, ipd * ipd * L * 0.5, p1, p1 * (hs * 0.5), c2 * (py * py)};
+ACCP_HD HxW hx_consts_w(double ipd, double L, double hs, double hsl4, double py, double h) {
+  const double c1 = ipd * L, c2 = hsl4 * ipd * ipd;
+  return HxW{c1, c1 * h, c2, hs * 0.5 * ipd, c2 * (py * py), py * (hs * 0.5)};
 }
-ACCP_HD void hxdrift_py(Part &p, const HxPy &c, double h, double py2) {
+ACCP_HD void hxdrift_w(Part &p, const HxW &c, double py, double py2, double &w) {
   const double x = p.x, px = p.px;
-  const double a = fma(h, x, 1.0);
+  const double g = fma(c.c1h, x, c.c1);
   const double s = fma(px, px, py2);
-  p.x = fma(px, fma(c.c2, px, fma(c.c1h, x, c.c1)), x - c.c2py2);
+  p.x = fma(px, fma(c.c2, px, g), x - c.c2py2);
   p.px = fma(-c.c3, s, px);
-  p.y = fma(a, fma(c.p1c4, px, c.p1), p.y);
-  p.z = fma(-(c.c5 * a), s, p.z);
+  p.y = fma(g, fma(c.pyc4, px, py), p.y);
+  w = fma(g, s, w);
 }
 // bndthinkick with PolyB[0] alone and delta fixed: px -= L (B0 + h h x) -
 // L h delta, as px + q - (L h h) x with q = L h delta - L B0 per particle
-// and element; z -= L h x (atpass.cpp:229-233).
-ACCP_HD void bend_kick_dipole(Part &p, double lhh, double q, double lh) {
+// and element; z -= L h x, as w += lw x with lw = 2 (1+delta) L h
+// (atpass.cpp:229-233).
+ACCP_HD void bend_kick_w(Part &p, double lhh, double q, double lw, double &w) {
   const double x = p.x;
   p.px = fma(-lhh, x, p.px + q);
-  p.z = fma(-lh, x, p.z);
+  w = fma(lw, x, w);
 }
+// z after the element: z - w / (2 (1+delta)).
+ACCP_HD void dipole_close(Part &p, double w) { p.z = fma(w, -0.5 * p.ipd, p.z); }
 // The kick of a pure sextupole term b (x + i y)^2: px -= k b (x^2 - y^2),
 // py += 2 k b x y, with kb = k b and kb2 = 2 k b (the reference's Horner sum
 // in another order: round-off differences only).
@@ -389,6 +396,26 @@ ACCP_HD void estrin9(const double (&b)[10], const double (&a)[10], double x, dou
   const double g0r = fma(f1r, z4r, fma(-f1i, z4i, f0r)), g0i = fma(f1r, z4i, fma(f1i, z4r, f0i));
   re = fma(er[4], z8r, fma(-ei[4], z8i, g0r));
   im = fma(er[4], z8i, fma(ei[4], z8r, g0i));
+}
+// The order-9 sum with c_0 = c_1 = 0 (c_n = b_n + i a_n):
+// z^2 sum_{m<=7} c_{m+2} z^m, the inner sum by Estrin's scheme (pairs,
+// then z^2 and z^4), then one complex product by z^2: 40 operations where
+// estrin9 takes 48 (the terms in another order: round-off differences only).
+ACCP_HD void estrin9_lo2(const double (&b)[10], const double (&a)[10], double x, double y, double &re,
+                         double &im) {
+  double er[4], ei[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    er[k] = fma(b[2 * k + 3], x, fma(-a[2 * k + 3], y, b[2 * k + 2]));
+    ei[k] = fma(b[2 * k + 3], y, fma(a[2 * k + 3], x, a[2 * k + 2]));
+  }
+  const double z2r = fma(x, x, -(y * y)), z2i = (x + x) * y;
+  const double z4r = fma(z2r, z2r, -(z2i * z2i)), z4i = (z2r + z2r) * z2i;
+  const double f0r = fma(er[1], z2r, fma(-ei[1], z2i, er[0])), f0i = fma(er[1], z2i, fma(ei[1], z2r, ei[0]));
+  const double f1r = fma(er[3], z2r, fma(-ei[3], z2i, er[2])), f1i = fma(er[3], z2i, fma(ei[3], z2r, ei[2]));
+  const double gr = fma(f1r, z4r, fma(-f1i, z4i, f0r)), gi = fma(f1r, z4i, fma(f1i, z4r, f0i));
+  re = fma(gr, z2r, -(gi * z2i));
+  im = fma(gr, z2i, gi * z2r);
 }
 // b (x+iy)^HI: horner_reg on the coefficients {0, ..., 0, b} and A = 0 with
 // the additions of those zeros and the first step's products by the zero

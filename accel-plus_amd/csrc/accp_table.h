@@ -59,6 +59,10 @@ constexpr int FUSE_POST_DELTA_SHIFT = 10;
 constexpr int FUSE_POST_MAX_DELTA = 1023;
 constexpr int FUSE_POST_HI_SHIFT = 20;
 constexpr int32_t FUSE_POST_SINGLE = 1 << 24;
+// The post kick is of order 9 with PolyB[0], PolyB[1], PolyA[0], PolyA[1]
+// zero (a field-error record of b3..b10, a3..a10): its sum is z^2 times a
+// polynomial of degree 7 (estrin9_lo2).
+constexpr int32_t FUSE_POST_LO2 = 1 << 25;
 
 // BendMPole header flags beside hi | plain << 8: an entry or exit edge with
 // non-zero focusing (psi1, psi2 != 0).

@@ -155,34 +155,40 @@ __device__ __forceinline__ void bendmp_fixed(Part (&p)[PPL], const Hdr h, const 
   const double c4a = hs1 * 0.5, c4b = hs2 * 0.5;
   const double lh1 = k1 * h0, lh2 = k2 * h0;
   if constexpr (PLAIN && MO == 0) {
-    // a pure dipole: py stays fixed through the slices (hxdrift_py)
+    // a pure dipole: py stays fixed through the slices, z is summed in w
+    // (hxdrift_w, bend_kick_w)
     const double lhh1 = lh1 * h0, lhh2 = lh2 * h0;
-    HxPy fa[PPL], fb[PPL];
-    double qa[PPL], qb[PPL], py2[PPL];
+    HxW fa[PPL], fb[PPL];
+    double qa[PPL], qb[PPL], wa[PPL], wb[PPL], py[PPL], py2[PPL], w[PPL];
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
       if (h.b & BM_EDGE_IN) edge(p[k], psi1);
+      py[k] = p[k].py;
       py2[k] = p[k].py * p[k].py;
-      fa[k] = hx_consts_py(p[k].ipd, l1, hs1, hq1, p[k].py, h0);
-      fb[k] = hx_consts_py(p[k].ipd, l2, hs2, hq2, p[k].py, h0);
+      fa[k] = hx_consts_w(p[k].ipd, l1, hs1, hq1, p[k].py, h0);
+      fb[k] = hx_consts_w(p[k].ipd, l2, hs2, hq2, p[k].py, h0);
       qa[k] = lh1 * p[k].d - k1 * b[0];
       qb[k] = lh2 * p[k].d - k2 * b[0];
+      const double opd2 = 2.0 * (1.0 + p[k].d);
+      wa[k] = opd2 * lh1;
+      wb[k] = opd2 * lh2;
+      w[k] = 0.0;
     }
     auto slice = [&]() {
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) hxdrift_py(p[k], fa[k], h0, py2[k]);
+      for (int k = 0; k < PPL; ++k) hxdrift_w(p[k], fa[k], py[k], py2[k], w[k]);
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) bend_kick_dipole(p[k], lhh1, qa[k], lh1);
+      for (int k = 0; k < PPL; ++k) bend_kick_w(p[k], lhh1, qa[k], wa[k], w[k]);
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) hxdrift_py(p[k], fb[k], h0, py2[k]);
+      for (int k = 0; k < PPL; ++k) hxdrift_w(p[k], fb[k], py[k], py2[k], w[k]);
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) bend_kick_dipole(p[k], lhh2, qb[k], lh2);
+      for (int k = 0; k < PPL; ++k) bend_kick_w(p[k], lhh2, qb[k], wb[k], w[k]);
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) hxdrift_py(p[k], fb[k], h0, py2[k]);
+      for (int k = 0; k < PPL; ++k) hxdrift_w(p[k], fb[k], py[k], py2[k], w[k]);
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) bend_kick_dipole(p[k], lhh1, qa[k], lh1);
+      for (int k = 0; k < PPL; ++k) bend_kick_w(p[k], lhh1, qa[k], wa[k], w[k]);
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) hxdrift_py(p[k], fa[k], h0, py2[k]);
+      for (int k = 0; k < PPL; ++k) hxdrift_w(p[k], fa[k], py[k], py2[k], w[k]);
     };
     if (ns == 10) {  // the reference's default nslice (accel.h:195): straight-line code
 #pragma unroll
@@ -190,9 +196,10 @@ __device__ __forceinline__ void bendmp_fixed(Part (&p)[PPL], const Hdr h, const 
     } else {
       for (int m = 0; m < ns; ++m) slice();
     }
-    if (h.b & BM_EDGE_OUT) {
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) edge(p[k], psi2);
+    for (int k = 0; k < PPL; ++k) {
+      dipole_close(p[k], w[k]);
+      if (h.b & BM_EDGE_OUT) edge(p[k], psi2);
     }
     return;
   }
@@ -312,6 +319,26 @@ __device__ __forceinline__ void strmp_empty(Part (&p)[PPL], const Hdr h, const P
       drift_c(p[k], n2, z2);
       drift_c(p[k], n1, z1);
     }
+  }
+}
+
+// A thin order-9 StrMPole kick with PolyB[0..1] and PolyA[0..1] zero
+// (FUSE_POST_LO2): one integrated kick (accel.cpp:824-827), the sum by
+// estrin9_lo2.
+template <int PPL, class PT>
+__device__ __forceinline__ void strmp_thin9_lo2(Part (&p)[PPL], const PT *C) {
+  double b[10], a[10];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    b[i] = i < 2 ? 0.0 : (double)C[i];
+    a[i] = i < 2 ? 0.0 : (double)C[10 + i];
+  }
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) {
+    double re, im;
+    estrin9_lo2(b, a, p[k].x, p[k].y, re, im);
+    p[k].px -= re;
+    p[k].py += im;
   }
 }
 
@@ -558,7 +585,8 @@ __device__ __forceinline__ void apply_element(Part (&p)[PPL], const Hdr h, const
     if (h.op & FUSE_POST) {
       const PT *Pk = Pm + ((h.op >> FUSE_POST_DELTA_SHIFT) & FUSE_POST_MAX_DELTA);
       const int pb = ((h.op >> FUSE_POST_HI_SHIFT) & 0xf) | ((h.op & FUSE_POST_SINGLE) ? STRMP_SINGLE : 0);
-      if (pb == 9) strmp_fixed<PPL, 9, false, false>(p, Hdr{OP_STRMP_THIN, 0, 0, pb}, Pk);
+      if (pb == 9 && (h.op & FUSE_POST_LO2)) strmp_thin9_lo2<PPL>(p, Pk);
+      else if (pb == 9) strmp_fixed<PPL, 9, false, false>(p, Hdr{OP_STRMP_THIN, 0, 0, pb}, Pk);
       else strmp_dispatch<PPL, false>(p, Hdr{OP_STRMP_THIN, 0, 0, pb}, Pk);
     }
   }

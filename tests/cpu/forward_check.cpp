@@ -1,13 +1,14 @@
 // CPU check of the regrouped forward maps of the tracking kernel
 // (accel-plus_amd/csrc/accp_physics.h, compiled here for the host): the
-// dipole pseudo-drift and kick (hxdrift_py, bend_kick_dipole) and the pure
+// dipole pseudo-drift and kick (hxdrift_w, bend_kick_w, dipole_close) and the pure
 // sextupole kick (sext_kick) against the same maps written in the
 // reference's order (hxdrift_c and bend_kick_c<0, true>, which follow
 // atpass.cpp:199-235, 343-355; horner_single<2>), a sextupole's 10 slices
 // with the drifts between slices merged (sext_slices) against the slice
 // sequence of accel.cpp:829-843, and the order-9 multipole
-// sum by Estrin's scheme (estrin9) against Horner's (horner_reg<9>, the
-// order of strthinkick, atpass.cpp:358-414).  They must agree to
+// sum by Estrin's scheme (estrin9, and estrin9_lo2 when orders 0 and 1 are
+// zero) against Horner's (horner_reg<9>, the order of strthinkick,
+// atpass.cpp:358-414).  They must agree to
 // round-off over random states; a dropped term or a wrong sign does not.
 // Prints one line per check and exits non-zero on a failure.
 #include <cmath>
@@ -61,18 +62,21 @@ int main() {
         bend_kick_c<0, true>(p, b, k1, h, lh1 * p.d, lh1, 0, 0);
         hxdrift_c(p, ca, h, hs1 * 0.5);
       }
-      // regrouped
-      const HxPy fa = hx_consts_py(q.ipd, l1, hs1, hq1, q.py, h), fb = hx_consts_py(q.ipd, l2, hs2, hq2, q.py, h);
-      const double py2 = q.py * q.py, qa = lh1 * q.d - k1 * b0, qb = lh2 * q.d - k2 * b0;
+      // regrouped, z summed in w
+      const HxW fa = hx_consts_w(q.ipd, l1, hs1, hq1, q.py, h), fb = hx_consts_w(q.ipd, l2, hs2, hq2, q.py, h);
+      const double py = q.py, py2 = q.py * q.py, qa = lh1 * q.d - k1 * b0, qb = lh2 * q.d - k2 * b0;
+      const double wa = 2.0 * (1.0 + q.d) * lh1, wb = 2.0 * (1.0 + q.d) * lh2;
+      double w = 0.0;
       for (int m = 0; m < 10; ++m) {
-        hxdrift_py(q, fa, h, py2);
-        bend_kick_dipole(q, lh1 * h, qa, lh1);
-        hxdrift_py(q, fb, h, py2);
-        bend_kick_dipole(q, lh2 * h, qb, lh2);
-        hxdrift_py(q, fb, h, py2);
-        bend_kick_dipole(q, lh1 * h, qa, lh1);
-        hxdrift_py(q, fa, h, py2);
+        hxdrift_w(q, fa, py, py2, w);
+        bend_kick_w(q, lh1 * h, qa, wa, w);
+        hxdrift_w(q, fb, py, py2, w);
+        bend_kick_w(q, lh2 * h, qb, wb, w);
+        hxdrift_w(q, fb, py, py2, w);
+        bend_kick_w(q, lh1 * h, qa, wa, w);
+        hxdrift_w(q, fa, py, py2, w);
       }
+      dipole_close(q, w);
       worst = std::fmax(worst, rel(p, q, sc));
     }
   }
@@ -144,6 +148,27 @@ int main() {
     worst = std::fmax(worst, std::fmax(std::fabs(hr - er), std::fabs(hi - ei)) / scale);
   }
   std::printf("order-9 multipole sum, Estrin vs Horner, 20000 cases: max difference %.2e of the largest term\n",
+              worst);
+  if (!(worst < 1e-14)) fail = 1;
+  // the same sum with orders 0 and 1 zero (a field-error record), by
+  // estrin9_lo2
+  worst = 0;
+  for (int it = 0; it < 20000; ++it) {
+    double b[10], a[10];
+    for (int i = 0; i < 10; ++i) {
+      b[i] = i < 2 ? 0.0 : U(-1, 1) * std::pow(10.0, 2 * i - 3);
+      a[i] = i < 2 ? 0.0 : U(-1, 1) * std::pow(10.0, 2 * i - 3);
+    }
+    const double x = U(-0.02, 0.02), y = U(-0.01, 0.01);
+    double hr, hi, er, ei;
+    horner_reg<9>(b, a, x, y, hr, hi);
+    estrin9_lo2(b, a, x, y, er, ei);
+    double scale = 0, zn = 1;
+    for (int i = 0; i < 10; ++i, zn *= std::hypot(x, y)) scale = std::fmax(scale, std::hypot(b[i], a[i]) * zn);
+    worst = std::fmax(worst, std::fmax(std::fabs(hr - er), std::fabs(hi - ei)) / scale);
+  }
+  std::printf("order-9 sum without orders 0 and 1, estrin9_lo2 vs Horner, 20000 cases: max difference %.2e of "
+              "the largest term\n",
               worst);
   if (!(worst < 1e-14)) fail = 1;
   std::printf(fail ? "FAILED\n" : "all regrouped maps ok\n");
