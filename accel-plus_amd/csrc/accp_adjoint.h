@@ -26,6 +26,13 @@
 // and of delta itself; given the angles after the loss, the delta before it
 // is the root near 1 + delta of a quadratic, so these inverses stay in
 // closed form too.
+//
+// The maps of the lean sweep (drift, edge, dipole pseudo-drift and kick,
+// multipole kick, series quad, K = 0 quad, cavity, corrector) carry NV
+// adjoint vectors at once (the *_n<NV> forms): the inverse and the partial
+// derivatives are formed once, then applied to each vector; knob
+// derivatives go through ga(vector, slot, value).  The plain names are the
+// one-vector forms.
 #pragma once
 #include "accp_physics.h"
 
@@ -33,6 +40,14 @@ namespace accp {
 
 struct Adj {
   double x, px, y, py, z, d;
+};
+
+// One-vector adapters: a single adjoint as an array of one, and a knob
+// gradient array as an accumulator.
+ACCP_HD Adj (&as_vec(Adj &l))[1] { return *reinterpret_cast<Adj(*)[1]>(&l); }
+struct GradAcc1 {
+  double *g;
+  ACCP_HD void operator()(int, int slot, double v) const { g[slot] += v; }
 };
 
 // Knob of one element: which gradient slot it feeds and what it moves.
@@ -45,17 +60,22 @@ struct Knob {
 
 // ---- drift with delta fixed: drift_c(p, nl, zc), nl = L/(1+d),
 // zc = nl/(2(1+d)) ------------------------------------------------------------
-ACCP_HD void drift_c_rev(Part &p, Adj &l, double nl, double zc) {
+template <int NV>
+ACCP_HD void drift_c_rev_n(Part &p, Adj (&l)[NV], double nl, double zc) {
   const double px = p.px, py = p.py, s = px * px + py * py;
   p.x -= nl * px;
   p.y -= nl * py;
   p.z += zc * s;
   const double ipd = p.ipd;
-  const double dnl = -nl * ipd, dzc = -2.0 * zc * ipd;
-  l.d += (l.x * px + l.y * py) * dnl - l.z * s * dzc;
-  l.px += l.x * nl - 2.0 * l.z * zc * px;
-  l.py += l.y * nl - 2.0 * l.z * zc * py;
+  const double dnl = -nl * ipd, zs = -2.0 * zc * ipd * s, zpx = 2.0 * zc * px, zpy = 2.0 * zc * py;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    l[v].d += (l[v].x * px + l[v].y * py) * dnl - l[v].z * zs;
+    l[v].px += l[v].x * nl - l[v].z * zpx;
+    l[v].py += l[v].y * nl - l[v].z * zpy;
+  }
 }
+ACCP_HD void drift_c_rev(Part &p, Adj &l, double nl, double zc) { drift_c_rev_n<1>(p, as_vec(l), nl, zc); }
 
 // ---- complex polynomial P(w) = sum (b_n + i a_n) w^n and P'(w) ------------
 template <int HI>
@@ -89,43 +109,68 @@ ACCP_HD void cpow(double x, double y, int j, double &wr, double &wi) {
 }
 
 // strthinkick with kick length K: px -= K Re P, py += K Im P.
-template <int HI>
-ACCP_HD void mpole_kick_rev(Part &p, Adj &l, const double (&b)[HI + 1], const double (&a)[HI + 1],
-                            double K, const Knob &kn, double *grad) {
+template <int NV, int HI, class GA>
+ACCP_HD void mpole_kick_rev_n(Part &p, Adj (&l)[NV], const double (&b)[HI + 1], const double (&a)[HI + 1],
+                              double K, const Knob &kn, const GA &ga) {
   const double x = p.x, y = p.y;
   double re, im, dre, dim;
   horner_d<HI>(b, a, x, y, re, im, dre, dim);
   p.px += K * re;
   p.py -= K * im;
-  const double lpx = l.px, lpy = l.py;
-  l.x += K * (-lpx * dre + lpy * dim);
-  l.y += K * (lpx * dim + lpy * dre);
+  const double kr = K * dre, ki = K * dim;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double lpx = l[v].px, lpy = l[v].py;
+    l[v].x += -lpx * kr + lpy * ki;
+    l[v].y += lpx * ki + lpy * kr;
+  }
   if (kn.kind >= 2) {
     double wr, wi;
     cpow(x, y, kn.j, wr, wi);
-    grad[kn.slot] += kn.kind == 2 ? K * (-lpx * wr + lpy * wi) : K * (lpx * wi + lpy * wr);
+    // (cx, cy) = d(px, py)/d(knob): (-K wr, K wi) for PolyB[j], (K wi, K wr)
+    // for PolyA[j] (d P/d b_j = w^j, d P/d a_j = i w^j)
+    const double cx = kn.kind == 2 ? -K * wr : K * wi, cy = kn.kind == 2 ? K * wi : K * wr;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) ga(v, kn.slot, l[v].px * cx + l[v].py * cy);
   }
+}
+template <int HI>
+ACCP_HD void mpole_kick_rev(Part &p, Adj &l, const double (&b)[HI + 1], const double (&a)[HI + 1],
+                            double K, const Knob &kn, double *grad) {
+  mpole_kick_rev_n<1, HI>(p, as_vec(l), b, a, K, kn, GradAcc1{grad});
 }
 
 // mpole kick when the element's field is empty but a knob sits on it: the
 // map is the identity, only the knob derivative survives.
-ACCP_HD void mpole_knob_only(const Part &p, const Adj &l, double K, const Knob &kn, double *grad) {
+template <int NV, class GA>
+ACCP_HD void mpole_knob_only_n(const Part &p, const Adj (&l)[NV], double K, const Knob &kn, const GA &ga) {
   if (kn.kind < 2) return;
   double wr, wi;
   cpow(p.x, p.y, kn.j, wr, wi);
-  grad[kn.slot] += kn.kind == 2 ? K * (-l.px * wr + l.py * wi) : K * (l.px * wi + l.py * wr);
+  const double cx = kn.kind == 2 ? -K * wr : K * wi, cy = kn.kind == 2 ? K * wi : K * wr;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) ga(v, kn.slot, l[v].px * cx + l[v].py * cy);
+}
+ACCP_HD void mpole_knob_only(const Part &p, const Adj &l, double K, const Knob &kn, double *grad) {
+  mpole_knob_only_n<1>(p, *reinterpret_cast<const Adj(*)[1]>(&l), K, kn, GradAcc1{grad});
 }
 
 // ---- hard-edge fringe ---------------------------------------------------------
-ACCP_HD void edge_rev(Part &p, Adj &l, double psi) {
+template <int NV>
+ACCP_HD void edge_rev_n(Part &p, Adj (&l)[NV], double psi) {
   p.px -= p.x * psi;
   p.py += p.y * psi;
-  l.x += l.px * psi;
-  l.y -= l.py * psi;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    l[v].x += l[v].px * psi;
+    l[v].y -= l[v].py * psi;
+  }
 }
+ACCP_HD void edge_rev(Part &p, Adj &l, double psi) { edge_rev_n<1>(p, as_vec(l), psi); }
 
 // ---- ATbendhxdrift6 with delta fixed (hxdrift_c) -------------------------------
-ACCP_HD void hxdrift_c_rev(Part &p, Adj &l, const HxC &c, double h, double c4) {
+template <int NV>
+ACCP_HD void hxdrift_c_rev_n(Part &p, Adj (&l)[NV], const HxC &c, double h, double c4) {
   const double i1 = p.ipd;
   // inverse: py unchanged; px solves px = q + c3 px^2 with q = px' + c3 py^2.
   // The fixed point contracts by 2 c3 |px| (about 1e-5 for a bend slice and
@@ -145,19 +190,29 @@ ACCP_HD void hxdrift_c_rev(Part &p, Adj &l, const HxC &c, double h, double c4) {
   const double f = 1.0 + px * c4;
   const double y = p.y - a * py * c.c1 * f;
   const double z = p.z + a * c.c5 * s;
-  // VJP at the input state
-  const Adj o = l;
-  l.x = o.x * (1.0 + hpc1) + o.y * h * py * c.c1 * f - o.z * h * c.c5 * s;
-  l.px = o.x * (a * c.c1 + 2.0 * c.c2 * px) + o.px * (1.0 - 2.0 * c.c3 * px) + o.y * a * py * c.c1 * c4 -
-         o.z * 2.0 * a * c.c5 * px;
-  l.py = -o.x * 2.0 * c.c2 * py - o.px * 2.0 * c.c3 * py + o.py + o.y * a * c.c1 * f -
-         o.z * 2.0 * a * c.c5 * py;
-  l.d = o.d + (-o.x * (a * px * c.c1 + 2.0 * c.c2 * d2) + o.px * s * c.c3 - o.y * a * py * f * c.c1 +
-               o.z * 2.0 * a * c.c5 * s) * i1;
+  // VJP at the input state: the partial derivatives of (x, px, y, z)
+  // after the step with respect to (x, px, py, delta) before it
+  const double ac1 = a * c.c1, a2c5 = 2.0 * a * c.c5;
+  const double xx = 1.0 + hpc1, yx = h * py * c.c1 * f, zx = -h * c.c5 * s;
+  const double xpx = ac1 + 2.0 * c.c2 * px, pxpx = 1.0 - 2.0 * c.c3 * px, ypx = ac1 * py * c4, zpx = -a2c5 * px;
+  const double xpy = -2.0 * c.c2 * py, pxpy = -2.0 * c.c3 * py, ypy = ac1 * f, zpy = -a2c5 * py;
+  const double xd = -(ac1 * px + 2.0 * c.c2 * d2) * i1, pxd = s * c.c3 * i1, yd = -ac1 * py * f * i1,
+               zd = a2c5 * s * i1;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const Adj o = l[v];
+    l[v].x = o.x * xx + o.y * yx + o.z * zx;
+    l[v].px = o.x * xpx + o.px * pxpx + o.y * ypx + o.z * zpx;
+    l[v].py = o.x * xpy + o.px * pxpy + o.py + o.y * ypy + o.z * zpy;
+    l[v].d = o.d + o.x * xd + o.px * pxd + o.y * yd + o.z * zd;
+  }
   p.x = x;
   p.px = px;
   p.y = y;
   p.z = z;
+}
+ACCP_HD void hxdrift_c_rev(Part &p, Adj &l, const HxC &c, double h, double c4) {
+  hxdrift_c_rev_n<1>(p, as_vec(l), c, h, c4);
 }
 
 // ---- bndthinkick, normal multipoles only, delta fixed (bend_kick_c<MO,PLAIN>) ----
@@ -167,9 +222,9 @@ ACCP_HD void hxdrift_c_rev(Part &p, Adj &l, const HxC &c, double h, double c4) {
 // 4/3 k2c x^2 y + c3 y^3), c3 = h/6 k1c - k2c/3; their x and y derivatives
 // join those of the multipole sum (Cauchy-Riemann: d re/dy = -Im P',
 // d im/dx = Im P').
-template <int MO, bool PLAIN>
-ACCP_HD void bend_kick_rev(Part &p, Adj &l, const double (&b)[MO + 1], double L, double h,
-                           double lhd, double lh, double k1c = 0.0, double k2c = 0.0) {
+template <int NV, int MO, bool PLAIN>
+ACCP_HD void bend_kick_rev_n(Part &p, Adj (&l)[NV], const double (&b)[MO + 1], double L, double h,
+                             double lhd, double lh, double k1c = 0.0, double k2c = 0.0) {
   const double x = p.x, y = p.y;
   double a0[MO + 1];
 #pragma unroll
@@ -180,10 +235,14 @@ ACCP_HD void bend_kick_rev(Part &p, Adj &l, const double (&b)[MO + 1], double L,
     p.px += L * (re + h * (h * x)) - lhd;
     if (MO > 0) p.py -= L * im;
     p.z += lh * x;
-    const double lpx = l.px, lpy = MO > 0 ? l.py : 0.0;
-    l.x += -lpx * L * (dre + h * h) + lpy * L * dim - l.z * lh;
-    if (MO > 0) l.y += lpx * L * dim + lpy * L * dre;
-    l.d += lpx * lh;
+    const double cxx = -L * (dre + h * h), cxy = L * dim, cyy = L * dre;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double lpx = l[v].px, lpy = MO > 0 ? l[v].py : 0.0;
+      l[v].x += lpx * cxx + lpy * cxy - l[v].z * lh;
+      if (MO > 0) l[v].y += lpx * cxy + lpy * cyy;
+      l[v].d += lpx * lh;
+    }
   } else {
     const double c3 = h / 6.0 * k1c - k2c / 3.0;
     p.px += L * (re + h * (h * x + k1c * (x * x - 0.5 * y * y) + k2c * (x * x * x - (4.0 / 3.0) * x * y * y))) -
@@ -194,11 +253,20 @@ ACCP_HD void bend_kick_rev(Part &p, Adj &l, const double (&b)[MO + 1], double L,
     const double fy = -dim - h * (k1c * y + (8.0 / 3.0) * k2c * x * y);
     const double gx = dim + h * (k1c * y + (8.0 / 3.0) * k2c * x * y);
     const double gy = dre + h * (k1c * x + (4.0 / 3.0) * k2c * x * x + 3.0 * c3 * y * y);
-    const double lpx = l.px, lpy = l.py;
-    l.x += L * (-lpx * fx + lpy * gx) - l.z * lh;
-    l.y += L * (-lpx * fy + lpy * gy);
-    l.d += lpx * lh;
+    const double lfx = -L * fx, lgx = L * gx, lfy = -L * fy, lgy = L * gy;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double lpx = l[v].px, lpy = l[v].py;
+      l[v].x += lpx * lfx + lpy * lgx - l[v].z * lh;
+      l[v].y += lpx * lfy + lpy * lgy;
+      l[v].d += lpx * lh;
+    }
   }
+}
+template <int MO, bool PLAIN>
+ACCP_HD void bend_kick_rev(Part &p, Adj &l, const double (&b)[MO + 1], double L, double h,
+                           double lhd, double lh, double k1c = 0.0, double k2c = 0.0) {
+  bend_kick_rev_n<1, MO, PLAIN>(p, as_vec(l), b, L, h, lhd, lh, k1c, k2c);
 }
 
 // ---- quad6 from the power series, with the K derivative ----------------------
@@ -222,8 +290,10 @@ ACCP_HD void ds_series(const PT *C, double u, double &sd, double &shd) {
 // quad6 reverse map from the functions of u = L^2 g, g = |K|/(1+delta):
 // c = cos sqrt u, ch = cosh sqrt u, s = sin sqrt u / sqrt u,
 // sh = sinh sqrt u / sqrt u and sd, shd = d s/du, d sh/du.
-ACCP_HD void quad_rev_core(Part &p, Adj &l, double c, double ch, double s, double sh, double sd, double shd,
-                           double L, double L2, bool foc, double g, double sK, const Knob &kn, double *grad) {
+template <int NV, class GA>
+ACCP_HD void quad_rev_core_n(Part &p, Adj (&l)[NV], double c, double ch, double s, double sh, double sd,
+                             double shd, double L, double L2, bool foc, double g, double sK, const Knob &kn,
+                             const GA &ga) {
   const double ipd = p.ipd, opd = 1.0 + p.d;
   // matrix and d/dg of its elements
   double mhd, m12, m21, mvd, m34, m43, mhd_g, m12_g, m21_g, mvd_g, m34_g, m43_g;
@@ -267,16 +337,18 @@ ACCP_HD void quad_rev_core(Part &p, Adj &l, double c, double ch, double s, doubl
   const double Y_d = Y_g * g_d + m34 * yp_d;
   const double PY_d = (m43 * y + mvd * yp) + opd * (PYg / opd * g_d + mvd * yp_d);
   const double Z_d = -T_g * g_d + Zxp * xp_d + Zyp * yp_d;
-  const Adj o = l;
-  l.x = o.x * mhd + o.px * opd * m21 + o.z * Zx;
-  l.px = (o.x * m12 + o.px * opd * mhd + o.z * Zxp) * ipd;
-  l.y = o.y * mvd + o.py * opd * m43 + o.z * Zy;
-  l.py = (o.y * m34 + o.py * opd * mvd + o.z * Zyp) * ipd;
-  l.z = o.z;
-  l.d = o.d + o.x * X_d + o.px * PX_d + o.y * Y_d + o.py * PY_d + o.z * Z_d;
-  if (kn.kind == 1) {
-    const double g_k = sK * ipd;  // dg/dK
-    grad[kn.slot] += (o.x * X_g + o.px * PXg + o.y * Y_g + o.py * PYg - o.z * T_g) * g_k;
+  const double om21 = opd * m21, omhd = opd * mhd, om43 = opd * m43, omvd = opd * mvd;
+  const double g_k = sK * ipd;  // dg/dK
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const Adj o = l[v];
+    l[v].x = o.x * mhd + o.px * om21 + o.z * Zx;
+    l[v].px = (o.x * m12 + o.px * omhd + o.z * Zxp) * ipd;
+    l[v].y = o.y * mvd + o.py * om43 + o.z * Zy;
+    l[v].py = (o.y * m34 + o.py * omvd + o.z * Zyp) * ipd;
+    l[v].z = o.z;
+    l[v].d = o.d + o.x * X_d + o.px * PX_d + o.y * Y_d + o.py * PY_d + o.z * Z_d;
+    if (kn.kind == 1) ga(v, kn.slot, (o.x * X_g + o.px * PXg + o.y * Y_g + o.py * PYg - o.z * T_g) * g_k);
   }
   p.x = x;
   p.px = xp * opd;
@@ -285,15 +357,20 @@ ACCP_HD void quad_rev_core(Part &p, Adj &l, double c, double ch, double s, doubl
   p.z = z;
 }
 
-template <int NT, class PT>
-ACCP_HD void quad_series_rev(Part &p, Adj &l, const PT *C, double L, double L2, bool foc, double aK,
-                             double sK, const Knob &kn, double *grad) {
+template <int NV, int NT, class PT, class GA>
+ACCP_HD void quad_series_rev_n(Part &p, Adj (&l)[NV], const PT *C, double L, double L2, bool foc, double aK,
+                               double sK, const Knob &kn, const GA &ga) {
   const double g = aK * p.ipd;
   const double u = L2 * g;
   double c, ch, s, sh, sd, shd;
   cs_series<NT>(C, u, c, ch, s, sh);
   ds_series<NT>(C, u, sd, shd);
-  quad_rev_core(p, l, c, ch, s, sh, sd, shd, L, L2, foc, g, sK, kn, grad);
+  quad_rev_core_n<NV>(p, l, c, ch, s, sh, sd, shd, L, L2, foc, g, sK, kn, ga);
+}
+template <int NT, class PT>
+ACCP_HD void quad_series_rev(Part &p, Adj &l, const PT *C, double L, double L2, bool foc, double aK,
+                             double sK, const Knob &kn, double *grad) {
+  quad_series_rev_n<1, NT>(p, as_vec(l), C, L, L2, foc, aK, sK, kn, GradAcc1{grad});
 }
 
 // The trigonometric path (quad, accp_physics.h), for u beyond the series
@@ -307,26 +384,33 @@ ACCP_HD void quad_trig_rev(Part &p, Adj &l, double L, double L2, bool foc, doubl
   double sn, cs;
   sincos(lt, &sn, &cs);
   const double c = cs, ch = cosh(lt), s = sn / lt, sh = sinh(lt) / lt;
-  quad_rev_core(p, l, c, ch, s, sh, (c - s) / (2.0 * u), (ch - sh) / (2.0 * u), L, L2, foc, g, sK, kn, grad);
+  quad_rev_core_n<1>(p, as_vec(l), c, ch, s, sh, (c - s) / (2.0 * u), (ch - sh) / (2.0 * u), L, L2, foc, g, sK,
+                     kn, GradAcc1{grad});
 }
 
 
 // ---- quad6 with K == 0: the reference's drift() (atpass.cpp:6-13) --------------
 // x += px L, y += py L, then z -= pn^2 L (px + y^2)^2 / 2 with the updated y
 // (its SQR(r6[1]+SQR(r6[2])) quirk), delta fixed.
-ACCP_HD void drift_kzero_rev(Part &p, Adj &l, double L) {
+template <int NV>
+ACCP_HD void drift_kzero_rev_n(Part &p, Adj (&l)[NV], double L) {
   const double pn = p.ipd, px = p.px, py = p.py, y1 = p.y;
   const double w = px + y1 * y1, c = pn * pn * L * 0.5;
   p.x -= px * L;
   p.y -= py * L;
   p.z += c * (w * w);
-  const Adj o = l;
   const double dzw = -2.0 * c * w;  // dz'/dw
-  l.px = o.x * L + o.px + o.z * dzw;
-  l.y = o.y + o.z * dzw * 2.0 * y1;
-  l.py = o.y * L + o.py + o.z * dzw * 2.0 * y1 * L;
-  l.d = o.d + o.z * L * pn * pn * pn * (w * w);
+  const double zy = dzw * 2.0 * y1, zpy = zy * L, zd = L * pn * pn * pn * (w * w);
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const Adj o = l[v];
+    l[v].px = o.x * L + o.px + o.z * dzw;
+    l[v].y = o.y + o.z * zy;
+    l[v].py = o.y * L + o.py + o.z * zpy;
+    l[v].d = o.d + o.z * zd;
+  }
 }
+ACCP_HD void drift_kzero_rev(Part &p, Adj &l, double L) { drift_kzero_rev_n<1>(p, as_vec(l), L); }
 
 // ---- bend6: linear sector bend with gradient (ABend, atpass.cpp:76-187) ----
 // Its map mixes delta into every output through g1, g2 and the dispersion
@@ -479,11 +563,13 @@ ACCP_HD void bend_linear_rev(Part &p, Adj &l, double L, double kx, double grd) {
 }
 
 // ---- RF cavity (ACavity::pass) ---------------------------------------------------
-ACCP_HD void drift_nl_rev(Part &p, Adj &l, double L) {
+template <int NV>
+ACCP_HD void drift_nl_rev_n(Part &p, Adj (&l)[NV], double L) {
   // drift_nl(p, L*ipd): nl = L/(1+d), zc = nl/(2(1+d))
   const double nl = L * p.ipd;
-  drift_c_rev(p, l, nl, nl * (0.5 * p.ipd));
+  drift_c_rev_n<NV>(p, l, nl, nl * (0.5 * p.ipd));
 }
+ACCP_HD void drift_nl_rev(Part &p, Adj &l, double L) { drift_nl_rev_n<1>(p, as_vec(l), L); }
 
 // ---- AMatrix: 6x6 map between half drifts (accel.cpp:1697-1729) -------------
 // Inverse by Gaussian elimination with partial pivoting on M r_in = r_out;
@@ -551,23 +637,35 @@ ACCP_HD void matrix_rev(Part &p, Adj &l, double L, const PT *m) {
   if (L != 0) drift_nl_rev(p, l, 0.5 * L);
 }
 
-ACCP_HD void cavity_rev(Part &p, Adj &l, double L, double nv, double w_c0, double phis, bool on) {
-  if (L != 0) drift_nl_rev(p, l, 0.5 * L);
+template <int NV>
+ACCP_HD void cavity_rev_n(Part &p, Adj (&l)[NV], double L, double nv, double w_c0, double phis, bool on) {
+  if (L != 0) drift_nl_rev_n<NV>(p, l, 0.5 * L);
   if (on) {
     const double arg = w_c0 * p.z + phis;
     set_delta(p, p.d - nv * sin(arg));
-    l.z += l.d * nv * w_c0 * cos(arg);
+    const double dz = nv * w_c0 * cos(arg);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) l[v].z += l[v].d * dz;
   }
-  if (L != 0) drift_nl_rev(p, l, 0.5 * L);
+  if (L != 0) drift_nl_rev_n<NV>(p, l, 0.5 * L);
+}
+ACCP_HD void cavity_rev(Part &p, Adj &l, double L, double nv, double w_c0, double phis, bool on) {
+  cavity_rev_n<1>(p, as_vec(l), L, nv, w_c0, phis, on);
 }
 
 // ---- ACorrector::pass -------------------------------------------------------------
-ACCP_HD void corrector_rev(Part &p, Adj &l, double L, double kx, double ky) {
-  if (L != 0) drift_nl_rev(p, l, 0.5 * L);
+template <int NV>
+ACCP_HD void corrector_rev_n(Part &p, Adj (&l)[NV], double L, double kx, double ky) {
+  if (L != 0) drift_nl_rev_n<NV>(p, l, 0.5 * L);
   p.px -= kx * p.ipd;
   p.py -= ky * p.ipd;
-  l.d += -(l.px * kx + l.py * ky) * p.ipd * p.ipd;
-  if (L != 0) drift_nl_rev(p, l, 0.5 * L);
+  const double i2 = p.ipd * p.ipd;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) l[v].d += -(l[v].px * kx + l[v].py * ky) * i2;
+  if (L != 0) drift_nl_rev_n<NV>(p, l, 0.5 * L);
+}
+ACCP_HD void corrector_rev(Part &p, Adj &l, double L, double kx, double ky) {
+  corrector_rev_n<1>(p, as_vec(l), L, kx, ky);
 }
 
 // ---- bndthinkickrad, classical loss (bend_kick_rad with flag 1) ----------------

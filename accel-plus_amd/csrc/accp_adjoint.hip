@@ -17,8 +17,8 @@ namespace accp {
 // BendMPole without radiation, PolyB[0..MO] in registers.  The record holds
 // PolyB[0..mo] (mo = MaxNPole - 1 <= MO); the orders above it are zeros,
 // which add exact zeros to the Horner sums.
-template <int MO, bool PLAIN, class PT>
-__device__ __forceinline__ void bendmp_rev(Part &p, Adj &l, const Hdr h, const PT *P) {
+template <int NV, int MO, bool PLAIN, class PT>
+__device__ __forceinline__ void bendmp_rev(Part &p, Adj (&l)[NV], const Hdr h, const PT *P) {
   const int ns = h.a, mo = h.b & 0xff;
   const double psi1 = P[BM_PSI1], psi2 = P[BM_PSI2], h0 = P[BM_IRHO];
   const double l1 = P[BM_L1], l2 = P[BM_L2], k1 = P[BM_K1], k2 = P[BM_K2];
@@ -28,24 +28,24 @@ __device__ __forceinline__ void bendmp_rev(Part &p, Adj &l, const Hdr h, const P
 #pragma unroll
   for (int i = 0; i <= MO; ++i) b[i] = i <= mo ? (double)P[BM_B + i] : 0.0;
   const double c4a = hs1 * 0.5, c4b = hs2 * 0.5, lh1 = k1 * h0, lh2 = k2 * h0;
-  if (psi2 != 0) edge_rev(p, l, psi2);
+  if (psi2 != 0) edge_rev_n<NV>(p, l, psi2);
   const HxC ca = hx_consts(p.ipd, l1, hs1, hq1), cb = hx_consts(p.ipd, l2, hs2, hq2);
   const double lhd1 = lh1 * p.d, lhd2 = lh2 * p.d;
   for (int m = 0; m < ns; ++m) {
-    hxdrift_c_rev(p, l, ca, h0, c4a);
-    bend_kick_rev<MO, PLAIN>(p, l, b, k1, h0, lhd1, lh1, k1c, k2c);
-    hxdrift_c_rev(p, l, cb, h0, c4b);
-    bend_kick_rev<MO, PLAIN>(p, l, b, k2, h0, lhd2, lh2, k1c, k2c);
-    hxdrift_c_rev(p, l, cb, h0, c4b);
-    bend_kick_rev<MO, PLAIN>(p, l, b, k1, h0, lhd1, lh1, k1c, k2c);
-    hxdrift_c_rev(p, l, ca, h0, c4a);
+    hxdrift_c_rev_n<NV>(p, l, ca, h0, c4a);
+    bend_kick_rev_n<NV, MO, PLAIN>(p, l, b, k1, h0, lhd1, lh1, k1c, k2c);
+    hxdrift_c_rev_n<NV>(p, l, cb, h0, c4b);
+    bend_kick_rev_n<NV, MO, PLAIN>(p, l, b, k2, h0, lhd2, lh2, k1c, k2c);
+    hxdrift_c_rev_n<NV>(p, l, cb, h0, c4b);
+    bend_kick_rev_n<NV, MO, PLAIN>(p, l, b, k1, h0, lhd1, lh1, k1c, k2c);
+    hxdrift_c_rev_n<NV>(p, l, ca, h0, c4a);
   }
-  if (psi1 != 0) edge_rev(p, l, psi1);
+  if (psi1 != 0) edge_rev_n<NV>(p, l, psi1);
 }
 
-template <int HI, bool THICK, class PT>
-__device__ __forceinline__ void strmp_rev(Part &p, Adj &l, const Hdr h, const PT *P, const Knob &kn,
-                                          double *grad) {
+template <int NV, int HI, bool THICK, class PT, class GA>
+__device__ __forceinline__ void strmp_rev(Part &p, Adj (&l)[NV], const Hdr h, const PT *P, const Knob &kn,
+                                          const GA &ga) {
   const PT *C = THICK ? P + 4 : P;
   double b[HI + 1], a[HI + 1];
 #pragma unroll
@@ -54,7 +54,7 @@ __device__ __forceinline__ void strmp_rev(Part &p, Adj &l, const Hdr h, const PT
     a[i] = C[HI + 1 + i];
   }
   if (!THICK) {
-    mpole_kick_rev<HI>(p, l, b, a, 1.0, kn, grad);
+    mpole_kick_rev_n<NV, HI>(p, l, b, a, 1.0, kn, ga);
     return;
   }
   const int ns = h.a;
@@ -62,22 +62,22 @@ __device__ __forceinline__ void strmp_rev(Part &p, Adj &l, const Hdr h, const PT
   const double n1 = l1 * p.ipd, n2 = l2 * p.ipd;
   const double z1 = n1 * (0.5 * p.ipd), z2 = n2 * (0.5 * p.ipd);
   for (int m = 0; m < ns; ++m) {
-    drift_c_rev(p, l, n1, z1);
-    mpole_kick_rev<HI>(p, l, b, a, k1, kn, grad);
-    drift_c_rev(p, l, n2, z2);
-    mpole_kick_rev<HI>(p, l, b, a, k2, kn, grad);
-    drift_c_rev(p, l, n2, z2);
-    mpole_kick_rev<HI>(p, l, b, a, k1, kn, grad);
-    drift_c_rev(p, l, n1, z1);
+    drift_c_rev_n<NV>(p, l, n1, z1);
+    mpole_kick_rev_n<NV, HI>(p, l, b, a, k1, kn, ga);
+    drift_c_rev_n<NV>(p, l, n2, z2);
+    mpole_kick_rev_n<NV, HI>(p, l, b, a, k2, kn, ga);
+    drift_c_rev_n<NV>(p, l, n2, z2);
+    mpole_kick_rev_n<NV, HI>(p, l, b, a, k1, kn, ga);
+    drift_c_rev_n<NV>(p, l, n1, z1);
   }
 }
 
 // StrMPole with no field: drifts, plus the knob derivative if a knob sits on it.
-template <class PT>
-__device__ __forceinline__ void strmp_empty_rev(Part &p, Adj &l, const Hdr h, const PT *P, bool thick,
-                                                const Knob &kn, double *grad) {
+template <int NV, class PT, class GA>
+__device__ __forceinline__ void strmp_empty_rev(Part &p, Adj (&l)[NV], const Hdr h, const PT *P, bool thick,
+                                                const Knob &kn, const GA &ga) {
   if (!thick) {
-    mpole_knob_only(p, l, 1.0, kn, grad);
+    mpole_knob_only_n<NV>(p, l, 1.0, kn, ga);
     return;
   }
   const int ns = h.a;
@@ -85,33 +85,33 @@ __device__ __forceinline__ void strmp_empty_rev(Part &p, Adj &l, const Hdr h, co
   const double n1 = l1 * p.ipd, n2 = l2 * p.ipd;
   const double z1 = n1 * (0.5 * p.ipd), z2 = n2 * (0.5 * p.ipd);
   for (int m = 0; m < ns; ++m) {
-    drift_c_rev(p, l, n1, z1);
-    mpole_knob_only(p, l, k1, kn, grad);
-    drift_c_rev(p, l, n2, z2);
-    mpole_knob_only(p, l, k2, kn, grad);
-    drift_c_rev(p, l, n2, z2);
-    mpole_knob_only(p, l, k1, kn, grad);
-    drift_c_rev(p, l, n1, z1);
+    drift_c_rev_n<NV>(p, l, n1, z1);
+    mpole_knob_only_n<NV>(p, l, k1, kn, ga);
+    drift_c_rev_n<NV>(p, l, n2, z2);
+    mpole_knob_only_n<NV>(p, l, k2, kn, ga);
+    drift_c_rev_n<NV>(p, l, n2, z2);
+    mpole_knob_only_n<NV>(p, l, k1, kn, ga);
+    drift_c_rev_n<NV>(p, l, n1, z1);
   }
 }
 
-template <bool THICK, class PT>
-__device__ __forceinline__ void strmp_rev_dispatch(Part &p, Adj &l, const Hdr h, const PT *P,
-                                                   const Knob &kn, double *grad) {
+template <int NV, bool THICK, class PT, class GA>
+__device__ __forceinline__ void strmp_rev_dispatch(Part &p, Adj (&l)[NV], const Hdr h, const PT *P,
+                                                   const Knob &kn, const GA &ga) {
   // the record keeps every coefficient up to hi; the STRMP_SINGLE flag only
   // serves the forward kernel
   switch (h.b == -1 ? -1 : (h.b & 0xff)) {
-    case -1: strmp_empty_rev(p, l, h, P, THICK, kn, grad); break;
-    case 0: strmp_rev<0, THICK>(p, l, h, P, kn, grad); break;
-    case 1: strmp_rev<1, THICK>(p, l, h, P, kn, grad); break;
-    case 2: strmp_rev<2, THICK>(p, l, h, P, kn, grad); break;
-    case 3: strmp_rev<3, THICK>(p, l, h, P, kn, grad); break;
-    case 4: strmp_rev<4, THICK>(p, l, h, P, kn, grad); break;
-    case 5: strmp_rev<5, THICK>(p, l, h, P, kn, grad); break;
-    case 6: strmp_rev<6, THICK>(p, l, h, P, kn, grad); break;
-    case 7: strmp_rev<7, THICK>(p, l, h, P, kn, grad); break;
-    case 8: strmp_rev<8, THICK>(p, l, h, P, kn, grad); break;
-    default: strmp_rev<9, THICK>(p, l, h, P, kn, grad); break;
+    case -1: strmp_empty_rev<NV>(p, l, h, P, THICK, kn, ga); break;
+    case 0: strmp_rev<NV, 0, THICK>(p, l, h, P, kn, ga); break;
+    case 1: strmp_rev<NV, 1, THICK>(p, l, h, P, kn, ga); break;
+    case 2: strmp_rev<NV, 2, THICK>(p, l, h, P, kn, ga); break;
+    case 3: strmp_rev<NV, 3, THICK>(p, l, h, P, kn, ga); break;
+    case 4: strmp_rev<NV, 4, THICK>(p, l, h, P, kn, ga); break;
+    case 5: strmp_rev<NV, 5, THICK>(p, l, h, P, kn, ga); break;
+    case 6: strmp_rev<NV, 6, THICK>(p, l, h, P, kn, ga); break;
+    case 7: strmp_rev<NV, 7, THICK>(p, l, h, P, kn, ga); break;
+    case 8: strmp_rev<NV, 8, THICK>(p, l, h, P, kn, ga); break;
+    default: strmp_rev<NV, 9, THICK>(p, l, h, P, kn, ga); break;
   }
 }
 
@@ -121,18 +121,22 @@ __device__ __forceinline__ void strmp_rev_dispatch(Part &p, Adj &l, const Hdr h,
 // table (WigTable), whose registers would halve the occupancy of every
 // lattice's sweep; the host picks the variant per lattice.  The host also
 // refuses what has no reverse map (noise, quantum excitation, wakes).
-template <bool FULL, class PT>
-__device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const PT *P, const Knob &kn,
-                                            double *grad) {
+//
+// NV adjoint vectors at once (l[0..NV-1], knob derivatives through ga) in
+// the lean variant; the full variant carries one.
+template <bool FULL, int NV, class PT, class GA>
+__device__ __forceinline__ void element_rev_body(Part &p, Adj (&l)[NV], const Hdr h, const PT *P,
+                                                 const Knob &kn, const GA &ga) {
+  static_assert(!FULL || NV == 1, "the full variant carries one adjoint vector");
   switch (h.op) {
     case OP_DRIFT:
-      drift_nl_rev(p, l, P[0]);
+      drift_nl_rev_n<NV>(p, l, P[0]);
       break;
     case OP_QUAD_K0:
-      drift_kzero_rev(p, l, P[0]);
+      drift_kzero_rev_n<NV>(p, l, P[0]);
       break;
     case OP_MATRIX:
-      if (FULL) matrix_rev(p, l, P[0], P + 1);
+      if constexpr (FULL) matrix_rev(p, l[0], P[0], P + 1);
       break;
     case OP_QUAD: {
       const double L = P[0], K = P[1], aK = P[2], L2 = P[3];
@@ -140,15 +144,15 @@ __device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const 
       const double sK = foc ? 1.0 : -1.0;
       switch (h.a) {
         case 0:
-          if (FULL) quad_trig_rev(p, l, L, L2, foc, aK, sK, kn, grad);
+          if constexpr (FULL) quad_trig_rev(p, l[0], L, L2, foc, aK, sK, kn, ga.g);
           break;
-        case 4: quad_series_rev<4>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
-        case 6: quad_series_rev<6>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
-        case 8: quad_series_rev<8>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
-        case 10: quad_series_rev<10>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
-        case 12: quad_series_rev<12>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
-        case 14: quad_series_rev<14>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
-        default: quad_series_rev<16>(p, l, P + 4, L, L2, foc, aK, sK, kn, grad); break;
+        case 4: quad_series_rev_n<NV, 4>(p, l, P + 4, L, L2, foc, aK, sK, kn, ga); break;
+        case 6: quad_series_rev_n<NV, 6>(p, l, P + 4, L, L2, foc, aK, sK, kn, ga); break;
+        case 8: quad_series_rev_n<NV, 8>(p, l, P + 4, L, L2, foc, aK, sK, kn, ga); break;
+        case 10: quad_series_rev_n<NV, 10>(p, l, P + 4, L, L2, foc, aK, sK, kn, ga); break;
+        case 12: quad_series_rev_n<NV, 12>(p, l, P + 4, L, L2, foc, aK, sK, kn, ga); break;
+        case 14: quad_series_rev_n<NV, 14>(p, l, P + 4, L, L2, foc, aK, sK, kn, ga); break;
+        default: quad_series_rev_n<NV, 16>(p, l, P + 4, L, L2, foc, aK, sK, kn, ga); break;
       }
       break;
     }
@@ -156,59 +160,84 @@ __device__ __forceinline__ void element_rev(Part &p, Adj &l, const Hdr h, const 
       // plain: both curvature coefficients zero (always so for MaxNPole = 1)
       const bool plain = (h.b >> 8) & 1;
       switch (h.b & 0xff) {
-        case 0: bendmp_rev<0, true>(p, l, h, P); break;
-        case 1: plain ? bendmp_rev<1, true>(p, l, h, P) : bendmp_rev<1, false>(p, l, h, P); break;
-        case 2: plain ? bendmp_rev<2, true>(p, l, h, P) : bendmp_rev<2, false>(p, l, h, P); break;
-        case 3: plain ? bendmp_rev<3, true>(p, l, h, P) : bendmp_rev<3, false>(p, l, h, P); break;
-        default: plain ? bendmp_rev<9, true>(p, l, h, P) : bendmp_rev<9, false>(p, l, h, P); break;
+        case 0: bendmp_rev<NV, 0, true>(p, l, h, P); break;
+        case 1: plain ? bendmp_rev<NV, 1, true>(p, l, h, P) : bendmp_rev<NV, 1, false>(p, l, h, P); break;
+        case 2: plain ? bendmp_rev<NV, 2, true>(p, l, h, P) : bendmp_rev<NV, 2, false>(p, l, h, P); break;
+        case 3: plain ? bendmp_rev<NV, 3, true>(p, l, h, P) : bendmp_rev<NV, 3, false>(p, l, h, P); break;
+        default: plain ? bendmp_rev<NV, 9, true>(p, l, h, P) : bendmp_rev<NV, 9, false>(p, l, h, P); break;
       }
       break;
     }
     case OP_BEND: {  // edge, bend6, edge (ABend::pass, accel.cpp:523-550)
-      if (!FULL) break;
-      const double psi1 = P[4], psi2 = P[5];
-      if (psi2 != 0) edge_rev(p, l, psi2);
-      bend_linear_rev(p, l, P[0], P[1], P[2]);
-      if (psi1 != 0) edge_rev(p, l, psi1);
+      if constexpr (FULL) {
+        const double psi1 = P[4], psi2 = P[5];
+        if (psi2 != 0) edge_rev(p, l[0], psi2);
+        bend_linear_rev(p, l[0], P[0], P[1], P[2]);
+        if (psi1 != 0) edge_rev(p, l[0], psi1);
+      }
       break;
     }
     case OP_STRMP:
-      strmp_rev_dispatch<true>(p, l, h, P, kn, grad);
+      strmp_rev_dispatch<NV, true>(p, l, h, P, kn, ga);
       break;
     case OP_STRMP_THIN:
-      strmp_rev_dispatch<false>(p, l, h, P, kn, grad);
+      strmp_rev_dispatch<NV, false>(p, l, h, P, kn, ga);
       break;
     case OP_CAVITY:
-      cavity_rev(p, l, P[0], P[1], P[2], P[3], h.a != 0);
+      cavity_rev_n<NV>(p, l, P[0], P[1], P[2], P[3], h.a != 0);
       break;
     case OP_APERTURE:
       // a particle that reaches the end alive passed every aperture: the
       // element acts on it as its drift (or the identity for len = 0)
-      if (P[0] != 0) drift_nl_rev(p, l, P[0]);
+      if (P[0] != 0) drift_nl_rev_n<NV>(p, l, P[0]);
       break;
     case OP_CORRECTOR:
-      corrector_rev(p, l, P[0], P[1], P[2]);
+      corrector_rev_n<NV>(p, l, P[0], P[1], P[2]);
       break;
     case OP_BENDMP_RAD:
-      if (FULL) bendmp_rad_rev(p, l, h, P);
+      if constexpr (FULL) bendmp_rad_rev(p, l[0], h, P);
       break;
     case OP_SYNCRAD:
-      if (FULL) syncrad_rev(p, l, P, (h.a & 1) != 0);
+      if constexpr (FULL) syncrad_rev(p, l[0], P, (h.a & 1) != 0);
       break;
     case OP_CRABCAV:
-      if (FULL) crabcav_rev(p, l, P, (h.a & 1) != 0);
+      if constexpr (FULL) crabcav_rev(p, l[0], P, (h.a & 1) != 0);
       break;
     case OP_RFMULT:
-      if (FULL) rfmult_rev(p, l, P, h.b);
+      if constexpr (FULL) rfmult_rev(p, l[0], P, h.b);
       break;
     case OP_IDEALWIG:
-      if (FULL) idealwig_rev(p, l, P, h.a, h.b & 0xff, ((h.b >> 8) & 1) != 0);
+      if constexpr (FULL) idealwig_rev(p, l[0], P, h.a, h.b & 0xff, ((h.b >> 8) & 1) != 0);
       break;
     case OP_WIGTABLE:
-      if (FULL) wigtable_rev(p, l, P, h.a, h.b & 0xffff, h.b >> 16);
+      if constexpr (FULL) wigtable_rev(p, l[0], P, h.a, h.b & 0xffff, h.b >> 16);
       break;
     default:
       break;
+  }
+}
+
+// The knob derivatives of one element (all to its one slot) summed in
+// registers and added to the gradient once after the element.
+template <int NV>
+struct ElemAcc {
+  double *a;
+  __device__ void operator()(int v, int, double x) const { a[v] += x; }
+};
+template <bool FULL, int NV, class PT, class GA>
+__device__ __forceinline__ void element_rev(Part &p, Adj (&l)[NV], const Hdr h, const PT *P, const Knob &kn,
+                                            const GA &ga) {
+  if constexpr (FULL) {
+    element_rev_body<FULL, NV>(p, l, h, P, kn, ga);
+  } else {
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    element_rev_body<FULL, NV>(p, l, h, P, kn, ElemAcc<NV>{acc});
+    if (kn.kind != 0) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) ga(v, kn.slot, acc[v]);
+    }
   }
 }
 
@@ -258,9 +287,9 @@ __device__ __forceinline__ RevTable stage_table(const AdjCtx &c, unsigned char *
 // particle inside the recomputed group of segments; at the start of every
 // turn the state is reset to the exact recomputed one.  The next element's
 // header and knob are fetched while the current element runs.
-template <bool LDS, bool FULL>
+template <bool LDS, bool FULL, int NV, class GA>
 __device__ __forceinline__ void reverse_turns(const AdjCtx &c, const RevTable &T, int64_t q, int t_lo,
-                                              int t_hi, Part &p, Adj &l, double *grad) {
+                                              int t_hi, Part &p, Adj (&l)[NV], const GA &ga) {
   for (int t = t_hi - 1; t >= t_lo; --t) {
     // exact state at the start of turn t, loaded early: nothing below
     // depends on it until the turn is done
@@ -274,7 +303,7 @@ __device__ __forceinline__ void reverse_turns(const AdjCtx &c, const RevTable &T
       Hdr hn;
       Knob kn_next;
       load_rev<LDS>(T.hdr, T.knob, e > c.st ? e - 1 : e, hn, kn_next);
-      element_rev<FULL>(p, l, h, T.par + h.off, kn, grad);
+      element_rev<FULL, NV>(p, l, h, T.par + h.off, kn, ga);
       h = hn;
       kn = kn_next;
     }
@@ -307,7 +336,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_reverse(AdjCtx c) {
   for (int t1 = c.t_hi; t1 > c.t_lo;) {
     const int seg = (t1 - 1) / c.ckpt_every;
     const int t0 = seg * c.ckpt_every > c.t_lo ? seg * c.ckpt_every : c.t_lo;
-    reverse_turns<LDS, FULL>(c, T, (int64_t)(seg - c.seg0) * c.n + i, t0, t1, p, l, grad);
+    reverse_turns<LDS, FULL, 1>(c, T, (int64_t)(seg - c.seg0) * c.n + i, t0, t1, p, as_vec(l),
+                                GradAcc1{grad});
     t1 = t0;
   }
   pe[0] = p.x; pe[1] = p.px; pe[2] = p.y; pe[3] = p.py; pe[4] = p.z; pe[5] = p.d;
@@ -339,10 +369,59 @@ __global__ __launch_bounds__(BLOCK) void k_adj_segjac(AdjCtx c) {
   for (int k = 0; k < ADJ_MAX_PARAM; ++k) grad[k] = 0.0;
   const int t0 = seg * c.ckpt_every;
   const int t1 = t0 + c.ckpt_every < c.nturn ? t0 + c.ckpt_every : c.nturn;
-  reverse_turns<LDS, FULL>(c, T, q, t0, t1, p, l, grad);
+  reverse_turns<LDS, FULL, 1>(c, T, q, t0, t1, p, as_vec(l), GradAcc1{grad});
   double *o = c.jac + (q * 6 + col) * (6 + c.nparam);
   o[0] = l.x; o[1] = l.px; o[2] = l.y; o[3] = l.py; o[4] = l.z; o[5] = l.d;
   for (int k = 0; k < c.nparam; ++k) o[6 + k] = grad[k];
+}
+
+// Knob gradients of NV adjoint vectors, G slots each.
+template <int NV, int G>
+struct GradAccN {
+  double (*g)[G];
+  __device__ void operator()(int v, int slot, double x) const { g[v][slot] += x; }
+};
+
+// Segment Jacobians, lean variant, NV columns per lane: one lane per
+// (segment, particle, group of NV columns) carries the adjoints of its
+// columns through each element together, so the element's inverse map and
+// partial derivatives are formed once for all of them (6 / NV lanes per
+// segment and particle instead of 6).  Same output as k_adj_segjac.
+template <int BLOCK, bool LDS, int NV, int G>
+__global__ __launch_bounds__(BLOCK) void k_adj_segjac_v(AdjCtx c) {
+  static_assert(6 % NV == 0, "NV divides the six columns");
+  constexpr int NG = 6 / NV;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const RevTable T = stage_table<BLOCK, LDS>(c, lds_raw);
+  const int64_t gid = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (gid >= c.group_n * NG) return;
+  const int part = (int)(gid % NG);
+  const int64_t q = gid / NG;  // segment-in-group * n + particle
+  const int seg = c.seg0 + (int)(q / c.n);
+  const double *pe = c.r_end + 6 * q;
+  Part p;
+  p.x = pe[0]; p.px = pe[1]; p.y = pe[2]; p.py = pe[3]; p.z = pe[4];
+  set_delta(p, pe[5]);
+  if (p.x != p.x) return;
+  Adj l[NV];
+  double grad[NV][G];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const int col = part * NV + v;
+    l[v] = Adj{col == 0 ? 1.0 : 0.0, col == 1 ? 1.0 : 0.0, col == 2 ? 1.0 : 0.0,
+               col == 3 ? 1.0 : 0.0, col == 4 ? 1.0 : 0.0, col == 5 ? 1.0 : 0.0};
+#pragma unroll
+    for (int k = 0; k < G; ++k) grad[v][k] = 0.0;
+  }
+  const int t0 = seg * c.ckpt_every;
+  const int t1 = t0 + c.ckpt_every < c.nturn ? t0 + c.ckpt_every : c.nturn;
+  reverse_turns<LDS, false, NV>(c, T, q, t0, t1, p, l, GradAccN<NV, G>{grad});
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    double *o = c.jac + (q * 6 + part * NV + v) * (6 + c.nparam);
+    o[0] = l[v].x; o[1] = l[v].px; o[2] = l[v].y; o[3] = l[v].py; o[4] = l[v].z; o[5] = l[v].d;
+    for (int k = 0; k < c.nparam; ++k) o[6 + k] = grad[v][k];
+  }
 }
 
 __global__ __launch_bounds__(64) void k_adj_combine(AdjCtx c) {
@@ -392,6 +471,19 @@ hipError_t launch_adj_reverse(const AdjCtx &c, hipStream_t s) {
 // the table (with a 64-lane workgroup a table of tens of KB would cap a CU
 // at two or three waves, below what the registers allow).
 hipError_t launch_adj_segjac(const AdjCtx &c, hipStream_t s) {
+  if (!c.full && c.nparam <= ADJ_SEGJAC_VEC_G) {
+    constexpr int NV = ADJ_SEGJAC_VEC_NV, G = ADJ_SEGJAC_VEC_G;
+    if (c.lds_bytes > 0) {
+      constexpr int B = 256;
+      const int64_t grid = (c.group_n * (6 / NV) + B - 1) / B;
+      hipLaunchKernelGGL((k_adj_segjac_v<B, true, NV, G>), dim3((unsigned)grid), dim3(B), c.lds_bytes, s, c);
+    } else {
+      constexpr int B = 64;
+      const int64_t grid = (c.group_n * (6 / NV) + B - 1) / B;
+      hipLaunchKernelGGL((k_adj_segjac_v<B, false, NV, G>), dim3((unsigned)grid), dim3(B), 0, s, c);
+    }
+    return hipGetLastError();
+  }
   if (c.lds_bytes > 0) {
     constexpr int B = 256;
     const int64_t grid = (c.group_n * 6 + B - 1) / B;

@@ -8,6 +8,15 @@
 namespace accp {
 
 constexpr int ADJ_MAX_PARAM = 16;
+// Segment Jacobians of lean lattices with at most ADJ_SEGJAC_VEC_G knobs:
+// ADJ_SEGJAC_VEC_NV columns per lane (k_adj_segjac_v).  All six in one lane
+// form each element's inverse and partial derivatives once per segment and
+// particle; on the 342-element, 8-knob lattice (4096 particles, 32 segments
+// of 16 turns) the sweep took 108 ms with one column per lane, 77 with two,
+// 62 with three and 49 with six, though six hold about 400 registers (one
+// wave per SIMD).
+constexpr int ADJ_SEGJAC_VEC_NV = 6;
+constexpr int ADJ_SEGJAC_VEC_G = 8;
 
 struct AdjCtx {
   const Hdr *hdr;
@@ -46,7 +55,8 @@ hipError_t launch_adj_reverse(const AdjCtx &c, hipStream_t s);
 // Segment Jacobians: one lane per (segment, particle, column c) walks its
 // segment back from the unit adjoint e_c, giving column c of the segment's
 // transposed Jacobian and of its knob derivatives; r_end holds each
-// segment's end state [segment][particle][6].
+// segment's end state [segment][particle][6].  Lean lattices with at most
+// ADJ_SEGJAC_VEC_G knobs take ADJ_SEGJAC_VEC_NV columns per lane.
 hipError_t launch_adj_segjac(const AdjCtx &c, hipStream_t s);
 // Chains the segment Jacobians of the group from the last segment to the
 // first: lam <- J^T lam, grad += G^T lam, one lane per particle.

@@ -256,7 +256,9 @@ int accp_batch_capture(accp_batch *b, double *d_r, const int64_t *offsets,
                        int nturn, const accp_tuning *tuning,
                        int32_t *d_loss_turn);
 int accp_batch_replay(accp_batch *b, void *stream);
-/* Time candidate launch shapes on a short run and return the fastest. */
+/* Time candidate launch shapes on a short run and return the fastest.  The
+ * candidates are tried from the smallest workgroup up; a later one replaces
+ * the best only when it is faster by more than 1%. */
 int accp_autotune(accp_batch *b, double *d_r, const int64_t *offsets,
                   int nturn_probe, accp_tuning *best, double *best_ms);
 int accp_batch_free(accp_batch *b);

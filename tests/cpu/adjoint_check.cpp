@@ -2,8 +2,9 @@
 // (accel-plus_amd/csrc/accp_adjoint.h, compiled here for the host): for each
 // element map, (1) the inverse recovers the input state, (2) the
 // vector-Jacobian product equals central finite differences of the forward
-// map, including the knob derivatives.  Prints one line per check and exits
-// non-zero on the first failure.
+// map, including the knob derivatives; (3) the forms that carry several
+// adjoint vectors give, vector by vector, what the one-vector form gives.
+// Prints one line per check and exits non-zero on a failure.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -93,6 +94,96 @@ static void check(const char *name, std::function<void(Part &, double)> fwd,
   const bool ok = rt < 1e-11 && worst < tol;
   std::printf("%-28s roundtrip %.2e  vjp-vs-fd %.2e  %s\n", name, rt, worst, ok ? "ok" : "FAIL");
   if (!ok) ++failures;
+}
+
+// The forms that carry several adjoint vectors (*_n<NV>) against the
+// one-vector form run once per vector from the same state: the same
+// inverse state and, vector by vector, the same adjoint and knob
+// derivative, bit for bit (the host build contracts nothing).
+struct GradRows {
+  double (*g)[4];
+  void operator()(int v, int slot, double x) const { g[v][slot] += x; }
+};
+static void check_vec(const char *name, std::function<void(Part &, Adj (&)[3], GradRows)> rev3,
+                      std::function<void(Part &, Adj &, double *)> rev1, const double in[6]) {
+  Adj l3[3];
+  double w[3][6];
+  for (int v = 0; v < 3; ++v) {
+    for (int k = 0; k < 6; ++k) w[v][k] = U(-1, 1);
+    l3[v] = Adj{w[v][0], w[v][1], w[v][2], w[v][3], w[v][4], w[v][5]};
+  }
+  double g3[3][4] = {};
+  Part p3 = mk(in);
+  rev3(p3, l3, GradRows{g3});
+  bool same = true;
+  for (int v = 0; v < 3; ++v) {
+    Part p1 = mk(in);
+    Adj l1{w[v][0], w[v][1], w[v][2], w[v][3], w[v][4], w[v][5]};
+    double g1[4] = {};
+    rev1(p1, l1, g1);
+    const double a[13] = {p1.x, p1.px, p1.y, p1.py, p1.z, p1.d, l1.x, l1.px, l1.y, l1.py, l1.z, l1.d, g1[1]};
+    const double b[13] = {p3.x, p3.px, p3.y, p3.py, p3.z, p3.d, l3[v].x, l3[v].px, l3[v].y, l3[v].py,
+                          l3[v].z, l3[v].d, g3[v][1]};
+    for (int k = 0; k < 13; ++k) same = same && (a[k] == b[k] || (a[k] != a[k] && b[k] != b[k]));
+  }
+  std::printf("%-28s three vectors = one vector three times: %s\n", name, same ? "ok" : "FAIL");
+  if (!same) ++failures;
+}
+
+static void vector_forms(const double s0[6]) {
+  check_vec("drift x3", [](Part &p, Adj (&l)[3], GradRows) { drift_nl_rev_n<3>(p, l, 0.7); },
+            [](Part &p, Adj &l, double *) { drift_nl_rev(p, l, 0.7); }, s0);
+  check_vec("quad K=0 x3", [](Part &p, Adj (&l)[3], GradRows) { drift_kzero_rev_n<3>(p, l, 0.3); },
+            [](Part &p, Adj &l, double *) { drift_kzero_rev(p, l, 0.3); }, s0);
+  check_vec("edge x3", [](Part &p, Adj (&l)[3], GradRows) { edge_rev_n<3>(p, l, 0.013); },
+            [](Part &p, Adj &l, double *) { edge_rev(p, l, 0.013); }, s0);
+  const double h = 0.157 / 1.5, L = 0.15 * DRIFT1, hs = h * L, hq = 1 / 4. * hs * L;
+  check_vec("hxdrift x3",
+            [=](Part &p, Adj (&l)[3], GradRows) { hxdrift_c_rev_n<3>(p, l, hx_consts(p.ipd, L, hs, hq), h, hs * 0.5); },
+            [=](Part &p, Adj &l, double *) { hxdrift_c_rev(p, l, hx_consts(p.ipd, L, hs, hq), h, hs * 0.5); }, s0);
+  const double kl = 0.15 * KICK1, lh = kl * h;
+  const double b0[1] = {2e-4}, b2[3] = {1e-4, -0.3, 4.0};
+  check_vec("bend kick MO=0 x3",
+            [=](Part &p, Adj (&l)[3], GradRows) { bend_kick_rev_n<3, 0, true>(p, l, b0, kl, h, lh * p.d, lh); },
+            [=](Part &p, Adj &l, double *) { bend_kick_rev<0, true>(p, l, b0, kl, h, lh * p.d, lh); }, s0);
+  check_vec("bend kick MO=2 x3",
+            [=](Part &p, Adj (&l)[3], GradRows) {
+              bend_kick_rev_n<3, 2, false>(p, l, b2, kl, h, lh * p.d, lh, -0.3, 4.0);
+            },
+            [=](Part &p, Adj &l, double *) { bend_kick_rev<2, false>(p, l, b2, kl, h, lh * p.d, lh, -0.3, 4.0); },
+            s0);
+  double b9[10], a9[10];
+  for (int i = 0; i < 10; ++i) {
+    b9[i] = U(-1, 1) * std::pow(10.0, i);
+    a9[i] = U(-1, 1) * std::pow(10.0, i);
+  }
+  for (int kind = 2; kind <= 3; ++kind) {
+    const Knob kn{1, kind, 4};
+    check_vec(kind == 2 ? "multipole kick, PolyB knob x3" : "multipole kick, PolyA knob x3",
+              [=](Part &p, Adj (&l)[3], GradRows g) { mpole_kick_rev_n<3, 9>(p, l, b9, a9, 0.02, kn, g); },
+              [=](Part &p, Adj &l, double *g) { mpole_kick_rev<9>(p, l, b9, a9, 0.02, kn, g); }, s0);
+    check_vec(kind == 2 ? "empty multipole, PolyB knob x3" : "empty multipole, PolyA knob x3",
+              [=](Part &p, Adj (&l)[3], GradRows g) { mpole_knob_only_n<3>(p, l, 0.02, kn, g); },
+              [=](Part &p, Adj &l, double *g) { mpole_knob_only(p, l, 0.02, kn, g); }, s0);
+  }
+  static const std::vector<double> C8 = quad_coefs(8);
+  for (double K : {1.2, -1.1}) {
+    check_vec(K > 0 ? "series quad K>0 x3" : "series quad K<0 x3",
+              [=](Part &p, Adj (&l)[3], GradRows g) {
+                quad_series_rev_n<3, 8>(p, l, C8.data(), 0.3, 0.09, K > 0, std::fabs(K), K > 0 ? 1.0 : -1.0,
+                                        Knob{1, 1, 0}, g);
+              },
+              [=](Part &p, Adj &l, double *g) {
+                quad_series_rev<8>(p, l, C8.data(), 0.3, 0.09, K > 0, std::fabs(K), K > 0 ? 1.0 : -1.0,
+                                   Knob{1, 1, 0}, g);
+              },
+              s0);
+  }
+  const double wc = TWOPI * 562e6 / C0;
+  check_vec("cavity x3", [=](Part &p, Adj (&l)[3], GradRows) { cavity_rev_n<3>(p, l, 0.4, 5e-4, wc, 0.1, true); },
+            [=](Part &p, Adj &l, double *) { cavity_rev(p, l, 0.4, 5e-4, wc, 0.1, true); }, s0);
+  check_vec("corrector x3", [](Part &p, Adj (&l)[3], GradRows) { corrector_rev_n<3>(p, l, 0.2, 1e-4, -2e-4); },
+            [](Part &p, Adj &l, double *) { corrector_rev(p, l, 0.2, 1e-4, -2e-4); }, s0);
 }
 
 int main() {
@@ -323,6 +414,11 @@ int main() {
       check("wig table sliced", [=](Part &p, double) { wigtable(p, C.data(), 4, mx, my); },
             [=](Part &p, Adj &l, double *) { wigtable_rev(p, l, C.data(), 4, mx, my); }, 0, false, s0);
     }
+  }
+  for (int trial = 0; trial < 3; ++trial) {
+    const double d = trial == 0 ? -0.03 : trial == 1 ? 0.0 : 0.027;
+    const double s0[6] = {U(-3e-3, 3e-3), U(-3e-4, 3e-4), U(-2e-3, 2e-3), U(-2e-4, 2e-4), U(-1e-3, 1e-3), d};
+    vector_forms(s0);
   }
   std::printf("%s\n", failures ? "FAILED" : "all adjoint primitives ok");
   return failures ? 1 : 0;

@@ -116,13 +116,19 @@ def test_adjoint_rejects_unsupported_lattices(gpu_lib):
         A.backward(np.zeros((4, 6)))
 
 
-def test_reverse_modes_agree(gpu_lib):
+@pytest.mark.parametrize("extra_knobs", [False, True], ids=["8 knobs", "10 knobs"])
+def test_reverse_modes_agree(gpu_lib, extra_knobs):
     """The sequential sweep and the segment-Jacobian chain are the same
     linear algebra in a different order: gradients and dL/dr0 agree to
     round-off, with a partial last segment (37 turns, checkpoints every 8),
-    a particle lost on input and particles lost during the run."""
+    a particle lost on input and particles lost during the run.  With 8
+    knobs the segment Jacobians take all six columns in one lane; with 10
+    (a quadrupole term on each sextupole family added) one column per lane."""
     ring = adjoint_lattice()
     knobs = lat.adjoint_knobs(ring)
+    if extra_knobs:
+        knobs += [(f"{n}_b1", el.FIELD_POLYB, 1, lat.family(ring, n, el.STRMPOLE)) for n in ("SF", "SD")]
+        assert len(knobs) == 10
     beam = lat.gaussian_beam(512, seed=3) * 0.5
     beam[5, 0] = np.nan
     beam[7:12, 0] = 0.03  # far outside the dynamic aperture
