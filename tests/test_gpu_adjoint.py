@@ -123,11 +123,17 @@ def test_reverse_modes_agree(gpu_lib, extra_knobs):
     round-off, with a partial last segment (37 turns, checkpoints every 8),
     a particle lost on input and particles lost during the run.  With 8
     knobs the segment Jacobians take all six columns in one lane; with 10
-    (a quadrupole term on each sextupole family added) one column per lane."""
+    (the QF and QD families each split into two knobs) one column per lane."""
     ring = adjoint_lattice()
     knobs = lat.adjoint_knobs(ring)
     if extra_knobs:
-        knobs += [(f"{n}_b1", el.FIELD_POLYB, 1, lat.family(ring, n, el.STRMPOLE)) for n in ("SF", "SD")]
+        split = []
+        for name, f, i, m in knobs:
+            if name in ("QF", "QD"):
+                split += [(name + "a", f, i, m[0::2]), (name + "b", f, i, m[1::2])]
+            else:
+                split.append((name, f, i, m))
+        knobs = split
         assert len(knobs) == 10
     beam = lat.gaussian_beam(512, seed=3) * 0.5
     beam[5, 0] = np.nan
