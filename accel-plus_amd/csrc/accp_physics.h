@@ -541,17 +541,21 @@ ACCP_HD void matrix_map(Part &p, double L, const PT *m) {
 // loss, flag 2 = loss plus gauss*sigma (quantum excitation).  The curvature
 // (Iselin) terms are added to the multipole sum before it enters both the
 // radiated power and the kick.  rcp(v) returns 1/v for the velocity norm
-// (atpass.cpp:242) and for 1/(1+delta) after the loss.
-template <class PT, class Rcp>
+// (atpass.cpp:242) and for 1/(1+delta) after the loss.  PLAIN: k1c = k2c =
+// 0, and the curvature terms, exact zeros then, are left out (the same
+// values up to the sign of a zero).
+template <bool PLAIN = false, class PT, class Rcp>
 ACCP_HD void bend_kick_rad(Part &p, const PT *B, int mo, double L, double h, double k1c, double k2c,
                            double crad, double cq, int flag, double gauss, Rcp rcp) {
   const double pn = p.ipd;
   const double x = p.x, xp = p.px * pn, y = p.y, yp = p.py * pn;
   double re, im;
   horner_b(B, mo, x, y, re, im);
-  const double hk = h * (k1c * h - k2c);
-  im += hk * y * y * y * (1.0 / 6.0);
-  re += -k1c * h * y * y * 0.5 + hk * x * y * y * 0.5;
+  if constexpr (!PLAIN) {
+    const double hk = h * (k1c * h - k2c);
+    im += hk * y * y * y * (1.0 / 6.0);
+    re += -k1c * h * y * y * 0.5 + hk * x * y * y * 0.5;
+  }
   const double a = 1.0 + x * h;
   const double vn2 = rcp(a * a + xp * xp + yp * yp);
   const double by = re + h, bx = im;
@@ -571,10 +575,15 @@ ACCP_HD void bend_kick_rad(Part &p, const PT *B, int mo, double L, double h, dou
   p.ipd = rcp(opn);
   p.px = xp * opn;
   p.py = yp * opn;
-  p.px -= L * (-h * dnew + re + h * (h * x + k1c * (x * x - 0.5 * y * y) +
-                                     k2c * (x * x * x - (4.0 / 3.0) * x * y * y)));
-  p.py += L * (im + h * (k1c * x * y + (4.0 / 3.0) * k2c * x * x * y +
-                         (h / 6.0 * k1c - k2c / 3.0) * y * y * y));
+  if constexpr (PLAIN) {
+    p.px -= L * (-h * dnew + re + h * (h * x));
+    p.py += L * im;
+  } else {
+    p.px -= L * (-h * dnew + re + h * (h * x + k1c * (x * x - 0.5 * y * y) +
+                                       k2c * (x * x * x - (4.0 / 3.0) * x * y * y)));
+    p.py += L * (im + h * (k1c * x * y + (4.0 / 3.0) * k2c * x * x * y +
+                           (h / 6.0 * k1c - k2c / 3.0) * y * y * y));
+  }
   p.z -= L * h * x;
 }
 

@@ -61,6 +61,79 @@ __device__ inline double gauss_kick(uint64_t seed, uint64_t pid, uint64_t turn, 
   return (slice & 1) ? g1 : g0;
 }
 
+// The slices of a radiating ABendMPole (accel.cpp:649-705): the 2nd-order
+// integrator with quantum excitation (flag 2, accel.cpp:693-700) or the
+// 4th-order one with the classical loss (flag 1).  PLAIN: no curvature
+// gradients (bend_kick_rad<true>).  Delta moves at every kick, so the
+// pseudo-drift constants of 1/(1+delta) (hx_consts) are formed after each
+// kick, once for the pseudo-drifts up to the next one.
+template <int PPL, bool PLAIN, class PT>
+__device__ __forceinline__ void bendmp_rad_slices(Part (&p)[PPL], const Hdr h, const PT *P,
+                                                  const KernelCtx &ctx, const uint64_t (&pid)[PPL],
+                                                  uint64_t turn, int eidx) {
+  const int ns = h.a, mo = h.b & 0xff;
+  const double h0 = P[BM_IRHO], k1c = P[BM_K1C], k2c = P[BM_K2C], crad = P[BM_CRAD];
+  const PT *B = P + BM_B;
+  if (h.op == OP_BENDMP_QE) {
+    const double sl = P[BM_SL], cq = P[BM_CQ];
+    const double half = 0.5 * sl, hs = P[BM_HSH], hq = P[BM_HQH];
+    // slices 2j and 2j+1 take the two normals of one Philox block, drawn once
+    // per pair of slices (a loop over single slices gets the draw if-converted
+    // into every slice)
+    const double c4 = hs * 0.5;
+    HxC c[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) c[k] = hx_consts(p[k].ipd, half, hs, hq);
+    for (int m = 0; m < ns; m += 2) {
+      double g0[PPL], g1[PPL];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) gauss_pair(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, (uint32_t)(m >> 1), g0[k], g1[k]);
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) hxdrift_c(p[k], c[k], h0, c4);
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        bend_kick_rad<PLAIN>(p[k], B, mo, sl, h0, k1c, k2c, crad, cq, 2, g0[k], RcpNear1{});
+        c[k] = hx_consts(p[k].ipd, half, hs, hq);
+      }
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) hxdrift_c(p[k], c[k], h0, c4);
+      if (m + 1 < ns) {
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) hxdrift_c(p[k], c[k], h0, c4);
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+          bend_kick_rad<PLAIN>(p[k], B, mo, sl, h0, k1c, k2c, crad, cq, 2, g1[k], RcpNear1{});
+          c[k] = hx_consts(p[k].ipd, half, hs, hq);
+        }
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) hxdrift_c(p[k], c[k], h0, c4);
+      }
+    }
+  } else {
+    const double l1 = P[BM_L1], l2 = P[BM_L2], k1 = P[BM_K1], k2 = P[BM_K2];
+    const double hs1 = P[BM_HS1], hq1 = P[BM_HQ1], hs2 = P[BM_HS2], hq2 = P[BM_HQ2];
+    const double c4a = hs1 * 0.5, c4b = hs2 * 0.5;
+    // a pseudo-drift of length l with the constants of the particle's delta
+    auto drift = [&](double l, double hs, double hq, double c4) {
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) hxdrift_c(p[k], hx_consts(p[k].ipd, l, hs, hq), h0, c4);
+    };
+    auto kick = [&](double kl) {
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) bend_kick_rad<PLAIN>(p[k], B, mo, kl, h0, k1c, k2c, crad, 0.0, 1, 0.0, RcpNear1{});
+    };
+    for (int m = 0; m < ns; ++m) {
+      drift(l1, hs1, hq1, c4a);
+      kick(k1);
+      drift(l2, hs2, hq2, c4b);
+      kick(k2);
+      drift(l2, hs2, hq2, c4b);
+      kick(k1);
+      drift(l1, hs1, hq1, c4a);
+    }
+  }
+}
+
 // ABendMPole::pass (accel.cpp:649-705): edge, nslice slices of the 4th-order
 // integrator (radiation off or classical, flag <= 1) or of the 2nd-order one
 // with quantum excitation (flag 2, accel.cpp:693-700), edge.
@@ -77,58 +150,25 @@ __device__ __forceinline__ void bendmp(Part (&p)[PPL], const Hdr h, const PT *P,
 #pragma unroll
     for (int k = 0; k < PPL; ++k) edge(p[k], psi1);
   }
-  if (h.op == OP_BENDMP_QE) {
-    const double sl = P[BM_SL], crad = P[BM_CRAD], cq = P[BM_CQ];
-    const double half = 0.5 * sl, hs = P[BM_HSH], hq = P[BM_HQH];
-    // slices 2j and 2j+1 take the two normals of one Philox block, drawn once
-    // per pair of slices (a loop over single slices gets the draw if-converted
-    // into every slice)
-    for (int m = 0; m < ns; m += 2) {
-      double g0[PPL], g1[PPL];
-#pragma unroll
-      for (int k = 0; k < PPL; ++k) gauss_pair(ctx.rng_seed, pid[k], turn, (uint32_t)eidx, (uint32_t)(m >> 1), g0[k], g1[k]);
-#pragma unroll
-      for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
-#pragma unroll
-      for (int k = 0; k < PPL; ++k) bend_kick_rad(p[k], B, mo, sl, h0, k1c, k2c, crad, cq, 2, g0[k], RcpNear1{});
-#pragma unroll
-      for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
-      if (m + 1 < ns) {
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) bend_kick_rad(p[k], B, mo, sl, h0, k1c, k2c, crad, cq, 2, g1[k], RcpNear1{});
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) hxdrift(p[k], half, h0, hs, hq);
-      }
-    }
+  if (h.op == OP_BENDMP_QE || h.op == OP_BENDMP_RAD) {
+    if (plain) bendmp_rad_slices<PPL, true>(p, h, P, ctx, pid, turn, eidx);
+    else bendmp_rad_slices<PPL, false>(p, h, P, ctx, pid, turn, eidx);
   } else {
     const double l1 = P[BM_L1], l2 = P[BM_L2], k1 = P[BM_K1], k2 = P[BM_K2];
     const double hs1 = P[BM_HS1], hq1 = P[BM_HQ1], hs2 = P[BM_HS2], hq2 = P[BM_HQ2];
-    const bool rad = h.op == OP_BENDMP_RAD;
-    const double crad = P[BM_CRAD];
     for (int m = 0; m < ns; ++m) {
 #pragma unroll
       for (int k = 0; k < PPL; ++k) hxdrift(p[k], l1, h0, hs1, hq1);
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) {
-        if (rad) bend_kick_rad(p[k], B, mo, k1, h0, k1c, k2c, crad, 0.0, 1, 0.0, RcpNear1{});
-        else bend_kick(p[k], B, mo, plain, k1, h0, k1c, k2c);
-      }
+      for (int k = 0; k < PPL; ++k) bend_kick(p[k], B, mo, plain, k1, h0, k1c, k2c);
 #pragma unroll
       for (int k = 0; k < PPL; ++k) hxdrift(p[k], l2, h0, hs2, hq2);
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) {
-        if (rad) bend_kick_rad(p[k], B, mo, k2, h0, k1c, k2c, crad, 0.0, 1, 0.0, RcpNear1{});
-        else bend_kick(p[k], B, mo, plain, k2, h0, k1c, k2c);
-      }
+      for (int k = 0; k < PPL; ++k) bend_kick(p[k], B, mo, plain, k2, h0, k1c, k2c);
 #pragma unroll
       for (int k = 0; k < PPL; ++k) hxdrift(p[k], l2, h0, hs2, hq2);
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) {
-        if (rad) bend_kick_rad(p[k], B, mo, k1, h0, k1c, k2c, crad, 0.0, 1, 0.0, RcpNear1{});
-        else bend_kick(p[k], B, mo, plain, k1, h0, k1c, k2c);
-      }
+      for (int k = 0; k < PPL; ++k) bend_kick(p[k], B, mo, plain, k1, h0, k1c, k2c);
 #pragma unroll
       for (int k = 0; k < PPL; ++k) hxdrift(p[k], l1, h0, hs1, hq1);
     }

@@ -8,8 +8,9 @@
 // sequence of accel.cpp:829-843, and the order-9 multipole
 // sum by Estrin's scheme (estrin9, and estrin9_lo2 when orders 0 and 1 are
 // zero) against Horner's (horner_reg<9>, the order of strthinkick,
-// atpass.cpp:358-414).  They must agree to
-// round-off over random states; a dropped term or a wrong sign does not.
+// atpass.cpp:358-414), and the radiation kick's form without curvature
+// gradients against its general form.  They must agree to round-off over
+// random states; a dropped term or a wrong sign does not.
 // Prints one line per check and exits non-zero on a failure.
 #include <cmath>
 #include <cstdio>
@@ -171,6 +172,30 @@ int main() {
               "the largest term\n",
               worst);
   if (!(worst < 1e-14)) fail = 1;
+  // the radiation kick without curvature gradients (k1c = k2c = 0): the
+  // plain form leaves out the terms that are exact zeros, so it equals the
+  // general form (compared with ==, which also equates -0 and +0)
+  {
+    struct Div {
+      double operator()(double v) const { return 1.0 / v; }
+    };
+    bool same = true;
+    const double hb = 6.283185307179586 / 40 / 1.5;
+    for (int it = 0; it < 20000; ++it) {
+      const double scale = it % 2 ? 1e-3 : 2e-2;
+      Part p = random_part(scale), q = p;
+      const double b[1] = {U(-1e-4, 1e-4)};
+      const int flag = 1 + it % 2;
+      const double g = U(-3, 3), crad = 8.85e-5 * 27 / 6.283185307179586, cq = flag == 2 ? 1e-9 : 0.0;
+      bend_kick_rad(p, b, 0, 0.15, hb, 0.0, 0.0, crad, cq, flag, g, Div{});
+      bend_kick_rad<true>(q, b, 0, 0.15, hb, 0.0, 0.0, crad, cq, flag, g, Div{});
+      same = same && p.x == q.x && p.px == q.px && p.y == q.y && p.py == q.py && p.z == q.z && p.d == q.d &&
+             p.ipd == q.ipd;
+    }
+    std::printf("radiation kick without curvature gradients, 20000 states: %s\n",
+                same ? "equal to the general form" : "DIFFERS");
+    if (!same) fail = 1;
+  }
   std::printf(fail ? "FAILED\n" : "all regrouped maps ok\n");
   return fail;
 }
