@@ -116,17 +116,22 @@ def test_adjoint_rejects_unsupported_lattices(gpu_lib):
         A.backward(np.zeros((4, 6)))
 
 
-@pytest.mark.parametrize("extra_knobs", [False, True], ids=["8 knobs", "10 knobs"])
-def test_reverse_modes_agree(gpu_lib, extra_knobs):
+@pytest.mark.parametrize("case", ["8 knobs", "10 knobs", "long line"])
+def test_reverse_modes_agree(gpu_lib, case):
     """The sequential sweep and the segment-Jacobian chain are the same
     linear algebra in a different order: gradients and dL/dr0 agree to
     round-off, with a partial last segment (37 turns, checkpoints every 8),
     a particle lost on input and particles lost during the run.  With 8
     knobs the segment Jacobians take all six columns in one lane; with 10
-    (the QF and QD families each split into two knobs) one column per lane."""
+    (the QF and QD families each split into two knobs) one column per lane;
+    the long line (the ring twice, 684 elements) holds a table beyond the
+    64 KB the reverse kernels stage in LDS, so they read it from global
+    memory."""
     ring = adjoint_lattice()
+    if case == "long line":
+        ring = ring + [e.copy() for e in ring]
     knobs = lat.adjoint_knobs(ring)
-    if extra_knobs:
+    if case == "10 knobs":
         split = []
         for name, f, i, m in knobs:
             if name in ("QF", "QD"):
