@@ -19,7 +19,8 @@ struct WakeElem {
   int nbins = 0;
   double zmin = 0, dz = 0, qm_charge = 0, e0ev = 0;
   double *d_wt = nullptr;    // W(d*dz/c), d = 0..nbins-1 (W(0) = alpha*Rs)
-  uint32_t *d_cnt = nullptr; // histogram of the current turn
+  uint32_t *d_cnt = nullptr; // histogram of the current pass, then a finish counter (nbins + 1);
+                             // cleared between passes by the kernel that reads them
   double *d_v = nullptr;     // wake potential per bin [V]
   // TransWake: length, strengths, decay and wave numbers per plane
   double len = 0, wx = 0, wy = 0, ax = 0, ay = 0, kbx = 0, kby = 0;

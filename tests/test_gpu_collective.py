@@ -21,20 +21,26 @@ from helpers import assert_coords_close
 pytestmark = pytest.mark.gpu
 
 
-def wake_elem(on=1):
-    return el.longwake(5e3, 1.0, 10e9, 2e-9, 256, -0.015, 0.015, energy=3000.0, on=on)
+def wake_elem(on=1, nbins=256):
+    return el.longwake(5e3, 1.0, 10e9, 2e-9, nbins, -0.015, 0.015, energy=3000.0, on=on)
 
 
-def test_wake_kick_matches_oracle_binned(gpu_lib, oracle):
+@pytest.mark.parametrize("nbins", [256, 1024])
+def test_wake_kick_matches_oracle_binned(gpu_lib, oracle, nbins):
+    """One pass and two passes in a row (the counts of the first must not
+    leak into the second) against the oracle; 256 bins form the potential
+    inside the kick kernel, 1024 in a kernel of their own."""
     beam = lat.gaussian_beam(20000, seed=8)
     beam[::1000, 0] = np.nan  # lost particles neither kick nor get kicked
-    w = [wake_elem()]
-    out = gpu_lib.lattice(w).track(beam, 1)
+    w = [wake_elem(nbins=nbins)]
+    L = gpu_lib.lattice(w)
+    out = L.track(beam, 1)
     ref = oracle.track(w, beam, 1)
     dd = out[:, 5] - beam[:, 5]
     assert np.abs(np.nan_to_num(dd)).max() > 1e-7  # the kick is there
     assert_coords_close(out, ref, 1e-12, what="wake kick")
-    off = gpu_lib.lattice([wake_elem(on=0)]).track(beam, 1)
+    assert_coords_close(L.track(beam, 2), oracle.track(w, beam, 2), 1e-12, what="two wake passes")
+    off = gpu_lib.lattice([wake_elem(on=0, nbins=nbins)]).track(beam, 1)
     assert np.array_equal(np.nan_to_num(off), np.nan_to_num(beam))
 
 
