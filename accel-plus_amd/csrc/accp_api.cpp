@@ -486,6 +486,12 @@ struct RunSpec {
   int32_t *d_loss_turn = nullptr;
   double *snap = nullptr;
   int nsnap = 0, mon_interval = 1;
+  // a run that is one piece of a call (the turns split at a wake): its first
+  // turn within the call, whether it starts the call, and the call's turn
+  // count (what survivors' loss turns read)
+  int turn_rel_base = 0;
+  bool first = true;
+  int loss_nturn = -1;
 };
 
 int resolve_tuning(const accp_tuning *t, size_t lds_need, int &block, int &ppl, bool &lds, int &chunk_max) {
@@ -555,7 +561,8 @@ int enqueue_run(const TableRef &T, const WsView &ws, int max_tiles, double *d_r,
   const int64_t *d_offs = ws.offs;
   int cur = 0;
   if (init) {
-    HIPCHK(launch_chunk_init(ws.idx[0], n, R.d_loss_turn, R.nturn, ws.cnt[0], d_offs, T.nseed, s));
+    HIPCHK(launch_chunk_init(ws.idx[0], n, R.d_loss_turn, R.loss_nturn >= 0 ? R.loss_nturn : R.nturn, ws.cnt[0],
+                             d_offs, T.nseed, s));
     HIPCHK(launch_make_tiles(ws.cnt[0], d_offs, T.nseed, per, ws.tiles, ws.ntiles, s));
   }
   KernelCtx c{};
@@ -583,7 +590,8 @@ int enqueue_run(const TableRef &T, const WsView &ws, int max_tiles, double *d_r,
     const int cn = compact ? std::min(chunk, R.nturn - t) : R.nturn;
     c.idx = ws.idx[cur];
     c.turn0 = R.turn0 + (uint64_t)t;
-    c.turn_rel0 = t;
+    c.turn_rel0 = R.turn_rel_base + t;
+    c.first_launch = R.first && t == 0;
     c.nturn = cn;
     if (spec) HIPCHK(launch_spec(spec, c, max_tiles, lds ? T.lds_need : 0, s));
     else HIPCHK(launch_track(c, max_tiles, block, ppl, lds, lds ? T.lds_need : 0, T.full, s));
@@ -778,21 +786,30 @@ static int track_device_impl(accp_lattice *L, double *d_r, int64_t n, const RunS
     ws = view_of(L->ws);
   }
   if (cuts.empty()) return enqueue_run(T, ws, max_tiles, d_r, n, Rg, tuning, s);
-  if (R.snap) return fail(ACCP_E_ARG, "monitor records with a wake element are not supported");
   accp_tuning one = tuning ? *tuning : accp_tuning{0, 0, 0, 0};
   one.compact = -1;  // every segment sees the whole beam
   bool init = true;  // the first segment builds the tile list, the others reuse it
+  const size_t beam_bytes = (size_t)n * 6 * sizeof(double);
   for (int t = 0; t < R.nturn; ++t) {
+    // monitor record of turn t: the beam as it starts the turn (a lost
+    // particle's frozen state is in d_r), the slot the tracker would fill
+    if (R.snap && t % R.mon_interval == 0 && t / R.mon_interval < R.nsnap)
+      HIPCHK(hipMemcpyAsync(R.snap + (size_t)(t / R.mon_interval) * n * 6, d_r, beam_bytes, hipMemcpyDeviceToDevice,
+                            s));
     RunSpec seg = R;
     seg.nturn = 1;
     seg.turn0 = R.turn0 + (uint64_t)t;
-    seg.d_loss_turn = nullptr;
+    seg.snap = nullptr;
+    seg.nsnap = 0;
+    seg.turn_rel_base = t;
+    seg.loss_nturn = R.nturn;
     int a = R.st;
     for (size_t k = 0; k <= cuts.size(); ++k) {
       const int b = k < cuts.size() ? cuts[k] : R.ed;
       if (b > a) {
         seg.st = a;
         seg.ed = b;
+        seg.first = init;
         if ((rc = enqueue_run(T, ws, max_tiles, d_r, n, seg, &one, s, init))) return rc;
         init = false;
       }

@@ -36,6 +36,7 @@ struct KernelCtx {
   int32_t mon_interval, nsnap;
   int32_t *loss_turn;        // per particle: turn lost in (relative to the call) or null
   int32_t turn_rel0;         // turn of this launch relative to the call start
+  int32_t first_launch;      // 1: the call's first launch (a particle lost on input gets -1)
 };
 
 }  // namespace accp

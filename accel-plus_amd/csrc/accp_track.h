@@ -773,7 +773,7 @@ __device__ __forceinline__ void track_tile(const KernelCtx &ctx) {
     set_delta(p[k], src[5]);
     // a particle already lost when this launch starts was recorded earlier,
     // unless this is the first launch of the call (lost on input: -1)
-    record[k] = inb[k] && ctx.loss_turn && (is_live(p[k]) || ctx.turn_rel0 == 0);
+    record[k] = inb[k] && ctx.loss_turn && (is_live(p[k]) || ctx.first_launch);
     lost_at[k] = 0x7fffffff;
     parked[k] = !inb[k] || !is_live(p[k]);
     if (!inb[k]) p[k].x = __builtin_nan("");  // padding lanes start lost, never written back

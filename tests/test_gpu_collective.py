@@ -55,6 +55,35 @@ def test_wake_lattice_tracks_whole_beam(gpu_lib, oracle):
     assert_coords_close(out, ref, 1e-10, what="ring with wake, 3 turns")
 
 
+def test_wake_lattice_monitor_and_loss_turns(gpu_lib, oracle):
+    """Monitor records and loss turns through a lattice whose turns split
+    at a wake, against the oracle turn by turn: statistics of the beam at
+    the start of turns 0, 4, 8 (lost particles in their frozen state), the
+    turn each particle was lost in (-1 lost on input, 9 survived)."""
+    ring = lat.da_ring(0)
+    ring.insert(120, wake_elem())
+    beam = lat.gaussian_beam(3000, seed=10)
+    beam[::250, 0] = np.nan  # lost on input
+    beam[7:11, 0] = 0.024  # inside the aperture, lost within a few turns
+    beam[7:11, 1] = 2e-3
+    out, recs, lt = gpu_lib.lattice(ring).track(beam, 9, mon_interval=4, loss_turn=True)
+    r = beam.copy()
+    ref_lt = np.where(np.isnan(beam[:, 0]), -1, 9)
+    assert len(recs) == 3
+    for t in range(9):
+        if t % 4 == 0:
+            o = oracle.beam_stat(r, t)
+            k = t // 4
+            assert recs[k][0] == t and recs[k][1] == o[1]
+            np.testing.assert_allclose(recs[k][2:], o[2:], rtol=1e-9, atol=1e-18)
+        nxt = oracle.track(ring, r, 1, turn0=t)
+        ref_lt[np.isnan(nxt[:, 0]) & ~np.isnan(r[:, 0])] = t
+        r = nxt
+    assert (ref_lt[7:11] < 9).all() and (ref_lt[7:11] >= 0).all()  # the run loses them
+    assert np.array_equal(lt, ref_lt), (lt[ref_lt != lt], ref_lt[ref_lt != lt])
+    assert_coords_close(out, r, 1e-10, what="ring with wake, 9 turns")
+
+
 def damping_partition(oracle, ring):
     """J_e = 2 + I4/I2 of the isomagnetic ring: periodic dispersion from the
     one-turn matrix, carried to every bend end with the transfer matrices
