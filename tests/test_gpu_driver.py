@@ -121,6 +121,37 @@ STOP
     np.testing.assert_allclose(st_s[:, 2:], st_w[:, 2:], rtol=1e-10, atol=1e-18)
 
 
+def test_track_with_wake_and_monitor(tmp_path, oracle):
+    """Do_Track on a ring holding a TransWake switched on by set_flags, with
+    monitor statistics (the turn splits at the wake): the Stat1 records and
+    the final coordinates against the oracle."""
+    ring = lat.fodo_ring() + [el.transwake(0.0, 2.0, 2.0e5, 1.5, 5.0, 2.0e5, 2.0, 3.0, on=1)]
+    el.write_lattice(ring, str(tmp_path / "tw.lat"))
+    beam = lat.gaussian_beam(400, seed=23)
+    el.write_plain_beam(beam, str(tmp_path / "beam.txt"))
+    out = run(tmp_path, """Lattice
+  file_name = tw.lat
+  set_flags: wake=on
+End
+Track
+  Distribution: Load beam.txt, format=plain
+  Monitor: interval=4, printflag=Stat1
+  Do_Track: NTurn=8, radiation=off, RFCavity=off, PrintFinal=Full
+End
+STOP
+""")
+    arr = el.to_array(ring)
+    stats = [numbers(s) for s in after(out, "$$$Track-Data: Stat1", 4)[2:4]]
+    r = beam.copy()
+    for k, rec in enumerate(stats):
+        ref = oracle.beam_stat(r)
+        assert rec[0] == 4 * k and rec[1] == ref[1]
+        assert np.allclose(rec[2:14], ref[2:14], rtol=1e-9, atol=1e-18), (k, rec, ref)
+        r = oracle.track(arr, r, 4, turn0=4 * k)
+    final = np.array([numbers(s)[1:7] for s in after(out, "$$$Track-Data: Full export", 401)[1:]])
+    assert_coords_close(final, r, 1e-9, what="8 turns with a transverse wake through the driver")
+
+
 def test_track_da_with_loss_turns(tmp_path):
     """TRACK_DA sets radiation (classical) and the cavity on, like the
     reference's DAMA block; with ERING phasing this is the trackda fixture's
