@@ -790,6 +790,8 @@ static int track_device_impl(accp_lattice *L, double *d_r, int64_t n, const RunS
   one.compact = -1;  // every segment sees the whole beam
   bool init = true;  // the first segment builds the tile list, the others reuse it
   const size_t beam_bytes = (size_t)n * 6 * sizeof(double);
+  // loss turns of a range that holds wakes alone (no tracking launch sets them)
+  if (R.d_loss_turn) HIPCHK(launch_loss_init(d_r, n, R.d_loss_turn, R.nturn, s));
   for (int t = 0; t < R.nturn; ++t) {
     // monitor record of turn t: the beam as it starts the turn (a lost
     // particle's frozen state is in d_r), the slot the tracker would fill

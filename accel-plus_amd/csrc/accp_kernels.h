@@ -19,6 +19,9 @@ hipError_t launch_chunk_init(int32_t *idx, int64_t n, int32_t *loss_turn, int32_
                              int32_t *cnt, const int64_t *offs, int nseed, hipStream_t s);
 hipError_t launch_make_tiles(const int32_t *cnt, const int64_t *offs, int nseed, int per,
                              TileDesc *tiles, int32_t *ntiles, hipStream_t s);
+// Loss turns from the beam alone: -1 for a particle lost on input, nturn for
+// the others (what a call that tracks no element leaves).
+hipError_t launch_loss_init(const double *r, int64_t n, int32_t *loss_turn, int32_t nturn, hipStream_t s);
 hipError_t launch_compact(const TileDesc *tiles, const int32_t *ntiles, int max_tiles,
                           const int32_t *idx_in, const double *r, const int64_t *offs,
                           int32_t *cnt_out, int32_t *idx_out, int nseed, hipStream_t s);

@@ -203,6 +203,17 @@ hipError_t launch_set_offsets(int64_t *offs, int64_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void k_loss_init(const double *r, int64_t n, int32_t *loss_turn, int32_t nturn) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    loss_turn[i] = r[6 * i] == r[6 * i] ? nturn : -1;
+}
+
+hipError_t launch_loss_init(const double *r, int64_t n, int32_t *loss_turn, int32_t nturn, hipStream_t s) {
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  if (grid > 0) hipLaunchKernelGGL(k_loss_init, dim3(grid), dim3(256), 0, s, r, n, loss_turn, nturn);
+  return hipGetLastError();
+}
+
 hipError_t launch_chunk_init(int32_t *idx, int64_t n, int32_t *loss_turn, int32_t nturn,
                              int32_t *cnt, const int64_t *offs, int nseed, hipStream_t s) {
   const int grid = (int)std::min<int64_t>((n + 255) / 256 + 1, 4096);

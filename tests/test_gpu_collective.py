@@ -66,7 +66,8 @@ def test_wake_lattice_monitor_and_loss_turns(gpu_lib, oracle):
     beam[::250, 0] = np.nan  # lost on input
     beam[7:11, 0] = 0.024  # inside the aperture, lost within a few turns
     beam[7:11, 1] = 2e-3
-    out, recs, lt = gpu_lib.lattice(ring).track(beam, 9, mon_interval=4, loss_turn=True)
+    L = gpu_lib.lattice(ring)
+    out, recs, lt = L.track(beam, 9, mon_interval=4, loss_turn=True)
     r = beam.copy()
     ref_lt = np.where(np.isnan(beam[:, 0]), -1, 9)
     assert len(recs) == 3
@@ -82,6 +83,9 @@ def test_wake_lattice_monitor_and_loss_turns(gpu_lib, oracle):
     assert (ref_lt[7:11] < 9).all() and (ref_lt[7:11] >= 0).all()  # the run loses them
     assert np.array_equal(lt, ref_lt), (lt[ref_lt != lt], ref_lt[ref_lt != lt])
     assert_coords_close(out, r, 1e-10, what="ring with wake, 9 turns")
+    # a range holding the wake alone: no tracking launch, the loss turns still set
+    _, lt_w = L.track(beam, 3, st=120, ed=121, loss_turn=True)
+    assert np.array_equal(lt_w, np.where(np.isnan(beam[:, 0]), -1, 3))
 
 
 def damping_partition(oracle, ring):
